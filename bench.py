@@ -1,0 +1,146 @@
+"""Throughput of the plain particle time step on one MI355X.
+
+Workload (2D LSM1 on an unstructured mesh): Delaunay triangulation of 1M
+jittered points in a 20 x 5 km channel (~2M elements, seed 7), two velocity
+frames of a cellular vortex (amplitude 1 m/s, wavelength 2 km) interpolated
+in time, depth 2-20 m, k-epsilon diffusivity K(x, y) in [0.1, 5] m2/s with
+its gradient drift, 10M FP64 particles, dt = 5 s, default Ziggurat normals.
+A "step" is one Solver.forward over the whole particle set.  Inputs are in
+HBM before the timed region, which brackets K steps with device syncs.
+
+Prints one JSON line (metric particle-steps/s) with the roofline of the
+fused step kernel and a CPU baseline (the oracle, one core, on a sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def c2_workload(npts_x=2000, npts_y=500, npart=10_000_000, seed=7):
+    from tests.cases import jittered_delaunay
+    lx, ly = 20000., 5000.
+    x, y, tri, mpl = jittered_delaunay(npts_x, npts_y, lx, ly, seed=seed)
+    n = x.size
+    k = 2. * np.pi / 2000.
+    times = np.array([0., 20000.])
+    u = np.stack([np.sin(k * x + ph) * np.cos(k * y) for ph in (0., 0.5)])
+    v = np.stack([-np.cos(k * x + ph) * np.sin(k * y) for ph in (0., 0.5)])
+    rng = np.random.default_rng(seed)
+    depth = 2. + 18. * (0.5 + 0.5 * np.sin(2 * np.pi * x / 7000. + 1.) * np.cos(2 * np.pi * y / 3000.))
+    K = 0.1 + 4.9 * (0.5 + 0.5 * np.cos(2 * np.pi * x / 5000.) * np.sin(2 * np.pi * y / 4000. + 0.3))
+    eps = np.full(n, 1e-3)
+    tke = np.sqrt(K * eps / (0.09 / 0.72))
+    fields = dict(dim=2, nlayers=1, times=times, elevation=np.stack([depth, depth]),
+                  velocity_x=u, velocity_y=v,
+                  opt_fields=np.stack([np.stack([tke, tke]), np.stack([eps, eps])]),
+                  opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
+    pos = np.stack([rng.uniform(0.01 * lx, 0.99 * lx, npart), rng.uniform(0.01 * ly, 0.99 * ly, npart)], 1)
+    params = dict(final_time=19000., time_step=5., model=1, diffusion_model=2)
+    return x, y, tri, fields, pos, params
+
+
+def bytes_per_particle_step(nt, nv, npart):
+    """Algorithmic bytes: evolving state read and written once (x, y f64 +
+    element id i32), plus the mesh and node fields each step must read once
+    (element record 84 B; node x, y, two frames of u, v, k, eps)."""
+    state = 2 * (16 + 4)
+    mesh = nt * 84 + nv * (16 + 2 * 2 * 16)
+    return state + mesh / npart, state, mesh
+
+
+def cpu_baseline(x, y, tri, fields, pos, params, nsample=100_000, nsteps=50):
+    """The oracle (serial restatement, one core) on a bounded sample."""
+    from oracle import refpy as R
+    mesh = R.RefMesh(x, y, tri)
+    fs = R.RefFields(mesh, **fields)
+    ps = R.RefParticles(pos[:nsample].copy(), dim=2)
+    s = R.RefSolver(fs, ps, params, rng=R.RefRng())
+    t0 = time.perf_counter()
+    s.forward(nsteps)
+    dt = time.perf_counter() - t0
+    return dict(value=nsample * nsteps / dt, unit="particle-steps/s", cores=1, kind="port",
+                sample="{} particles x {} steps of the same workload ({:.1f} s)".format(nsample, nsteps, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--npart", type=int, default=10_000_000)
+    ap.add_argument("--sort-interval", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the step runs on one MI355X holding the whole mesh "
+              "and particle set; --gpus {} is not supported".format(args.gpus))
+        sys.exit(2)
+    import cylag_amd as cy
+    t0 = time.time()
+    x, y, tri, fields, pos, params = c2_workload(npart=args.npart)
+    log("workload built: {} nodes, {} triangles, {} particles ({:.1f} s)".format(
+        x.size, tri.shape[0], pos.shape[0], time.time() - t0))
+    mesh = cy.TriangularMesh(x, y, tri)
+    fs = cy.EulerianFieldSet(mesh, **fields)
+    ps = cy.LagrangianParticleSet(pos, dim=2)
+    cy.pcg32_seed(42, 0xda3e06d11bb2b203)
+    t0 = time.time()
+    solver = cy.Solver(fs, ps, params)
+    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 256)
+    log("solver ready ({:.1f} s): {}".format(time.time() - t0, solver.ctx.device_info()[0]))
+    solver._run(args.warmup)
+    log("warmup done")
+    t0 = time.perf_counter()
+    solver._run(args.steps)
+    wall = time.perf_counter() - t0
+    k_ms, tot_ms, nlaunch = solver.ctx.last_timing()
+    nact = ps.npart_active if not ps._stale else None
+    ps._sync()
+    nact = ps.npart_active
+    psteps = nact * args.steps / wall
+    b_alg, b_state, b_mesh = bytes_per_particle_step(mesh.nt, mesh.nv, nact)
+    avg_ms = k_ms / max(nlaunch, 1)
+    achieved = b_alg * nact / (avg_ms * 1e-3)
+    line = {
+        "metric": "particle-steps/s (plain step, 1 GPU)",
+        "value": psteps,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D LSM1 k-eps, Delaunay 1M pts / {} elements, {} particles, 2 frames, "
+                               "dt=5 s, Ziggurat".format(mesh.nt, nact),
+                   "elements": int(mesh.nt), "nodes": int(mesh.nv), "particles": int(nact),
+                   "rng_method": 2, "sort_interval": args.sort_interval},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "B/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_step<2,1,false>", "kernel_avg_ms": avg_ms,
+                     "bytes_per_particle_step": b_alg},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(x, y, tri, fields, pos, params)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,21 @@
+"""cylag_amd: MI355X-native particle time step behind CyLag's Python API.
+
+Drop-in names: TriangularMesh, EulerianFieldSet, LagrangianParticleSet,
+Parameters, SimuTime, Solver, read_cli and the PCG32 module functions
+(cylag/__init__.py, cylag/core/__init__.py, cylag/geom/__init__.py,
+cylag/lsm/__init__.py).  The per-particle step runs in libcylag_hip.so.
+"""
+__version__ = "1.0"
+
+from .constants import *  # noqa: F401,F403
+from .cylag_dico import CYLAG_DEFAULT_PARAMETERS  # noqa: F401
+from .parameters import Parameters  # noqa: F401
+from .simutime import SimuTime  # noqa: F401
+from .mesh import TriangularMesh, read_cli  # noqa: F401
+from .fields import EulerianFieldSet  # noqa: F401
+from .particles import LagrangianParticleSet  # noqa: F401
+from .solver import Solver  # noqa: F401
+from .random_utils import (pcg32_seed, pcg32_advance, random_uniform, random_gaussian,  # noqa: F401
+                           random_gaussian_pair, random_gaussian_ziggurat,
+                           random_gaussian_ziggurat_pair, generate_random_pair,
+                           generate_random_single)

@@ -1,0 +1,1339 @@
+// Host driver and C ABI of libcylag_hip.so (see include/cylag_hip.h).
+//
+// One context = one HIP stream on one GPU, holding the whole mesh, every field
+// record and the particle pool in HBM.  cyl_step enqueues, per time step:
+//   time interpolation -> (active-rank scan if rows were deleted) ->
+//   random-stream event tables (Ziggurat / polar) -> fused step kernel ->
+//   guarded general-mode rerun (count, pool-order scan, events, step) that
+//   only does work when the fused kernel saw a draw count other than the
+//   uniform prediction -> end-of-step stream advance.
+// Nothing in a multi-step call waits on the host; results, statistics and the
+// random state are read back once at the end.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cylag_hip.h"
+#include "cyl_kernels.h"
+
+using namespace cyl;
+
+namespace {
+
+struct Bufs {
+  std::vector<void*> ptrs;
+  void free_all() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+  }
+};
+
+}  // namespace
+
+struct cyl_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int errcode = 0, errparticle = -1;
+  std::string errmsg;
+
+  // mesh
+  MeshDev mesh{};
+  bool has_mesh = false;
+  Bufs mesh_bufs;
+
+  // fields
+  FieldsDev fields{};
+  bool has_fields = false;
+  Bufs field_bufs;
+  std::vector<double> times;
+  int ntimes = 0;
+  int64_t fsize = 0;
+  std::vector<int> fids_present;            // field ids with data
+  std::vector<double*> frame_ptr;           // [fid*ntimes + rec]
+  double* d_tmp[4 + MAX_OPT] = {};
+  double* d_prev[4] = {};
+  int record = 0, interp_done = 0, prev_valid = 0, frozen = 1;
+  const double** d_ti_src = nullptr;        // [ntimes][nti] frame pointers of the interpolated fields
+  double** d_ti_dst = nullptr;              // [2][nti] destination per buffer parity
+  int nti = 0, tmp_parity = 0;
+  double* d_tmpbuf2[4] = {};                // second buffer of the main fields (previous values)
+
+  // particles
+  int n = 0, dim = 2;
+  PoolDev pool[2]{};
+  int cur = 0;
+  Bufs part_bufs;
+  int *d_gpool = nullptr, *d_rank = nullptr, *d_init_tri = nullptr, *d_del_step = nullptr;
+  int64_t* d_off = nullptr;
+  double *d_diam = nullptr, *d_vol = nullptr, *d_depth = nullptr;
+  ParticleConst pc{};
+  bool pending_init = false;   // rows with tri == -2 exist
+  bool have_last = false;
+
+  // scenarios (a single one for the plain solver)
+  int nscen = 1;
+  std::vector<int> scen_base_h{0, 0};
+  int* d_scen_base = nullptr;
+  int* d_scen = nullptr;
+  StreamDev* d_stream = nullptr;
+  JumpTable* d_jt = nullptr;
+  unsigned long long *d_stream_end = nullptr, *d_err = nullptr, *d_err_step = nullptr;
+  int64_t *d_scen_units = nullptr, *d_scen_active = nullptr;
+  int* d_flag = nullptr;
+  int* d_n_deleted = nullptr;
+  long long* d_n_deleted_total = nullptr;
+  int* d_step_done = nullptr;
+  std::vector<uint64_t> rng_state, rng_inc;
+  bool rng_dirty = true;
+  ZigTables* d_zt = nullptr;
+
+  // pool-order scan scratch
+  int* d_vals = nullptr;
+  int64_t *d_scan = nullptr, *d_bsum = nullptr, *d_total = nullptr;
+
+  // random-stream events
+  int64_t ev_units_cap = 0, nchunks = 0, ev_cap = 0;
+  Bufs ev_bufs;
+  int64_t* d_chunk_base = nullptr;
+  int* d_chunk_count = nullptr;
+  int64_t *d_chunk_off = nullptr, *d_ev_pos = nullptr, *d_ev_len = nullptr, *d_reach = nullptr;
+  int64_t *d_bmax = nullptr, *d_E = nullptr, *d_C = nullptr, *d_key = nullptr, *d_xtra = nullptr;
+  int64_t *d_ev_begin = nullptr, *d_nev = nullptr, *d_ev_bsum = nullptr, *d_xscratch = nullptr;
+  int* d_on = nullptr;
+
+  // element re-sort
+  int sort_interval = 0, block = BLK, steps_since_sort = 0;
+  int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
+  int64_t* d_cursor = nullptr;
+  void* d_tmpbuf = nullptr;
+
+  int step_counter = 0;
+  int* d_bnd = nullptr;
+  size_t bnd_cap = 0;
+  int nopen = 0;
+
+  double last_kernel_ms = 0., last_total_ms = 0., last_step_kernel_ms = 0.;
+  int last_step_launches = 0;
+  std::vector<hipEvent_t> kev;              // start/stop pairs around the fused step kernel
+  int last_launches = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  int set_err(int code, int particle, const std::string& msg) {
+    errcode = code;
+    errparticle = particle;
+    errmsg = msg;
+    return code;
+  }
+};
+
+#define HIPCHK(x)                                                                             \
+  do {                                                                                        \
+    hipError_t e_ = (x);                                                                      \
+    if (e_ != hipSuccess)                                                                     \
+      return ctx->set_err(CYL_E_HIP, -1, std::string(#x) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+template <class T>
+static hipError_t dalloc(Bufs& b, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e == hipSuccess) b.ptrs.push_back(q);
+  *p = (T*)q;
+  return e;
+}
+
+static inline unsigned nblk(int64_t n, int b = BLK) { return (unsigned)((n + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int cyl_ctx_create(int device, int flags, cyl_ctx** out) {
+  (void)flags;
+  if (!out) return CYL_E_ARG;
+  cyl_ctx* ctx = new cyl_ctx();
+  ctx->device = device;
+  *out = ctx;
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipStreamCreate(&ctx->stream));
+  HIPCHK(hipEventCreate(&ctx->ev0));
+  HIPCHK(hipEventCreate(&ctx->ev1));
+  // Ziggurat tables with the host libm (random_utils.pyx:118-146)
+  ZigTables z;
+  {
+    double m1 = 2147483648.0, dn = 3.442619855899, tn = dn, vn = 9.91256303526217e-3;
+    double q = vn / std::exp(-0.5 * dn * dn);
+    z.kn[0] = (uint32_t)((dn / q) * m1);
+    z.kn[1] = 0;
+    z.wn[0] = q / m1;
+    z.wn[127] = dn / m1;
+    z.fn[0] = 1.0;
+    z.fn[127] = std::exp(-0.5 * dn * dn);
+    for (int i = 126; i > 0; --i) {
+      dn = std::sqrt(-2.0 * std::log(vn / dn + std::exp(-0.5 * dn * dn)));
+      z.kn[i + 1] = (uint32_t)((dn / tn) * m1);
+      tn = dn;
+      z.fn[i] = std::exp(-0.5 * dn * dn);
+      z.wn[i] = dn / m1;
+    }
+  }
+  HIPCHK(hipMalloc(&ctx->d_zt, sizeof(ZigTables)));
+  HIPCHK(hipMemcpy(ctx->d_zt, &z, sizeof(ZigTables), hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&ctx->d_flag, 8 * sizeof(int)));
+  HIPCHK(hipMemset(ctx->d_flag, 0, 8 * sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_n_deleted, sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_n_deleted_total, sizeof(long long)));
+  HIPCHK(hipMalloc(&ctx->d_step_done, sizeof(int)));
+  HIPCHK(hipMemset(ctx->d_n_deleted, 0, sizeof(int)));
+  HIPCHK(hipMemset(ctx->d_n_deleted_total, 0, sizeof(long long)));
+  ctx->rng_state.assign(1, 0);
+  ctx->rng_inc.assign(1, 1);
+  // module default seed, cylag/lsm/__init__.py:6-10
+  Pcg g;
+  pcg_seed(g, 42, 0xda3e06d11bb2b203ULL);
+  ctx->rng_state[0] = g.state;
+  ctx->rng_inc[0] = g.inc;
+  return CYL_OK;
+}
+
+extern "C" int cyl_ctx_destroy(cyl_ctx* ctx) {
+  if (!ctx) return CYL_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->mesh_bufs.free_all();
+  ctx->field_bufs.free_all();
+  ctx->part_bufs.free_all();
+  ctx->ev_bufs.free_all();
+  for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
+  void* singles[] = {ctx->d_zt, ctx->d_flag, ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->d_step_done, ctx->d_bnd};
+  for (void* p : singles)
+    if (p) (void)hipFree(p);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CYL_OK;
+}
+
+extern "C" int cyl_last_error(cyl_ctx* ctx, int* code, int* particle, char* msg, size_t n) {
+  if (!ctx) return CYL_E_ARG;
+  if (code) *code = ctx->errcode;
+  if (particle) *particle = ctx->errparticle;
+  if (msg && n) {
+    std::strncpy(msg, ctx->errmsg.c_str(), n - 1);
+    msg[n - 1] = 0;
+  }
+  return CYL_OK;
+}
+
+extern "C" int cyl_device_info(cyl_ctx* ctx, char* name, size_t n, size_t* free_b, size_t* total_b) {
+  if (!ctx) return CYL_E_ARG;
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+  if (name && n) {
+    std::snprintf(name, n, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  }
+  size_t f = 0, t = 0;
+  HIPCHK(hipMemGetInfo(&f, &t));
+  if (free_b) *free_b = f;
+  if (total_b) *total_b = t;
+  return CYL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mesh
+// ---------------------------------------------------------------------------
+extern "C" int cyl_mesh_upload(cyl_ctx* ctx, int nv, const double* x, const double* y, int nt, const int* tri,
+                               int ne, const int* edges, const int* edgetri, const int* triedge,
+                               const int* edge_labels, const double* det, const double* gradx,
+                               const double* grady, int nopen) {
+  if (!ctx || nv <= 0 || nt <= 0 || ne <= 0) return CYL_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->mesh_bufs.free_all();
+  MeshDev m{};
+  m.nv = nv;
+  m.nt = nt;
+  m.ne = ne;
+  m.nopen = nopen;
+  ctx->nopen = nopen;
+  auto up = [&](auto** dst, const auto* src, size_t cnt) -> hipError_t {
+    hipError_t e = dalloc(ctx->mesh_bufs, dst, cnt);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src, cnt * sizeof(**dst), hipMemcpyHostToDevice);
+  };
+  double *dx, *dy, *ddet, *dgx, *dgy, *dcen;
+  int *dtri, *ded, *det2, *dte, *dlab;
+  HIPCHK(up(&dx, x, nv));
+  HIPCHK(up(&dy, y, nv));
+  HIPCHK(up(&dtri, tri, (size_t)nt * 3));
+  HIPCHK(up(&ded, edges, (size_t)ne * 2));
+  HIPCHK(up(&det2, edgetri, (size_t)ne * 2));
+  HIPCHK(up(&dte, triedge, (size_t)nt * 3));
+  HIPCHK(up(&dlab, edge_labels, ne));
+  HIPCHK(up(&ddet, det, (size_t)nt * 3));
+  HIPCHK(up(&dgx, gradx, (size_t)nt * 3));
+  HIPCHK(up(&dgy, grady, (size_t)nt * 3));
+  // centroids (triangular_mesh.pyx:621-626) and the bucket grid for fallback search
+  std::vector<double> cen((size_t)nt * 2);
+  double xmin = x[0], xmax = x[0], ymin = y[0], ymax = y[0];
+  for (int i = 0; i < nv; ++i) {
+    xmin = std::min(xmin, x[i]);
+    xmax = std::max(xmax, x[i]);
+    ymin = std::min(ymin, y[i]);
+    ymax = std::max(ymax, y[i]);
+  }
+  for (int k = 0; k < nt; ++k) {
+    int a = tri[3 * k], b = tri[3 * k + 1], c = tri[3 * k + 2];
+    cen[2 * k] = (x[a] + x[b] + x[c]) / 3.0;
+    cen[2 * k + 1] = (y[a] + y[b] + y[c]) / 3.0;
+  }
+  HIPCHK(up(&dcen, cen.data(), cen.size()));
+  double w = std::max(xmax - xmin, 1e-300), h = std::max(ymax - ymin, 1e-300);
+  double cell = std::sqrt(w * h / nt) * 1.5;
+  if (!(cell > 0.)) cell = std::max(w, h);
+  int gnx = (int)std::min<double>(std::max(1.0, std::ceil(w / cell)), 1 << 14);
+  int gny = (int)std::min<double>(std::max(1.0, std::ceil(h / cell)), 1 << 14);
+  double inv = 1.0 / cell;
+  auto cidx = [&](double v, double v0, int nmax) {
+    double f = std::floor((v - v0) * inv);
+    if (f < 0.) f = 0.;
+    if (f > nmax - 1) f = nmax - 1;
+    return (int)f;
+  };
+  std::vector<int> start((size_t)gnx * gny + 1, 0);
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<int> fill;
+    if (pass == 1) {
+      for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
+      fill.assign(start.begin(), start.end() - 1);
+    }
+    static std::vector<int> items;
+    if (pass == 1) items.assign(start.back(), 0);
+    for (int k = 0; k < nt; ++k) {
+      int a = tri[3 * k], b = tri[3 * k + 1], c = tri[3 * k + 2];
+      double bx0 = std::min(x[a], std::min(x[b], x[c])), bx1 = std::max(x[a], std::max(x[b], x[c]));
+      double by0 = std::min(y[a], std::min(y[b], y[c])), by1 = std::max(y[a], std::max(y[b], y[c]));
+      int ix0 = cidx(bx0, xmin, gnx), ix1 = cidx(bx1, xmin, gnx);
+      int iy0 = cidx(by0, ymin, gny), iy1 = cidx(by1, ymin, gny);
+      for (int iy = iy0; iy <= iy1; ++iy)
+        for (int ix = ix0; ix <= ix1; ++ix) {
+          size_t cc = (size_t)iy * gnx + ix;
+          if (pass == 0) start[cc + 1]++;
+          else items[fill[cc]++] = k;
+        }
+    }
+    if (pass == 1) {
+      int *dst, *dit;
+      HIPCHK(up(&dst, start.data(), start.size()));
+      HIPCHK(up(&dit, items.data(), std::max<size_t>(items.size(), 1)));
+      m.g_start = dst;
+      m.g_items = dit;
+      items.clear();
+      items.shrink_to_fit();
+    }
+  }
+  m.x = dx; m.y = dy; m.tri = dtri; m.edges = ded; m.edgetri = det2; m.triedge = dte; m.edge_labels = dlab;
+  m.det = ddet; m.gx = dgx; m.gy = dgy; m.centroid = dcen;
+  m.g_x0 = xmin; m.g_y0 = ymin; m.g_inv = inv; m.g_nx = gnx; m.g_ny = gny;
+  ctx->mesh = m;
+  ctx->has_mesh = true;
+  return CYL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fields
+// ---------------------------------------------------------------------------
+extern "C" int cyl_fields_upload(cyl_ctx* ctx, int dim, int nlayers, int npoints, int ntimes, const double* times,
+                                 const double* elev, const double* vx, const double* vy, const double* vz, int nopt,
+                                 const double* opt, const int* fids, int friction_model, double friction_coef) {
+  if (!ctx || (dim != 2 && dim != 3) || nlayers < 1 || npoints < 1 || ntimes < 1 || nopt < 0 || nopt > MAX_OPT)
+    return CYL_E_ARG;
+  if (!elev || !vx || !vy || (dim == 3 && !vz) || (nopt > 0 && !opt)) return CYL_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->field_bufs.free_all();
+  int64_t size = (int64_t)npoints * nlayers;
+  ctx->fsize = size;
+  ctx->ntimes = ntimes;
+  ctx->times.assign(times, times + ntimes);
+  int nf = 4 + nopt;
+  ctx->frame_ptr.assign((size_t)nf * ntimes, nullptr);
+  ctx->fids_present.clear();
+  const double* src[4 + MAX_OPT] = {elev, vx, vy, vz};
+  for (int q = 0; q < nopt; ++q) src[4 + q] = opt + (size_t)q * ntimes * size;
+  FieldsDev f{};
+  f.dim = dim;
+  f.nlayers = nlayers;
+  f.npoints = npoints;
+  f.nopt = nopt;
+  f.fid_zs = fids[0]; f.fid_ux = fids[1]; f.fid_uy = fids[2]; f.fid_uz = fids[3];
+  f.fid_turbeng = fids[4]; f.fid_dissip = fids[5]; f.fid_zb = fids[6];
+  f.friction_model = friction_model;
+  f.friction_coefficient = friction_coef;
+  for (int fid = 0; fid < nf; ++fid) {
+    if (!src[fid]) continue;
+    ctx->fids_present.push_back(fid);
+    double* d;
+    HIPCHK(dalloc(ctx->field_bufs, &d, (size_t)ntimes * size));
+    HIPCHK(hipMemcpy(d, src[fid], (size_t)ntimes * size * sizeof(double), hipMemcpyHostToDevice));
+    for (int r = 0; r < ntimes; ++r) ctx->frame_ptr[(size_t)fid * ntimes + r] = d + (size_t)r * size;
+    double* t;
+    HIPCHK(dalloc(ctx->field_bufs, &t, size));
+    // _init_tmp: record 0 (eulerian_field_set.pyx:743-776)
+    HIPCHK(hipMemcpy(t, d, size * sizeof(double), hipMemcpyDeviceToDevice));
+    ctx->d_tmp[fid] = t;
+    f.tmp[fid] = t;
+    if (fid < 4) {
+      double* pv;
+      HIPCHK(dalloc(ctx->field_bufs, &pv, size));
+      HIPCHK(hipMemcpy(pv, t, size * sizeof(double), hipMemcpyDeviceToDevice));
+      ctx->d_tmpbuf2[fid] = pv;
+      ctx->d_prev[fid] = pv;
+      f.prev[fid] = pv;
+    }
+  }
+  // time-interpolation pointer tables: fields 0..dim (main) and every optional field
+  std::vector<int> ti;
+  int ntmp = dim == 2 ? 3 : 4;
+  for (int fid : ctx->fids_present)
+    if (!(fid < 4 && fid >= ntmp)) ti.push_back(fid);
+  int nti = (int)ti.size();
+  ctx->nti = nti;
+  std::vector<const double*> srct((size_t)ntimes * nti);
+  for (int r = 0; r < ntimes; ++r)
+    for (int q = 0; q < nti; ++q) srct[(size_t)r * nti + q] = ctx->frame_ptr[(size_t)ti[q] * ntimes + r];
+  std::vector<double*> dstt((size_t)2 * nti);
+  for (int q = 0; q < nti; ++q) {
+    int fid = ti[q];
+    dstt[q] = ctx->d_tmp[fid];
+    dstt[nti + q] = fid < 4 ? ctx->d_tmpbuf2[fid] : ctx->d_tmp[fid];
+  }
+  HIPCHK(dalloc(ctx->field_bufs, &ctx->d_ti_src, srct.size()));
+  HIPCHK(hipMemcpy(ctx->d_ti_src, srct.data(), srct.size() * sizeof(void*), hipMemcpyHostToDevice));
+  HIPCHK(dalloc(ctx->field_bufs, &ctx->d_ti_dst, dstt.size()));
+  HIPCHK(hipMemcpy(ctx->d_ti_dst, dstt.data(), dstt.size() * sizeof(void*), hipMemcpyHostToDevice));
+  ctx->tmp_parity = 0;
+  f.frozen = ntimes == 1;
+  ctx->frozen = f.frozen;
+  ctx->record = 0;
+  ctx->interp_done = 0;
+  ctx->prev_valid = 0;
+  ctx->fields = f;
+  ctx->has_fields = true;
+  return CYL_OK;
+}
+
+static int lower_index(const std::vector<double>& a, double v, int last) {
+  // zlocalizer.pyx:125-151
+  int n = (int)a.size();
+  for (int i = last; i < n - 1; ++i)
+    if (v >= a[i] && v < a[i + 1]) return i;
+  if (n > 1 && v >= a[n - 1]) return n - 2;
+  return 0;
+}
+
+extern "C" int cyl_fields_reset(cyl_ctx* ctx, int frozen, double t0) {
+  if (!ctx || !ctx->has_fields) return CYL_E_STATE;
+  ctx->frozen = frozen ? 1 : 0;
+  ctx->fields.frozen = ctx->frozen;
+  if (!ctx->frozen) ctx->record = lower_index(ctx->times, t0, 0);
+  ctx->interp_done = 0;
+  ctx->prev_valid = 0;
+  ctx->fields.prev_valid = 0;
+  return CYL_OK;
+}
+
+extern "C" int cyl_deleted_count(cyl_ctx* ctx, long long* n) {
+  if (!ctx || !n) return CYL_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(n, ctx->d_n_deleted_total, sizeof(long long), hipMemcpyDeviceToHost));
+  return CYL_OK;
+}
+
+// enqueue _time_interpolation(time) (eulerian_field_set.pyx:866-938).  The
+// main fields are double-buffered: the previous interpolation stays in the
+// other buffer, which is what the reference copies into _tmp_*0.
+static int enqueue_time_interp(cyl_ctx* ctx, double time, int& launches) {
+  if (ctx->frozen) return CYL_OK;
+  ctx->prev_valid = ctx->interp_done;
+  ctx->interp_done = 1;
+  ctx->record = lower_index(ctx->times, time, ctx->record);
+  int rec = ctx->record;
+  if (rec + 1 >= ctx->ntimes) return ctx->set_err(CYL_E_STEP, -1, "time interpolation beyond the last record");
+  double dtt = ctx->times[rec] - ctx->times[rec + 1];
+  if (std::fabs(dtt) < EPS)
+    return ctx->set_err(CYL_E_STEP, -1, "Time interpolation failed: duplicate or too-close time values detected");
+  double tf = (time - ctx->times[rec + 1]) / dtt;
+  int par = 1 - ctx->tmp_parity;  // write into the buffer that held the older values
+  dim3 g(nblk(ctx->fsize), ctx->nti);
+  k_time_interp<<<g, BLK, 0, ctx->stream>>>(ctx->d_ti_src + (size_t)rec * ctx->nti,
+                                           ctx->d_ti_src + (size_t)(rec + 1) * ctx->nti,
+                                           ctx->d_ti_dst + (size_t)par * ctx->nti, ctx->nti, ctx->fsize, tf);
+  ++launches;
+  ctx->tmp_parity = par;
+  for (int fid = 0; fid < 4; ++fid) {
+    if (!ctx->d_tmp[fid] || !ctx->d_tmpbuf2[fid]) continue;
+    double* cur = par == 0 ? ctx->d_tmp[fid] : ctx->d_tmpbuf2[fid];
+    double* old = par == 0 ? ctx->d_tmpbuf2[fid] : ctx->d_tmp[fid];
+    ctx->fields.tmp[fid] = cur;
+    ctx->fields.prev[fid] = old;
+  }
+  ctx->fields.prev_valid = ctx->prev_valid;
+  return CYL_OK;
+}
+
+extern "C" int cyl_fields_interpolate(cyl_ctx* ctx, double time) {
+  if (!ctx || !ctx->has_fields) return CYL_E_STATE;
+  HIPCHK(hipSetDevice(ctx->device));
+  int l = 0;
+  int e = enqueue_time_interp(ctx, time, l);
+  if (e) return e;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return CYL_OK;
+}
+
+extern "C" int cyl_fields_download_tmp(cyl_ctx* ctx, int fid, double* out) {
+  if (!ctx || !ctx->has_fields || fid < 0 || fid >= 4 + MAX_OPT || !ctx->fields.tmp[fid]) return CYL_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(out, ctx->fields.tmp[fid], ctx->fsize * sizeof(double), hipMemcpyDeviceToHost));
+  return CYL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// particles
+// ---------------------------------------------------------------------------
+static int alloc_pool(cyl_ctx* ctx, PoolDev& P) {
+  Bufs& b = ctx->part_bufs;
+  int n = ctx->n;
+  HIPCHK(dalloc(b, &P.x, n));
+  HIPCHK(dalloc(b, &P.y, n));
+  HIPCHK(dalloc(b, &P.z, ctx->dim == 3 ? n : 1));
+  for (int c = 0; c < 3; ++c) {
+    HIPCHK(dalloc(b, &P.up[c], c < ctx->dim ? n : 1));
+    HIPCHK(dalloc(b, &P.us[c], c < ctx->dim ? n : 1));
+  }
+  HIPCHK(dalloc(b, &P.tri, n));
+  HIPCHK(dalloc(b, &P.ll, n));
+  HIPCHK(dalloc(b, &P.inactive, n));
+  return CYL_OK;
+}
+
+static int ensure_scen_tables(cyl_ctx* ctx) {
+  Bufs& b = ctx->part_bufs;
+  int ns = ctx->nscen;
+  HIPCHK(dalloc(b, &ctx->d_scen_base, ns + 1));
+  HIPCHK(hipMemcpy(ctx->d_scen_base, ctx->scen_base_h.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(dalloc(b, &ctx->d_stream, ns));
+  HIPCHK(dalloc(b, &ctx->d_jt, ns));
+  HIPCHK(dalloc(b, &ctx->d_stream_end, ns));
+  HIPCHK(dalloc(b, &ctx->d_err, ns));
+  HIPCHK(dalloc(b, &ctx->d_err_step, ns));
+  HIPCHK(hipMemset(ctx->d_err_step, 0xff, ns * sizeof(unsigned long long)));
+  HIPCHK(dalloc(b, &ctx->d_scen_units, ns));
+  HIPCHK(dalloc(b, &ctx->d_scen_active, ns));
+  HIPCHK(hipMemset(ctx->d_stream_end, 0, ns * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(ctx->d_err, 0xff, ns * sizeof(unsigned long long)));
+  std::vector<StreamDev> st(ns);
+  std::memset(st.data(), 0, ns * sizeof(StreamDev));
+  HIPCHK(hipMemcpy(ctx->d_stream, st.data(), ns * sizeof(StreamDev), hipMemcpyHostToDevice));
+  ctx->rng_dirty = true;
+  return CYL_OK;
+}
+
+extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const double* pos, const double* vel,
+                                    const double* us, const int* inactive, const int* tri, const int* ll,
+                                    const double* diameter, const double* volume, const int* init_tri,
+                                    const cyl_pconst* pconst) {
+  if (!ctx || npart <= 0 || (dim != 2 && dim != 3) || !pos || !inactive || !tri) return CYL_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->part_bufs.free_all();
+  ctx->ev_bufs.free_all();
+  ctx->ev_units_cap = 0;
+  ctx->n = npart;
+  ctx->dim = dim;
+  ctx->cur = 0;
+  if (pconst) {
+    ParticleConst& p = ctx->pc;
+    p.rho0 = pconst->rho0; p.density = pconst->density; p.drag_coef = pconst->drag_coef;
+    p.added_mass_coef = pconst->added_mass_coef; p.buoyancy_velocity_value = pconst->buoyancy_velocity_value;
+    p.drag_coef_model = pconst->drag_coef_model; p.added_mass_force = pconst->added_mass_force;
+    p.buoyancy_velocity_model = pconst->buoyancy_velocity_model;
+    p.a0c = pconst->a0c; p.a1c = pconst->a1c; p.a2c = pconst->a2c; p.a3c = pconst->a3c;
+  }
+  for (int b = 0; b < 2; ++b) {
+    int e = alloc_pool(ctx, ctx->pool[b]);
+    if (e) return e;
+  }
+  Bufs& bb = ctx->part_bufs;
+  int n = npart;
+  HIPCHK(dalloc(bb, &ctx->d_gpool, n));
+  HIPCHK(dalloc(bb, &ctx->d_rank, n));
+  HIPCHK(dalloc(bb, &ctx->d_off, n));
+  HIPCHK(dalloc(bb, &ctx->d_init_tri, n));
+  HIPCHK(dalloc(bb, &ctx->d_del_step, n));
+  HIPCHK(dalloc(bb, &ctx->d_diam, n));
+  HIPCHK(dalloc(bb, &ctx->d_vol, n));
+  HIPCHK(dalloc(bb, &ctx->d_depth, dim == 3 ? n : 1));
+  HIPCHK(dalloc(bb, &ctx->d_vals, n + 1));
+  HIPCHK(dalloc(bb, &ctx->d_scan, n + 1));
+  HIPCHK(dalloc(bb, &ctx->d_bsum, nblk(n + 1) + 1));
+  HIPCHK(dalloc(bb, &ctx->d_total, 2));
+  HIPCHK(dalloc(bb, &ctx->d_perm, n));
+  HIPCHK(dalloc(bb, &ctx->d_skey, n));
+  {
+    double* tb;
+    HIPCHK(dalloc(bb, &tb, (size_t)n));
+    ctx->d_tmpbuf = tb;
+  }
+  ctx->nscen = 1;
+  ctx->scen_base_h = {0, n};
+  ctx->d_scen = nullptr;
+  int e = ensure_scen_tables(ctx);
+  if (e) return e;
+  // SoA conversion on the host, physical order = pool order
+  std::vector<double> col(n);
+  std::vector<int> icol(n);
+  PoolDev& A = ctx->pool[0];
+  auto putd = [&](double* dst, const double* aos, int c, int stride) -> hipError_t {
+    for (int i = 0; i < n; ++i) col[i] = aos ? aos[(size_t)i * stride + c] : 0.;
+    return hipMemcpy(dst, col.data(), n * sizeof(double), hipMemcpyHostToDevice);
+  };
+  HIPCHK(putd(A.x, pos, 0, dim));
+  HIPCHK(putd(A.y, pos, 1, dim));
+  if (dim == 3) HIPCHK(putd(A.z, pos, 2, dim));
+  for (int c = 0; c < dim; ++c) {
+    HIPCHK(putd(A.up[c], vel, c, dim));
+    HIPCHK(putd(A.us[c], us, c, dim));
+  }
+  HIPCHK(putd(ctx->d_diam, diameter, 0, 1));
+  HIPCHK(putd(ctx->d_vol, volume, 0, 1));
+  bool pend = false;
+  for (int i = 0; i < n; ++i) {
+    icol[i] = tri[i];
+    if (!inactive[i] && tri[i] == -2) pend = true;
+  }
+  ctx->pending_init = pend;
+  HIPCHK(hipMemcpy(A.tri, icol.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  for (int i = 0; i < n; ++i) icol[i] = ll ? ll[i] : 0;
+  HIPCHK(hipMemcpy(A.ll, icol.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  for (int i = 0; i < n; ++i) icol[i] = inactive[i] ? 1 : 0;
+  HIPCHK(hipMemcpy(A.inactive, icol.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  for (int i = 0; i < n; ++i) icol[i] = init_tri ? init_tri[i] : -3;
+  HIPCHK(hipMemcpy(ctx->d_init_tri, icol.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  for (int i = 0; i < n; ++i) icol[i] = i;
+  HIPCHK(hipMemcpy(ctx->d_gpool, icol.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(ctx->d_del_step, 0xff, n * sizeof(int)));
+  HIPCHK(hipMemset(ctx->d_depth, 0, (dim == 3 ? n : 1) * sizeof(double)));
+  // the second buffer starts as a copy (inactive rows are never rewritten there)
+  PoolDev& B = ctx->pool[1];
+  HIPCHK(hipMemcpy(B.inactive, A.inactive, n * sizeof(int), hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(B.tri, A.tri, n * sizeof(int), hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemcpy(B.ll, A.ll, n * sizeof(int), hipMemcpyDeviceToDevice));
+  HIPCHK(hipMemset(ctx->d_flag, 0, 8 * sizeof(int)));
+  int one = 1;
+  HIPCHK(hipMemcpy(ctx->d_flag + 1, &one, sizeof(int), hipMemcpyHostToDevice));  // rank dirty
+  HIPCHK(hipMemcpy(ctx->d_flag + 3, &one, sizeof(int), hipMemcpyHostToDevice));  // constant 1
+  HIPCHK(hipMemset(ctx->d_n_deleted, 0, sizeof(int)));
+  HIPCHK(hipMemset(ctx->d_n_deleted_total, 0, sizeof(long long)));
+  ctx->have_last = false;
+  ctx->steps_since_sort = 0;
+  return CYL_OK;
+}
+
+extern "C" int cyl_particles_download(cyl_ctx* ctx, int npart, double* pos, double* vel, double* us, int* inactive,
+                                      int* tri, int* ll, double* lpos, double* lvel, int* ltri, double* depth) {
+  if (!ctx || npart != ctx->n) return CYL_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int n = ctx->n, dim = ctx->dim;
+  std::vector<int> gp(n), ina(n);
+  HIPCHK(hipMemcpy(gp.data(), ctx->d_gpool, n * sizeof(int), hipMemcpyDeviceToHost));
+  const PoolDev& P = ctx->pool[ctx->cur];
+  const PoolDev& L = ctx->pool[1 - ctx->cur];
+  HIPCHK(hipMemcpy(ina.data(), P.inactive, n * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<double> col(n);
+  std::vector<int> icol(n);
+  auto getd = [&](const double* src, double* aos, int c, int stride, bool zero_inactive) -> hipError_t {
+    hipError_t e = hipMemcpy(col.data(), src, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (int j = 0; j < n; ++j) aos[(size_t)gp[j] * stride + c] = (zero_inactive && ina[j]) ? 0. : col[j];
+    return hipSuccess;
+  };
+  const double* px[3] = {P.x, P.y, P.z};
+  const double* lx[3] = {L.x, L.y, L.z};
+  for (int c = 0; c < dim; ++c) {
+    if (pos) HIPCHK(getd(px[c], pos, c, dim, true));
+    if (vel) HIPCHK(getd(P.up[c], vel, c, dim, true));
+    if (us) HIPCHK(getd(P.us[c], us, c, dim, true));
+    if (lpos) HIPCHK(getd(ctx->have_last ? lx[c] : px[c], lpos, c, dim, true));
+    if (lvel) HIPCHK(getd(ctx->have_last ? L.up[c] : P.up[c], lvel, c, dim, true));
+  }
+  if (depth && dim == 3) HIPCHK(getd(ctx->d_depth, depth, 0, 1, true));
+  if (inactive)
+    for (int j = 0; j < n; ++j) inactive[gp[j]] = ina[j] ? 1 : 0;
+  auto geti = [&](const int* src, int* out, int dflt) -> hipError_t {
+    hipError_t e = hipMemcpy(icol.data(), src, n * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (int j = 0; j < n; ++j) out[gp[j]] = ina[j] ? dflt : icol[j];
+    return hipSuccess;
+  };
+  if (tri) HIPCHK(geti(P.tri, tri, -1));
+  if (ll) HIPCHK(geti(P.ll, ll, 0));
+  if (ltri) HIPCHK(geti(ctx->have_last ? L.tri : P.tri, ltri, -1));
+  return CYL_OK;
+}
+
+extern "C" int cyl_particles_deleted(cyl_ctx* ctx, int* pool_index, int cap, int* nout) {
+  if (!ctx || !nout) return CYL_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int n = ctx->n;
+  std::vector<int> gp(n), ina(n), ds(n);
+  const PoolDev& P = ctx->pool[ctx->cur];
+  HIPCHK(hipMemcpy(gp.data(), ctx->d_gpool, n * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ina.data(), P.inactive, n * sizeof(int), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ds.data(), ctx->d_del_step, n * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<std::pair<int, int>> del;
+  for (int j = 0; j < n; ++j)
+    if (ina[j] == 2) {
+      del.push_back({ds[j], gp[j]});
+      ina[j] = 1;
+    }
+  std::sort(del.begin(), del.end());
+  *nout = (int)del.size();
+  for (int q = 0; q < (int)del.size() && q < cap; ++q) pool_index[q] = del[q].second;
+  if (!del.empty()) {
+    HIPCHK(hipMemcpy(P.inactive, ina.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->pool[1 - ctx->cur].inactive, ina.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  }
+  return CYL_OK;
+}
+
+extern "C" int cyl_particles_add(cyl_ctx* ctx, int nadd, const int* slots, const double* pos, const double* diam,
+                                 const double* vol, const int* init_tri) {
+  if (!ctx || nadd < 0 || (nadd > 0 && (!slots || !pos))) return CYL_E_ARG;
+  if (nadd == 0) return CYL_OK;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  int n = ctx->n, dim = ctx->dim;
+  std::vector<int> gp(n), slot_of(n);
+  HIPCHK(hipMemcpy(gp.data(), ctx->d_gpool, n * sizeof(int), hipMemcpyDeviceToHost));
+  for (int j = 0; j < n; ++j) slot_of[gp[j]] = j;
+  PoolDev& P = ctx->pool[ctx->cur];
+  for (int q = 0; q < nadd; ++q) {
+    if (slots[q] < 0 || slots[q] >= n) return CYL_E_ARG;
+    int j = slot_of[slots[q]];
+    double* px[3] = {P.x, P.y, P.z};
+    for (int c = 0; c < dim; ++c) {
+      double v = pos[(size_t)q * dim + c], zero = 0.;
+      HIPCHK(hipMemcpy(px[c] + j, &v, sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(P.up[c] + j, &zero, sizeof(double), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(P.us[c] + j, &zero, sizeof(double), hipMemcpyHostToDevice));
+    }
+    int m2 = -2, z0 = 0, it = init_tri ? init_tri[q] : -3;
+    HIPCHK(hipMemcpy(P.tri + j, &m2, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P.ll + j, &z0, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(P.inactive + j, &z0, sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_init_tri + j, &it, sizeof(int), hipMemcpyHostToDevice));
+    if (diam) HIPCHK(hipMemcpy(ctx->d_diam + j, diam + q, sizeof(double), hipMemcpyHostToDevice));
+    if (vol) HIPCHK(hipMemcpy(ctx->d_vol + j, vol + q, sizeof(double), hipMemcpyHostToDevice));
+  }
+  ctx->pending_init = true;
+  int one = 1;
+  HIPCHK(hipMemcpy(ctx->d_flag + 1, &one, sizeof(int), hipMemcpyHostToDevice));
+  return CYL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// random stream
+// ---------------------------------------------------------------------------
+extern "C" int cyl_rng_set(cyl_ctx* ctx, uint64_t state, uint64_t inc) {
+  if (!ctx) return CYL_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->rng_state.assign(ctx->nscen, state);
+  ctx->rng_inc.assign(ctx->nscen, inc);
+  // scenario s gets the stream jumped by s<<48 (solver_parallel.pyx:113-115 idiom)
+  for (int s = 1; s < ctx->nscen; ++s) {
+    Pcg g{state, inc};
+    pcg_advance(g, (uint64_t)s << 48);
+    ctx->rng_state[s] = g.state;
+  }
+  ctx->rng_dirty = true;
+  return CYL_OK;
+}
+
+extern "C" int cyl_rng_get(cyl_ctx* ctx, uint64_t* state, uint64_t* inc) {
+  if (!ctx || !state || !inc) return CYL_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (!ctx->rng_dirty && ctx->d_stream) {
+    StreamDev st;
+    HIPCHK(hipMemcpy(&st, ctx->d_stream, sizeof(StreamDev), hipMemcpyDeviceToHost));
+    ctx->rng_state[0] = st.base;
+  }
+  *state = ctx->rng_state[0];
+  *inc = ctx->rng_inc[0];
+  return CYL_OK;
+}
+
+extern "C" uint32_t cyl_pcg32_next(uint64_t* state, uint64_t inc) {
+  Pcg g{*state, inc};
+  uint32_t r = pcg_next(g);
+  *state = g.state;
+  return r;
+}
+
+extern "C" void cyl_pcg32_seed(uint64_t* state, uint64_t* inc, uint64_t seed, uint64_t stream) {
+  Pcg g;
+  pcg_seed(g, seed, stream);
+  *state = g.state;
+  *inc = g.inc;
+}
+
+extern "C" void cyl_pcg32_advance(uint64_t* state, uint64_t inc, uint64_t delta) {
+  Pcg g{*state, inc};
+  pcg_advance(g, delta);
+  *state = g.state;
+}
+
+static ZigTables& host_zig() {
+  static ZigTables z;
+  static bool ready = false;
+  if (!ready) {
+    double m1 = 2147483648.0, dn = 3.442619855899, tn = dn, vn = 9.91256303526217e-3;
+    double q = vn / std::exp(-0.5 * dn * dn);
+    z.kn[0] = (uint32_t)((dn / q) * m1);
+    z.kn[1] = 0;
+    z.wn[0] = q / m1;
+    z.wn[127] = dn / m1;
+    z.fn[0] = 1.0;
+    z.fn[127] = std::exp(-0.5 * dn * dn);
+    for (int i = 126; i > 0; --i) {
+      dn = std::sqrt(-2.0 * std::log(vn / dn + std::exp(-0.5 * dn * dn)));
+      z.kn[i + 1] = (uint32_t)((dn / tn) * m1);
+      tn = dn;
+      z.fn[i] = std::exp(-0.5 * dn * dn);
+      z.wn[i] = dn / m1;
+    }
+    ready = true;
+  }
+  return z;
+}
+
+extern "C" double cyl_random_normal(uint64_t* state, uint64_t inc, int method) {
+  Pcg g{*state, inc};
+  double r;
+  auto ex = [](double v) { return std::exp(v); };
+  auto lg = [](double v) { return std::log(v); };
+  auto sq = [](double v) { return std::sqrt(v); };
+  if (method == 1) r = rect_normal(g);
+  else if (method == 2) r = zig_normal_t(g, host_zig(), ex, lg);
+  else {
+    double a, b;
+    polar_pair_t(g, a, b, lg, sq);
+    r = a;
+  }
+  *state = g.state;
+  return r;
+}
+
+extern "C" void cyl_random_pair(uint64_t* state, uint64_t inc, int method, double* a, double* b) {
+  Pcg g{*state, inc};
+  auto ex = [](double v) { return std::exp(v); };
+  auto lg = [](double v) { return std::log(v); };
+  auto sq = [](double v) { return std::sqrt(v); };
+  if (method == 1) {
+    *a = rect_normal(g);
+    *b = rect_normal(g);
+  } else if (method == 2) {
+    *a = zig_normal_t(g, host_zig(), ex, lg);
+    *b = zig_normal_t(g, host_zig(), ex, lg);
+  } else {
+    polar_pair_t(g, *a, *b, lg, sq);
+  }
+  *state = g.state;
+}
+
+// ---------------------------------------------------------------------------
+// step
+// ---------------------------------------------------------------------------
+static ParamsDev to_dev(const cyl_params* p) {
+  ParamsDev d{};
+  d.model = p->model; d.time_scheme = p->time_scheme; d.particle_velocity_init = p->particle_velocity_init;
+  d.rng_method = p->rng_method; d.diffusion_model = p->diffusion_model;
+  d.diffusion_lsm2_option = p->diffusion_lsm2_option; d.horizontal_diffusivity = p->horizontal_diffusivity;
+  d.vertical_diffusivity = p->vertical_diffusivity; d.schmidt_number = p->schmidt_number;
+  d.boundary_conditions = p->boundary_conditions; d.boundary_conditions_type = p->boundary_conditions_type;
+  d.max_wall_rebounds = p->max_wall_rebounds; d.bnd_statistics = p->bnd_statistics;
+  d.rebound_damping_coef = p->rebound_damping_coef; d.wall_roughness_angle = p->wall_roughness_angle;
+  d.stranding_probability = p->stranding_probability; d.compute_depth = p->compute_depth;
+  return d;
+}
+
+// Draw units one active row takes in a step when nothing is data-dependent;
+// -1 when no uniform prediction is made.  `umax` bounds the units of any row.
+static int predict_units(const cyl_params* p, int dim, double dt, int& umax) {
+  int pair = p->rng_method == 3 ? 1 : 2, single = 1;
+  double sqrtdt = std::sqrt(dt);
+  int dm = p->diffusion_model;
+  umax = 2 * pair + 2 * single + 3 * pair;
+  if (p->model == 0) return 0;
+  if (p->model == 1 || (p->model == 2 && (p->diffusion_lsm2_option == 1 || dm == 0))) {
+    if (dm == 0) return 0;
+    int u = 0;
+    if (dm == 2 || dm == 3) u += pair;  // K > 0 predicted
+    else if (std::sqrt(2. * p->horizontal_diffusivity) * sqrtdt != 0.) u += pair;
+    if (dim == 3) {
+      double dz = (dm == 3) ? 1. : std::sqrt(2. * p->vertical_diffusivity) * sqrtdt;
+      if (dm == 3 || dz != 0.) u += single;
+    }
+    return u;
+  }
+  if (p->model == 2) {
+    if (p->time_scheme == 1) return pair + (dim == 3 ? single : 0);
+    return dim * pair;  // direct integrator: one pair per component with nonzero amplitude
+  }
+  return -1;
+}
+
+static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
+  if (method == 1) return CYL_OK;
+  int64_t need = units_per_scen_cap;
+  if (ctx->ev_units_cap >= need && ctx->ev_bufs.ptrs.size()) return CYL_OK;
+  ctx->ev_bufs.free_all();
+  ctx->ev_units_cap = need;
+  int ns = ctx->nscen;
+  std::vector<int64_t> cb(ns);
+  int64_t nch = 0;
+  // chunk capacity per scenario, proportional to its pool size
+  std::vector<int64_t> cap_units(ns);
+  for (int s = 0; s < ns; ++s) {
+    int64_t rows = ctx->scen_base_h[s + 1] - ctx->scen_base_h[s];
+    cap_units[s] = std::max<int64_t>(rows, 1) * (need / std::max<int64_t>(ctx->n, 1) + 1);
+  }
+  int64_t evcap = 0;
+  for (int s = 0; s < ns; ++s) {
+    cb[s] = nch;
+    int64_t u = cap_units[s];
+    int64_t w = method == 2 ? u + u / 8 + 4096 : 2 * (u + u / 2 + 4096);
+    nch += (w + ZCHUNK - 1) / ZCHUNK + 1;
+    evcap += method == 2 ? w / 24 + 4096 : w / 4 + 4096;
+  }
+  ctx->nchunks = nch;
+  ctx->ev_cap = evcap;
+  Bufs& b = ctx->ev_bufs;
+  HIPCHK(dalloc(b, &ctx->d_chunk_base, ns));
+  HIPCHK(hipMemcpy(ctx->d_chunk_base, cb.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(dalloc(b, &ctx->d_chunk_count, nch + 1));
+  HIPCHK(dalloc(b, &ctx->d_chunk_off, nch + 1));
+  HIPCHK(dalloc(b, &ctx->d_ev_bsum, nblk(std::max(nch, evcap) + 1) + 1));
+  HIPCHK(dalloc(b, &ctx->d_ev_pos, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_ev_len, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_reach, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_bmax, nblk(evcap + 1) + 1));
+  HIPCHK(dalloc(b, &ctx->d_E, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_C, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_key, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_xtra, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_xscratch, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_on, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_ev_begin, ns + 1));
+  HIPCHK(dalloc(b, &ctx->d_nev, 1));
+  return CYL_OK;
+}
+
+// exclusive scan of n values (int or int64) into out (int64, n entries) with total
+template <class T>
+static void enqueue_scan(cyl_ctx* ctx, const T* in, int64_t* out, int64_t n, int64_t* bsum, int64_t* total,
+                         const int64_t* ndev, const int* cond, int& launches) {
+  unsigned nb = nblk(n);
+  k_scan_blocks<T><<<nb, BLK, 0, ctx->stream>>>(in, out, bsum, n, ndev, cond);
+  k_scan_totals<<<1, 1024, 0, ctx->stream>>>(bsum, nb, total, cond);
+  k_scan_add<<<nb, BLK, 0, ctx->stream>>>(out, bsum, n, cond);
+  launches += 3;
+}
+
+// active rank of every row within its scenario (pool order)
+static void enqueue_rank(cyl_ctx* ctx, const int* cond, int& launches) {
+  int n = ctx->n;
+  const PoolDev& A = ctx->pool[ctx->cur];
+  k_scatter_active<<<nblk(n), BLK, 0, ctx->stream>>>(A.inactive, ctx->d_gpool, n, ctx->d_vals, cond);
+  enqueue_scan(ctx, ctx->d_vals, ctx->d_scan, (int64_t)n, ctx->d_bsum, ctx->d_total, (const int64_t*)nullptr, cond,
+               launches);
+  k_gather_rank<<<nblk(n), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_gpool, ctx->d_scen, ctx->d_scen_base, n,
+                                                  ctx->d_rank, nullptr, cond);
+  k_scen_totals<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_total, ctx->d_scen_base, ctx->nscen,
+                                                           n, ctx->d_scen_active, cond);
+  k_clear_flag<<<1, 1, 0, ctx->stream>>>(ctx->d_flag, 1);
+  launches += 4;
+}
+
+// event tables of the step's stream window (rng_method 2 and 3)
+static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launches) {
+  if (method == 1) return;
+  EventArgs e{};
+  e.stream = ctx->d_stream;
+  e.jt = ctx->d_jt;
+  e.zt = ctx->d_zt;
+  e.scen_units = ctx->d_scen_units;
+  e.chunk_base = ctx->d_chunk_base;
+  e.nscen = ctx->nscen;
+  e.nchunks = ctx->nchunks;
+  e.method = method;
+  e.chunk_count = ctx->d_chunk_count;
+  e.chunk_off = ctx->d_chunk_off;
+  e.ev_pos = ctx->d_ev_pos;
+  e.ev_len = ctx->d_ev_len;
+  e.ev_cap = ctx->ev_cap;
+  e.overflow = ctx->d_flag + 2;
+  e.cond = cond;
+  hipStream_t s = ctx->stream;
+  k_rng_events<false><<<nblk(ctx->nchunks), BLK, 0, s>>>(e);
+  enqueue_scan(ctx, ctx->d_chunk_count, ctx->d_chunk_off, ctx->nchunks, ctx->d_ev_bsum, ctx->d_nev,
+               (const int64_t*)nullptr, cond, launches);
+  k_rng_events<true><<<nblk(ctx->nchunks), BLK, 0, s>>>(e);
+  k_ev_begin<<<nblk(ctx->nscen + 1), BLK, 0, s>>>(ctx->d_chunk_off, ctx->d_chunk_base, ctx->nscen, ctx->d_nev,
+                                                  ctx->d_ev_begin, cond);
+  launches += 3;
+  int64_t cap = ctx->ev_cap;
+  if (method == 2) {
+    unsigned nb = nblk(cap);
+    k_reach_key<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, cap,
+                                   ctx->d_xscratch, cond);
+    k_mscan_blocks<<<nb, BLK, 0, s>>>(ctx->d_xscratch, ctx->d_reach, ctx->d_bmax, cap, cond);
+    k_mscan_totals<<<1, 1024, 0, s>>>(ctx->d_bmax, nb, cond);
+    k_reach_decode<<<nb, BLK, 0, s>>>(ctx->d_reach, ctx->d_bmax, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, cap, cond);
+    k_zig_resolve<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->d_reach, ctx->d_nev,
+                                     ctx->nscen, ctx->d_on, cond);
+    k_zig_extra<<<nblk(cap + 1), BLK, 0, s>>>(ctx->d_on, ctx->d_ev_len, ctx->d_nev, cap + 1, ctx->d_xscratch, cond);
+    enqueue_scan(ctx, ctx->d_xscratch, ctx->d_E, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, (const int64_t*)nullptr,
+                 cond, launches);
+    enqueue_scan(ctx, ctx->d_on, ctx->d_C, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, ctx->d_nev, cond, launches);
+    k_zig_compact<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_on, ctx->d_E, ctx->d_C, ctx->d_ev_begin,
+                                     ctx->nscen, ctx->d_nev, ctx->d_key, ctx->d_xtra, cond);
+    launches += 7;
+  } else {
+    k_polar_keys<<<nblk(cap), BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, ctx->d_key, cond);
+    launches += 1;
+  }
+  k_stream_events<<<nblk(ctx->nscen), BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_C, method,
+                                                   ctx->d_key, ctx->d_xtra, cond);
+  launches += 1;
+}
+
+template <int DIM, int MODEL, bool COUNT>
+static void launch_step_t(cyl_ctx* ctx, const StepArgs& a) {
+  k_step<DIM, MODEL, COUNT><<<nblk(a.n, BLK), BLK, 0, ctx->stream>>>(a);
+}
+
+static void launch_step(cyl_ctx* ctx, const StepArgs& a, int model, bool count) {
+  int d = ctx->dim;
+  if (d == 2) {
+    if (model == 0) count ? launch_step_t<2, 0, true>(ctx, a) : launch_step_t<2, 0, false>(ctx, a);
+    else if (model == 1) count ? launch_step_t<2, 1, true>(ctx, a) : launch_step_t<2, 1, false>(ctx, a);
+    else count ? launch_step_t<2, 2, true>(ctx, a) : launch_step_t<2, 2, false>(ctx, a);
+  } else {
+    if (model == 0) count ? launch_step_t<3, 0, true>(ctx, a) : launch_step_t<3, 0, false>(ctx, a);
+    else if (model == 1) count ? launch_step_t<3, 1, true>(ctx, a) : launch_step_t<3, 1, false>(ctx, a);
+    else count ? launch_step_t<3, 2, true>(ctx, a) : launch_step_t<3, 2, false>(ctx, a);
+  }
+}
+
+static int upload_streams(cyl_ctx* ctx, int units_per_part) {
+  std::vector<StreamDev> st(ctx->nscen);
+  HIPCHK(hipMemcpy(st.data(), ctx->d_stream, ctx->nscen * sizeof(StreamDev), hipMemcpyDeviceToHost));
+  for (int s = 0; s < ctx->nscen; ++s) {
+    if (ctx->rng_dirty) {
+      st[s].base = ctx->rng_state[s];
+      st[s].inc = ctx->rng_inc[s];
+    }
+    st[s].units_per_part = units_per_part;
+  }
+  if (ctx->rng_dirty) {
+    std::vector<JumpTable> jt(ctx->nscen);
+    for (int s = 0; s < ctx->nscen; ++s) make_jump_table(ctx->rng_inc[s], jt[s]);
+    HIPCHK(hipMemcpy(ctx->d_jt, jt.data(), ctx->nscen * sizeof(JumpTable), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(ctx->d_stream, st.data(), ctx->nscen * sizeof(StreamDev), hipMemcpyHostToDevice));
+  ctx->rng_dirty = false;
+  return CYL_OK;
+}
+
+static int enqueue_sort(cyl_ctx* ctx, int& launches);
+
+static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, double dt, int nsteps, int* bnd_out,
+                     bool init_only) {
+  if (!ctx->has_mesh || !ctx->has_fields || ctx->n == 0) return ctx->set_err(CYL_E_STATE, -1, "context not ready");
+  if (p->model < 0 || p->model > 2)
+    return ctx->set_err(CYL_E_ARG, -1, "LSM3 (model 3) is not available in the device step");
+  if (p->rng_method < 1 || p->rng_method > 3) return ctx->set_err(CYL_E_ARG, -1, "Unknown RNG method");
+  HIPCHK(hipSetDevice(ctx->device));
+  auto t_start = std::chrono::steady_clock::now();
+  int umax = 0;
+  int D = predict_units(p, ctx->dim, dt, umax);
+  bool can_uniform = D >= 0 && !init_only;
+  int launches = 0;
+  int e = upload_streams(ctx, can_uniform ? D : 0);
+  if (e) return e;
+  e = ensure_events(ctx, (int64_t)umax * ctx->n, p->rng_method);
+  if (e) return e;
+  int nstat = ctx->nscen * std::max(ctx->nopen, 1);
+  size_t need_bnd = (size_t)std::max(nsteps, 1) * nstat;
+  if (need_bnd > ctx->bnd_cap) {
+    if (ctx->d_bnd) HIPCHK(hipFree(ctx->d_bnd));
+    HIPCHK(hipMalloc(&ctx->d_bnd, need_bnd * sizeof(int)));
+    ctx->bnd_cap = need_bnd;
+  }
+  HIPCHK(hipMemsetAsync(ctx->d_bnd, 0, need_bnd * sizeof(int), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_err, 0xff, ctx->nscen * sizeof(unsigned long long), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_flag + 2, 0, sizeof(int), ctx->stream));
+  StepArgs a{};
+  a.m = ctx->mesh;
+  a.par = to_dev(p);
+  a.pc = ctx->pc;
+  a.init_tri = ctx->d_init_tri;
+  a.del_step = ctx->d_del_step;
+  a.depth = ctx->dim == 3 ? ctx->d_depth : nullptr;
+  a.n = ctx->n;
+  a.dim = ctx->dim;
+  a.dt = dt;
+  a.stream = ctx->d_stream;
+  a.jt = ctx->d_jt;
+  a.zt = ctx->d_zt;
+  a.nopen = std::max(ctx->nopen, 1);
+  a.nopen_real = ctx->nopen;
+  a.n_deleted = ctx->d_n_deleted;
+  a.err = ctx->d_err_step;
+  a.stream_end = ctx->d_stream_end;
+  a.flag = ctx->d_flag;
+  a.units_pool = ctx->d_vals;
+  a.gpool = ctx->d_gpool;
+  a.init_only = init_only ? 1 : 0;
+  a.slot.pool = ctx->d_gpool;
+  a.slot.rank = ctx->d_rank;
+  a.slot.off = ctx->d_off;
+  a.slot.scen = ctx->d_scen;
+  a.slot.diameter = ctx->d_diam;
+  a.slot.volume = ctx->d_vol;
+  double t = *time_inout;
+  int* one = ctx->d_flag + 3;
+  float kms = 0.f;
+  int n_timed = 0;
+  for (int step = 0; step < nsteps; ++step) {
+    bool do_sort = !init_only && ctx->sort_interval > 0 && ctx->steps_since_sort >= ctx->sort_interval;
+    if (do_sort) {
+      e = enqueue_sort(ctx, launches);
+      if (e) return e;
+      ctx->steps_since_sort = 0;
+    }
+    if (!init_only) {
+      e = enqueue_time_interp(ctx, t, launches);
+      if (e) return e;
+    }
+    a.f = ctx->fields;
+    a.A = ctx->pool[ctx->cur];
+    a.B = ctx->pool[1 - ctx->cur];
+    a.step_id = ctx->step_counter;
+    a.bnd_stats = p->bnd_statistics ? ctx->d_bnd + (size_t)step * nstat : nullptr;
+    // active rank when rows were deleted or added since the last scan
+    enqueue_rank(ctx, ctx->d_flag + 1, launches);
+    bool uniform = can_uniform && !ctx->pending_init;
+    if (step == 0) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (uniform) {
+      k_uniform_units<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, ctx->nscen,
+                                                                 ctx->d_scen_units);
+      ++launches;
+      enqueue_events(ctx, p->rng_method, one, launches);
+      a.general = 0;
+      a.guarded = 0;
+      while (ctx->kev.size() < (size_t)2 * (step + 1)) {
+        hipEvent_t ev;
+        HIPCHK(hipEventCreate(&ev));
+        ctx->kev.push_back(ev);
+      }
+      HIPCHK(hipEventRecord(ctx->kev[2 * step], ctx->stream));
+      launch_step(ctx, a, p->model, false);
+      HIPCHK(hipEventRecord(ctx->kev[2 * step + 1], ctx->stream));
+      ++launches;
+      ++n_timed;
+    } else {
+      HIPCHK(hipMemsetAsync(ctx->d_flag, 0xff, sizeof(int), ctx->stream));  // force the general path
+    }
+    // general mode: exact per-row unit counts, pool-order scan, rerun
+    int* g = ctx->d_flag;
+    k_rerun_reset<<<nblk(std::max(nstat, ctx->nscen)), BLK, 0, ctx->stream>>>(
+        ctx->d_stream_end, ctx->d_err_step, a.bnd_stats, nstat, ctx->nscen, ctx->d_n_deleted, g);
+    a.general = 0;
+    a.guarded = 1;
+    launch_step(ctx, a, p->model, true);
+    enqueue_scan(ctx, ctx->d_vals, ctx->d_scan, (int64_t)ctx->n, ctx->d_bsum, ctx->d_total, (const int64_t*)nullptr,
+                 g, launches);
+    k_gather_rank<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_gpool, ctx->d_scen, ctx->d_scen_base,
+                                                         ctx->n, nullptr, ctx->d_off, g);
+    k_scen_totals<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_total, ctx->d_scen_base,
+                                                             ctx->nscen, ctx->n, ctx->d_scen_units, g);
+    launches += 4;
+    enqueue_events(ctx, p->rng_method, g, launches);
+    a.general = 1;
+    a.guarded = 1;
+    launch_step(ctx, a, p->model, false);
+    ++launches;
+    if (step == nsteps - 1) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    k_finish<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_stream_end, ctx->nscen,
+                                                        ctx->d_flag, ctx->d_n_deleted, ctx->d_n_deleted_total,
+                                                        ctx->step_counter, ctx->d_step_done, ctx->d_err_step,
+                                                        ctx->d_err);
+    ++launches;
+    ctx->cur = 1 - ctx->cur;
+    ctx->have_last = !init_only;
+    ctx->pending_init = false;
+    ++ctx->step_counter;
+    ++ctx->steps_since_sort;
+    if (!init_only) t += dt;  // SimuTime.increment, simutime.pyx:81
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (nsteps > 0) HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
+  ctx->last_kernel_ms = kms;
+  ctx->last_launches = launches;
+  double sk = 0.;
+  for (int q = 0; q < n_timed; ++q) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, ctx->kev[2 * q], ctx->kev[2 * q + 1]));
+    sk += ms;
+  }
+  ctx->last_step_kernel_ms = sk;
+  ctx->last_step_launches = n_timed;
+  if (bnd_out && p->bnd_statistics && ctx->nopen > 0) {
+    std::vector<int> h(need_bnd);
+    HIPCHK(hipMemcpy(h.data(), ctx->d_bnd, need_bnd * sizeof(int), hipMemcpyDeviceToHost));
+    for (int s = 0; s < nsteps; ++s)
+      for (int q = 0; q < ctx->nopen; ++q) bnd_out[(size_t)s * ctx->nopen + q] = h[(size_t)s * nstat + q];
+  }
+  int flags[4];
+  HIPCHK(hipMemcpy(flags, ctx->d_flag, 4 * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<unsigned long long> errs(ctx->nscen);
+  HIPCHK(hipMemcpy(errs.data(), ctx->d_err, ctx->nscen * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  *time_inout = t;
+  ctx->last_total_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (flags[2]) return ctx->set_err(CYL_E_STEP, -1, "random-stream event table overflow");
+  for (int s = 0; s < ctx->nscen; ++s) {
+    if (errs[s] != ~0ull) {
+      int code = (int)(errs[s] & 0xff), pidx = (int)(errs[s] >> 8);
+      static const char* msgs[] = {"",
+                                   "Failed interpolation, point not in triangulation",
+                                   "Failed interpolation, point is below bottom",
+                                   "Failed interpolation, point is above surface",
+                                   "3D interpolation failed: degenerate vertical layer detected",
+                                   "LSM2 requires finite a1 >= 0, dt >= 0 and a1*dt",
+                                   "Wrong time scheme",
+                                   "random-stream window too small",
+                                   "draw-count misprediction",
+                                   "neighbourhood search needs triangle_adjacency",
+                                   "LSM3 is not available in the device step"};
+      std::string m = code > 0 && code <= 10 ? msgs[code] : "device error";
+      ctx->errcode = code;
+      return ctx->set_err(CYL_E_STEP, pidx, m);
+    }
+  }
+  return CYL_OK;
+}
+
+extern "C" int cyl_step(cyl_ctx* ctx, const cyl_params* p, double* time_inout, double dt, int nsteps, int* bnd) {
+  if (!ctx || !p || !time_inout || nsteps < 0) return CYL_E_ARG;
+  ctx->fields.frozen = ctx->frozen;
+  return run_steps(ctx, p, time_inout, dt, nsteps, bnd, false);
+}
+
+extern "C" int cyl_init_particles(cyl_ctx* ctx, const cyl_params* p, double time, double dt) {
+  if (!ctx || !p) return CYL_E_ARG;
+  double t = time;
+  ctx->fields.frozen = ctx->frozen;
+  int e = run_steps(ctx, p, &t, dt, 1, nullptr, true);
+  ctx->have_last = false;
+  return e;
+}
+
+extern "C" int cyl_set_tuning(cyl_ctx* ctx, int sort_interval, int block_size) {
+  if (!ctx || sort_interval < 0) return CYL_E_ARG;
+  ctx->sort_interval = sort_interval;
+  if (block_size > 0) ctx->block = block_size;
+  return CYL_OK;
+}
+
+extern "C" int cyl_last_timing(cyl_ctx* ctx, double* kms, double* tms, int* launches) {
+  if (!ctx) return CYL_E_ARG;
+  if (kms) *kms = ctx->last_step_kernel_ms;
+  if (tms) *tms = ctx->last_total_ms;
+  if (launches) *launches = ctx->last_step_launches;
+  return CYL_OK;
+}
+
+// Counting sort of the slots by element, then permutation of both state
+// buffers' current copy and of every per-slot array.
+static int enqueue_sort(cyl_ctx* ctx, int& launches) {
+  int n = ctx->n, nt = ctx->mesh.nt;
+  hipStream_t s = ctx->stream;
+  if (!ctx->d_scnt) {
+    HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scnt, nt + 3));
+    HIPCHK(dalloc(ctx->part_bufs, &ctx->d_cursor, nt + 3));
+  }
+  PoolDev& A = ctx->pool[ctx->cur];
+  PoolDev& B = ctx->pool[1 - ctx->cur];
+  HIPCHK(hipMemsetAsync(ctx->d_scnt, 0, (nt + 3) * sizeof(int), s));
+  k_sort_hist<<<nblk(n), BLK, 0, s>>>(A.tri, A.inactive, n, nt, ctx->d_scnt, ctx->d_skey);
+  enqueue_scan(ctx, ctx->d_scnt, ctx->d_cursor, (int64_t)nt + 3, ctx->d_bsum, ctx->d_total,
+               (const int64_t*)nullptr, (const int*)nullptr, launches);
+  k_sort_place<<<nblk(n), BLK, 0, s>>>(ctx->d_skey, n, ctx->d_cursor, ctx->d_perm);
+  launches += 2;
+  // state: A -> B via perm, then swap so the sorted copy is current
+  auto pd = [&](double* src, double* dst) { k_permute<double><<<nblk(n), BLK, 0, s>>>(src, dst, ctx->d_perm, n); };
+  auto pi = [&](int* src, int* dst) { k_permute<int><<<nblk(n), BLK, 0, s>>>(src, dst, ctx->d_perm, n); };
+  pd(A.x, B.x);
+  pd(A.y, B.y);
+  if (ctx->dim == 3) pd(A.z, B.z);
+  for (int c = 0; c < ctx->dim; ++c) {
+    pd(A.up[c], B.up[c]);
+    pd(A.us[c], B.us[c]);
+  }
+  pi(A.tri, B.tri);
+  pi(A.ll, B.ll);
+  pi(A.inactive, B.inactive);
+  launches += 5 + 2 * ctx->dim;
+  ctx->cur = 1 - ctx->cur;
+  // per-slot arrays through the scratch buffer
+  int* ti = (int*)ctx->d_tmpbuf;
+  double* td = (double*)ctx->d_tmpbuf;
+  int* islots[] = {ctx->d_gpool, ctx->d_init_tri, ctx->d_del_step};
+  for (int* arr : islots) {
+    pi(arr, ti);
+    HIPCHK(hipMemcpyAsync(arr, ti, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+    ++launches;
+  }
+  if (ctx->d_scen) {
+    pi(ctx->d_scen, ti);
+    HIPCHK(hipMemcpyAsync(ctx->d_scen, ti, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+    ++launches;
+  }
+  double* dslots[] = {ctx->d_diam, ctx->d_vol};
+  for (double* arr : dslots) {
+    pd(arr, td);
+    HIPCHK(hipMemcpyAsync(arr, td, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ++launches;
+  }
+  if (ctx->dim == 3) {
+    pd(ctx->d_depth, td);
+    HIPCHK(hipMemcpyAsync(ctx->d_depth, td, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    ++launches;
+  }
+  // ranks follow their rows: recompute
+  int one = 1;
+  HIPCHK(hipMemcpyAsync(ctx->d_flag + 1, &one, sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  ctx->have_last = false;
+  return CYL_OK;
+}

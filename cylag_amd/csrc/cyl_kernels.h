@@ -1,0 +1,627 @@
+// Kernels of the step pipeline.
+//
+//  k_step        fused particle update: arrival init, LSM update, walk,
+//                boundary conditions; one thread per pool slot (physical order)
+//  k_time_interp tmp = tf*f[rec] + (1-tf)*f[rec+1] on every node of every field
+//  pool scans    scatter a per-slot value to pool order, exclusive scan with a
+//                reset at every scenario, gather back (active rank, draw offsets)
+//  rng events    slow Ziggurat starts / rejected polar attempts in the step's
+//                stream window, resolved into the serial chain of normals
+//  sort          counting sort of slots by element, then permutation of state
+#pragma once
+#include "cyl_particle.h"
+
+namespace cyl {
+
+constexpr int BLK = 256;
+
+struct StepArgs {
+  MeshDev m;
+  FieldsDev f;
+  ParamsDev par;
+  ParticleConst pc;
+  PoolDev A, B;       // input / output state
+  SlotDev slot;
+  const int* init_tri;
+  int* del_step;      // per slot: step at which the row was deleted
+  double* depth;      // per slot (compute_depth) or null
+  int n;              // physical slots
+  int dim;
+  double dt;
+  int step_id;
+  StreamDev* stream;  // device copy, one per scenario
+  const JumpTable* jt;
+  const ZigTables* zt;
+  int* bnd_stats;     // [nscen][nopen] of this step
+  int nopen;       // stride of bnd_stats
+  int nopen_real;  // open boundaries of the mesh
+  int* n_deleted;
+  unsigned long long* err;        // per scenario
+  unsigned long long* stream_end; // per scenario: max end position
+  int* flag;          // [0] mispredict, [1] general-mode request consumed
+  int* units_pool;    // COUNT mode: units per global pool index
+  const int* gpool;   // global pool index of each slot
+  int init_only;      // 1: only initialize tri == -2 rows
+  int general;        // 1: offsets come from slot.off
+  int guarded;        // general-mode launch that runs only if flag[0] is set
+};
+
+__device__ __forceinline__ void load_part(const PoolDev& A, const SlotDev& s, int j, int dim, Part& p) {
+  p.x[0] = A.x[j];
+  p.x[1] = A.y[j];
+  p.x[2] = dim == 3 ? A.z[j] : 0.;
+  for (int c = 0; c < 3; ++c) {
+    p.up[c] = c < dim ? A.up[c][j] : 0.;
+    p.us[c] = c < dim ? A.us[c][j] : 0.;
+    p.lx[c] = p.x[c];
+    p.lup[c] = p.up[c];
+  }
+  p.tri = A.tri[j];
+  p.ll = A.ll[j];
+  p.last_tri = p.tri;
+  p.inactive = A.inactive[j];
+  p.deleted = 0;
+  p.nrebound = 0;
+  p.diameter = s.diameter ? s.diameter[j] : 0.;
+  p.volume = s.volume ? s.volume[j] : 0.;
+}
+
+__device__ __forceinline__ void store_part(const PoolDev& B, int j, int dim, const Part& p, int inactive_code) {
+  B.x[j] = p.x[0];
+  B.y[j] = p.x[1];
+  if (dim == 3) B.z[j] = p.x[2];
+  for (int c = 0; c < dim; ++c) {
+    B.up[c][j] = p.up[c];
+    B.us[c][j] = p.us[c];
+  }
+  B.tri[j] = p.tri;
+  B.ll[j] = p.ll;
+  B.inactive[j] = inactive_code;
+}
+
+template <int DIM, int MODEL, bool COUNT>
+__global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
+  if (a.guarded && a.flag[0] == 0) return;
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= a.n) return;
+  Part p;
+  load_part(a.A, a.slot, j, DIM, p);
+  const int scen = a.slot.scen ? a.slot.scen[j] : 0;
+  if (p.inactive) {
+    if (COUNT) a.units_pool[a.gpool[j]] = 0;
+    if (!COUNT) {
+      // keep the inactive row (deleted rows keep their "deleted" mark)
+      a.B.inactive[j] = p.inactive;
+      a.B.tri[j] = -1;
+      a.B.ll[j] = 0;
+    }
+    return;
+  }
+  const StreamDev& st = a.stream[scen];
+  Ctx c{a.m, a.f, a.par, a.pc, a.dt, DIM, 0};
+  Draws dr;
+  dr.st = &st;
+  dr.jt = &a.jt[scen];
+  dr.zt = a.zt;
+  dr.method = a.par.rng_method;
+  dr.dry = COUNT;
+  dr.nunits = 0;
+  dr.ncalls = 0;
+  dr.pos0 = 0;
+  dr.live = false;
+  dr.err = 0;
+  if (!COUNT) {
+    if (a.general) dr.unit0 = a.slot.off[j];
+    else dr.unit0 = (int64_t)st.units_per_part * a.slot.rank[j];
+  }
+  int pool = a.slot.pool[j];
+  int bnd_j = -1;
+  bool was_new = p.tri == -2;
+  if (was_new) init_particle(c, p, dr, a.init_tri ? a.init_tri[j] : -3);
+  if (!p.inactive && !c.err && !a.init_only) {
+    if (MODEL == 0) bnd_j = update_test(c, p);
+    else if (MODEL == 1) bnd_j = update_lsm1(c, p, dr);
+    else bnd_j = update_lsm2(c, p, dr);
+    if (!c.err) {
+      if (COUNT) {
+        // units are counted before the boundary conditions (which use a sub-stream)
+      } else if (a.par.boundary_conditions) {
+        BcDraws bd;
+        bd.method = a.par.rng_method;
+        bd.zt = a.zt;
+        bd.g.inc = 0x6c62272e07bb0143ULL | 1u;
+        bd.g.state = st.base ^ ((uint64_t)(unsigned)pool * 0x9e3779b97f4a7c15ULL);
+        compute_bc(c, p, bnd_j, bd, a.bnd_stats ? a.bnd_stats + scen * a.nopen : nullptr, a.nopen_real);
+      } else {
+        if (p.tri == PART_LOC_O) delete_part(p, DIM);
+        if (p.ll == PART_LOC_B) delete_part(p, DIM);
+        if (p.ll == PART_LOC_A) delete_part(p, DIM);
+      }
+      if (!COUNT && a.par.compute_depth && !p.inactive && DIM == 3 && a.depth) {
+        a.depth[j] = field2d(a.f, a.m, a.f.fid_zs, p.tri, a.f.nlayers - 1, p.x[0], p.x[1]) - p.x[2];
+      }
+    }
+  }
+  if (was_new && !p.inactive && a.par.compute_depth && DIM == 3 && a.depth && !COUNT && !c.err &&
+      a.init_only) {
+    a.depth[j] = field2d(a.f, a.m, a.f.fid_zs, p.tri, a.f.nlayers - 1, p.x[0], p.x[1]) - p.x[2];
+  }
+  if (dr.err && !c.err) c.err = dr.err;
+  if (COUNT) {
+    a.units_pool[a.gpool[j]] = (int)dr.nunits;
+    if (c.err) atomicMin(&a.err[scen], ((unsigned long long)(unsigned)pool << 8) | (unsigned)c.err);
+    return;
+  }
+  if (c.err) atomicMin(&a.err[scen], ((unsigned long long)(unsigned)pool << 8) | (unsigned)c.err);
+  if (!a.general && !a.init_only && dr.nunits != st.units_per_part) atomicOr(&a.flag[0], 1);
+  if (dr.live) atomicMax(&a.stream_end[scen], (unsigned long long)(dr.pos0 + dr.ncalls));
+  int code = 0;
+  if (p.deleted) {
+    code = 2;
+    atomicAdd(a.n_deleted, 1);
+    if (a.del_step) a.del_step[j] = a.step_id;
+  } else if (p.inactive) {
+    code = 1;
+  }
+  store_part(a.B, j, DIM, p, code);
+}
+
+// tmp[fid][i] = tf*frame_rec[i] + (1-tf)*frame_rec1[i]  (eulerian_field_set.pyx:926-938)
+__global__ void __launch_bounds__(BLK) k_time_interp(const double* const* f0, const double* const* f1,
+                                                     double* const* out, int nf, int64_t size, double tf) {
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  int q = blockIdx.y;
+  if (i >= size || q >= nf) return;
+  out[q][i] = tf * f0[q][i] + (1. - tf) * f1[q][i];
+}
+
+// ---------------------------------------------------------------------------
+// Pool-order scans
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(BLK) k_scatter_active(const int* inactive, const int* gpool, int n,
+                                                        int* vals, const int* cond) {
+  if (cond && *cond == 0) return;
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j < n) vals[gpool[j]] = inactive[j] == 0 ? 1 : 0;
+}
+
+// block-local exclusive scan into int64, block totals out; elements at or past
+// the device count *ndev (if given) read as zero
+template <class T>
+__global__ void __launch_bounds__(BLK) k_scan_blocks(const T* in, int64_t* out, int64_t* block_sum, int64_t n,
+                                                     const int64_t* ndev, const int* cond) {
+  if (cond && *cond == 0) return;
+  __shared__ int64_t s[BLK];
+  int64_t lim = ndev ? (*ndev < n ? *ndev : n) : n;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  int64_t v = i < lim ? (int64_t)in[i] : 0;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < BLK; o <<= 1) {
+    int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < n) out[i] = s[threadIdx.x] - v;
+  if (threadIdx.x == BLK - 1) block_sum[blockIdx.x] = s[BLK - 1];
+}
+
+// exclusive scan of the block totals by one block (strided chunks)
+__global__ void __launch_bounds__(1024) k_scan_totals(int64_t* bs, int64_t nb, int64_t* total, const int* cond) {
+  if (cond && *cond == 0) return;
+  __shared__ int64_t s[1024];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < nb; base += 1024) {
+    int64_t i = base + threadIdx.x;
+    int64_t v = i < nb ? bs[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
+      __syncthreads();
+      s[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nb) bs[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ void __launch_bounds__(BLK) k_scan_add(int64_t* out, const int64_t* bs, int64_t n, const int* cond) {
+  if (cond && *cond == 0) return;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) out[i] += bs[blockIdx.x];
+}
+
+// slot value = scan[gpool] - scan[scenario base]; as int rank or int64 offset
+__global__ void __launch_bounds__(BLK) k_gather_rank(const int64_t* scan, const int* gpool, const int* scen,
+                                                     const int* scen_base, int n, int* rank, int64_t* off,
+                                                     const int* cond) {
+  if (cond && *cond == 0) return;
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= n) return;
+  int s = scen ? scen[j] : 0;
+  int64_t v = scan[gpool[j]] - scan[scen_base[s]];
+  if (rank) rank[j] = (int)v;
+  if (off) off[j] = v;
+}
+
+// total units of each scenario after a scan: scan[end] - scan[base]
+__global__ void k_scen_totals(const int64_t* scan, const int64_t* total, const int* scen_base, int nscen,
+                              int npool, int64_t* scen_units, const int* cond) {
+  if (cond && *cond == 0) return;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nscen) return;
+  int e = scen_base[s + 1];
+  int64_t end = e >= npool ? *total : scan[e];
+  scen_units[s] = end - scan[scen_base[s]];
+}
+
+// ---------------------------------------------------------------------------
+// Random-stream events of one step (per scenario window)
+// ---------------------------------------------------------------------------
+constexpr int ZCHUNK = 64;
+
+struct EventArgs {
+  StreamDev* stream;          // [nscen]
+  const JumpTable* jt;        // [nscen]
+  const ZigTables* zt;
+  const int64_t* scen_units;  // units needed per scenario this step
+  const int64_t* chunk_base;  // first chunk of each scenario (host-sized capacity)
+  int nscen;
+  int64_t nchunks;
+  int method;
+  int* chunk_count;           // events per chunk
+  int64_t* chunk_off;         // scanned
+  int64_t* ev_pos;            // candidate position (relative to window start)
+  int64_t* ev_len;            // positions consumed by the normal starting there (zig)
+  int64_t ev_cap;
+  int* overflow;
+  const int* cond;            // run only when *cond != 0
+};
+
+__device__ __forceinline__ int scen_of_chunk(const EventArgs& e, int64_t ch) {
+  int lo = 0, hi = e.nscen - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (e.chunk_base[mid] <= ch) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// window needed for u units: zig normals take ~1.03 positions on average
+__device__ __forceinline__ int64_t window_for(int method, int64_t units) {
+  if (units <= 0) return 0;
+  if (method == 2) return units + units / 8 + 4096;
+  return 2 * (units + units / 2 + 4096);  // polar: attempts of 2 positions, ~21% rejected
+}
+
+// Ziggurat: positions consumed by a normal that starts at state g (before the first call)
+__device__ inline int64_t zig_length(Pcg g, const ZigTables& z) {
+  Draws d;
+  d.zt = &z;
+  d.g = g;
+  d.ncalls = 0;
+  d.zig();
+  return d.ncalls;
+}
+
+template <bool WRITE>
+__global__ void __launch_bounds__(BLK) k_rng_events(EventArgs e) {
+  if (e.cond && *e.cond == 0) return;
+  int64_t ch = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (ch >= e.nchunks) return;
+  int s = scen_of_chunk(e, ch);
+  int64_t local = ch - e.chunk_base[s];
+  int64_t W = window_for(e.method, e.scen_units[s]);
+  int64_t cap_chunks = (s + 1 < e.nscen ? e.chunk_base[s + 1] : e.nchunks) - e.chunk_base[s];
+  if (!WRITE && threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int q = 0; q < e.nscen; ++q) e.stream[q].window = window_for(e.method, e.scen_units[q]);
+  }
+  int64_t p0 = local * ZCHUNK;
+  int cnt = 0;
+  int64_t w = 0;
+  if (WRITE) w = e.chunk_off[ch];
+  if (p0 < W) {
+    if (!WRITE && (local + 1) * ZCHUNK < W && local + 1 == cap_chunks) atomicOr(e.overflow, 1);
+    const StreamDev& st = e.stream[s];
+    Pcg g;
+    g.inc = st.inc;
+    if (e.method == 2) {
+      g.state = pcg_jump(st.base, (uint64_t)p0, e.jt[s]);
+      for (int q = 0; q < ZCHUNK; ++q) {
+        Pcg at = g;
+        uint32_t u = pcg_next(g);
+        int32_t hz = (int32_t)u;
+        uint32_t iz = (uint32_t)hz & 127u;
+        uint32_t ahz = hz < 0 ? (uint32_t)(-(int64_t)hz) : (uint32_t)hz;
+        if (ahz < e.zt->kn[iz]) continue;
+        if (WRITE) {
+          if (w < e.ev_cap) {
+            e.ev_pos[w] = p0 + q;
+            e.ev_len[w] = zig_length(at, *e.zt);
+          } else atomicOr(e.overflow, 2);
+          ++w;
+        } else ++cnt;
+      }
+    } else {
+      // polar: attempt a uses positions 2a, 2a+1
+      int64_t a0 = p0 / 2;
+      g.state = pcg_jump(st.base, (uint64_t)(2 * a0), e.jt[s]);
+      for (int q = 0; q < ZCHUNK / 2; ++q) {
+        uint32_t u1 = pcg_next(g), u2 = pcg_next(g);
+        if (polar_accept(u1, u2)) continue;
+        if (WRITE) {
+          if (w < e.ev_cap) e.ev_pos[w] = a0 + q;
+          else atomicOr(e.overflow, 2);
+          ++w;
+        } else ++cnt;
+      }
+    }
+  }
+  if (!WRITE) e.chunk_count[ch] = cnt;
+}
+
+// Ziggurat chain resolution.  Position p is off the chain iff an on-chain slow
+// start s < p has s + len(s) - 1 >= p.  A candidate is surely on-chain when no
+// earlier candidate reaches it; from there the chain is followed forward.
+__global__ void __launch_bounds__(BLK) k_zig_resolve(const int64_t* pos, const int64_t* len, const int64_t* ev_begin,
+                                                     const int64_t* reach_excl, const int64_t* nev, int nscen,
+                                                     int* on_chain, const int* cond) {
+  if (cond && *cond == 0) return;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= *nev) return;
+  int lo = 0, hi = nscen - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ev_begin[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  int64_t b = ev_begin[lo];
+  int64_t j0 = i;
+  while (j0 > b && reach_excl[j0] >= pos[j0]) --j0;
+  int64_t cur_end = pos[j0] + len[j0] - 1;
+  int on = 1;
+  for (int64_t l = j0 + 1; l <= i; ++l) {
+    if (pos[l] > cur_end) {
+      cur_end = pos[l] + len[l] - 1;
+      on = 1;
+    } else on = 0;
+  }
+  on_chain[i] = (i == j0) ? 1 : on;
+}
+
+// reach key of event i: (scenario << 40) | (pos + len - 1); the exclusive
+// max-scan of the keys, decoded per scenario, is the reach of earlier events
+constexpr int REACH_SHIFT = 40;
+
+__global__ void __launch_bounds__(BLK) k_reach_key(const int64_t* pos, const int64_t* len, const int64_t* ev_begin,
+                                                   int nscen, const int64_t* nev, int64_t cap, int64_t* key,
+                                                   const int* cond) {
+  if (cond && *cond == 0) return;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= cap) return;
+  if (i >= *nev) {
+    key[i] = -1;
+    return;
+  }
+  int lo = 0, hi = nscen - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ev_begin[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  key[i] = ((int64_t)lo << REACH_SHIFT) | (pos[i] + len[i] - 1);
+}
+
+__global__ void __launch_bounds__(BLK) k_mscan_blocks(const int64_t* in, int64_t* out, int64_t* bmax, int64_t n,
+                                                      const int* cond) {
+  if (cond && *cond == 0) return;
+  __shared__ int64_t s[BLK];
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  int64_t v = i < n ? in[i] : -1;
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 1; o < BLK; o <<= 1) {
+    int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : -1;
+    __syncthreads();
+    s[threadIdx.x] = t > s[threadIdx.x] ? t : s[threadIdx.x];
+    __syncthreads();
+  }
+  int64_t excl = threadIdx.x > 0 ? s[threadIdx.x - 1] : -1;
+  if (i < n) out[i] = excl;
+  if (threadIdx.x == BLK - 1) bmax[blockIdx.x] = s[BLK - 1];
+}
+
+__global__ void __launch_bounds__(1024) k_mscan_totals(int64_t* bm, int64_t nb, const int* cond) {
+  if (cond && *cond == 0) return;
+  __shared__ int64_t s[1024];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = -1;
+  __syncthreads();
+  for (int64_t base = 0; base < nb; base += 1024) {
+    int64_t i = base + threadIdx.x;
+    int64_t v = i < nb ? bm[i] : -1;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : -1;
+      __syncthreads();
+      s[threadIdx.x] = t > s[threadIdx.x] ? t : s[threadIdx.x];
+      __syncthreads();
+    }
+    int64_t excl = threadIdx.x > 0 ? s[threadIdx.x - 1] : -1;
+    if (i < nb) bm[i] = excl > carry ? excl : carry;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = s[1023] > carry ? s[1023] : carry;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_reach_decode(int64_t* out, const int64_t* bmax, const int64_t* ev_begin,
+                                                      int nscen, const int64_t* nev, int64_t cap, const int* cond) {
+  if (cond && *cond == 0) return;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= cap || i >= *nev) return;
+  int64_t m = out[i], bm = bmax[blockIdx.x];
+  m = bm > m ? bm : m;
+  int lo = 0, hi = nscen - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ev_begin[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  out[i] = (m >= 0 && (m >> REACH_SHIFT) == lo) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
+}
+
+// Sort key for the element re-sort: active rows by element, then the rest.
+__global__ void __launch_bounds__(BLK) k_sort_hist(const int* tri, const int* inactive, int n, int nt, int* cnt,
+                                                   int* key) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= n) return;
+  int t = tri[j];
+  int k = inactive[j] ? nt + 1 : (t >= 0 ? t : nt);
+  key[j] = k;
+  atomicAdd(&cnt[k], 1);
+}
+
+__global__ void __launch_bounds__(BLK) k_sort_place(const int* key, int n, int64_t* cursor, int* perm) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= n) return;
+  unsigned long long d = atomicAdd((unsigned long long*)&cursor[key[j]], 1ull);
+  perm[d] = j;
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLK) k_permute(const T* src, T* dst, const int* perm, int n) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j < n) dst[j] = src[perm[j]];
+}
+
+__global__ void __launch_bounds__(BLK) k_slot_of_pool(const int* gpool, int n, int* slot_of_pool) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j < n) slot_of_pool[gpool[j]] = j;
+}
+
+__device__ __forceinline__ int scen_of_event(const int64_t* ev_begin, int nscen, int64_t i) {
+  int lo = 0, hi = nscen - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (ev_begin[mid] <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ev_begin[s] = first event of scenario s; ev_begin[nscen] = number of events
+__global__ void k_ev_begin(const int64_t* chunk_off, const int64_t* chunk_base, int nscen, const int64_t* total,
+                           int64_t* ev_begin, const int* cond) {
+  if (cond && *cond == 0) return;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nscen) ev_begin[s] = chunk_off[chunk_base[s]];
+  if (s == 0) ev_begin[nscen] = *total;
+}
+
+__global__ void __launch_bounds__(BLK) k_zig_extra(const int* on, const int64_t* len, const int64_t* nev,
+                                                   int64_t cap, int64_t* extra, const int* cond) {
+  if (cond && *cond == 0) return;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= cap) return;
+  extra[i] = (i < *nev && on[i]) ? len[i] - 1 : 0;
+}
+
+// compact on-chain events: normal index and cumulative extra positions (inclusive)
+__global__ void __launch_bounds__(BLK) k_zig_compact(const int64_t* pos, const int64_t* len, const int* on,
+                                                     const int64_t* E, const int64_t* Cidx, const int64_t* ev_begin,
+                                                     int nscen, const int64_t* nev, int64_t* key, int64_t* xtra,
+                                                     const int* cond) {
+  if (cond && *cond == 0) return;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= *nev || !on[i]) return;
+  int s = scen_of_event(ev_begin, nscen, i);
+  int64_t eb = E[ev_begin[s]];
+  int64_t c = Cidx[i];
+  key[c] = pos[i] - (E[i] - eb);
+  xtra[c] = (E[i] - eb) + len[i] - 1;
+}
+
+// polar: key_j = r_j - j (j counted within the scenario)
+__global__ void __launch_bounds__(BLK) k_polar_keys(const int64_t* pos, const int64_t* ev_begin, int nscen,
+                                                    const int64_t* nev, int64_t* key, const int* cond) {
+  if (cond && *cond == 0) return;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= *nev) return;
+  int s = scen_of_event(ev_begin, nscen, i);
+  key[i] = pos[i] - (i - ev_begin[s]);
+}
+
+// point each scenario's stream at its slice of the event tables
+__global__ void k_stream_events(StreamDev* st, int nscen, const int64_t* ev_begin, const int64_t* Cidx, int method,
+                                int64_t* key, int64_t* xtra, const int* cond) {
+  if (cond && *cond == 0) return;
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nscen) return;
+  if (method == 2) {
+    int64_t c0 = Cidx[ev_begin[s]], c1 = Cidx[ev_begin[s + 1]];
+    st[s].ev_key = key + c0;
+    st[s].ev_extra = xtra + c0;
+    st[s].n_ev = c1 - c0;
+  } else {
+    st[s].ev_key = key + ev_begin[s];
+    st[s].ev_extra = nullptr;
+    st[s].n_ev = ev_begin[s + 1] - ev_begin[s];
+  }
+}
+
+// end of step: advance each scenario's stream past its last draw, reset counters
+__global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long* stream_end, int nscen, int* flag,
+                         int* n_deleted, long long* n_deleted_total, int step_id, int* step_done,
+                         unsigned long long* err_step, unsigned long long* err) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nscen) {
+    unsigned long long e = stream_end[s];
+    if (e) st[s].base = pcg_jump(st[s].base, e, jt[s]);
+    stream_end[s] = 0;
+    if (err_step[s] < err[s]) err[s] = err_step[s];
+    err_step[s] = ~0ull;
+  }
+  if (s == 0) {
+    if (*n_deleted) flag[1] = 1;
+    *n_deleted_total += *n_deleted;
+    *n_deleted = 0;
+    flag[0] = 0;
+    *step_done = step_id + 1;
+  }
+}
+
+// reset the per-step outputs before a general-mode rerun of the step
+__global__ void k_rerun_reset(unsigned long long* stream_end, unsigned long long* err, int* bnd_stats, int nstat,
+                              int nscen, int* n_deleted, const int* cond) {
+  if (cond && *cond == 0) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nscen) {
+    stream_end[i] = 0;
+    err[i] = ~0ull;
+  }
+  if (bnd_stats && i < nstat) bnd_stats[i] = 0;
+  if (i == 0) *n_deleted = 0;
+}
+
+// units needed this step in uniform mode: units_per_part * active rows of the scenario
+__global__ void k_uniform_units(const StreamDev* st, const int64_t* scen_active, int nscen, int64_t* scen_units) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nscen) scen_units[s] = (int64_t)st[s].units_per_part * scen_active[s];
+}
+
+__global__ void k_clear_flag(int* flag, int idx) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) flag[idx] = 0;
+}
+
+}  // namespace cyl

@@ -1,0 +1,926 @@
+// One particle's time step on the device: random draws at the particle's
+// serial offset, LSM-1 / test-kernel / LSM-2 updates, boundary conditions and
+// arrival initialization.  The control flow and expression order restate
+// cylag/lsm/update_state_lsm1.pyx, update_state_test.pyx,
+// update_state_lsm2.pyx, init_lsm2_fluctuations.pyx,
+// cylag/boundary_conditions/compute_bc.pyx and cylag/core/solver.pyx:238-549.
+#pragma once
+#include "cyl_geom.h"
+#include "cyl_interp.h"
+
+namespace cyl {
+
+// Thread-local copy of one pool row (LagrangianParticleSet arrays of row i).
+struct Part {
+  double x[3], up[3], us[3];
+  double lx[3], lup[3];  // last_position / last_velocity
+  int tri, ll, last_tri;
+  int inactive, deleted, nrebound;
+  double diameter, volume;
+};
+
+__device__ inline void delete_part(Part& p, int dim) {
+  // delete_single, lagrangian_particle_set.pyx:754-788 (tag renumbering is done on the host)
+  if (p.inactive) return;
+  p.inactive = 1;
+  p.deleted = 1;
+  p.diameter = 0.;
+  p.volume = 0.;
+  for (int c = 0; c < 3; ++c) {
+    p.x[c] = 0.; p.up[c] = 0.; p.us[c] = 0.; p.lx[c] = 0.; p.lup[c] = 0.;
+  }
+  p.tri = -1;
+  p.ll = 0;
+  p.last_tri = -1;
+}
+
+// ---------------------------------------------------------------------------
+// Random draws.  A "unit" is one normal for rng_method 1 and 2 and one call of
+// the polar sampler for rng_method 3.  The particle's first unit comes from the
+// pool-order scan; its stream position is found through the slow-event table
+// of the step (Ziggurat, polar) and the particle then samples sequentially.
+// ---------------------------------------------------------------------------
+struct Draws {
+  const StreamDev* st;
+  const JumpTable* jt;
+  const ZigTables* zt;
+  int method;
+  bool dry;
+  int64_t unit0;   // first unit of this particle
+  int64_t nunits;  // units consumed
+  int64_t pos0;    // stream position of the first draw
+  int64_t ncalls;  // generator calls since pos0
+  bool live;
+  int err;
+  Pcg g;
+
+  __device__ uint32_t next() {
+    ++ncalls;
+    return pcg_next(g);
+  }
+
+  __device__ int64_t zig_position(int64_t u) const {
+    // number of on-chain slow events with normal index < u, then extra positions
+    int64_t lo = 0, hi = st->n_ev;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (st->ev_key[mid] < u) lo = mid + 1;
+      else hi = mid;
+    }
+    return u + (lo > 0 ? st->ev_extra[lo - 1] : 0);
+  }
+
+  __device__ int64_t polar_position(int64_t q) const {
+    // first attempt of call q = q + #{rejected attempts j : r_j - j < q}
+    int64_t lo = 0, hi = st->n_ev;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (st->ev_key[mid] < q) lo = mid + 1;
+      else hi = mid;
+    }
+    return 2 * (q + lo);
+  }
+
+  __device__ void start() {
+    live = true;
+    if (method == 1) pos0 = unit0;
+    else if (method == 2) pos0 = zig_position(unit0);
+    else pos0 = polar_position(unit0);
+    if (method != 1 && pos0 >= st->window) err = E_RNG_WINDOW;
+    g.inc = st->inc;
+    g.state = pcg_jump(st->base, (uint64_t)pos0, *jt);
+    ncalls = 0;
+  }
+
+  __device__ double zig() {
+    // random_gaussian_ziggurat, random_utils.pyx:154-194
+    for (;;) {
+      int32_t hz = (int32_t)next();
+      uint32_t iz = (uint32_t)hz & 127u;
+      uint32_t ahz = hz < 0 ? (uint32_t)(-(int64_t)hz) : (uint32_t)hz;
+      if (ahz < zt->kn[iz]) return hz * zt->wn[iz];
+      if (iz == 0) {
+        for (;;) {
+          double x = -log(u32_to_uniform(next())) * 0.2904764;
+          double y = -log(u32_to_uniform(next()));
+          if (y + y >= x * x) return hz > 0 ? ZIG_R + x : -ZIG_R - x;
+        }
+      }
+      double x = hz * zt->wn[iz];
+      if (zt->fn[iz] + u32_to_uniform(next()) * (zt->fn[iz - 1] - zt->fn[iz]) < exp(-0.5 * x * x)) return x;
+    }
+  }
+
+  __device__ void polar(double& a, double& b) {
+    // random_gaussian_pair, random_utils.pyx:90-108
+    double x1 = 0., x2 = 0., w = 2.0;
+    while (w >= 1.0 || w == 0.0) {
+      x1 = 2.0 * u32_to_uniform(next()) - 1.0;
+      x2 = 2.0 * u32_to_uniform(next()) - 1.0;
+      w = x1 * x1 + x2 * x2;
+    }
+    w = sqrt((-2.0 * log(w)) / w);
+    a = x1 * w;
+    b = x2 * w;
+  }
+
+  // generate_random_pair, random_utils.pyx:202-233
+  __device__ void pair(double& a, double& b) {
+    if (dry) {
+      nunits += method == 3 ? 1 : 2;
+      a = b = 0.;
+      return;
+    }
+    if (!live) start();
+    if (method == 1) {
+      a = (u32_to_uniform(next()) - 0.5) * TWO_SQRT3;
+      b = (u32_to_uniform(next()) - 0.5) * TWO_SQRT3;
+      nunits += 2;
+    } else if (method == 2) {
+      a = zig();
+      b = zig();
+      nunits += 2;
+    } else {
+      polar(a, b);
+      nunits += 1;
+    }
+  }
+
+  // generate_random_single, random_utils.pyx:235-262
+  __device__ double single() {
+    if (dry) {
+      nunits += 1;
+      return 0.;
+    }
+    if (!live) start();
+    nunits += 1;
+    if (method == 1) return (u32_to_uniform(next()) - 0.5) * TWO_SQRT3;
+    if (method == 2) return zig();
+    double a, b;
+    polar(a, b);
+    return a;
+  }
+};
+
+// Boundary-condition draws (stranding, rough-wall normal perturbation) come
+// from a per-particle sub-stream: their serial position depends on the
+// particle's own diffusion draws, which a parallel step cannot know in advance.
+struct BcDraws {
+  Pcg g;
+  int method;
+  const ZigTables* zt;
+  __device__ double uniform() { return u32_to_uniform(pcg_next(g)); }
+  __device__ double normal() {
+    Draws d;
+    d.method = method;
+    d.zt = zt;
+    d.dry = false;
+    d.live = true;
+    d.g = g;
+    d.nunits = 0;
+    d.ncalls = 0;
+    double r = d.single();
+    g = d.g;
+    return r;
+  }
+};
+
+struct Ctx {
+  const MeshDev& m;
+  const FieldsDev& f;
+  const ParamsDev& par;
+  const ParticleConst& pc;
+  double dt;
+  int pdim;  // particle-set dimension
+  int err;
+};
+
+#define CYL_CHK_TRI(k)              \
+  if ((k) < 0) {                    \
+    c.err = E_NOT_IN_TRI;           \
+    return;                         \
+  }
+#define CYL_RET_ERR() \
+  if (c.err) return;
+
+// mean fluid velocity at a point, with pseudo-3D corrections (update_state_lsm1.pyx:149-164)
+__device__ inline void mean_velocity(Ctx& c, int k, int ll, double x, double y, double z, double& ux,
+                                     double& uy, double& uz) {
+  CYL_CHK_TRI(k);
+  if (c.f.dim == 2) {
+    vel2d(c.f, c.m, k, x, y, ux, uy);
+    uz = 0.;
+    if (c.pdim == 3) {
+      double w, hor;
+      pseudo3d(c.f, c.m, k, x, y, z, c.dt, ux, uy, w, hor);
+      uz += w;
+      ux *= hor;
+      uy *= hor;
+    }
+  } else {
+    vel3d(c.f, c.m, k, ll, x, y, z, ux, uy, uz, c.err);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LSM-1
+// ---------------------------------------------------------------------------
+// RK2/RK3/RK4 (update_state_lsm1.pyx:179-656); returns bnd_j of an early exit
+__device__ inline int rk_lsm1(Ctx& c, Part& p, int order) {
+  const int d = c.pdim;
+  double dt = c.dt, ux = 0., uy = 0., uz = 0.;
+  double kx[4] = {0., 0., 0., 0.}, ky[4] = {0., 0., 0., 0.}, kz[4] = {0., 0., 0., 0.};
+  mean_velocity(c, p.tri, d == 3 ? p.ll : 0, p.x[0], p.x[1], d == 3 ? p.x[2] : 0., ux, uy, uz);
+  if (c.err) return -1;
+  p.up[0] = ux; p.up[1] = uy; p.us[0] = ux; p.us[1] = uy;
+  if (d == 3) { p.up[2] = uz; p.us[2] = uz; }
+  for (int st = 0; st < order - 1; ++st) {
+    kx[st] = ux * dt; ky[st] = uy * dt; kz[st] = d == 3 ? uz * dt : 0.;
+    double xt, yt, zt = 0.;
+    if (order == 2) {
+      xt = p.x[0] + 0.5 * ux * dt;
+      yt = p.x[1] + 0.5 * uy * dt;
+      if (d == 3) zt = p.x[2] + 0.5 * uz * dt;
+    } else if (order == 3) {
+      double cc = st == 0 ? (1. / 3.) : (2. / 3.);
+      xt = p.x[0] + cc * kx[st];
+      yt = p.x[1] + cc * ky[st];
+      zt = d == 3 ? p.x[2] + cc * kz[st] : 0.;
+    } else {
+      double cc = st < 2 ? 0.5 : 1.;
+      if (st < 2) {
+        xt = p.x[0] + cc * kx[st];
+        yt = p.x[1] + cc * ky[st];
+        zt = d == 3 ? p.x[2] + cc * kz[st] : 0.;
+      } else {
+        xt = p.x[0] + kx[st];
+        yt = p.x[1] + ky[st];
+        zt = d == 3 ? p.x[2] + kz[st] : 0.;
+      }
+    }
+    p.last_tri = p.tri;
+    int bnd = -1, tl = 0;
+    int tt = xy_localize(c.m, p.last_tri, p.lx[0], p.lx[1], xt, yt, 3, bnd, c.err);
+    if (d == 3) tl = z_localize(c.f, c.m, tt, xt, yt, zt);
+    if (bnd != -1 || (d == 3 && (tl == -2 || tl == -3))) {
+      p.tri = tt;
+      if (d == 3) p.ll = tl;
+      p.x[0] += p.up[0] * dt;
+      p.x[1] += p.up[1] * dt;
+      if (d == 3) p.x[2] += p.up[2] * dt;
+      return bnd;
+    }
+    mean_velocity(c, tt, tl, xt, yt, zt, ux, uy, uz);
+    if (c.err) return -1;
+  }
+  if (order == 2) {
+    p.x[0] += ux * dt;
+    p.x[1] += uy * dt;
+    if (d == 3) p.x[2] += uz * dt;
+  } else if (order == 3) {
+    kx[2] = ux * dt; ky[2] = uy * dt; kz[2] = d == 3 ? uz * dt : 0.;
+    p.x[0] += (0.25 * kx[0] + 0.75 * kx[2]);
+    p.x[1] += (0.25 * ky[0] + 0.75 * ky[2]);
+    if (d == 3) p.x[2] += (0.25 * kz[0] + 0.75 * kz[2]);
+  } else {
+    kx[3] = ux * dt; ky[3] = uy * dt; kz[3] = d == 3 ? uz * dt : 0.;
+    p.x[0] += (kx[0] + 2. * kx[1] + 2. * kx[2] + kx[3]) / 6.;
+    p.x[1] += (ky[0] + 2. * ky[1] + 2. * ky[2] + ky[3]) / 6.;
+    if (d == 3) p.x[2] += (kz[0] + 2. * kz[1] + 2. * kz[2] + kz[3]) / 6.;
+  }
+  return -1;
+}
+
+// diffusion_lsm1 (update_state_lsm1.pyx:658-726) == diffusion_lsm2 (update_state_lsm2.pyx:1023-1086)
+__device__ inline void position_diffusion(Ctx& c, Part& p, Draws& dr, int tri0, int ll0) {
+  const int d = c.pdim;
+  int dm = c.par.diffusion_model;
+  double dt = c.dt, sqrtdt = sqrt(dt);
+  double xix = 0., xiy = 0., xiz = 0., diffx = 0., diffy = 0., diffz = 0.;
+  double driftx = 0., drifty = 0., driftz = 0.;
+  if (dm == 2 || dm == 3) {
+    CYL_CHK_TRI(tri0);
+    double Kp = 0., dkdx = 0., dkdy = 0., dkdz = 0.;
+    diffusivity_gradient(c.f, c.m, tri0, ll0, p.lx[0], p.lx[1], d == 3 ? p.lx[2] : 0., c.par.schmidt_number,
+                         Kp, dkdx, dkdy, dkdz, c.err);
+    CYL_RET_ERR();
+    diffx = sqrt(2. * Kp) * sqrtdt;
+    diffy = diffx;
+    driftx = dkdx * dt;
+    drifty = dkdy * dt;
+    if (d == 3) {
+      if (dm == 2) diffz = sqrt(2. * c.par.vertical_diffusivity) * sqrtdt;
+      else {
+        diffz = diffx;
+        driftz = dkdz * dt;
+      }
+    }
+  } else {
+    diffx = sqrt(2. * c.par.horizontal_diffusivity) * sqrtdt;
+    diffy = diffx;
+    if (d == 3) diffz = sqrt(2. * c.par.vertical_diffusivity) * sqrtdt;
+  }
+  if (diffx != 0.) dr.pair(xix, xiy);
+  if (d == 3 && diffz != 0.) xiz = dr.single();
+  p.x[0] += driftx + diffx * xix;
+  p.x[1] += drifty + diffy * xiy;
+  if (d == 3) p.x[2] += driftz + diffz * xiz;
+}
+
+// buoyancy_velocity, lagrangian_particle_set.pyx:834-890 (model_hr = 0)
+__device__ inline double buoyancy_velocity(const ParticleConst& pc, double dp) {
+  if (pc.buoyancy_velocity_model == 0) return pc.buoyancy_velocity_value;
+  double s = pc.density / pc.rho0, wb;
+  if (dp < 1.e-4) wb = fabs(s - 1.) * (GRAV * (dp * dp)) / (18. * NU0);
+  else if (dp > 1.e-3) wb = sqrt(fabs(s - 1.) * (4. * dp * GRAV) / (3. * pc.drag_coef));
+  else wb = (10. * NU0 / dp) * (sqrt(1. + fabs(s - 1.) * GRAV * pow(dp, 3.0) / (100. * (NU0 * NU0))) - 1.);
+  if (s > 1.) wb *= -1.;
+  return wb;
+}
+
+// update_state_lsm1, update_state_lsm1.pyx:30-133
+__device__ inline int update_lsm1(Ctx& c, Part& p, Draws& dr) {
+  const int d = c.pdim;
+  int bnd_j = -1, tri0 = p.tri, ll0 = d == 3 ? p.ll : 0;
+  for (int q = 0; q < d; ++q) { p.lx[q] = p.x[q]; p.lup[q] = p.up[q]; }
+  int ts = c.par.time_scheme;
+  if (ts == 1) {
+    double ux = 0., uy = 0., uz = 0.;
+    mean_velocity(c, p.tri, d == 3 ? p.ll : 0, p.x[0], p.x[1], d == 3 ? p.x[2] : 0., ux, uy, uz);
+    if (c.err) return -1;
+    p.x[0] += ux * c.dt;
+    p.x[1] += uy * c.dt;
+    p.up[0] = ux; p.up[1] = uy; p.us[0] = ux; p.us[1] = uy;
+    if (d == 3) { p.x[2] += uz * c.dt; p.up[2] = uz; p.us[2] = uz; }
+  } else if (ts >= 2 && ts <= 4) {
+    bnd_j = rk_lsm1(c, p, ts);
+    if (c.err) return -1;
+  } else {
+    c.err = E_SCHEME;
+    return -1;
+  }
+  if (c.par.diffusion_model != 0) {
+    position_diffusion(c, p, dr, tri0, ll0);
+    if (c.err) return -1;
+  }
+  if (d == 3) {
+    double wb = buoyancy_velocity(c.pc, p.diameter);
+    p.x[2] += wb * c.dt;
+    p.up[2] += wb;
+  }
+  if (bnd_j == -1) {
+    p.last_tri = p.tri;
+    p.tri = xy_localize(c.m, p.last_tri, p.lx[0], p.lx[1], p.x[0], p.x[1], 3, bnd_j, c.err);
+    if (d == 3) p.ll = z_localize(c.f, c.m, p.tri, p.x[0], p.x[1], p.x[2]);
+  }
+  return bnd_j;
+}
+
+// update_state_test, update_state_test.pyx:28-87
+__device__ inline int update_test(Ctx& c, Part& p) {
+  const int d = c.pdim;
+  int bnd_j = -1;
+  for (int q = 0; q < d; ++q) { p.lx[q] = p.x[q]; p.lup[q] = p.up[q]; }
+  for (int q = 0; q < d; ++q) p.x[q] += p.up[q] * c.dt;
+  p.last_tri = p.tri;
+  p.tri = xy_localize(c.m, p.last_tri, p.lx[0], p.lx[1], p.x[0], p.x[1], 2, bnd_j, c.err);
+  if (d == 3) p.ll = z_localize(c.f, c.m, p.tri, p.x[0], p.x[1], p.x[2]);
+  return bnd_j;
+}
+
+// ---------------------------------------------------------------------------
+// LSM-2
+// ---------------------------------------------------------------------------
+// drag_coefficient, lagrangian_particle_set.pyx:790-832
+__device__ inline double drag_coefficient(const ParticleConst& pc, double U, double dp) {
+  double Re = fmax(EPS, U * dp / NU0);
+  if (pc.drag_coef_model == 0) return pc.drag_coef;
+  if (pc.drag_coef_model == 1) return Re <= 1000. ? (24. / Re) * (1. + 0.15 * pow(Re, 0.687)) : 0.44;
+  if (pc.drag_coef_model == 2) {
+    double phi1 = pow(24. / Re, 10.) + pow(21. * pow(Re, -0.67), 10.) + pow(4. * pow(Re, -0.33), 10.) + pow(0.4, 10.);
+    double phi2 = 1. / (pow(0.148 * pow(Re, 0.11), -10.) + pow(0.5, -10.));
+    double phi3 = pow(1.57 * pow(Re, -1.625) * 1.e8, 10.);
+    double phi4 = 1. / (pow(6e-17 * pow(Re, 2.63), -10.) + pow(0.2, -10.));
+    return pow(1. / (pow(phi1 + phi2, -1.) + pow(phi3, -1.)) + phi4, 1 / 10.);
+  }
+  return 0.44;
+}
+
+__device__ inline double relax_rate(const Ctx& c, const Part& p, double unorm) {
+  unorm = fmax(EPS_LSM2, unorm);
+  double Cd = drag_coefficient(c.pc, unorm, p.diameter);
+  return c.pc.a1c * (1.5 / p.diameter) * Cd * unorm;
+}
+
+__device__ inline double rel_norm(int d, double ux, double uy, double uz, double a, double b, double e) {
+  double dx = ux - a, dy = uy - b;
+  if (d == 2) return sqrt(dx * dx + dy * dy);
+  double dz = uz - e;
+  return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ inline void accel_at(Ctx& c, int k, int ll, double x, double y, double z, double ux, double uy,
+                                double uz, double& ax, double& ay, double& az) {
+  ax = ay = az = 0.;
+  if (!c.pc.added_mass_force) return;
+  if (c.f.dim == 2) accel2d(c.f, c.m, k, x, y, c.dt, ux, uy, ax, ay);
+  else accel3d(c.f, c.m, k, ll, x, y, z, c.dt, ux, uy, uz, ax, ay, az, c.err);
+}
+
+// euler_lsm2 (update_state_lsm2.pyx:141-217) and rk2/rk3/rk4_lsm2 (:219-1021)
+__device__ inline int rk_lsm2(Ctx& c, Part& p, int order) {
+  const int d = c.pdim;
+  double dt = c.dt, a2c = c.pc.a2c, a0c = c.pc.a0c;
+  double ux = 0., uy = 0., uz = 0., ax, ay, az, a1 = 0.;
+  double kx[4], ky[4], kz[4], fx[4], fy[4], fz[4];
+  int tri = p.tri, ll = d == 3 ? p.ll : 0;
+  double xt = p.x[0], yt = p.x[1], zt = d == 3 ? p.x[2] : 0.;
+  for (int st = 0; st < order; ++st) {
+    mean_velocity(c, tri, ll, xt, yt, zt, ux, uy, uz);
+    if (c.err) return -1;
+    accel_at(c, tri, ll, xt, yt, zt, ux, uy, uz, ax, ay, az);
+    if (c.err) return -1;
+    if (st == 0) {
+      a1 = relax_rate(c, p, rel_norm(d, ux, uy, uz, p.up[0], p.up[1], p.up[2]));
+      kx[0] = p.up[0] * dt;
+      ky[0] = p.up[1] * dt;
+      fx[0] = (ux - p.up[0]) * dt * a1 + ax * a2c * dt;
+      fy[0] = (uy - p.up[1]) * dt * a1 + ay * a2c * dt;
+      if (d == 3) {
+        kz[0] = p.up[2] * dt;
+        fz[0] = (uz - p.up[2]) * dt * a1 + az * a2c * dt - GRAV * a0c * dt;
+      }
+    } else {
+      // velocity fraction applied to the previous stage's increment
+      double cv = order == 3 ? (st == 1 ? (1. / 3.) : (2. / 3.)) : (order == 4 && st == 3 ? 1. : 0.5);
+      double ex = p.up[0] + cv * fx[st - 1], ey = p.up[1] + cv * fy[st - 1], ez = 0.;
+      double rx = ux - p.up[0] - cv * fx[st - 1], ry = uy - p.up[1] - cv * fy[st - 1];
+      double un;
+      if (d == 2) un = sqrt(rx * rx + ry * ry);
+      else {
+        double rz = uz - p.up[2] - cv * fz[st - 1];
+        un = sqrt(rx * rx + ry * ry + rz * rz);
+        ez = p.up[2] + cv * fz[st - 1];
+      }
+      a1 = relax_rate(c, p, un);
+      kx[st] = ex * dt;
+      ky[st] = ey * dt;
+      fx[st] = (ux - ex) * dt * a1 + ax * a2c * dt;
+      fy[st] = (uy - ey) * dt * a1 + ay * a2c * dt;
+      if (d == 3) {
+        kz[st] = ez * dt;
+        fz[st] = (uz - ez) * dt * a1 + az * a2c * dt - GRAV * a0c * dt;
+      }
+    }
+    if (st == order - 1) break;
+    double cp = order == 3 ? (st == 0 ? (1. / 3.) : (2. / 3.)) : (order == 4 && st == 2 ? 1. : 0.5);
+    if (order == 4 && st == 2) {
+      xt = p.x[0] + kx[st];
+      yt = p.x[1] + ky[st];
+      zt = d == 3 ? p.x[2] + kz[st] : 0.;
+    } else {
+      xt = p.x[0] + cp * kx[st];
+      yt = p.x[1] + cp * ky[st];
+      zt = d == 3 ? p.x[2] + cp * kz[st] : 0.;
+    }
+    p.last_tri = p.tri;
+    int bnd = -1;
+    tri = xy_localize(c.m, p.last_tri, p.lx[0], p.lx[1], xt, yt, 3, bnd, c.err);
+    if (d == 3) ll = z_localize(c.f, c.m, tri, xt, yt, zt);
+    if (bnd != -1 || (d == 3 && (ll == -2 || ll == -3))) {
+      p.tri = tri;
+      if (d == 3) p.ll = ll;
+      p.us[0] = ux;
+      p.us[1] = uy;
+      p.up[0] += (ux - p.up[0]) * dt * a1 + ax * a2c * dt;
+      p.up[1] += (uy - p.up[1]) * dt * a1 + ay * a2c * dt;
+      if (d == 3) {
+        p.us[2] = uz;
+        p.up[2] += (uz - p.up[2]) * dt * a1 + az * a2c * dt - GRAV * a0c * dt;
+      }
+      p.x[0] += p.up[0] * dt;
+      p.x[1] += p.up[1] * dt;
+      if (d == 3) p.x[2] += p.up[2] * dt;
+      return bnd;
+    }
+  }
+  p.us[0] = ux;
+  p.us[1] = uy;
+  if (d == 3) p.us[2] = uz;
+  if (order == 1) {
+    p.up[0] += fx[0];
+    p.up[1] += fy[0];
+    p.x[0] += p.up[0] * dt;
+    p.x[1] += p.up[1] * dt;
+    if (d == 3) {
+      p.up[2] += fz[0];
+      p.x[2] += p.up[2] * dt;
+    }
+  } else if (order == 2) {
+    p.up[0] += fx[1];
+    p.up[1] += fy[1];
+    p.x[0] += kx[1];
+    p.x[1] += ky[1];
+    if (d == 3) { p.up[2] += fz[1]; p.x[2] += kz[1]; }
+  } else if (order == 3) {
+    p.up[0] += (0.25 * fx[0] + 0.75 * fx[2]);
+    p.up[1] += (0.25 * fy[0] + 0.75 * fy[2]);
+    p.x[0] += (0.25 * kx[0] + 0.75 * kx[2]);
+    p.x[1] += (0.25 * ky[0] + 0.75 * ky[2]);
+    if (d == 3) {
+      p.up[2] += (0.25 * fz[0] + 0.75 * fz[2]);
+      p.x[2] += (0.25 * kz[0] + 0.75 * kz[2]);
+    }
+  } else {
+    p.up[0] += (1. / 6.) * (fx[0] + 2. * fx[1] + 2. * fx[2] + fx[3]);
+    p.up[1] += (1. / 6.) * (fy[0] + 2. * fy[1] + 2. * fy[2] + fy[3]);
+    p.x[0] += (1. / 6.) * (kx[0] + 2. * kx[1] + 2. * kx[2] + kx[3]);
+    p.x[1] += (1. / 6.) * (ky[0] + 2. * ky[1] + 2. * ky[2] + ky[3]);
+    if (d == 3) {
+      p.up[2] += (1. / 6.) * (fz[0] + 2. * fz[1] + 2. * fz[2] + fz[3]);
+      p.x[2] += (1. / 6.) * (kz[0] + 2. * kz[1] + 2. * kz[2] + kz[3]);
+    }
+  }
+  return -1;
+}
+
+// turbulent energy and dissipation at the particle (update_state_lsm2.pyx:1164-1176)
+__device__ inline void keps_at(Ctx& c, const Part& p, double x, double y, double z, double& k, double& e) {
+  if (c.f.dim == 2) {
+    k = field2d(c.f, c.m, c.f.fid_turbeng, p.tri, -1, x, y);
+    e = field2d(c.f, c.m, c.f.fid_dissip, p.tri, -1, x, y);
+  } else {
+    k = field3d(c.f, c.m, c.f.fid_turbeng, p.tri, p.ll, x, y, z, c.err);
+    e = field3d(c.f, c.m, c.f.fid_dissip, p.tri, p.ll, x, y, z, c.err);
+  }
+}
+
+// lsm2_direct_coefficients, update_state_lsm2.pyx:1240-1327
+__device__ inline bool lsm2_coefficients(double a1, double dt, double& beta, double& D1, double& D2, double& fxc,
+                                         double& quu, double& qxu, double& qxx) {
+  double r = a1 * dt;
+  if (!isfinite(a1) || !isfinite(dt) || a1 < 0. || dt < 0. || !isfinite(r)) return false;
+  beta = exp(-r);
+  if (r < 0.1) {
+    double phi1 = 1., phi2 = 0.5, term = 1.;
+    for (int n = 1; n < 13; ++n) {
+      term *= -r / (n + 1.);
+      phi1 += term;
+      phi2 += term / (n + 2.);
+    }
+    D1 = dt * phi1;
+    D2 = dt * r * phi2;
+    fxc = dt * dt * phi2;
+  } else {
+    D1 = -expm1(-r) / a1;
+    D2 = dt - D1;
+    fxc = D2 / a1;
+  }
+  quu = (r == 0.) ? dt : dt * (-expm1(-2. * r) / (2. * r));
+  qxu = 0.5 * D1 * D1;
+  if (r < 1.) {
+    double term = 1. / 6., power2 = 4., q = 1. / 3.;
+    for (int n = 1; n < 24; ++n) {
+      term *= -r / (n + 3.);
+      power2 *= 2.;
+      q += (power2 - 2.) * term;
+    }
+    q *= dt * dt * dt;
+    qxx = q;
+  } else {
+    qxx = ((dt - 2. * D1 + quu) / a1) / a1;
+  }
+  return true;
+}
+
+// di_lsm2, update_state_lsm2.pyx:1329-1543
+__device__ inline void di_lsm2(Ctx& c, Part& p, Draws& dr, int dm, double nux, double nuz, int nuopt) {
+  const int d = c.pdim;
+  double dt = c.dt, x0[3] = {p.x[0], p.x[1], p.x[2]}, u0[3] = {p.up[0], p.up[1], p.up[2]};
+  double ux = 0., uy = 0., uz = 0., ax, ay, az;
+  int ll = d == 3 ? p.ll : 0;
+  double z0 = d == 3 ? x0[2] : 0.;
+  mean_velocity(c, p.tri, ll, x0[0], x0[1], z0, ux, uy, uz);
+  CYL_RET_ERR();
+  accel_at(c, p.tri, ll, x0[0], x0[1], z0, ux, uy, uz, ax, ay, az);
+  CYL_RET_ERR();
+  double a1 = relax_rate(c, p, rel_norm(d, ux, uy, uz, u0[0], u0[1], u0[2]));
+  double beta, D1, D2, fxc, quu, qxu, qxx;
+  if (!lsm2_coefficients(a1, dt, beta, D1, D2, fxc, quu, qxu, qxx)) {
+    c.err = E_LSM2_COEF;
+    return;
+  }
+  double um_beta = -expm1(-a1 * dt);
+  double b2[3] = {0., 0., 0.};
+  if (dm == 2 || dm == 3) {
+    double tk, ep;
+    keps_at(c, p, x0[0], x0[1], z0, tk, ep);
+    CYL_RET_ERR();
+    ep = fmax(ep, EPS);
+    double tml = (tk / ep) / (0.5 + (3. / 4.) * C0KOLM);
+    b2[0] = a1 * a1 * C0KOLM * ep * (tml * tml);
+    b2[1] = b2[0];
+    if (dm == 2) b2[2] = nuopt == 2 ? nuz * nuz : 2. * nuz * a1 * a1;
+    else b2[2] = b2[0];
+  } else if (dm != 0) {
+    if (nuopt == 2) { b2[0] = nux * nux; b2[1] = b2[0]; b2[2] = nuz * nuz; }
+    else if (nuopt == 3) { b2[0] = 2. * nux * a1 * a1; b2[1] = b2[0]; b2[2] = 2. * nuz * a1 * a1; }
+  }
+  for (int k = 0; k < d; ++k) {
+    double uk = k == 0 ? ux : (k == 1 ? uy : uz);
+    double Fp = k == 0 ? c.pc.a2c * ax : (k == 1 ? c.pc.a2c * ay : c.pc.a2c * az - GRAV * c.pc.a0c);
+    double upm = u0[k] * beta + uk * um_beta + Fp * D1;
+    double xpm = x0[k] + u0[k] * D1 + uk * D2 + Fp * fxc;
+    if (b2[k] == 0. || dt == 0.) {
+      p.us[k] = uk;
+      p.up[k] = upm;
+      p.x[k] = xpm;
+      continue;
+    }
+    double vgp = b2[k] * quu, cov = b2[k] * qxu, vgx = b2[k] * qxx;
+    double Lpp = sqrt(fmax(0., vgp));
+    double Lxp = Lpp > 0. ? cov / Lpp : 0.;
+    double Lxx = sqrt(fmax(0., vgx - Lxp * Lxp));
+    double xix, xiy;
+    dr.pair(xix, xiy);
+    p.us[k] = uk;
+    p.up[k] = upm + Lpp * xix;
+    p.x[k] = xpm + (Lxp * xix + Lxx * xiy);
+  }
+}
+
+// euler_maruyama_lsm2_u, update_state_lsm2.pyx:1088-1230
+__device__ inline void em_lsm2_u(Ctx& c, Part& p, Draws& dr) {
+  const int d = c.pdim;
+  int dm = c.par.diffusion_model, nuopt = c.par.diffusion_lsm2_option;
+  double dt = c.dt, sqrtdt = sqrt(dt), ux = 0., uy = 0., uz = 0., ax, ay, az;
+  double nux = c.par.horizontal_diffusivity, nuz = c.par.vertical_diffusivity;
+  int ll = d == 3 ? p.ll : 0;
+  double z = d == 3 ? p.x[2] : 0.;
+  mean_velocity(c, p.tri, ll, p.x[0], p.x[1], z, ux, uy, uz);
+  CYL_RET_ERR();
+  accel_at(c, p.tri, ll, p.x[0], p.x[1], z, ux, uy, uz, ax, ay, az);
+  CYL_RET_ERR();
+  double a1 = relax_rate(c, p, rel_norm(d, ux, uy, uz, p.up[0], p.up[1], p.up[2]));
+  double xix = 0., xiy = 0., xiz = 0., diffx = 0., diffy = 0., diffz = 0.;
+  if (dm != 0) {
+    dr.pair(xix, xiy);
+    if (d == 3) xiz = dr.single();
+  }
+  if (dm == 2 || dm == 3) {
+    double tk, ep;
+    keps_at(c, p, p.x[0], p.x[1], z, tk, ep);
+    CYL_RET_ERR();
+    ep = ep > EPS ? ep : EPS;
+    double tml = (tk / ep) / (0.5 + (3. / 4.) * C0KOLM);
+    diffx = a1 * tml * sqrt(C0KOLM * ep) * sqrtdt;
+    diffy = diffx;
+    if (d == 3) diffz = dm == 2 ? (nuopt == 2 ? nuz * sqrtdt : a1 * sqrt(2. * nuz) * sqrtdt) : diffx;
+  } else if (nuopt == 2) {
+    diffx = nux * sqrtdt; diffy = diffx;
+    if (d == 3) diffz = nuz * sqrtdt;
+  } else if (nuopt == 3) {
+    diffx = a1 * sqrt(2. * nux) * sqrtdt; diffy = diffx;
+    if (d == 3) diffz = a1 * sqrt(2. * nuz) * sqrtdt;
+  }
+  double a2c = c.pc.a2c, a0c = c.pc.a0c;
+  p.us[0] = ux;
+  p.us[1] = uy;
+  p.up[0] += (ux - p.up[0]) * dt * a1 + ax * a2c * dt + diffx * xix;
+  p.up[1] += (uy - p.up[1]) * dt * a1 + ay * a2c * dt + diffy * xiy;
+  p.x[0] += p.up[0] * dt;
+  p.x[1] += p.up[1] * dt;
+  if (d == 3) {
+    p.us[2] = uz;
+    p.up[2] += (uz - p.up[2]) * dt * a1 + az * a2c * dt - GRAV * a0c * dt + diffz * xiz;
+    p.x[2] += p.up[2] * dt;
+  }
+}
+
+// update_state_lsm2, update_state_lsm2.pyx:30-139
+__device__ inline int update_lsm2(Ctx& c, Part& p, Draws& dr) {
+  const int d = c.pdim;
+  int bnd_j = -1, tri0 = p.tri, ll0 = d == 3 ? p.ll : 0;
+  int ts = c.par.time_scheme, dm = c.par.diffusion_model, nuopt = c.par.diffusion_lsm2_option;
+  for (int q = 0; q < d; ++q) { p.lx[q] = p.x[q]; p.lup[q] = p.up[q]; }
+  if (nuopt == 1 || (nuopt >= 2 && dm == 0)) {
+    if (ts >= 1 && ts <= 4) bnd_j = rk_lsm2(c, p, ts);
+    else if (ts == 5) di_lsm2(c, p, dr, 0, 0., 0., 2);
+    else c.err = E_SCHEME;
+    if (c.err) return -1;
+    if (dm != 0) position_diffusion(c, p, dr, tri0, ll0);
+  } else if (nuopt >= 2 && dm != 0) {
+    if (ts == 1) em_lsm2_u(c, p, dr);
+    else if (ts == 5) di_lsm2(c, p, dr, dm, c.par.horizontal_diffusivity, c.par.vertical_diffusivity, nuopt);
+    else c.err = E_SCHEME;
+  } else c.err = E_SCHEME;
+  if (c.err) return -1;
+  p.last_tri = p.tri;
+  p.tri = xy_localize(c.m, p.last_tri, p.lx[0], p.lx[1], p.x[0], p.x[1], 3, bnd_j, c.err);
+  if (d == 3) p.ll = z_localize(c.f, c.m, p.tri, p.x[0], p.x[1], p.x[2]);
+  return bnd_j;
+}
+
+// update_lsm2_fluctuations, init_lsm2_fluctuations.pyx:28-129
+__device__ inline void lsm2_fluctuations(Ctx& c, Part& p, Draws& dr, double& f1, double& f2, double& f3) {
+  f1 = f2 = f3 = 0.;
+  int dm = c.par.diffusion_model, nuopt = c.par.diffusion_lsm2_option;
+  if (dm == 0) return;
+  double unorm = EPS_LSM2;
+  double Cd = drag_coefficient(c.pc, unorm, p.diameter);
+  double a1 = c.pc.a1c * (1.5 / p.diameter) * Cd * unorm;
+  double nux = c.par.horizontal_diffusivity, nuz = c.par.vertical_diffusivity;
+  if (a1 == 0.) {
+    if (nuopt == 2 && (((dm != 2 && dm != 3) && nux != 0.) || (c.pdim == 3 && dm != 3 && nuz != 0.)))
+      c.err = E_LSM2_COEF;
+    return;
+  }
+  double b0, b1, bz;
+  if (dm == 2 || dm == 3) {
+    double tk, ep;
+    keps_at(c, p, p.x[0], p.x[1], c.pdim == 3 ? p.x[2] : 0., tk, ep);
+    CYL_RET_ERR();
+    ep = fmax(ep, EPS);
+    double tml = (tk / ep) / (0.5 + (3. / 4.) * C0KOLM);
+    b0 = a1 * C0KOLM * ep * (tml * tml);
+    b1 = b0;
+    if (dm == 2) bz = nuopt == 2 ? (nuz * nuz) / a1 : 2. * nuz * a1;
+    else bz = b0;
+  } else if (nuopt == 2) {
+    b0 = (nux * nux) / a1; b1 = b0; bz = (nuz * nuz) / a1;
+  } else {
+    b0 = 2. * nux * a1; b1 = b0; bz = 2. * nuz * a1;
+  }
+  if (b0 > 0.) {
+    double xix, xiy;
+    dr.pair(xix, xiy);
+    f1 = sqrt(0.5 * b0) * xix;
+    f2 = sqrt(0.5 * b1) * xiy;
+  }
+  if (c.pdim == 3 && bz > 0.) f3 = sqrt(0.5 * bz) * dr.single();
+}
+
+// ---------------------------------------------------------------------------
+// Arrival: initialize_particle_state, solver.pyx:238-380.  `tri_found` is the
+// element found on the host (trifinder) or -3 to search on the device.
+// ---------------------------------------------------------------------------
+__device__ inline void init_particle(Ctx& c, Part& p, Draws& dr, int tri_found) {
+  const int d = c.pdim;
+  int bj;
+  p.tri = tri_found != -3 ? tri_found : xy_localize(c.m, -1, 0., 0., p.x[0], p.x[1], 0, bj, c.err);
+  if (p.tri == -1) delete_part(p, d);
+  if (d == 3 && !p.inactive) {
+    double zs = field2d(c.f, c.m, c.f.fid_zs, p.tri, c.f.nlayers - 1, p.x[0], p.x[1]);
+    double zb = c.f.dim == 2 ? field2d(c.f, c.m, c.f.fid_zb, p.tri, 0, p.x[0], p.x[1])
+                             : field2d(c.f, c.m, c.f.fid_zs, p.tri, 0, p.x[0], p.x[1]);
+    p.ll = lower_layer_index(c.f.nlayers, zs, zb, p.x[2]);
+    if (p.x[2] < zb) p.x[2] = zb + EPS;
+    if (p.x[2] > zs) p.x[2] = zs - EPS;
+  }
+  if (p.inactive) return;
+  if (c.par.particle_velocity_init == 1) {
+    double ux, uy, uz = 0.;
+    if (c.f.dim == 2) {
+      vel2d(c.f, c.m, p.tri, p.x[0], p.x[1], ux, uy);
+    } else {
+      vel3d(c.f, c.m, p.tri, p.ll, p.x[0], p.x[1], p.x[2], ux, uy, uz, c.err);
+      CYL_RET_ERR();
+    }
+    p.up[0] = ux; p.up[1] = uy; p.us[0] = ux; p.us[1] = uy;
+    if (d == 3 && c.f.dim == 3) { p.up[2] = uz; p.us[2] = uz; }
+    if (c.par.model == 2 && c.par.diffusion_lsm2_option >= 2 && c.par.diffusion_model != 0) {
+      double f1, f2, f3;
+      lsm2_fluctuations(c, p, dr, f1, f2, f3);
+      CYL_RET_ERR();
+      p.up[0] += f1;
+      p.up[1] += f2;
+      if (d == 3) p.up[2] += f3;
+    }
+    for (int q = 0; q < d; ++q) p.us[q] = p.up[q];
+    if (c.par.model == 3) c.err = E_LSM3;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Boundary conditions, compute_bc.pyx:35-395
+// ---------------------------------------------------------------------------
+__device__ inline void reflect_velocity(double* v, const double nn[2], double damp, int d, bool damped) {
+  double up0 = v[0];
+  v[0] = up0 * nn[0] + v[1] * nn[1];
+  v[1] = -up0 * nn[1] + v[1] * nn[0];
+  v[0] = -1. * v[0];
+  up0 = v[0];
+  v[0] = up0 * nn[0] - v[1] * nn[1];
+  v[1] = up0 * nn[1] + v[1] * nn[0];
+  if (damped) {
+    v[0] *= 1. - damp;
+    v[1] *= 1. - damp;
+    if (d == 3) v[2] *= 1. - damp;
+  }
+}
+
+__device__ inline void rough_normal(const Ctx& c, BcDraws& bd, double nn[2]) {
+  double rough = c.par.wall_roughness_angle * DEG_TO_RAD, max_theta = 85.0 * DEG_TO_RAD;
+  if (c.par.boundary_conditions_type == 2 && rough > 0.) {
+    double th = bd.normal() * rough;
+    th = fmax(-max_theta, fmin(max_theta, th));
+    double ct = cos(th), st = sin(th);
+    double a = nn[0] * ct - nn[1] * st, b = nn[0] * st + nn[1] * ct;
+    nn[0] = a;
+    nn[1] = b;
+  }
+}
+
+__device__ inline void compute_bc(Ctx& c, Part& p, int bnd_j, BcDraws& bd, int* bnd_stats, int nopen) {
+  const int d = c.pdim;
+  const MeshDev& m = c.m;
+  double damp = fmin(c.par.rebound_damping_coef, 1.0 - EPS_REBOUND);
+  double strand = c.par.stranding_probability;
+  int strand_cond = 0;
+  if (bnd_j != -1) {
+    int lab = m.edge_labels[bnd_j];
+    if (lab == 0 && strand > 0.) strand_cond = bd.uniform() < strand;
+    if (lab == 0 && !strand_cond) {
+      int bjj = bnd_j, rc = 0;
+      while (p.tri == PART_LOC_O && rc < c.par.max_wall_rebounds) {
+        int i0 = m.edges[2 * bjj], i1 = m.edges[2 * bjj + 1];
+        double v0x = m.x[i0], v0y = m.y[i0], v1x = m.x[i1], v1y = m.y[i1];
+        // compute_symetric_point2d, intersections.pyx:27-62
+        double n0 = v0y - v1y, n1 = v1x - v0x;
+        double norm2 = n0 * n0 + n1 * n1;
+        double dist = (p.x[0] - v0x) * n0 + (p.x[1] - v0y) * n1;
+        double coef = 1. - damp / 2.;
+        double px = p.x[0] - 2. * ((dist * coef) / norm2) * n0;
+        double py = p.x[1] - 2. * ((dist * coef) / norm2) * n1;
+        p.x[0] = px;
+        p.x[1] = py;
+        p.tri = xy_localize(m, p.last_tri, p.lx[0], p.lx[1], px, py, 3, bjj, c.err);
+        ++p.nrebound;
+        if (p.tri == PART_LOC_O && bjj != -1 && m.edge_labels[bjj] >= 1) {
+          if (c.par.bnd_statistics && bnd_stats && m.edge_labels[bjj] - 1 < nopen)
+            atomicAdd(&bnd_stats[m.edge_labels[bjj] - 1], 1);
+          delete_part(p, d);
+          break;
+        }
+        int model = c.par.model;
+        if (model == 0 || model >= 2) {
+          // normal2u, utils.pyx:99-119 (x**2 as x*x)
+          double ex = v1x - v0x, ey = v1y - v0y;
+          double nrm = sqrt(ex * ex + ey * ey);
+          double nn[2] = {(v0y - v1y) / nrm, (v1x - v0x) / nrm};
+          double nv[2] = {nn[0], nn[1]};
+          rough_normal(c, bd, nv);
+          reflect_velocity(p.up, nv, damp, d, true);
+          if (model == 3) {
+            double ns[2] = {nn[0], nn[1]};
+            rough_normal(c, bd, ns);
+            reflect_velocity(p.us, ns, damp, d, false);
+          }
+        }
+        ++rc;
+        if (p.tri == PART_LOC_O && bjj == -1) break;
+      }
+      if (d == 3 && p.tri != PART_LOC_O && !p.inactive) p.ll = z_localize(c.f, m, p.tri, p.x[0], p.x[1], p.x[2]);
+    }
+    if (lab >= 1) {
+      delete_part(p, d);
+      if (c.par.bnd_statistics && bnd_stats && lab - 1 < nopen) atomicAdd(&bnd_stats[lab - 1], 1);
+    }
+  }
+  if (d == 3) {
+    for (int pass = 0; pass < 2; ++pass) {
+      bool below = pass == 0;
+      int want = below ? PART_LOC_B : PART_LOC_A;
+      if (!(p.ll == want && p.tri != PART_LOC_O) || p.inactive) continue;
+      double zbp, zsp;
+      if (c.f.dim == 2) {
+        zbp = field2d(c.f, m, c.f.fid_zb, p.tri, 0, p.x[0], p.x[1]);
+        zsp = field2d(c.f, m, c.f.fid_zs, p.tri, 0, p.x[0], p.x[1]);
+      } else {
+        zbp = field2d(c.f, m, c.f.fid_zs, p.tri, 0, p.x[0], p.x[1]);
+        zsp = field2d(c.f, m, c.f.fid_zs, p.tri, c.f.nlayers - 1, p.x[0], p.x[1]);
+      }
+      double hp = fmax(0., zsp - zbp);
+      if (below) {
+        double dz = fabs(zbp - p.x[2]);
+        p.x[2] = fmin(zbp + (1. - damp) * dz, zbp + 0.5 * hp);
+        p.ll = lower_layer_index(c.f.nlayers, zsp, zbp, p.x[2]);
+        if (c.par.model >= 2) p.up[2] = fabs(p.up[2]) * (1. - damp);
+        if (c.par.model == 3) p.us[2] = fabs(p.us[2]) * (1. - damp);
+      } else {
+        double dz = fabs(p.x[2] - zsp);
+        p.x[2] = fmax(zsp - (1. - damp) * dz, zsp - 0.5 * hp);
+        p.ll = lower_layer_index(c.f.nlayers, zsp, zbp, p.x[2]);
+        if (c.par.model >= 2) p.up[2] = -fabs(p.up[2]) * (1. - damp);
+        if (c.par.model == 3) p.us[2] = -fabs(p.us[2]) * (1. - damp);
+      }
+    }
+  }
+  if (!p.inactive) {
+    if (p.tri == PART_LOC_O) delete_part(p, d);
+    if (p.ll == PART_LOC_B) delete_part(p, d);
+    if (p.ll == PART_LOC_A) delete_part(p, d);
+  }
+}
+
+}  // namespace cyl

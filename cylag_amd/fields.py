@@ -1,0 +1,168 @@
+"""Eulerian field set: mesh, time records and node fields
+(cylag/core/eulerian_field_set.pyx:125-315, 606-622, 1770-1885).
+
+Fields are (ntimes, npoints*nlayers), layer-major; optional fields are
+(nopt, ntimes, npoints*nlayers) and resolved by name.  The step uploads every
+record to the GPU once and time-interpolates on the device.
+"""
+import numpy as np
+
+from .constants import EPSILON
+
+_TURB = ("TURBULENT ENERG.", "TURBULENT ENERGY", "ENERGIE TURBUL.")
+
+
+class EulerianFieldSet:
+    def __init__(self, triangular_mesh, dim=2, nlayers=1, times=None, elevation=None,
+                 velocity_x=None, velocity_y=None, velocity_z=None, opt_fields=None,
+                 opt_fields_names=None, trifinder=None, friction_model=1, friction_coefficient=40.):
+        self.triangular_mesh = triangular_mesh
+        self.npoints = triangular_mesh.nv
+        self.nlayers = nlayers
+        self.dim = dim
+        self.trifinder = trifinder
+        self.times = np.ascontiguousarray(times, dtype=np.float64)
+        self.ntimes = len(self.times)
+        self.record = 0
+        self.time_interp_factor = 1.
+        if self.ntimes == 1:
+            self.frozen_hydro = True
+        else:
+            self.frozen_hydro = False
+            for i in range(self.ntimes - 1):
+                dt = self.times[i + 1] - self.times[i]
+                if dt <= EPSILON:
+                    raise ValueError(
+                        "Invalid times array: times must be strictly increasing. "
+                        "Found times[{}]={:.6e} and times[{}]={:.6e} with dt={:.6e} "
+                        "(must be > {:.6e}). Check input data for duplicate or reversed time "
+                        "values.".format(i, self.times[i], i + 1, self.times[i + 1], dt, EPSILON))
+        self.check_dim(elevation)
+        self.check_dim(velocity_x)
+        self.check_dim(velocity_y)
+        if self.dim == 3:
+            self.check_dim(velocity_z)
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        self.elevation = c(elevation)
+        self.velocity_x = c(velocity_x)
+        self.velocity_y = c(velocity_y)
+        self.velocity_z = c(velocity_z)
+        if self.dim == 3:
+            self.check_regular_layers()
+        self.field_names = ['ELEVATION', 'VELOCITY U', 'VELOCITY V', 'VELOCITY W']
+        if opt_fields is not None:
+            assert opt_fields_names is not None
+            self.nopt = len(opt_fields_names)
+        else:
+            self.nopt = 0
+        self.opt_fields = c(opt_fields)
+        self.opt_fields_names = opt_fields_names
+        self.fid_zs, self.fid_ux, self.fid_uy, self.fid_uz = 0, 1, 2, 3
+        self.fid_turbeng = self.fid_dissip = self.fid_zb = self.fid_windx = self.fid_windy = -1
+        if opt_fields_names is not None:
+            names = list(opt_fields_names)
+            pick = lambda cands: next((4 + names.index(n) for n in cands if n in names), -1)
+            self.fid_turbeng = pick(_TURB)
+            self.fid_dissip = pick(("DISSIPATION",))
+            self.fid_zb = pick(("BOTTOM", "FOND"))
+            self.fid_windx = pick(("WIND X", "VENT X"))
+            self.fid_windy = pick(("WIND Y", "VENT Y"))
+        self.friction_model = friction_model
+        self.friction_coefficient = friction_coefficient
+
+    def check_dim(self, field):
+        ref_shape = (self.ntimes, self.npoints) if self.dim == 2 else \
+            (self.ntimes, self.npoints * self.nlayers)
+        shape = np.shape(field)
+        if shape != ref_shape:
+            raise ValueError("Field have wrong dimension, expected {}, found {}".format(
+                (self.ntimes, self.npoints), shape))
+
+    def check_regular_layers(self, rtol=1e-3):
+        """Uniform sigma layers at the first record (eulerian_field_set.pyx:281-314)."""
+        nl = self.nlayers
+        if self.dim != 3 or nl < 2:
+            return
+        frac = (np.arange(nl) / (nl - 1.))[:, None]
+        z = np.asarray(self.elevation)[0].reshape(nl, self.npoints)
+        zb = z[0]
+        hl = (z[-1] - zb) / (nl - 1.)
+        expected = zb[None, :] + frac * (z[-1] - zb)[None, :]
+        tol = rtol * np.abs(hl) + 1e-6 * (1. + np.abs(zb))
+        err = np.abs(z - expected) - tol[None, :]
+        if np.any(err > 0.):
+            l, i = np.unravel_index(np.argmax(err), err.shape)
+            raise ValueError(
+                "Vertical layers are not regularly spaced (CyLag requires uniform sigma layers): "
+                "at first time record, node {}, layer {}, elevation is {:.6e}, expected "
+                "{:.6e}.".format(i, l, z[l, i], expected[l, i]))
+
+    # ------------------------------------------------------------------
+    def field_by_id(self, field_id):
+        """(ntimes, npoints*nlayers) array of a field id (eulerian_field_set.pyx:673-706)."""
+        if field_id == 0:
+            return self.elevation
+        if field_id == 1:
+            return self.velocity_x
+        if field_id == 2:
+            return self.velocity_y
+        if field_id == 3:
+            return self.velocity_z
+        if 3 < field_id <= 3 + self.nopt:
+            return self.opt_fields[field_id - 4]
+        raise ValueError("Wrong field id")
+
+    def get_mtri_triangulation(self):
+        import matplotlib.tri as mtri
+        m = self.triangular_mesh
+        return mtri.Triangulation(m.x, m.y, m.triangles)
+
+    def get_field_list(self):
+        field_list = self.field_names + list(self.opt_fields_names or [])
+        return field_list, list(range(len(field_list)))
+
+    def _interp_record(self, field, time):
+        """Field values at `time` (record search and lerp of
+        eulerian_field_set.pyx:845-864,940-992)."""
+        if self.frozen_hydro or self.ntimes == 1:
+            return np.asarray(field[0])
+        t = self.times
+        rec = 0
+        for i in range(self.ntimes - 1):
+            if t[i] <= time < t[i + 1]:
+                rec = i
+                break
+        else:
+            if time >= t[-1]:
+                rec = self.ntimes - 2
+        tf = (time - t[rec + 1]) / (t[rec] - t[rec + 1])
+        return tf * field[rec] + (1. - tf) * field[rec + 1]
+
+    def get_field(self, field_id, time, layer=-1):
+        """Field at a time and layer; pre/post-processing only (:1770-1795)."""
+        res = self._interp_record(self.field_by_id(field_id), time)
+        if self.dim == 2 or layer == -1:
+            return np.asarray(res)
+        return np.asarray(res[layer * self.npoints:(layer + 1) * self.npoints])
+
+    def _p1(self, k, x, y):
+        m = self.triangular_mesh
+        f = m.det[k] + x * m.gradx[k] + y * m.grady[k]
+        return m.triangles[k], f
+
+    def interpolate_field(self, field_id, point, time, k=None):
+        """P1 interpolation at a point (2D; :1842-1885)."""
+        if k is None:
+            k = int(self.trifinder(point[0], point[1]))
+        if k == -1:
+            raise ValueError("Failed interpolation (5), point not in triangulation")
+        nodes, f = self._p1(k, point[0], point[1])
+        fld = self.get_field(field_id, time, -1 if self.dim == 2 else 0)
+        return fld[nodes[0]] * f[0] + fld[nodes[1]] * f[1] + fld[nodes[2]] * f[2]
+
+    def interpolate_velocity(self, point, time, k=None):
+        if k is None:
+            k = int(self.trifinder(point[0], point[1]))
+        if k == -1:
+            raise ValueError("Failed interpolation (4), point not in triangulation")
+        return (self.interpolate_field(1, point, time, k), self.interpolate_field(2, point, time, k))

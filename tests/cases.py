@@ -1,0 +1,164 @@
+"""Synthetic workloads shared by the parity tests, smoke() and bench.py.
+
+Each case builds the same mesh, fields and initial particles for the product
+(cylag_amd, HIP step) and for the oracle (oracle/refpy.py, CPU restatement),
+starts both from the same global PCG32 state, and returns pool-order rows.
+"""
+import os
+from types import SimpleNamespace
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SEED = (42, 0xda3e06d11bb2b203)
+
+
+def structured_square(n, length=100.):
+    """n x n nodes on [0, length]^2, each cell split by one diagonal."""
+    xs = np.linspace(0., length, n)
+    X, Y = np.meshgrid(xs, xs)
+    x, y = X.ravel().copy(), Y.ravel().copy()
+    idx = np.arange(n * n).reshape(n, n)
+    a, b = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel()
+    c, d = idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
+    tri = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)]).astype(np.int32)
+    return x, y, tri
+
+
+def jittered_delaunay(nx, ny, lx, ly, seed=7, jitter=0.35):
+    """Delaunay triangulation of a jittered grid (matplotlib/qhull)."""
+    from matplotlib.tri import Triangulation
+    rng = np.random.default_rng(seed)
+    xs, ys = np.linspace(0., lx, nx), np.linspace(0., ly, ny)
+    X, Y = np.meshgrid(xs, ys)
+    dx, dy = lx / (nx - 1), ly / (ny - 1)
+    inner = np.ones_like(X, bool)
+    inner[[0, -1], :] = False
+    inner[:, [0, -1]] = False
+    X = X + inner * rng.uniform(-jitter, jitter, X.shape) * dx
+    Y = Y + inner * rng.uniform(-jitter, jitter, Y.shape) * dy
+    tri = Triangulation(X.ravel(), Y.ravel())
+    return tri.x.copy(), tri.y.copy(), tri.triangles.astype(np.int32), tri
+
+
+class Case:
+    """mesh + fields + initial rows; `fields` is a dict of EulerianFieldSet kwargs."""
+
+    def __init__(self, x, y, tri, fields, position, velocity=None, pool=None, boundary_points=None,
+                 nopen=0, use_trifinder=True, pset_kw=None, mpl_tri=None):
+        self.x, self.y, self.tri = x, y, tri
+        self.fields = fields
+        self.position = np.asarray(position, dtype=np.float64)
+        self.velocity = velocity
+        self.pool = pool
+        self.boundary_points = boundary_points
+        self.nopen = nopen
+        self.use_trifinder = use_trifinder
+        self.pset_kw = pset_kw or {}
+        self._mpl = mpl_tri
+
+    def trifinder(self):
+        if not self.use_trifinder:
+            return None
+        if self._mpl is None:
+            from matplotlib.tri import Triangulation
+            self._mpl = Triangulation(self.x, self.y, self.tri)
+        return self._mpl.get_trifinder()
+
+    # -- product -------------------------------------------------------------------
+    def build_gpu(self, cy, params):
+        mesh = cy.TriangularMesh(self.x, self.y, self.tri, boundary_points=self.boundary_points,
+                                 nopen=self.nopen)
+        fs = cy.EulerianFieldSet(mesh, trifinder=self.trifinder(), **self.fields)
+        ps = cy.LagrangianParticleSet(self.position, self.velocity, dim=self.fields.get("dim", 2),
+                                      initial_pool_size=self.pool, **self.pset_kw)
+        cy.pcg32_seed(*SEED)
+        return cy.Solver(fs, ps, params)
+
+    def run_gpu(self, cy, params, nsteps=None):
+        s = self.build_gpu(cy, params)
+        if nsteps is None:
+            s.solve()
+        else:
+            for _ in range(nsteps):
+                s.forward(s.simutime.time_step)
+        ps = s.particle_set
+        return SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(),
+                               tri=ps.tri.copy(), inactive=ps.inactive.copy(),
+                               lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy(), solver=s)
+
+    # -- oracle -----------------------------------------------------------------------
+    def run_oracle(self, R, params, nsteps=None):
+        mesh = R.RefMesh(self.x, self.y, self.tri, boundary_points=self.boundary_points,
+                         nopen=self.nopen)
+        f = dict(self.fields)
+        fs = R.RefFields(mesh, **{k: v for k, v in f.items() if k != "trifinder"})
+        kw = {k: v for k, v in self.pset_kw.items()}
+        ps = R.RefParticles(self.position, self.velocity, dim=f.get("dim", 2),
+                            initial_pool_size=self.pool, **kw)
+        rng = R.RefRng(*SEED)
+        s = R.RefSolver(fs, ps, params, trifinder=self.trifinder(), rng=rng)
+        if nsteps is None:
+            s.solve()
+        else:
+            s.forward(nsteps)
+        return SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(),
+                               tri=ps.tri.copy(), inactive=ps.inactive.copy(),
+                               lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy(), rng=rng,
+                               solver=s)
+
+
+def square_case(n=101, npart=10000, u=0.1, v=0.0, depth=1.0, centre=50., length=100.):
+    """The plain 2D isotropic-diffusion workload: uniform flow on a structured
+    square, all particles released at the centre (a mesh vertex when n is odd)."""
+    x, y, tri = structured_square(n, length)
+    nn = x.size
+    fields = dict(dim=2, nlayers=1, times=np.zeros(1), elevation=np.full((1, nn), depth),
+                  velocity_x=np.full((1, nn), u), velocity_y=np.full((1, nn), v))
+    pos = np.full((npart, 2), centre)
+    return Case(x, y, tri, fields, pos)
+
+
+def reference_square_case(position, velocity=None, pool=None):
+    """The reference tests' square mesh with a solid-body rotation field
+    (tests/test_20_lsm1_analytical_advection_2d.py:15-66)."""
+    x = np.loadtxt(os.path.join(GOLDEN, "square_meshx.txt"))
+    y = np.loadtxt(os.path.join(GOLDEN, "square_meshy.txt"))
+    tri = np.loadtxt(os.path.join(GOLDEN, "square_meshtri.txt"), dtype=np.int32)
+    from oracle.refpy import read_cli
+    bp, _ = read_cli(os.path.join(GOLDEN, "square_geo.cli"))
+    om = 2. * np.pi / 3600.
+    r, phi = np.sqrt(x ** 2 + y ** 2), np.arctan2(x, y)
+    vx, vy = om * r * np.cos(phi), -om * r * np.sin(phi)
+    n = x.size
+    fields = dict(dim=2, nlayers=1, times=np.zeros(1), elevation=np.ones((1, n)),
+                  velocity_x=vx[None], velocity_y=vy[None])
+    return Case(x, y, tri, fields, position, velocity, pool, boundary_points=bp)
+
+
+def vortex_case(nx=41, ny=11, lx=2000., ly=500., npart=2000, seed=3, ntimes=2, keps=True,
+                wavelength=400.):
+    """Small version of the unstructured 2D workload: jittered Delaunay mesh,
+    two frames of a cellular vortex, k-epsilon diffusivity varying in space."""
+    x, y, tri, mpl = jittered_delaunay(nx, ny, lx, ly, seed=7)
+    k = 2. * np.pi / wavelength
+    n = x.size
+    frames_u, frames_v = [], []
+    for t in range(ntimes):
+        ph = 0.3 * t
+        frames_u.append(np.sin(k * x + ph) * np.cos(k * y))
+        frames_v.append(-np.cos(k * x + ph) * np.sin(k * y))
+    times = np.linspace(0., 10000., ntimes)
+    depth = 2. + 18. * (0.5 + 0.5 * np.sin(x / lx * 3.1) * np.cos(y / ly * 2.3))
+    K = 0.1 + 4.9 * (0.5 + 0.5 * np.cos(x / lx * 5.) * np.sin(y / ly * 4.))
+    eps = np.full(n, 1e-3)
+    tke = np.sqrt(K * eps / (0.09 / 0.72))
+    fields = dict(dim=2, nlayers=1, times=times, elevation=np.tile(depth, (ntimes, 1)),
+                  velocity_x=np.array(frames_u), velocity_y=np.array(frames_v))
+    if keps:
+        opt = np.stack([np.tile(tke, (ntimes, 1)), np.tile(eps, (ntimes, 1))])
+        fields.update(opt_fields=opt, opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
+    rng = np.random.default_rng(seed)
+    pos = np.stack([rng.uniform(0.05 * lx, 0.95 * lx, npart),
+                    rng.uniform(0.05 * ly, 0.95 * ly, npart)], 1)
+    return Case(x, y, tri, fields, pos, mpl_tri=mpl)

@@ -1,0 +1,112 @@
+"""HIP step (through the C ABI) against the oracle on the same seeded inputs:
+identical element ids, pool flags, tags and random-stream state; positions
+within 1e-9 relative (FP64)."""
+import numpy as np
+import pytest
+
+import cylag_amd as cy
+from oracle import refpy as R
+from tests.cases import reference_square_case, square_case, vortex_case
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def assert_parity(g, r, check_velocity=True):
+    assert np.array_equal(g.inactive, r.inactive), "pool flags differ"
+    assert np.array_equal(g.tri, r.tri), "element ids differ at rows {}".format(
+        np.nonzero(g.tri != r.tri)[0][:10])
+    assert np.array_equal(g.lowerlayer, r.lowerlayer)
+    assert np.array_equal(g.tag, r.tag), "tags differ"
+    scale = np.maximum(np.abs(r.position), 1.)
+    err = np.max(np.abs(g.position - r.position) / scale) if r.position.size else 0.
+    assert err <= TOL, "position error {:.3e}".format(err)
+    if check_velocity:
+        verr = np.max(np.abs(g.velocity - r.velocity) / np.maximum(np.abs(r.velocity), 1.))
+        assert verr <= TOL, "velocity error {:.3e}".format(verr)
+    assert cy.random_utils.get_state() == r.rng.state(), "random stream state differs"
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+def test_plain_diffusion_from_a_vertex(rng_method):
+    """All particles start on a mesh vertex: the first walk circles the vertex fan."""
+    case = square_case(n=21, npart=400)
+    params = dict(final_time=60., time_step=1., frozen_eulerian_fields=True, model=1,
+                  diffusion_model=1, horizontal_diffusivity=0.5, rng_method=rng_method)
+    assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
+
+
+@pytest.mark.parametrize("scheme", [1, 2, 3, 4])
+def test_lsm1_schemes_reference_mesh(scheme):
+    pos = np.array([[5000., 0.], [5000., 500.], [1000., 0.], [1000., 500.], [-7000., 3000.]])
+    case = reference_square_case(pos, pool=8)
+    params = dict(final_time=3600., time_step=100., time_scheme=scheme,
+                  frozen_eulerian_fields=True, model=1, diffusion_model=1,
+                  horizontal_diffusivity=20.)
+    assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
+
+
+def test_lsm1_golden_rk4():
+    # tests/test_20_lsm1_analytical_advection_2d.py:242-246 through the product
+    pos = np.array([[5000., 0.], [5000., 500.], [1000., 0.], [1000., 500.]])
+    case = reference_square_case(pos, pool=1000)
+    params = dict(final_time=3600., time_step=200., time_scheme=4, frozen_eulerian_fields=True,
+                  model=1, diffusion_model=0)
+    g = case.run_gpu(cy, params)
+    np.testing.assert_allclose(g.position[g.inactive == 0], np.array(
+        [[4.99888531e+03, 3.71825833e+00], [4.99851348e+03, 5.03606789e+02],
+         [9.99777062e+02, 7.43651666e-01], [9.99405236e+02, 5.00632183e+02]]), rtol=1e-7)
+
+
+def test_model0_rebound_golden():
+    pos = np.full((11, 2), 7500.)
+    vel = 10. * np.array([[1., 0.1 * k] for k in range(11)])
+    case = reference_square_case(pos, vel, pool=12)
+    params = dict(final_time=5000., time_step=50., time_scheme=1, frozen_eulerian_fields=True,
+                  model=0, particle_velocity_init=0, boundary_conditions=True,
+                  max_wall_rebounds=4, rebound_damping_coef=0.)
+    g = case.run_gpu(cy, params)
+    r = case.run_oracle(R, params)
+    assert_parity(g, r)
+    xp = g.position[np.argsort(g.inactive, kind="stable")][:int((g.inactive == 0).sum())]
+    from tests.test_oracle_golden import REBOUND_EXPECTED
+    np.testing.assert_allclose(xp, REBOUND_EXPECTED, rtol=1e-7)
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+def test_unstructured_keps_two_frames(rng_method):
+    case = vortex_case(npart=3000)
+    params = dict(final_time=200., time_step=5., model=1, diffusion_model=2,
+                  rng_method=rng_method)
+    assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
+
+
+def test_open_boundary_deletions_and_tags():
+    """Particles leave through the reference mesh's open boundaries: deletions,
+    tag renumbering and the pool-order random stream after them."""
+    rng = np.random.default_rng(5)
+    pos = np.stack([rng.uniform(-9000., 9000., 300), rng.uniform(-9000., 9000., 300)], 1)
+    case = reference_square_case(pos, pool=320)
+    case.nopen = 2
+    params = dict(final_time=4000., time_step=20., frozen_eulerian_fields=True, model=1,
+                  diffusion_model=1, horizontal_diffusivity=3000., rng_method=2,
+                  bnd_statistics=True)
+    g = case.run_gpu(cy, params)
+    r = case.run_oracle(R, params)
+    assert (r.inactive[:300] == 1).sum() > 0, "the case must delete particles"
+    assert_parity(g, r)
+
+
+def test_element_resort_does_not_change_results():
+    case = vortex_case(npart=2000)
+    params = dict(final_time=150., time_step=5., model=1, diffusion_model=2)
+    s = case.build_gpu(cy, params)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 256)
+    s.solve()
+    r = case.run_oracle(R, params)
+    ps = s.particle_set
+    from types import SimpleNamespace
+    g = SimpleNamespace(position=ps.position, velocity=ps.velocity, tri=ps.tri,
+                        inactive=ps.inactive, lowerlayer=ps.lowerlayer, tag=ps.tag)
+    assert_parity(g, r)
