@@ -61,18 +61,31 @@ def bytes_per_particle_step(nt, nv, npart):
     return state + mesh / npart, state, mesh
 
 
-def cpu_baseline(x, y, tri, fields, pos, params, nsample=100_000, nsteps=50):
-    """The oracle (serial restatement, one core) on a bounded sample."""
+def cpu_baseline(x, y, tri, fields, pos, params, nsample=100_000, budget_s=15.):
+    """The oracle (serial restatement, one core) on a bounded sample: the first
+    `nsample` particles of the same workload, stepped until ~budget_s of CPU time."""
+    from matplotlib.tri import Triangulation
     from oracle import refpy as R
+    t0 = time.perf_counter()
     mesh = R.RefMesh(x, y, tri)
     fs = R.RefFields(mesh, **fields)
     ps = R.RefParticles(pos[:nsample].copy(), dim=2)
-    s = R.RefSolver(fs, ps, params, rng=R.RefRng())
+    tf = Triangulation(x, y, tri).get_trifinder()
+    s = R.RefSolver(fs, ps, params, trifinder=tf, rng=R.RefRng())
+    log("cpu baseline: oracle ready ({:.1f} s)".format(time.perf_counter() - t0))
     t0 = time.perf_counter()
-    s.forward(nsteps)
-    dt = time.perf_counter() - t0
-    return dict(value=nsample * nsteps / dt, unit="particle-steps/s", cores=1, kind="port",
-                sample="{} particles x {} steps of the same workload ({:.1f} s)".format(nsample, nsteps, dt))
+    nsteps = 0
+    while True:
+        s.forward(1)
+        nsteps += 1
+        el = time.perf_counter() - t0
+        if nsteps % 5 == 0:
+            log("cpu baseline: {} steps, {:.1f} s".format(nsteps, el))
+        if el >= budget_s:
+            break
+    return dict(value=nsample * nsteps / el, unit="particle-steps/s", cores=1, kind="port",
+                sample="{} particles x {} steps of the same workload ({:.1f} s, oracle/cpu_ref.c "
+                       "-O2, serial pool-order loop)".format(nsample, nsteps, el))
 
 
 def main():
@@ -107,9 +120,10 @@ def main():
     solver._run(args.steps)
     wall = time.perf_counter() - t0
     k_ms, tot_ms, nlaunch = solver.ctx.last_timing()
-    nact = ps.npart_active if not ps._stale else None
     ps._sync()
     nact = ps.npart_active
+    log("timed: {} steps in {:.3f} s, step kernel {:.3f} ms/launch over {} launches".format(
+        args.steps, wall, k_ms / max(nlaunch, 1), nlaunch))
     psteps = nact * args.steps / wall
     b_alg, b_state, b_mesh = bytes_per_particle_step(mesh.nt, mesh.nv, nact)
     avg_ms = k_ms / max(nlaunch, 1)
@@ -131,7 +145,7 @@ def main():
                                "dt=5 s, Ziggurat".format(mesh.nt, nact),
                    "elements": int(mesh.nt), "nodes": int(mesh.nv), "particles": int(nact),
                    "rng_method": 2, "sort_interval": args.sort_interval},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "B/s",
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": "k_step<2,1,false>", "kernel_avg_ms": avg_ms,
                      "bytes_per_particle_step": b_alg},

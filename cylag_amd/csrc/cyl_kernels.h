@@ -79,23 +79,21 @@ __device__ __forceinline__ void store_part(const PoolDev& B, int j, int dim, con
   B.inactive[j] = inactive_code;
 }
 
+// One pool slot; returns the stream position after the slot's last draw (0 if none).
 template <int DIM, int MODEL, bool COUNT>
-__global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
-  if (a.guarded && a.flag[0] == 0) return;
-  int j = blockIdx.x * BLK + threadIdx.x;
-  if (j >= a.n) return;
+__device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j, int& scen) {
   Part p;
   load_part(a.A, a.slot, j, DIM, p);
-  const int scen = a.slot.scen ? a.slot.scen[j] : 0;
+  scen = a.slot.scen ? a.slot.scen[j] : 0;
   if (p.inactive) {
     if (COUNT) a.units_pool[a.gpool[j]] = 0;
-    if (!COUNT) {
+    else {
       // keep the inactive row (deleted rows keep their "deleted" mark)
       a.B.inactive[j] = p.inactive;
       a.B.tri[j] = -1;
       a.B.ll[j] = 0;
     }
-    return;
+    return 0;
   }
   const StreamDev& st = a.stream[scen];
   Ctx c{a.m, a.f, a.par, a.pc, a.dt, DIM, 0};
@@ -110,6 +108,7 @@ __global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
   dr.pos0 = 0;
   dr.live = false;
   dr.err = 0;
+  dr.unit0 = 0;
   if (!COUNT) {
     if (a.general) dr.unit0 = a.slot.off[j];
     else dr.unit0 = (int64_t)st.units_per_part * a.slot.rank[j];
@@ -122,10 +121,9 @@ __global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
     if (MODEL == 0) bnd_j = update_test(c, p);
     else if (MODEL == 1) bnd_j = update_lsm1(c, p, dr);
     else bnd_j = update_lsm2(c, p, dr);
-    if (!c.err) {
-      if (COUNT) {
-        // units are counted before the boundary conditions (which use a sub-stream)
-      } else if (a.par.boundary_conditions) {
+    if (!c.err && !COUNT) {
+      // draws are counted before the boundary conditions, which use a sub-stream
+      if (a.par.boundary_conditions) {
         BcDraws bd;
         bd.method = a.par.rng_method;
         bd.zt = a.zt;
@@ -137,24 +135,19 @@ __global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
         if (p.ll == PART_LOC_B) delete_part(p, DIM);
         if (p.ll == PART_LOC_A) delete_part(p, DIM);
       }
-      if (!COUNT && a.par.compute_depth && !p.inactive && DIM == 3 && a.depth) {
+      if (a.par.compute_depth && !p.inactive && DIM == 3 && a.depth)
         a.depth[j] = field2d(a.f, a.m, a.f.fid_zs, p.tri, a.f.nlayers - 1, p.x[0], p.x[1]) - p.x[2];
-      }
     }
   }
-  if (was_new && !p.inactive && a.par.compute_depth && DIM == 3 && a.depth && !COUNT && !c.err &&
-      a.init_only) {
+  if (was_new && !p.inactive && a.par.compute_depth && DIM == 3 && a.depth && !COUNT && !c.err && a.init_only)
     a.depth[j] = field2d(a.f, a.m, a.f.fid_zs, p.tri, a.f.nlayers - 1, p.x[0], p.x[1]) - p.x[2];
-  }
   if (dr.err && !c.err) c.err = dr.err;
+  if (c.err) atomicMin(&a.err[scen], ((unsigned long long)(unsigned)pool << 8) | (unsigned)c.err);
   if (COUNT) {
     a.units_pool[a.gpool[j]] = (int)dr.nunits;
-    if (c.err) atomicMin(&a.err[scen], ((unsigned long long)(unsigned)pool << 8) | (unsigned)c.err);
-    return;
+    return 0;
   }
-  if (c.err) atomicMin(&a.err[scen], ((unsigned long long)(unsigned)pool << 8) | (unsigned)c.err);
   if (!a.general && !a.init_only && dr.nunits != st.units_per_part) atomicOr(&a.flag[0], 1);
-  if (dr.live) atomicMax(&a.stream_end[scen], (unsigned long long)(dr.pos0 + dr.ncalls));
   int code = 0;
   if (p.deleted) {
     code = 2;
@@ -164,6 +157,29 @@ __global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
     code = 1;
   }
   store_part(a.B, j, DIM, p, code);
+  return dr.live ? (unsigned long long)(dr.pos0 + dr.ncalls) : 0ull;
+}
+
+template <int DIM, int MODEL, bool COUNT>
+__global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
+  if (a.guarded && a.flag[0] == 0) return;
+  int j = blockIdx.x * BLK + threadIdx.x;
+  int scen = -1;
+  unsigned long long e = 0;
+  if (j < a.n) e = step_slot<DIM, MODEL, COUNT>(a, j, scen);
+  if (COUNT) return;
+  // stream end: one atomic per wavefront when its rows share a scenario
+  int s0 = __shfl(scen, 0);
+  bool same = __all(scen == s0 || scen < 0);
+  if (same) {
+    for (int o = 32; o > 0; o >>= 1) {
+      unsigned long long q = __shfl_xor(e, o);
+      e = q > e ? q : e;
+    }
+    if ((threadIdx.x & 63) == 0 && e && s0 >= 0) atomicMax(&a.stream_end[s0], e);
+  } else if (e) {
+    atomicMax(&a.stream_end[scen], e);
+  }
 }
 
 // tmp[fid][i] = tf*frame_rec[i] + (1-tf)*frame_rec1[i]  (eulerian_field_set.pyx:926-938)

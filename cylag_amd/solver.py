@@ -124,6 +124,7 @@ class Solver:
         self._bnd_stats_times = np.zeros(nt_buf)
         self._bnd_stats_data = np.zeros((nt_buf, max(1, nopen)), dtype=np.int32)
         self._bnd_stats_count = 0
+        self._bnd_flushed = []
         self._csec_stats_times = np.zeros(nt_buf)
         self._csec_stats_data = np.zeros((nt_buf, max(1, p.ncsec)), dtype=np.int32)
         self._csec_stats_count = 0
@@ -299,10 +300,20 @@ class Solver:
         self.flush_statistics()
 
     def flush_statistics(self):
-        """Statistics buffers are kept in memory; file output is not part of the step."""
+        """Move the buffered per-step statistics to the flushed record
+        (solver.pyx:587-630 writes them to files and resets the buffers)."""
+        n = self._bnd_stats_count
+        if n:
+            self._bnd_flushed.append((self._bnd_stats_times[:n].copy(),
+                                      self._bnd_stats_data[:n].copy()))
         self._bnd_stats_count = 0
         self._csec_stats_count = 0
 
     @property
     def bnd_stats(self):
-        return self._bnd_stats_times[:self._bnd_stats_count], self._bnd_stats_data[:self._bnd_stats_count]
+        """(times, counts[nstep, nopen]) of every step run so far."""
+        parts = self._bnd_flushed + [(self._bnd_stats_times[:self._bnd_stats_count],
+                                      self._bnd_stats_data[:self._bnd_stats_count])]
+        nopen = self.field_set.triangular_mesh.nopen
+        return (np.concatenate([t for t, _ in parts]),
+                np.concatenate([d for _, d in parts])[:, :max(nopen, 1)])

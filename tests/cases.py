@@ -162,3 +162,20 @@ def vortex_case(nx=41, ny=11, lx=2000., ly=500., npart=2000, seed=3, ntimes=2, k
     pos = np.stack([rng.uniform(0.05 * lx, 0.95 * lx, npart),
                     rng.uniform(0.05 * ly, 0.95 * ly, npart)], 1)
     return Case(x, y, tri, fields, pos, mpl_tri=mpl)
+
+
+def open_channel_case(n=21, npart=400, u=0.8, seed=5):
+    """Structured square with open boundaries on the right (label 1) and left
+    (label 2) sides, walls elsewhere: particles advected to the right leave
+    the domain (deletions, tag renumbering, boundary statistics)."""
+    x, y, tri = structured_square(n)
+    nn = x.size
+    right = np.nonzero(np.isclose(x, 100.) & (y > 0.) & (y < 100.))[0]
+    left = np.nonzero(np.isclose(x, 0.) & (y > 0.) & (y < 100.))[0]
+    bp = np.concatenate([np.stack([right, np.full(right.size, 1)], 1),
+                         np.stack([left, np.full(left.size, 2)], 1)]).astype(np.int32)
+    fields = dict(dim=2, nlayers=1, times=np.zeros(1), elevation=np.ones((1, nn)),
+                  velocity_x=np.full((1, nn), u), velocity_y=np.zeros((1, nn)))
+    rng = np.random.default_rng(seed)
+    pos = np.stack([rng.uniform(5., 95., npart), rng.uniform(5., 95., npart)], 1)
+    return Case(x, y, tri, fields, pos, pool=npart + 16, boundary_points=bp, nopen=2)

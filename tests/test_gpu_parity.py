@@ -6,7 +6,7 @@ import pytest
 
 import cylag_amd as cy
 from oracle import refpy as R
-from tests.cases import reference_square_case, square_case, vortex_case
+from tests.cases import open_channel_case, reference_square_case, square_case, vortex_case
 
 pytestmark = pytest.mark.gpu
 
@@ -82,20 +82,21 @@ def test_unstructured_keps_two_frames(rng_method):
     assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
 
 
-def test_open_boundary_deletions_and_tags():
-    """Particles leave through the reference mesh's open boundaries: deletions,
-    tag renumbering and the pool-order random stream after them."""
-    rng = np.random.default_rng(5)
-    pos = np.stack([rng.uniform(-9000., 9000., 300), rng.uniform(-9000., 9000., 300)], 1)
-    case = reference_square_case(pos, pool=320)
-    case.nopen = 2
-    params = dict(final_time=4000., time_step=20., frozen_eulerian_fields=True, model=1,
-                  diffusion_model=1, horizontal_diffusivity=3000., rng_method=2,
+@pytest.mark.parametrize("rng_method", [1, 2])
+def test_open_boundary_deletions_and_tags(rng_method):
+    """Particles leave through open boundaries: deletions, tag renumbering,
+    boundary statistics and the pool-order random stream after them."""
+    case = open_channel_case()
+    params = dict(final_time=70., time_step=1., frozen_eulerian_fields=True, model=1,
+                  diffusion_model=1, horizontal_diffusivity=2., rng_method=rng_method,
                   bnd_statistics=True)
     g = case.run_gpu(cy, params)
     r = case.run_oracle(R, params)
-    assert (r.inactive[:300] == 1).sum() > 0, "the case must delete particles"
+    assert (r.inactive[:400] == 1).sum() > 50, "the case must delete particles"
     assert_parity(g, r)
+    gs = g.solver.bnd_stats[1]
+    assert gs.sum() == np.sum(r.solver.bnd_history)
+    np.testing.assert_array_equal(gs, np.array(r.solver.bnd_history))
 
 
 def test_element_resort_does_not_change_results():
