@@ -147,7 +147,7 @@ def main():
                    "rng_method": 2, "sort_interval": args.sort_interval},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_step<2,1,false>", "kernel_avg_ms": avg_ms,
+                     "kernel": "k_lsm1_2d<2>", "kernel_avg_ms": avg_ms,
                      "bytes_per_particle_step": b_alg},
     }
     if not args.no_cpu:

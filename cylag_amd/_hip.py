@@ -68,6 +68,7 @@ SIGNATURES = {
     "cyl_step": (C.c_int, [C.c_void_p, C.POINTER(CylParams), _dp, C.c_double, C.c_int, _ip]),
     "cyl_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "cyl_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
+    "cyl_last_prep_timing": (C.c_int, [C.c_void_p, _dp]),
 }
 
 _lib = None
@@ -144,6 +145,11 @@ class Context:
         k, t, n = C.c_double(), C.c_double(), C.c_int()
         self.lib.cyl_last_timing(self.h, C.byref(k), C.byref(t), C.byref(n))
         return k.value, t.value, n.value
+
+    def last_prep_ms(self):
+        q = C.c_double()
+        self.lib.cyl_last_prep_timing(self.h, C.byref(q))
+        return q.value
 
 
 def contiguous(a, dtype):

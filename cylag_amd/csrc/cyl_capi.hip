@@ -20,7 +20,7 @@
 #include <vector>
 
 #include "../../include/cylag_hip.h"
-#include "cyl_kernels.h"
+#include "cyl_fast.h"
 
 using namespace cyl;
 
@@ -46,6 +46,11 @@ struct cyl_ctx {
   MeshDev mesh{};
   bool has_mesh = false;
   Bufs mesh_bufs;
+  // packed records of the fused 2D LSM-1 kernel (built on first use)
+  bool fast_ready = false;
+  Bufs fast_bufs;
+  FastMesh fm{};
+  NodeF* d_nodes = nullptr;
 
   // fields
   FieldsDev fields{};
@@ -70,6 +75,7 @@ struct cyl_ctx {
   int cur = 0;
   Bufs part_bufs;
   int *d_gpool = nullptr, *d_rank = nullptr, *d_init_tri = nullptr, *d_del_step = nullptr;
+  double2* d_xi = nullptr;            // the step's normals per active rank (fused 2D step)
   int64_t* d_off = nullptr;
   double *d_diam = nullptr, *d_vol = nullptr, *d_depth = nullptr;
   ParticleConst pc{};
@@ -83,6 +89,8 @@ struct cyl_ctx {
   int* d_scen = nullptr;
   StreamDev* d_stream = nullptr;
   JumpTable* d_jt = nullptr;
+  JumpTable8* d_jt8 = nullptr;
+  int ncu = 256;
   unsigned long long *d_stream_end = nullptr, *d_err = nullptr, *d_err_step = nullptr;
   int64_t *d_scen_units = nullptr, *d_scen_active = nullptr;
   int* d_flag = nullptr;
@@ -104,13 +112,24 @@ struct cyl_ctx {
   int* d_chunk_count = nullptr;
   int64_t *d_chunk_off = nullptr, *d_ev_pos = nullptr, *d_ev_len = nullptr, *d_reach = nullptr;
   int64_t *d_bmax = nullptr, *d_E = nullptr, *d_C = nullptr, *d_key = nullptr, *d_xtra = nullptr;
+  double* d_ev_val = nullptr;
   int64_t *d_ev_begin = nullptr, *d_nev = nullptr, *d_ev_bsum = nullptr, *d_xscratch = nullptr;
   int* d_on = nullptr;
+  int* d_bucket = nullptr;          // bucket index of the event keys, all scenarios
+  int64_t* d_bucket_base = nullptr;
+  int64_t nbucket = 0;
+  int bshift = 6;
 
   // element re-sort
   int sort_interval = 0, block = BLK, steps_since_sort = 0;
   int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
   int64_t* d_cursor = nullptr;
+  int64_t* d_sort_tiles = nullptr;
+  int sort_nt = 0;
+  // spare per-slot buffers the sort gathers into (pointers swapped after)
+  int *sp_gpool = nullptr, *sp_init_tri = nullptr, *sp_del_step = nullptr, *sp_inactive = nullptr;
+  int* sp_rank = nullptr;
+  double *sp_diam = nullptr, *sp_vol = nullptr, *sp_depth = nullptr;
   void* d_tmpbuf = nullptr;
 
   int step_counter = 0;
@@ -118,7 +137,10 @@ struct cyl_ctx {
   size_t bnd_cap = 0;
   int nopen = 0;
 
-  double last_kernel_ms = 0., last_total_ms = 0., last_step_kernel_ms = 0.;
+  double last_kernel_ms = 0., last_total_ms = 0., last_step_kernel_ms = 0., last_prep_ms = 0.;
+  int64_t* d_pos0 = nullptr;     // per slot: first-draw stream position (fused 2D step)
+  uint64_t* d_state0 = nullptr;  // per slot: generator state there
+  int start_cap = 0;
   int last_step_launches = 0;
   std::vector<hipEvent_t> kev;              // start/stop pairs around the fused step kernel
   int last_launches = 0;
@@ -161,6 +183,7 @@ extern "C" int cyl_ctx_create(int device, int flags, cyl_ctx** out) {
   *out = ctx;
   HIPCHK(hipSetDevice(device));
   HIPCHK(hipStreamCreate(&ctx->stream));
+  HIPCHK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device));
   HIPCHK(hipEventCreate(&ctx->ev0));
   HIPCHK(hipEventCreate(&ctx->ev1));
   // Ziggurat tables with the host libm (random_utils.pyx:118-146)
@@ -205,7 +228,12 @@ extern "C" int cyl_ctx_destroy(cyl_ctx* ctx) {
   if (!ctx) return CYL_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->pool[1].inactive = nullptr;
   ctx->mesh_bufs.free_all();
+  ctx->fast_bufs.free_all();
+  ctx->fast_ready = false;
+  if (ctx->d_pos0) (void)hipFree(ctx->d_pos0);
+  if (ctx->d_state0) (void)hipFree(ctx->d_state0);
   ctx->field_bufs.free_all();
   ctx->part_bufs.free_all();
   ctx->ev_bufs.free_all();
@@ -256,6 +284,8 @@ extern "C" int cyl_mesh_upload(cyl_ctx* ctx, int nv, const double* x, const doub
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->mesh_bufs.free_all();
+  ctx->fast_bufs.free_all();
+  ctx->fast_ready = false;
   MeshDev m{};
   m.nv = nv;
   m.nt = nt;
@@ -520,7 +550,7 @@ static int alloc_pool(cyl_ctx* ctx, PoolDev& P) {
   }
   HIPCHK(dalloc(b, &P.tri, n));
   HIPCHK(dalloc(b, &P.ll, n));
-  HIPCHK(dalloc(b, &P.inactive, n));
+  if (&P == &ctx->pool[0]) HIPCHK(dalloc(b, &P.inactive, n));
   return CYL_OK;
 }
 
@@ -531,6 +561,7 @@ static int ensure_scen_tables(cyl_ctx* ctx) {
   HIPCHK(hipMemcpy(ctx->d_scen_base, ctx->scen_base_h.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(dalloc(b, &ctx->d_stream, ns));
   HIPCHK(dalloc(b, &ctx->d_jt, ns));
+  HIPCHK(dalloc(b, &ctx->d_jt8, ns));
   HIPCHK(dalloc(b, &ctx->d_stream_end, ns));
   HIPCHK(dalloc(b, &ctx->d_err, ns));
   HIPCHK(dalloc(b, &ctx->d_err_step, ns));
@@ -554,6 +585,12 @@ extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const doub
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->part_bufs.free_all();
+  ctx->d_scnt = nullptr;
+  ctx->d_cursor = nullptr;
+  ctx->d_sort_tiles = nullptr;
+  ctx->sort_nt = 0;
+  ctx->sp_gpool = ctx->sp_init_tri = ctx->sp_del_step = ctx->sp_inactive = ctx->sp_rank = nullptr;
+  ctx->sp_diam = ctx->sp_vol = ctx->sp_depth = nullptr;
   ctx->ev_bufs.free_all();
   ctx->ev_units_cap = 0;
   ctx->n = npart;
@@ -571,10 +608,14 @@ extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const doub
     int e = alloc_pool(ctx, ctx->pool[b]);
     if (e) return e;
   }
+  // the pool flags are per slot, not double-buffered: a row's flag is only
+  // changed by its own thread (deletion), in place
+  ctx->pool[1].inactive = ctx->pool[0].inactive;
   Bufs& bb = ctx->part_bufs;
   int n = npart;
   HIPCHK(dalloc(bb, &ctx->d_gpool, n));
   HIPCHK(dalloc(bb, &ctx->d_rank, n));
+  HIPCHK(dalloc(bb, &ctx->d_xi, n));
   HIPCHK(dalloc(bb, &ctx->d_off, n));
   HIPCHK(dalloc(bb, &ctx->d_init_tri, n));
   HIPCHK(dalloc(bb, &ctx->d_del_step, n));
@@ -633,7 +674,6 @@ extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const doub
   HIPCHK(hipMemset(ctx->d_depth, 0, (dim == 3 ? n : 1) * sizeof(double)));
   // the second buffer starts as a copy (inactive rows are never rewritten there)
   PoolDev& B = ctx->pool[1];
-  HIPCHK(hipMemcpy(B.inactive, A.inactive, n * sizeof(int), hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(B.tri, A.tri, n * sizeof(int), hipMemcpyDeviceToDevice));
   HIPCHK(hipMemcpy(B.ll, A.ll, n * sizeof(int), hipMemcpyDeviceToDevice));
   HIPCHK(hipMemset(ctx->d_flag, 0, 8 * sizeof(int)));
@@ -708,10 +748,7 @@ extern "C" int cyl_particles_deleted(cyl_ctx* ctx, int* pool_index, int cap, int
   std::sort(del.begin(), del.end());
   *nout = (int)del.size();
   for (int q = 0; q < (int)del.size() && q < cap; ++q) pool_index[q] = del[q].second;
-  if (!del.empty()) {
-    HIPCHK(hipMemcpy(P.inactive, ina.data(), n * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->pool[1 - ctx->cur].inactive, ina.data(), n * sizeof(int), hipMemcpyHostToDevice));
-  }
+  if (!del.empty()) HIPCHK(hipMemcpy(P.inactive, ina.data(), n * sizeof(int), hipMemcpyHostToDevice));
   return CYL_OK;
 }
 
@@ -925,7 +962,16 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   }
   ctx->nchunks = nch;
   ctx->ev_cap = evcap;
+  // about one Ziggurat slow event (rate ~1.2%) per 64 normals, and about two
+  // rejected polar attempts (rate 1 - pi/4) per 8 calls
+  ctx->bshift = method == 2 ? 6 : 3;
+  std::vector<int64_t> bb(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) bb[s + 1] = bb[s] + (cap_units[s] >> ctx->bshift) + 2;
+  ctx->nbucket = bb[ns];
   Bufs& b = ctx->ev_bufs;
+  HIPCHK(dalloc(b, &ctx->d_bucket, ctx->nbucket));
+  HIPCHK(dalloc(b, &ctx->d_bucket_base, ns + 1));
+  HIPCHK(hipMemcpy(ctx->d_bucket_base, bb.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
   HIPCHK(dalloc(b, &ctx->d_chunk_base, ns));
   HIPCHK(hipMemcpy(ctx->d_chunk_base, cb.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice));
   HIPCHK(dalloc(b, &ctx->d_chunk_count, nch + 1));
@@ -939,6 +985,7 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(dalloc(b, &ctx->d_C, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_key, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_xtra, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_ev_val, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_xscratch, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_on, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_begin, ns + 1));
@@ -950,10 +997,10 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
 template <class T>
 static void enqueue_scan(cyl_ctx* ctx, const T* in, int64_t* out, int64_t n, int64_t* bsum, int64_t* total,
                          const int64_t* ndev, const int* cond, int& launches) {
-  unsigned nb = nblk(n);
-  k_scan_blocks<T><<<nb, BLK, 0, ctx->stream>>>(in, out, bsum, n, ndev, cond);
-  k_scan_totals<<<1, 1024, 0, ctx->stream>>>(bsum, nb, total, cond);
-  k_scan_add<<<nb, BLK, 0, ctx->stream>>>(out, bsum, n, cond);
+  unsigned nb = nblk(n, SCAN_TILE);
+  k_scan_tiles<T, OpSum><<<nb, BLK, 0, ctx->stream>>>(in, out, bsum, n, ndev, cond);
+  k_scan_top<OpSum><<<1, 1024, 0, ctx->stream>>>(bsum, nb, total, cond);
+  k_scan_fix<<<nb, BLK, 0, ctx->stream>>>(out, bsum, n, cond);
   launches += 3;
 }
 
@@ -1004,8 +1051,9 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
     unsigned nb = nblk(cap);
     k_reach_key<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, cap,
                                    ctx->d_xscratch, cond);
-    k_mscan_blocks<<<nb, BLK, 0, s>>>(ctx->d_xscratch, ctx->d_reach, ctx->d_bmax, cap, cond);
-    k_mscan_totals<<<1, 1024, 0, s>>>(ctx->d_bmax, nb, cond);
+    unsigned nt = nblk(cap, SCAN_TILE);
+    k_scan_tiles<int64_t, OpMax><<<nt, BLK, 0, s>>>(ctx->d_xscratch, ctx->d_reach, ctx->d_bmax, cap, nullptr, cond);
+    k_scan_top<OpMax><<<1, 1024, 0, s>>>(ctx->d_bmax, nt, nullptr, cond);
     k_reach_decode<<<nb, BLK, 0, s>>>(ctx->d_reach, ctx->d_bmax, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, cap, cond);
     k_zig_resolve<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->d_reach, ctx->d_nev,
                                      ctx->nscen, ctx->d_on, cond);
@@ -1014,15 +1062,19 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
                  cond, launches);
     enqueue_scan(ctx, ctx->d_on, ctx->d_C, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, ctx->d_nev, cond, launches);
     k_zig_compact<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_on, ctx->d_E, ctx->d_C, ctx->d_ev_begin,
-                                     ctx->nscen, ctx->d_nev, ctx->d_key, ctx->d_xtra, cond);
+                                     ctx->nscen, ctx->d_nev, ctx->d_key, ctx->d_xtra, ctx->d_ev_val, ctx->d_stream,
+                                     ctx->d_jt, ctx->d_zt, cond);
     launches += 7;
   } else {
     k_polar_keys<<<nblk(cap), BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, ctx->d_key, cond);
     launches += 1;
   }
   k_stream_events<<<nblk(ctx->nscen), BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_C, method,
-                                                   ctx->d_key, ctx->d_xtra, cond);
-  launches += 1;
+                                                   ctx->d_key, ctx->d_xtra, ctx->d_ev_val, ctx->d_bucket,
+                                                   ctx->d_bucket_base, ctx->bshift, cond);
+  k_ev_buckets<<<nblk(ctx->nbucket), BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_bucket_base, ctx->nbucket,
+                                                  ctx->d_bucket, cond);
+  launches += 2;
 }
 
 template <int DIM, int MODEL, bool COUNT>
@@ -1055,15 +1107,154 @@ static int upload_streams(cyl_ctx* ctx, int units_per_part) {
   }
   if (ctx->rng_dirty) {
     std::vector<JumpTable> jt(ctx->nscen);
-    for (int s = 0; s < ctx->nscen; ++s) make_jump_table(ctx->rng_inc[s], jt[s]);
+    std::vector<JumpTable8> jt8(ctx->nscen);
+    for (int s = 0; s < ctx->nscen; ++s) {
+      make_jump_table(ctx->rng_inc[s], jt[s]);
+      make_jump_table8(ctx->rng_inc[s], jt8[s]);
+    }
     HIPCHK(hipMemcpy(ctx->d_jt, jt.data(), ctx->nscen * sizeof(JumpTable), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_jt8, jt8.data(), ctx->nscen * sizeof(JumpTable8), hipMemcpyHostToDevice));
   }
   HIPCHK(hipMemcpy(ctx->d_stream, st.data(), ctx->nscen * sizeof(StreamDev), hipMemcpyHostToDevice));
   ctx->rng_dirty = false;
   return CYL_OK;
 }
 
-static int enqueue_sort(cyl_ctx* ctx, int& launches);
+// The fused 2D LSM-1 kernel covers the Euler scheme with diffusion models
+// 0-3 on 2D fields for one scenario whose rows are all initialized; every
+// other configuration runs the generic k_step.
+static bool fast_eligible(const cyl_ctx* ctx, const cyl_params* p, bool init_only) {
+  const FieldsDev& f = ctx->fields;
+  if (init_only || ctx->pending_init || ctx->nscen != 1 || ctx->dim != 2 || f.dim != 2) return false;
+  if (p->model != 1 || p->time_scheme != 1 || p->diffusion_model < 0 || p->diffusion_model > 3) return false;
+  if (f.npoints != ctx->mesh.nv || f.fid_ux < 0 || f.fid_uy < 0) return false;
+  if (p->diffusion_model >= 2 && (f.fid_turbeng < 0 || f.fid_dissip < 0)) return false;
+  return true;
+}
+
+static int ensure_fast(cyl_ctx* ctx) {
+  if (ctx->fast_ready) return CYL_OK;
+  const MeshDev& m = ctx->mesh;
+  ElemF* ef;
+  int4 *en, *te, *ed;
+  double2* xy;
+  HIPCHK(dalloc(ctx->fast_bufs, &ef, m.nt));
+  HIPCHK(dalloc(ctx->fast_bufs, &en, m.nt));
+  HIPCHK(dalloc(ctx->fast_bufs, &te, m.nt));
+  HIPCHK(dalloc(ctx->fast_bufs, &ed, m.ne));
+  HIPCHK(dalloc(ctx->fast_bufs, &xy, m.nv));
+  HIPCHK(dalloc(ctx->fast_bufs, &ctx->d_nodes, m.nv));
+  k_pack_elems<<<nblk(m.nt), BLK, 0, ctx->stream>>>(m, ef, en, te);
+  k_pack_edges<<<nblk(std::max(m.ne, m.nv)), BLK, 0, ctx->stream>>>(m, ed, xy);
+  HIPCHK(hipGetLastError());
+  ctx->fm.ef = ef;
+  ctx->fm.en = en;
+  ctx->fm.te = te;
+  ctx->fm.ed = ed;
+  ctx->fm.xy = xy;
+  ctx->fm.edge_labels = m.edge_labels;
+  ctx->fm.nt = m.nt;
+  ctx->fast_ready = true;
+  return CYL_OK;
+}
+
+static int launch_fast(cyl_ctx* ctx, const StepArgs& a, const cyl_params* p, hipEvent_t mid) {
+  if (ctx->start_cap < ctx->n) {
+    if (ctx->d_pos0) HIPCHK(hipFree(ctx->d_pos0));
+    if (ctx->d_state0) HIPCHK(hipFree(ctx->d_state0));
+    ctx->d_pos0 = nullptr;
+    ctx->d_state0 = nullptr;
+    ctx->start_cap = 0;
+    HIPCHK(hipMalloc(&ctx->d_pos0, (size_t)ctx->n * sizeof(int64_t)));
+    HIPCHK(hipMalloc(&ctx->d_state0, (size_t)ctx->n * sizeof(uint64_t)));
+    ctx->start_cap = ctx->n;
+  }
+  FastArgs f{};
+  f.fm = ctx->fm;
+  f.m = ctx->mesh;
+  f.nodes = ctx->d_nodes;
+  f.xA = a.A.x;
+  f.yA = a.A.y;
+  f.triA = a.A.tri;
+  f.xB = a.B.x;
+  f.yB = a.B.y;
+  f.triB = a.B.tri;
+  f.upB0 = a.B.up[0];
+  f.upB1 = a.B.up[1];
+  f.usB0 = a.B.us[0];
+  f.usB1 = a.B.us[1];
+  f.inactive = ctx->pool[0].inactive;
+  f.del_step = ctx->d_del_step;
+  f.rank = ctx->d_rank;
+  f.off = ctx->d_off;
+  f.pool = ctx->d_gpool;
+  f.n = ctx->n;
+  f.dt = a.dt;
+  f.sqrtdt = sqrt(a.dt);
+  f.diffusion_model = p->diffusion_model >= 2 ? 2 : p->diffusion_model;
+  f.diff_const = sqrt(2. * p->horizontal_diffusivity) * f.sqrtdt;  // diffusion_lsm1, update_state_lsm1.pyx:700
+  f.sigc = p->schmidt_number;
+  f.rng_method = p->rng_method;
+  f.boundary_conditions = p->boundary_conditions;
+  f.max_wall_rebounds = p->max_wall_rebounds;
+  f.bnd_statistics = p->bnd_statistics;
+  f.nopen = ctx->nopen;
+  f.damp = fmin(p->rebound_damping_coef, 1.0 - EPS_REBOUND);
+  f.strand = p->stranding_probability;
+  f.general = a.general;
+  f.guarded = a.guarded;
+  f.flag = a.flag;
+  f.stream = a.stream;
+  f.jt = a.jt;
+  f.jt8 = a.jt8;
+  f.zt = a.zt;
+  f.bnd_stats = a.bnd_stats;
+  f.n_deleted = a.n_deleted;
+  f.err = a.err;
+  f.stream_end = a.stream_end;
+  f.step_id = a.step_id;
+  f.pos0 = ctx->d_pos0;
+  f.state0 = ctx->d_state0;
+  // persistent grids: as many blocks as fit on the device at once, a multiple of 8
+  auto grid = [&](const void* fn) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    unsigned g = std::min<unsigned>(nblk(ctx->n), (unsigned)(ctx->ncu * per_cu));
+    return std::max<unsigned>(8, g / 8 * 8);
+  };
+  f.xi = ctx->d_xi;
+  f.nact = ctx->d_scen_active;
+  int m = p->rng_method;
+  if (!a.general) {
+    // uniform draw assignment: the step's normals in pool order, then the step
+    int64_t nch = ((int64_t)ctx->n + GEN_PER - 1) / GEN_PER;
+    auto ggrid = [&](const void* fn) {
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+      return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(nch), (int64_t)ctx->ncu * per_cu));
+    };
+    if (p->diffusion_model != 0) {
+      if (m == 1) k_gen_normals<1><<<ggrid((const void*)k_gen_normals<1>), BLK, 0, ctx->stream>>>(f);
+      else if (m == 2) k_gen_normals<2><<<ggrid((const void*)k_gen_normals<2>), BLK, 0, ctx->stream>>>(f);
+      else k_gen_normals<3><<<ggrid((const void*)k_gen_normals<3>), BLK, 0, ctx->stream>>>(f);
+    }
+    if (mid) HIPCHK(hipEventRecord(mid, ctx->stream));
+    if (m == 1) k_lsm1_2d<1, true><<<grid((const void*)k_lsm1_2d<1, true>), BLK, 0, ctx->stream>>>(f);
+    else if (m == 2) k_lsm1_2d<2, true><<<grid((const void*)k_lsm1_2d<2, true>), BLK, 0, ctx->stream>>>(f);
+    else k_lsm1_2d<3, true><<<grid((const void*)k_lsm1_2d<3, true>), BLK, 0, ctx->stream>>>(f);
+    return CYL_OK;
+  }
+  if (m == 1) k_draw_start<1><<<grid((const void*)k_draw_start<1>), BLK, 0, ctx->stream>>>(f);
+  else if (m == 2) k_draw_start<2><<<grid((const void*)k_draw_start<2>), BLK, 0, ctx->stream>>>(f);
+  else k_draw_start<3><<<grid((const void*)k_draw_start<3>), BLK, 0, ctx->stream>>>(f);
+  if (mid) HIPCHK(hipEventRecord(mid, ctx->stream));
+  if (m == 1) k_lsm1_2d<1, false><<<grid((const void*)k_lsm1_2d<1, false>), BLK, 0, ctx->stream>>>(f);
+  else if (m == 2) k_lsm1_2d<2, false><<<grid((const void*)k_lsm1_2d<2, false>), BLK, 0, ctx->stream>>>(f);
+  else k_lsm1_2d<3, false><<<grid((const void*)k_lsm1_2d<3, false>), BLK, 0, ctx->stream>>>(f);
+  return CYL_OK;
+}
+
+static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
 
 static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, double dt, int nsteps, int* bnd_out,
                      bool init_only) {
@@ -1103,6 +1294,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   a.dt = dt;
   a.stream = ctx->d_stream;
   a.jt = ctx->d_jt;
+  a.jt8 = ctx->d_jt8;
   a.zt = ctx->d_zt;
   a.nopen = std::max(ctx->nopen, 1);
   a.nopen_real = ctx->nopen;
@@ -1124,9 +1316,10 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   float kms = 0.f;
   int n_timed = 0;
   for (int step = 0; step < nsteps; ++step) {
+    bool fast = fast_eligible(ctx, p, init_only);
     bool do_sort = !init_only && ctx->sort_interval > 0 && ctx->steps_since_sort >= ctx->sort_interval;
     if (do_sort) {
-      e = enqueue_sort(ctx, launches);
+      e = enqueue_sort(ctx, !fast, launches);
       if (e) return e;
       ctx->steps_since_sort = 0;
     }
@@ -1134,7 +1327,26 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       e = enqueue_time_interp(ctx, t, launches);
       if (e) return e;
     }
+    // per-slot arrays (their buffers are swapped by the sort)
+    a.init_tri = ctx->d_init_tri;
+    a.del_step = ctx->d_del_step;
+    a.depth = ctx->dim == 3 ? ctx->d_depth : nullptr;
+    a.gpool = ctx->d_gpool;
+    a.slot.pool = ctx->d_gpool;
+    a.slot.rank = ctx->d_rank;
+    a.slot.diameter = ctx->d_diam;
+    a.slot.volume = ctx->d_vol;
     a.f = ctx->fields;
+    if (fast) {
+      e = ensure_fast(ctx);
+      if (e) return e;
+      const FieldsDev& fd = ctx->fields;
+      bool keps = p->diffusion_model >= 2;
+      k_pack_nodes<<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(
+          fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], keps ? fd.tmp[fd.fid_turbeng] : nullptr,
+          keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / p->schmidt_number, fd.npoints, ctx->d_nodes);
+      ++launches;
+    }
     a.A = ctx->pool[ctx->cur];
     a.B = ctx->pool[1 - ctx->cur];
     a.step_id = ctx->step_counter;
@@ -1150,14 +1362,20 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       enqueue_events(ctx, p->rng_method, one, launches);
       a.general = 0;
       a.guarded = 0;
-      while (ctx->kev.size() < (size_t)2 * (step + 1)) {
+      while (ctx->kev.size() < (size_t)3 * (step + 1)) {
         hipEvent_t ev;
         HIPCHK(hipEventCreate(&ev));
         ctx->kev.push_back(ev);
       }
-      HIPCHK(hipEventRecord(ctx->kev[2 * step], ctx->stream));
-      launch_step(ctx, a, p->model, false);
-      HIPCHK(hipEventRecord(ctx->kev[2 * step + 1], ctx->stream));
+      HIPCHK(hipEventRecord(ctx->kev[3 * step], ctx->stream));
+      if (fast) {
+        e = launch_fast(ctx, a, p, ctx->kev[3 * step + 1]);
+        if (e) return e;
+      } else {
+        HIPCHK(hipEventRecord(ctx->kev[3 * step + 1], ctx->stream));
+        launch_step(ctx, a, p->model, false);
+      }
+      HIPCHK(hipEventRecord(ctx->kev[3 * step + 2], ctx->stream));
       ++launches;
       ++n_timed;
     } else {
@@ -1167,6 +1385,8 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     int* g = ctx->d_flag;
     k_rerun_reset<<<nblk(std::max(nstat, ctx->nscen)), BLK, 0, ctx->stream>>>(
         ctx->d_stream_end, ctx->d_err_step, a.bnd_stats, nstat, ctx->nscen, ctx->d_n_deleted, g);
+    k_undelete<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->pool[0].inactive, ctx->d_del_step, ctx->n,
+                                                      ctx->step_counter, g);
     a.general = 0;
     a.guarded = 1;
     launch_step(ctx, a, p->model, true);
@@ -1180,7 +1400,12 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     enqueue_events(ctx, p->rng_method, g, launches);
     a.general = 1;
     a.guarded = 1;
-    launch_step(ctx, a, p->model, false);
+    if (fast) {
+      e = launch_fast(ctx, a, p, nullptr);
+      if (e) return e;
+    } else {
+      launch_step(ctx, a, p->model, false);
+    }
     ++launches;
     if (step == nsteps - 1) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     k_finish<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_stream_end, ctx->nscen,
@@ -1199,13 +1424,16 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   if (nsteps > 0) HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
   ctx->last_kernel_ms = kms;
   ctx->last_launches = launches;
-  double sk = 0.;
+  double sk = 0., sp = 0.;
   for (int q = 0; q < n_timed; ++q) {
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, ctx->kev[2 * q], ctx->kev[2 * q + 1]));
+    float ms = 0.f, mp = 0.f;
+    HIPCHK(hipEventElapsedTime(&mp, ctx->kev[3 * q], ctx->kev[3 * q + 1]));
+    HIPCHK(hipEventElapsedTime(&ms, ctx->kev[3 * q + 1], ctx->kev[3 * q + 2]));
     sk += ms;
+    sp += mp;
   }
   ctx->last_step_kernel_ms = sk;
+  ctx->last_prep_ms = sp;
   ctx->last_step_launches = n_timed;
   if (bnd_out && p->bnd_statistics && ctx->nopen > 0) {
     std::vector<int> h(need_bnd);
@@ -1234,8 +1462,9 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
                                    "random-stream window too small",
                                    "draw-count misprediction",
                                    "neighbourhood search needs triangle_adjacency",
-                                   "LSM3 is not available in the device step"};
-      std::string m = code > 0 && code <= 10 ? msgs[code] : "device error";
+                                   "LSM3 is not available in the device step",
+                                   "random-stream event table inconsistent"};
+      std::string m = code > 0 && code <= 11 ? msgs[code] : "device error";
       ctx->errcode = code;
       return ctx->set_err(CYL_E_STEP, pidx, m);
     }
@@ -1273,67 +1502,87 @@ extern "C" int cyl_last_timing(cyl_ctx* ctx, double* kms, double* tms, int* laun
   return CYL_OK;
 }
 
-// Counting sort of the slots by element, then permutation of both state
-// buffers' current copy and of every per-slot array.
-static int enqueue_sort(cyl_ctx* ctx, int& launches) {
+extern "C" int cyl_last_prep_timing(cyl_ctx* ctx, double* prep_ms) {
+  if (!ctx || !prep_ms) return CYL_E_ARG;
+  *prep_ms = ctx->last_prep_ms;
+  return CYL_OK;
+}
+
+// Counting sort of the slots by element, then one gather pass that moves the
+// current state copy and every per-slot array into the spare buffers, whose
+// pointers are then swapped in.  The fluid velocity seen is carried only when
+// the next step reads it (the fused 2D LSM-1 kernel rewrites it).  Active ranks
+// are pool-order quantities and move with their slots.
+static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
   int n = ctx->n, nt = ctx->mesh.nt;
   hipStream_t s = ctx->stream;
-  if (!ctx->d_scnt) {
-    HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scnt, nt + 3));
-    HIPCHK(dalloc(ctx->part_bufs, &ctx->d_cursor, nt + 3));
+  Bufs& bb = ctx->part_bufs;
+  int nbin = (nt >> SORT_SHIFT) + 3;
+  if (!ctx->d_scnt || ctx->sort_nt != nt) {
+    HIPCHK(dalloc(bb, &ctx->d_scnt, nbin));
+    HIPCHK(dalloc(bb, &ctx->d_cursor, nbin));
+    HIPCHK(dalloc(bb, &ctx->d_sort_tiles, nblk(nbin, SCAN_TILE) + 1));
+    ctx->sort_nt = nt;
+  }
+  if (!ctx->sp_gpool) {
+    HIPCHK(dalloc(bb, &ctx->sp_gpool, n));
+    HIPCHK(dalloc(bb, &ctx->sp_init_tri, n));
+    HIPCHK(dalloc(bb, &ctx->sp_del_step, n));
+    HIPCHK(dalloc(bb, &ctx->sp_inactive, n));
+    HIPCHK(dalloc(bb, &ctx->sp_rank, n));
+    HIPCHK(dalloc(bb, &ctx->sp_diam, n));
+    HIPCHK(dalloc(bb, &ctx->sp_vol, n));
+    HIPCHK(dalloc(bb, &ctx->sp_depth, ctx->dim == 3 ? n : 1));
   }
   PoolDev& A = ctx->pool[ctx->cur];
   PoolDev& B = ctx->pool[1 - ctx->cur];
-  HIPCHK(hipMemsetAsync(ctx->d_scnt, 0, (nt + 3) * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(ctx->d_scnt, 0, nbin * sizeof(int), s));
   k_sort_hist<<<nblk(n), BLK, 0, s>>>(A.tri, A.inactive, n, nt, ctx->d_scnt, ctx->d_skey);
-  enqueue_scan(ctx, ctx->d_scnt, ctx->d_cursor, (int64_t)nt + 3, ctx->d_bsum, ctx->d_total,
+  enqueue_scan(ctx, ctx->d_scnt, ctx->d_cursor, (int64_t)nbin, ctx->d_sort_tiles, ctx->d_total,
                (const int64_t*)nullptr, (const int*)nullptr, launches);
   k_sort_place<<<nblk(n), BLK, 0, s>>>(ctx->d_skey, n, ctx->d_cursor, ctx->d_perm);
-  launches += 2;
-  // state: A -> B via perm, then swap so the sorted copy is current
-  auto pd = [&](double* src, double* dst) { k_permute<double><<<nblk(n), BLK, 0, s>>>(src, dst, ctx->d_perm, n); };
-  auto pi = [&](int* src, int* dst) { k_permute<int><<<nblk(n), BLK, 0, s>>>(src, dst, ctx->d_perm, n); };
+  PermArgs pa{};
+  pa.perm = ctx->d_perm;
+  pa.n = n;
+  auto pd = [&](const double* src, double* dst) {
+    pa.dsrc[pa.nd] = src;
+    pa.ddst[pa.nd++] = dst;
+  };
+  auto pi = [&](const int* src, int* dst) {
+    pa.isrc[pa.ni] = src;
+    pa.idst[pa.ni++] = dst;
+  };
   pd(A.x, B.x);
   pd(A.y, B.y);
   if (ctx->dim == 3) pd(A.z, B.z);
   for (int c = 0; c < ctx->dim; ++c) {
     pd(A.up[c], B.up[c]);
-    pd(A.us[c], B.us[c]);
+    if (need_us) pd(A.us[c], B.us[c]);
   }
+  pd(ctx->d_diam, ctx->sp_diam);
+  pd(ctx->d_vol, ctx->sp_vol);
+  if (ctx->dim == 3) pd(ctx->d_depth, ctx->sp_depth);
   pi(A.tri, B.tri);
-  pi(A.ll, B.ll);
-  pi(A.inactive, B.inactive);
-  launches += 5 + 2 * ctx->dim;
+  if (ctx->dim == 3) pi(A.ll, B.ll);
+  pi(ctx->d_gpool, ctx->sp_gpool);
+  pi(ctx->d_init_tri, ctx->sp_init_tri);
+  pi(ctx->d_del_step, ctx->sp_del_step);
+  pi(ctx->pool[0].inactive, ctx->sp_inactive);
+  pi(ctx->d_rank, ctx->sp_rank);  // a slot's active rank moves with it
+  if (ctx->d_scen) return ctx->set_err(CYL_E_STATE, -1, "element sort of a batched pool is not supported");
+  k_permute_all<<<nblk(n), BLK, 0, s>>>(pa);
+  launches += 3;
+  HIPCHK(hipGetLastError());
   ctx->cur = 1 - ctx->cur;
-  // per-slot arrays through the scratch buffer
-  int* ti = (int*)ctx->d_tmpbuf;
-  double* td = (double*)ctx->d_tmpbuf;
-  int* islots[] = {ctx->d_gpool, ctx->d_init_tri, ctx->d_del_step};
-  for (int* arr : islots) {
-    pi(arr, ti);
-    HIPCHK(hipMemcpyAsync(arr, ti, n * sizeof(int), hipMemcpyDeviceToDevice, s));
-    ++launches;
-  }
-  if (ctx->d_scen) {
-    pi(ctx->d_scen, ti);
-    HIPCHK(hipMemcpyAsync(ctx->d_scen, ti, n * sizeof(int), hipMemcpyDeviceToDevice, s));
-    ++launches;
-  }
-  double* dslots[] = {ctx->d_diam, ctx->d_vol};
-  for (double* arr : dslots) {
-    pd(arr, td);
-    HIPCHK(hipMemcpyAsync(arr, td, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    ++launches;
-  }
-  if (ctx->dim == 3) {
-    pd(ctx->d_depth, td);
-    HIPCHK(hipMemcpyAsync(ctx->d_depth, td, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    ++launches;
-  }
-  // ranks follow their rows: recompute
-  int one = 1;
-  HIPCHK(hipMemcpyAsync(ctx->d_flag + 1, &one, sizeof(int), hipMemcpyHostToDevice, s));
-  HIPCHK(hipStreamSynchronize(s));
+  std::swap(ctx->d_gpool, ctx->sp_gpool);
+  std::swap(ctx->d_init_tri, ctx->sp_init_tri);
+  std::swap(ctx->d_del_step, ctx->sp_del_step);
+  std::swap(ctx->d_diam, ctx->sp_diam);
+  std::swap(ctx->d_vol, ctx->sp_vol);
+  std::swap(ctx->d_depth, ctx->sp_depth);
+  std::swap(ctx->pool[0].inactive, ctx->sp_inactive);
+  std::swap(ctx->d_rank, ctx->sp_rank);
+  ctx->pool[1].inactive = ctx->pool[0].inactive;
   ctx->have_last = false;
   return CYL_OK;
 }

@@ -1,0 +1,510 @@
+// Fused 2D LSM-1 step (Euler advection + Euler-Maruyama diffusion + walk +
+// rebounds) on packed, cache-line-sized mesh records.  This is the common
+// production configuration; every other configuration runs k_step.
+//
+// Packed layout in HBM:
+//   ElemF  (128 B, one L2 line): P1 coefficients det/gradx/grady and the three
+//          vertex coordinates of an element
+//   int4   element node ids, element edge ids (triedge order), edge records
+//          {vertex0, vertex1, element0, element1}
+//   NodeF  (32 B): time-interpolated u, v and nodal k-epsilon diffusivity of a
+//          node, rewritten once per step by k_pack_nodes
+// The arithmetic is the same, operation for operation, as update_state_lsm1.pyx
+// (euler_lsm1 :135-177, diffusion_lsm1 :658-726), xylocalizer.pyx:338-435 and
+// compute_bc.pyx:35-395 for model 1.
+#pragma once
+#include "cyl_kernels.h"
+
+namespace cyl {
+
+struct __align__(128) ElemF {
+  double det[3], gx[3], gy[3];
+  double vx[3], vy[3];
+  double pad;
+};
+
+struct __align__(32) NodeF {
+  double u, v, K, pad;
+};
+
+struct FastMesh {
+  const ElemF* ef;
+  const int4* en;    // node ids
+  const int4* te;    // edge ids of the element, increasing
+  const int4* ed;    // edge: v0, v1, t0, t1
+  const double2* xy; // node coordinates
+  const int* edge_labels;
+  int nt;
+};
+
+// tmp -> packed node records; K = (CMU/sigc) k^2 / max(eps, EPSILON) per node
+// (eulerian_field_set.pyx:1995-1997), the value every particle would compute
+__global__ void __launch_bounds__(BLK) k_pack_nodes(const double* u, const double* v, const double* kt,
+                                                    const double* ep, double fac, int n, NodeF* out) {
+  int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  NodeF r;
+  r.u = u[i];
+  r.v = v[i];
+  r.K = kt ? fac * kt[i] * kt[i] / fmax(ep[i], EPS) : 0.;
+  r.pad = 0.;
+  out[i] = r;
+}
+
+// element records from the mesh tables (built once per mesh)
+__global__ void __launch_bounds__(BLK) k_pack_elems(MeshDev m, ElemF* ef, int4* en, int4* te) {
+  int k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= m.nt) return;
+  ElemF r;
+  for (int i = 0; i < 3; ++i) {
+    int v = m.tri[3 * k + i];
+    r.det[i] = m.det[3 * k + i];
+    r.gx[i] = m.gx[3 * k + i];
+    r.gy[i] = m.gy[3 * k + i];
+    r.vx[i] = m.x[v];
+    r.vy[i] = m.y[v];
+  }
+  r.pad = 0.;
+  ef[k] = r;
+  en[k] = make_int4(m.tri[3 * k], m.tri[3 * k + 1], m.tri[3 * k + 2], 0);
+  te[k] = make_int4(m.triedge[3 * k], m.triedge[3 * k + 1], m.triedge[3 * k + 2], 0);
+}
+
+__global__ void __launch_bounds__(BLK) k_pack_edges(MeshDev m, int4* ed, double2* xy) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j < m.ne) ed[j] = make_int4(m.edges[2 * j], m.edges[2 * j + 1], m.edgetri[2 * j], m.edgetri[2 * j + 1]);
+  if (j < m.nv) xy[j] = make_double2(m.x[j], m.y[j]);
+}
+
+__device__ __forceinline__ bool in_elem(const ElemF& e, double x, double y) {
+  // point_in_triangle_dotp (utils.pyx:359-385) on the element's vertices
+  return pintri(x, y, e.vx[0], e.vy[0], e.vx[1], e.vy[1], e.vx[2], e.vy[2]) >= 0 &&
+         pintri(x, y, e.vx[1], e.vy[1], e.vx[2], e.vy[2], e.vx[0], e.vy[0]) >= 0 &&
+         pintri(x, y, e.vx[2], e.vy[2], e.vx[0], e.vy[0], e.vx[1], e.vy[1]) >= 0;
+}
+
+__device__ __forceinline__ int in_tri_fast(const FastMesh& fm, int k, double x, double y) {
+  int kk = k < 0 ? k + fm.nt : k;
+  return in_elem(fm.ef[kk], x, y) ? k : -1;
+}
+
+// xy_localize_point_from_path on the packed tables (xylocalizer.pyx:338-435)
+__device__ inline int walk_fast(const FastMesh& fm, int k, double p0x, double p0y, double x, double y,
+                                int& bnd_j) {
+  int new_k = -1, niter = 0, last_edge = -1;
+  bnd_j = -1;
+  while (new_k == -1 && niter <= 50) {
+    bool hit = false;
+    int kk = k < 0 ? k + fm.nt : k;
+    int4 te = fm.te[kk];
+    int tev[3] = {te.x, te.y, te.z};
+    for (int i = 0; i < 3; ++i) {
+      int j = tev[i];
+      if (j == last_edge) continue;
+      int4 ed = fm.ed[j];
+      double2 a = fm.xy[ed.x], b = fm.xy[ed.y];
+      if (seg_intersect(p0x, p0y, x, y, a.x, a.y, b.x, b.y)) {
+        hit = true;
+        int kn = (ed.z == k) ? ed.w : ed.z;
+        if (kn == -1) {
+          bnd_j = j;
+          return -1;
+        }
+        new_k = in_tri_fast(fm, kn, x, y);
+        if (new_k != -1) return new_k;
+        k = kn;
+        last_edge = j;
+        break;
+      }
+    }
+    if (!hit) return in_tri_fast(fm, k, x, y) == -1 ? -1 : k;
+    ++niter;
+  }
+  return new_k;
+}
+
+// xy_localize with method 3 (xylocalizer.pyx:65-120)
+__device__ __forceinline__ int localize_fast(const FastMesh& fm, const MeshDev& m, int k, double p0x, double p0y,
+                                             double x, double y, int& bnd_j) {
+  bnd_j = -1;
+  int nk = in_tri_fast(fm, k, x, y);
+  if (nk != -1) return nk;
+  nk = walk_fast(fm, k, p0x, p0y, x, y, bnd_j);
+  if (nk == -1 && bnd_j == -1) nk = locate_grid(m, x, y);
+  return nk;
+}
+
+struct FastArgs {
+  FastMesh fm;
+  MeshDev m;               // grid for the fallback search
+  const NodeF* nodes;
+  const double* xA;
+  const double* yA;
+  const int* triA;
+  double* xB;
+  double* yB;
+  int* triB;
+  double* upB0;
+  double* upB1;
+  double* usB0;
+  double* usB1;
+  int* inactive;
+  int* del_step;
+  const int* rank;
+  const int64_t* off;
+  const int* pool;
+  int n;
+  double dt, sqrtdt;
+  int diffusion_model;     // 0, 1 (constant), 2 (k-epsilon)
+  double diff_const;       // sqrt(2*nux)*sqrtdt for model 1
+  double sigc;
+  int rng_method;
+  int boundary_conditions, max_wall_rebounds, bnd_statistics, nopen;
+  double damp, strand;
+  int general, guarded;
+  int* flag;
+  StreamDev* stream;
+  const JumpTable* jt;
+  const JumpTable8* jt8;
+  const ZigTables* zt;
+  int* bnd_stats;
+  int* n_deleted;
+  unsigned long long* err;
+  unsigned long long* stream_end;
+  int step_id;
+  int64_t* pos0;      // per slot: stream position of the first draw
+  uint64_t* state0;   // per slot: generator state at that position
+  double2* xi;        // per active rank: the step's two normals (pre-generated)
+  const int64_t* nact;  // active rows (device)
+};
+
+
+// Grid-stride walk over the slots for a persistent grid (a multiple of 8
+// blocks).  Blocks are dispatched round-robin over the 8 XCDs, so block b
+// takes the chunks of the (b % 8)-th eighth of the slots: neighbouring slots
+// (neighbouring elements once sorted) share one XCD's L2.
+template <class F>
+__device__ __forceinline__ void for_slots_xcd(int n, F&& f) {
+  const int nx = 8;
+  int xcd = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
+  int64_t nch = ((int64_t)n + BLK - 1) / BLK;
+  int64_t cpx = (nch + nx - 1) / nx;
+  for (int64_t c = lb; c < cpx; c += per) {
+    int64_t chunk = xcd * cpx + c;
+    int64_t j = chunk * BLK + threadIdx.x;
+    if (chunk < nch && j < n) f((int)j);
+  }
+}
+
+__device__ __forceinline__ void load_jump8(JumpTable8& t8, const JumpTable8* src) {
+  const uint64_t* s = reinterpret_cast<const uint64_t*>(src);
+  uint64_t* d = reinterpret_cast<uint64_t*>(&t8);
+  for (int q = threadIdx.x; q < (int)(sizeof(JumpTable8) / 8); q += BLK) d[q] = s[q];
+}
+
+// First draw of every active slot: the serial stream position of its first
+// unit (uniform: units_per_part * active rank; general: pool-order scan) and
+// the generator state there.  Lookups of neighbouring slots are independent,
+// so this small kernel runs at high occupancy and hides their latency.
+template <int RNG>
+__global__ void __launch_bounds__(BLK) k_draw_start(FastArgs a) {
+  if (a.guarded && a.flag[0] == 0) return;
+  __shared__ JumpTable8 t8;
+  load_jump8(t8, a.jt8);
+  __syncthreads();
+  const StreamDev& st = a.stream[0];
+  for_slots_xcd(a.n, [&](int j) {
+    if (a.inactive[j] != 0) return;
+    int64_t u = a.general ? a.off[j] : (int64_t)st.units_per_part * a.rank[j];
+    Draws d;
+    d.st = &st;
+    int64_t pos;
+    if (RNG == 1) pos = u;
+    else if (RNG == 2) pos = d.zig_position(u);
+    else pos = d.polar_position(u);
+    // a position past the scanned window is reported if the slot draws (-1)
+    a.pos0[j] = (RNG != 1 && pos >= st.window) ? -1 : pos;
+    a.state0[j] = pcg_jump8(st.base, (uint64_t)pos, t8, *a.jt);
+  });
+}
+
+// Normals of the step in pool order: thread c serves the active ranks
+// [GEN_PER c, GEN_PER (c+1)), i.e. a contiguous run of the serial stream.  It
+// finds the run's first position once (event table), jumps there, then draws
+// sequentially: a Ziggurat normal that starts on a slow event takes the
+// event's precomputed value and skips its positions; every other normal is a
+// one-draw fast normal.  The pair of rank r = GEN_PER c + k is stored at
+// xi[k * nchunk + c] so that each of the thread's stores is coalesced across
+// the wavefront (see xi_index).
+constexpr int GEN_PER = 32;
+
+__device__ __forceinline__ int64_t xi_index(int64_t r, int64_t nact) {
+  int64_t nchunk = (nact + GEN_PER - 1) / GEN_PER;
+  return (r % GEN_PER) * nchunk + r / GEN_PER;
+}
+
+template <int RNG>
+__global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
+  __shared__ JumpTable8 t8;
+  __shared__ ZigTables zt;
+  load_jump8(t8, a.jt8);
+  if (RNG == 2) {
+    for (int q = threadIdx.x; q < 128; q += BLK) {
+      zt.kn[q] = a.zt->kn[q];
+      zt.wn[q] = a.zt->wn[q];
+      zt.fn[q] = a.zt->fn[q];
+    }
+  }
+  __syncthreads();
+  const StreamDev& st = a.stream[0];
+  int64_t nact = *a.nact;
+  int64_t nchunk = (nact + GEN_PER - 1) / GEN_PER;
+  const int64_t NONE = 0x7fffffffffffffffLL;
+  unsigned long long emax = 0;
+  for (int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x; c < nchunk; c += (int64_t)gridDim.x * BLK) {
+    int64_t r0 = c * GEN_PER, r1 = r0 + GEN_PER < nact ? r0 + GEN_PER : nact;
+    int64_t u = r0 * st.units_per_part;
+    Draws d;
+    d.st = &st;
+    int64_t pos, ev = 0, next_key = NONE;
+    if (RNG == 1) {
+      pos = u;
+    } else if (RNG == 2) {
+      ev = d.events_before(u);
+      pos = u + (ev > 0 ? st.ev_extra[ev - 1] : 0);
+      next_key = ev < st.n_ev ? st.ev_key[ev] : NONE;
+    } else {
+      pos = 2 * (u + d.events_before(u));
+    }
+    int err = (RNG != 1 && pos >= st.window) ? E_RNG_WINDOW : 0;
+    Pcg g;
+    g.inc = st.inc;
+    g.state = pcg_jump8(st.base, (uint64_t)pos, t8, *a.jt);
+    for (int64_t r = r0; r < r1; ++r) {
+      double v[2];
+      if (RNG == 1) {
+        v[0] = rect_normal(g);
+        v[1] = rect_normal(g);
+        pos += 2;
+      } else if (RNG == 2) {
+        for (int k = 0; k < 2; ++k, ++u) {
+          if (pos >= st.window) err = E_RNG_WINDOW;
+          if (u == next_key) {
+            int64_t len = st.ev_extra[ev] - (ev > 0 ? st.ev_extra[ev - 1] : 0) + 1;
+            v[k] = st.ev_val[ev];
+            for (int64_t q = 0; q < len; ++q) pcg_next(g);
+            pos += len;
+            ++ev;
+            next_key = ev < st.n_ev ? st.ev_key[ev] : NONE;
+          } else {
+            int32_t hz = (int32_t)pcg_next(g);
+            uint32_t iz = (uint32_t)hz & 127u;
+            uint32_t ahz = hz < 0 ? (uint32_t)(-(int64_t)hz) : (uint32_t)hz;
+            if (ahz >= zt.kn[iz]) err = E_RNG_EVENTS;  // a slow start missing from the table
+            v[k] = hz * zt.wn[iz];
+            pos += 1;
+          }
+        }
+      } else {
+        double x1 = 0., x2 = 0., w = 2.0;
+        while (w >= 1.0 || w == 0.0) {
+          x1 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
+          x2 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
+          w = x1 * x1 + x2 * x2;
+          pos += 2;
+        }
+        w = sqrt((-2.0 * log(w)) / w);
+        v[0] = x1 * w;
+        v[1] = x2 * w;
+      }
+      a.xi[(r - r0) * nchunk + c] = make_double2(v[0], v[1]);
+    }
+    // reported against pool index 0: the rank's slot is not known here
+    if (err) atomicMin(&a.err[0], (unsigned long long)(unsigned)err);
+    emax = (unsigned long long)pos > emax ? (unsigned long long)pos : emax;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long q = __shfl_xor(emax, o);
+    emax = q > emax ? q : emax;
+  }
+  if ((threadIdx.x & 63) == 0 && emax) atomicMax(&a.stream_end[0], emax);
+}
+
+// One particle of the step; returns the stream position after its last draw
+// (0 if it drew nothing or its normals were generated beforehand).
+// PREGEN: the step's two normals of the slot are xi[rank] (uniform draw
+// assignment); otherwise they are sampled here from the prepared start.
+template <int RNG, bool PREGEN>
+__device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, int j, const ZigTables& zt) {
+  if (a.inactive[j] != 0) return 0;
+  const StreamDev& st = a.stream[0];
+  int tri0 = a.triA[j];
+  double x0 = a.xA[j], y0 = a.yA[j];
+  if (tri0 < 0) {
+    // the particle has no element: the reference raises in the velocity lookup
+    atomicMin(&a.err[0], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)E_NOT_IN_TRI);
+    a.xB[j] = x0;
+    a.yB[j] = y0;
+    a.triB[j] = tri0;
+    return 0;
+  }
+  // the pair of normals of this slot's active rank (load issued early)
+  double2 xi = make_double2(0., 0.);
+  if (PREGEN && a.diffusion_model != 0) xi = a.xi[xi_index(a.rank[j], *a.nact)];
+  const ElemF& ef = a.fm.ef[tri0];
+  int4 en = a.fm.en[tri0];
+  // P1 factors at the start position (eulerian_field_set.pyx:1103-1105)
+  double f0 = ef.det[0] + x0 * ef.gx[0] + y0 * ef.gy[0];
+  double f1 = ef.det[1] + x0 * ef.gx[1] + y0 * ef.gy[1];
+  double f2 = ef.det[2] + x0 * ef.gx[2] + y0 * ef.gy[2];
+  NodeF n0 = a.nodes[en.x], n1 = a.nodes[en.y], n2 = a.nodes[en.z];
+  double ux = n0.u * f0 + n1.u * f1 + n2.u * f2;
+  double uy = n0.v * f0 + n1.v * f1 + n2.v * f2;
+  // euler_lsm1
+  double x = x0 + ux * a.dt;
+  double y = y0 + uy * a.dt;
+  Draws dr;
+  dr.st = &st;
+  dr.jt = a.jt;
+  dr.jt8 = nullptr;
+  dr.pre_pos = PREGEN ? nullptr : a.pos0;
+  dr.pre_state = a.state0;
+  dr.slot = j;
+  dr.zt = &zt;
+  dr.method = RNG;
+  dr.dry = false;
+  dr.nunits = 0;
+  dr.ncalls = 0;
+  dr.pos0 = 0;
+  dr.live = false;
+  dr.err = 0;
+  dr.unit0 = 0;
+  // diffusion_lsm1 with K and grad K at the start-of-step element
+  if (a.diffusion_model != 0) {
+    double diffx, driftx = 0., drifty = 0.;
+    if (a.diffusion_model == 2) {
+      double K = n0.K * f0 + n1.K * f1 + n2.K * f2;
+      double dkdx = n0.K * ef.gx[0] + n1.K * ef.gx[1] + n2.K * ef.gx[2];
+      double dkdy = n0.K * ef.gy[0] + n1.K * ef.gy[1] + n2.K * ef.gy[2];
+      diffx = sqrt(2. * fmax(0., K)) * a.sqrtdt;
+      driftx = dkdx * a.dt;
+      drifty = dkdy * a.dt;
+    } else {
+      diffx = a.diff_const;
+    }
+    double xix = 0., xiy = 0.;
+    if (diffx != 0.) {
+      if (PREGEN) {
+        xix = xi.x;
+        xiy = xi.y;
+        dr.nunits = RNG == 3 ? 1 : 2;  // one pair call (polar) or two normals
+      } else {
+        dr.pair(xix, xiy);
+      }
+    }
+    x += driftx + diffx * xix;
+    y += drifty + diffx * xiy;
+  }
+  // localization from (tri0, x0) along the path (update_state_lsm1.pyx:113-124):
+  // xy_localize method 3 after its point-in-triangle test of tri0
+  int bnd_j = -1, tri = tri0;
+  if (!in_elem(ef, x, y)) {
+    tri = walk_fast(a.fm, tri0, x0, y0, x, y, bnd_j);
+    if (tri == -1 && bnd_j == -1) tri = locate_grid(a.m, x, y);
+  }
+  bool deleted = false;
+  if (a.boundary_conditions) {
+    if (bnd_j != -1) {
+      int lab = a.fm.edge_labels[bnd_j];
+      int strand_cond = 0;
+      if (lab == 0 && a.strand > 0.) {
+        BcDraws bd;
+        bd.method = RNG;
+        bd.zt = &zt;
+        bd.g.inc = 0x6c62272e07bb0143ULL | 1u;
+        bd.g.state = st.base ^ ((uint64_t)(unsigned)a.pool[j] * 0x9e3779b97f4a7c15ULL);
+        strand_cond = bd.uniform() < a.strand;
+      }
+      if (lab == 0 && !strand_cond) {
+        int bjj = bnd_j, rc = 0;
+        while (tri == PART_LOC_O && rc < a.max_wall_rebounds) {
+          int4 ed = a.fm.ed[bjj];
+          double2 v0 = a.fm.xy[ed.x], v1 = a.fm.xy[ed.y];
+          // compute_symetric_point2d, intersections.pyx:27-62
+          double nx = v0.y - v1.y, ny = v1.x - v0.x;
+          double norm2 = nx * nx + ny * ny;
+          double dist = (x - v0.x) * nx + (y - v0.y) * ny;
+          double coef = 1. - a.damp / 2.;
+          double px = x - 2. * ((dist * coef) / norm2) * nx;
+          double py = y - 2. * ((dist * coef) / norm2) * ny;
+          x = px;
+          y = py;
+          tri = localize_fast(a.fm, a.m, tri0, x0, y0, px, py, bjj);
+          if (tri == PART_LOC_O && bjj != -1 && a.fm.edge_labels[bjj] >= 1) {
+            int l2 = a.fm.edge_labels[bjj];
+            if (a.bnd_statistics && a.bnd_stats && l2 - 1 < a.nopen) atomicAdd(&a.bnd_stats[l2 - 1], 1);
+            deleted = true;
+            break;
+          }
+          ++rc;
+          if (tri == PART_LOC_O && bjj == -1) break;
+        }
+      }
+      if (lab >= 1 && !deleted) {
+        deleted = true;
+        if (a.bnd_statistics && a.bnd_stats && lab - 1 < a.nopen) atomicAdd(&a.bnd_stats[lab - 1], 1);
+      }
+    }
+    if (!deleted && tri == PART_LOC_O) deleted = true;
+  } else if (tri == PART_LOC_O) {
+    deleted = true;
+  }
+  if (dr.err) atomicMin(&a.err[0], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)dr.err);
+  if (!a.general && dr.nunits != st.units_per_part) atomicOr(&a.flag[0], 1);
+  if (deleted) {
+    a.inactive[j] = 2;
+    a.del_step[j] = a.step_id;
+    atomicAdd(a.n_deleted, 1);
+    x = 0.;
+    y = 0.;
+    tri = -1;
+    ux = 0.;
+    uy = 0.;
+  }
+  a.xB[j] = x;
+  a.yB[j] = y;
+  a.triB[j] = tri;
+  a.upB0[j] = ux;
+  a.upB1[j] = uy;
+  a.usB0[j] = ux;
+  a.usB1[j] = uy;
+  return dr.live ? (unsigned long long)(dr.pos0 + dr.ncalls) : 0ull;
+}
+
+// Persistent launch over the slots (see for_slots_xcd); the Ziggurat tables
+// are staged in LDS once per block.
+template <int RNG, bool PREGEN>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_lsm1_2d(FastArgs a) {
+  if (a.guarded && a.flag[0] == 0) return;
+  __shared__ ZigTables zt;
+  if (RNG == 2 && !PREGEN) {
+    for (int q = threadIdx.x; q < 128; q += BLK) {
+      zt.kn[q] = a.zt->kn[q];
+      zt.wn[q] = a.zt->wn[q];
+      zt.fn[q] = a.zt->fn[q];
+    }
+  }
+  __syncthreads();
+  unsigned long long e = 0;
+  for_slots_xcd(a.n, [&](int j) {
+    unsigned long long q = lsm1_particle<RNG, PREGEN>(a, j, zt);
+    e = q > e ? q : e;
+  });
+  for (int o = 32; o > 0; o >>= 1) {
+    unsigned long long q = __shfl_xor(e, o);
+    e = q > e ? q : e;
+  }
+  if ((threadIdx.x & 63) == 0 && e) atomicMax(&a.stream_end[0], e);
+}
+
+}  // namespace cyl

@@ -31,6 +31,7 @@ struct StepArgs {
   int step_id;
   StreamDev* stream;  // device copy, one per scenario
   const JumpTable* jt;
+  const JumpTable8* jt8;
   const ZigTables* zt;
   int* bnd_stats;     // [nscen][nopen] of this step
   int nopen;       // stride of bnd_stats
@@ -67,6 +68,7 @@ __device__ __forceinline__ void load_part(const PoolDev& A, const SlotDev& s, in
 }
 
 __device__ __forceinline__ void store_part(const PoolDev& B, int j, int dim, const Part& p, int inactive_code) {
+  if (inactive_code) B.inactive[j] = inactive_code;  // per-slot flags are updated in place
   B.x[j] = p.x[0];
   B.y[j] = p.x[1];
   if (dim == 3) B.z[j] = p.x[2];
@@ -76,7 +78,6 @@ __device__ __forceinline__ void store_part(const PoolDev& B, int j, int dim, con
   }
   B.tri[j] = p.tri;
   B.ll[j] = p.ll;
-  B.inactive[j] = inactive_code;
 }
 
 // One pool slot; returns the stream position after the slot's last draw (0 if none).
@@ -87,12 +88,6 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
   scen = a.slot.scen ? a.slot.scen[j] : 0;
   if (p.inactive) {
     if (COUNT) a.units_pool[a.gpool[j]] = 0;
-    else {
-      // keep the inactive row (deleted rows keep their "deleted" mark)
-      a.B.inactive[j] = p.inactive;
-      a.B.tri[j] = -1;
-      a.B.ll[j] = 0;
-    }
     return 0;
   }
   const StreamDev& st = a.stream[scen];
@@ -100,6 +95,10 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
   Draws dr;
   dr.st = &st;
   dr.jt = &a.jt[scen];
+  dr.jt8 = a.jt8 ? &a.jt8[scen] : nullptr;
+  dr.pre_pos = nullptr;
+  dr.pre_state = nullptr;
+  dr.slot = j;
   dr.zt = a.zt;
   dr.method = a.par.rng_method;
   dr.dry = COUNT;
@@ -203,56 +202,106 @@ __global__ void __launch_bounds__(BLK) k_scatter_active(const int* inactive, con
 
 // block-local exclusive scan into int64, block totals out; elements at or past
 // the device count *ndev (if given) read as zero
-template <class T>
-__global__ void __launch_bounds__(BLK) k_scan_blocks(const T* in, int64_t* out, int64_t* block_sum, int64_t n,
-                                                     const int64_t* ndev, const int* cond) {
+// Exclusive scans (sum, max) in three passes: tiles of BLK x SCAN_ITEMS
+// elements, one block over the tile totals, then the tile prefix is folded in.
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = BLK * SCAN_ITEMS;
+
+struct OpSum {
+  static constexpr int64_t ident = 0;
+  __device__ __forceinline__ int64_t operator()(int64_t a, int64_t b) const { return a + b; }
+};
+struct OpMax {
+  static constexpr int64_t ident = -1;  // reach keys are >= -1
+  __device__ __forceinline__ int64_t operator()(int64_t a, int64_t b) const { return a > b ? a : b; }
+};
+
+// exclusive scan of one value per thread over a block of NT threads; *total
+// receives the block's reduction.  lds holds NT/64 entries.
+template <class Op, int NT>
+__device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t& total, int64_t* lds) {
+  Op op;
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t incl = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int64_t t = __shfl_up(incl, o);
+    if (lane >= o) incl = op(incl, t);
+  }
+  if (lane == 63) lds[w] = incl;
+  __syncthreads();
+  int64_t wpre = Op::ident, tot = Op::ident;
+  for (int q = 0; q < NT / 64; ++q) {
+    int64_t x = lds[q];
+    if (q < w) wpre = op(wpre, x);
+    tot = op(tot, x);
+  }
+  int64_t excl = __shfl_up(incl, 1);
+  if (lane == 0) excl = Op::ident;
+  __syncthreads();
+  total = tot;
+  return op(wpre, excl);
+}
+
+template <class T, class Op>
+__global__ void __launch_bounds__(BLK) k_scan_tiles(const T* in, int64_t* out, int64_t* tile_sum, int64_t n,
+                                                    const int64_t* ndev, const int* cond) {
   if (cond && *cond == 0) return;
-  __shared__ int64_t s[BLK];
+  __shared__ int64_t lds[BLK / 64];
+  Op op;
   int64_t lim = ndev ? (*ndev < n ? *ndev : n) : n;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  int64_t v = i < lim ? (int64_t)in[i] : 0;
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = 1; o < BLK; o <<= 1) {
-    int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
-    __syncthreads();
-    s[threadIdx.x] += t;
-    __syncthreads();
+  int64_t i0 = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int64_t v[SCAN_ITEMS];
+  int64_t run = Op::ident;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = i0 + k;
+    v[k] = i < lim ? (int64_t)in[i] : Op::ident;
+    int64_t x = v[k];
+    v[k] = run;  // exclusive within the thread
+    run = op(run, x);
   }
-  if (i < n) out[i] = s[threadIdx.x] - v;
-  if (threadIdx.x == BLK - 1) block_sum[blockIdx.x] = s[BLK - 1];
+  int64_t total;
+  int64_t pre = block_excl_scan<Op, BLK>(run, total, lds);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = i0 + k;
+    if (i < n) out[i] = op(pre, v[k]);
+  }
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
 }
 
-// exclusive scan of the block totals by one block (strided chunks)
-__global__ void __launch_bounds__(1024) k_scan_totals(int64_t* bs, int64_t nb, int64_t* total, const int* cond) {
+// exclusive scan of the tile totals by one block of 1024 threads
+template <class Op>
+__global__ void __launch_bounds__(1024) k_scan_top(int64_t* ts, int64_t nt, int64_t* total, const int* cond) {
   if (cond && *cond == 0) return;
-  __shared__ int64_t s[1024];
-  __shared__ int64_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < nb; base += 1024) {
-    int64_t i = base + threadIdx.x;
-    int64_t v = i < nb ? bs[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0;
-      __syncthreads();
-      s[threadIdx.x] += t;
-      __syncthreads();
+  __shared__ int64_t lds[16];
+  Op op;
+  int64_t per = (nt + 1023) / 1024;
+  int64_t i0 = (int64_t)threadIdx.x * per;
+  int64_t run = Op::ident;
+  for (int64_t k = 0; k < per; ++k)
+    if (i0 + k < nt) run = op(run, ts[i0 + k]);
+  int64_t tot;
+  int64_t pre = block_excl_scan<Op, 1024>(run, tot, lds);
+  for (int64_t k = 0; k < per; ++k) {
+    if (i0 + k < nt) {
+      int64_t x = ts[i0 + k];
+      ts[i0 + k] = pre;
+      pre = op(pre, x);
     }
-    if (i < nb) bs[i] = carry + s[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += s[1023];
-    __syncthreads();
   }
-  if (threadIdx.x == 0 && total) *total = carry;
+  if (threadIdx.x == 0 && total) *total = tot;
 }
 
-__global__ void __launch_bounds__(BLK) k_scan_add(int64_t* out, const int64_t* bs, int64_t n, const int* cond) {
+__global__ void __launch_bounds__(BLK) k_scan_fix(int64_t* out, const int64_t* ts, int64_t n, const int* cond) {
   if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i < n) out[i] += bs[blockIdx.x];
+  int64_t add = ts[blockIdx.x];
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = base + k * BLK + threadIdx.x;
+    if (i < n) out[i] += add;
+  }
 }
 
 // slot value = scan[gpool] - scan[scenario base]; as int rank or int64 offset
@@ -323,6 +372,7 @@ __device__ __forceinline__ int64_t window_for(int method, int64_t units) {
 __device__ inline int64_t zig_length(Pcg g, const ZigTables& z) {
   Draws d;
   d.zt = &z;
+  d.pre_pos = nullptr;
   d.g = g;
   d.ncalls = 0;
   d.zig();
@@ -437,56 +487,12 @@ __global__ void __launch_bounds__(BLK) k_reach_key(const int64_t* pos, const int
   key[i] = ((int64_t)lo << REACH_SHIFT) | (pos[i] + len[i] - 1);
 }
 
-__global__ void __launch_bounds__(BLK) k_mscan_blocks(const int64_t* in, int64_t* out, int64_t* bmax, int64_t n,
-                                                      const int* cond) {
-  if (cond && *cond == 0) return;
-  __shared__ int64_t s[BLK];
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  int64_t v = i < n ? in[i] : -1;
-  s[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = 1; o < BLK; o <<= 1) {
-    int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : -1;
-    __syncthreads();
-    s[threadIdx.x] = t > s[threadIdx.x] ? t : s[threadIdx.x];
-    __syncthreads();
-  }
-  int64_t excl = threadIdx.x > 0 ? s[threadIdx.x - 1] : -1;
-  if (i < n) out[i] = excl;
-  if (threadIdx.x == BLK - 1) bmax[blockIdx.x] = s[BLK - 1];
-}
-
-__global__ void __launch_bounds__(1024) k_mscan_totals(int64_t* bm, int64_t nb, const int* cond) {
-  if (cond && *cond == 0) return;
-  __shared__ int64_t s[1024];
-  __shared__ int64_t carry;
-  if (threadIdx.x == 0) carry = -1;
-  __syncthreads();
-  for (int64_t base = 0; base < nb; base += 1024) {
-    int64_t i = base + threadIdx.x;
-    int64_t v = i < nb ? bm[i] : -1;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      int64_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : -1;
-      __syncthreads();
-      s[threadIdx.x] = t > s[threadIdx.x] ? t : s[threadIdx.x];
-      __syncthreads();
-    }
-    int64_t excl = threadIdx.x > 0 ? s[threadIdx.x - 1] : -1;
-    if (i < nb) bm[i] = excl > carry ? excl : carry;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry = s[1023] > carry ? s[1023] : carry;
-    __syncthreads();
-  }
-}
-
 __global__ void __launch_bounds__(BLK) k_reach_decode(int64_t* out, const int64_t* bmax, const int64_t* ev_begin,
                                                       int nscen, const int64_t* nev, int64_t cap, const int* cond) {
   if (cond && *cond == 0) return;
   int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= cap || i >= *nev) return;
-  int64_t m = out[i], bm = bmax[blockIdx.x];
+  int64_t m = out[i], bm = bmax[i / SCAN_TILE];
   m = bm > m ? bm : m;
   int lo = 0, hi = nscen - 1;
   while (lo < hi) {
@@ -497,22 +503,59 @@ __global__ void __launch_bounds__(BLK) k_reach_decode(int64_t* out, const int64_
   out[i] = (m >= 0 && (m >> REACH_SHIFT) == lo) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
 }
 
-// Sort key for the element re-sort: active rows by element, then the rest.
+// Element re-sort: counting sort of the slots by a key that groups 2^SORT_SHIFT
+// neighbouring element ids (active rows), then rows without element, then
+// inactive rows.  The input is nearly sorted after the first sort, so equal
+// keys come in runs of adjacent lanes: the first lane of a run does one atomic
+// for the whole run.
+constexpr int SORT_SHIFT = 2;
+
+__device__ __forceinline__ int sort_nbins(int nt) { return (nt >> SORT_SHIFT) + 3; }
+
+__device__ __forceinline__ int sort_key(int t, int inactive, int nt) {
+  int nb = (nt >> SORT_SHIFT) + 1;
+  return inactive ? nb + 1 : (t >= 0 ? t >> SORT_SHIFT : nb);
+}
+
+struct WaveRun {
+  int head_lane, len, rank;
+  bool head;
+};
+
+__device__ __forceinline__ WaveRun wave_run(int key) {
+  int lane = threadIdx.x & 63;
+  int prev = __shfl_up(key, 1);
+  WaveRun r;
+  r.head = lane == 0 || prev != key;
+  unsigned long long heads = __ballot(r.head);
+  unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  r.head_lane = 63 - __clzll(heads & upto);
+  unsigned long long above = heads & ~upto;
+  int next = above ? __ffsll((long long)above) - 1 : 64;
+  r.len = next - r.head_lane;
+  r.rank = lane - r.head_lane;
+  return r;
+}
+
 __global__ void __launch_bounds__(BLK) k_sort_hist(const int* tri, const int* inactive, int n, int nt, int* cnt,
                                                    int* key) {
   int j = blockIdx.x * BLK + threadIdx.x;
-  if (j >= n) return;
-  int t = tri[j];
-  int k = inactive[j] ? nt + 1 : (t >= 0 ? t : nt);
-  key[j] = k;
-  atomicAdd(&cnt[k], 1);
+  bool valid = j < n;
+  int k = valid ? sort_key(tri[j], inactive[j], nt) : -1 - (int)(threadIdx.x & 63);
+  if (valid) key[j] = k;
+  WaveRun r = wave_run(k);
+  if (valid && r.head) atomicAdd(&cnt[k], r.len);
 }
 
 __global__ void __launch_bounds__(BLK) k_sort_place(const int* key, int n, int64_t* cursor, int* perm) {
   int j = blockIdx.x * BLK + threadIdx.x;
-  if (j >= n) return;
-  unsigned long long d = atomicAdd((unsigned long long*)&cursor[key[j]], 1ull);
-  perm[d] = j;
+  bool valid = j < n;
+  int k = valid ? key[j] : -1 - (int)(threadIdx.x & 63);
+  WaveRun r = wave_run(k);
+  unsigned long long base = 0;
+  if (valid && r.head) base = atomicAdd((unsigned long long*)&cursor[k], (unsigned long long)r.len);
+  base = __shfl(base, r.head_lane);
+  if (valid) perm[base + r.rank] = j;
 }
 
 template <class T>
@@ -553,11 +596,13 @@ __global__ void __launch_bounds__(BLK) k_zig_extra(const int* on, const int64_t*
   extra[i] = (i < *nev && on[i]) ? len[i] - 1 : 0;
 }
 
-// compact on-chain events: normal index and cumulative extra positions (inclusive)
+// compact on-chain events: normal index, cumulative extra positions (inclusive)
+// and the value of the slow normal (sampled from its start position)
 __global__ void __launch_bounds__(BLK) k_zig_compact(const int64_t* pos, const int64_t* len, const int* on,
                                                      const int64_t* E, const int64_t* Cidx, const int64_t* ev_begin,
                                                      int nscen, const int64_t* nev, int64_t* key, int64_t* xtra,
-                                                     const int* cond) {
+                                                     double* val, const StreamDev* st, const JumpTable* jt,
+                                                     const ZigTables* zt, const int* cond) {
   if (cond && *cond == 0) return;
   int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= *nev || !on[i]) return;
@@ -566,6 +611,13 @@ __global__ void __launch_bounds__(BLK) k_zig_compact(const int64_t* pos, const i
   int64_t c = Cidx[i];
   key[c] = pos[i] - (E[i] - eb);
   xtra[c] = (E[i] - eb) + len[i] - 1;
+  Draws d;
+  d.zt = zt;
+  d.pre_pos = nullptr;
+  d.g.inc = st[s].inc;
+  d.g.state = pcg_jump(st[s].base, (uint64_t)pos[i], jt[s]);
+  d.ncalls = 0;
+  val[c] = d.zig();
 }
 
 // polar: key_j = r_j - j (j counted within the scenario)
@@ -580,20 +632,45 @@ __global__ void __launch_bounds__(BLK) k_polar_keys(const int64_t* pos, const in
 
 // point each scenario's stream at its slice of the event tables
 __global__ void k_stream_events(StreamDev* st, int nscen, const int64_t* ev_begin, const int64_t* Cidx, int method,
-                                int64_t* key, int64_t* xtra, const int* cond) {
+                                int64_t* key, int64_t* xtra, double* val, const int* bucket,
+                                const int64_t* bucket_base, int bshift, const int* cond) {
   if (cond && *cond == 0) return;
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nscen) return;
+  st[s].ev_bucket = bucket + bucket_base[s];
+  st[s].n_bucket = bucket_base[s + 1] - bucket_base[s];
+  st[s].ev_bshift = bshift;
   if (method == 2) {
     int64_t c0 = Cidx[ev_begin[s]], c1 = Cidx[ev_begin[s + 1]];
     st[s].ev_key = key + c0;
     st[s].ev_extra = xtra + c0;
+    st[s].ev_val = val + c0;
     st[s].n_ev = c1 - c0;
   } else {
     st[s].ev_key = key + ev_begin[s];
     st[s].ev_extra = nullptr;
+    st[s].ev_val = nullptr;
     st[s].n_ev = ev_begin[s + 1] - ev_begin[s];
   }
+}
+
+// bucket index of every scenario's event keys: entry b = lower_bound(b << shift)
+__global__ void __launch_bounds__(BLK) k_ev_buckets(const StreamDev* st, int nscen, const int64_t* bucket_base,
+                                                    int64_t ntot, int* bucket, const int* cond) {
+  if (cond && *cond == 0) return;
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= ntot) return;
+  int s = 0;
+  while (s + 1 < nscen && bucket_base[s + 1] <= i) ++s;
+  int64_t b = i - bucket_base[s], u = b << st[s].ev_bshift;
+  const int64_t* key = st[s].ev_key;
+  int64_t lo = 0, hi = st[s].n_ev;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (key[mid] < u) lo = mid + 1;
+    else hi = mid;
+  }
+  bucket[i] = (int)lo;
 }
 
 // end of step: advance each scenario's stream past its last draw, reset counters
@@ -636,8 +713,43 @@ __global__ void k_uniform_units(const StreamDev* st, const int64_t* scen_active,
   if (s < nscen) scen_units[s] = (int64_t)st[s].units_per_part * scen_active[s];
 }
 
+// a general-mode rerun starts from the step's input: undo its deletion marks
+__global__ void __launch_bounds__(BLK) k_undelete(int* inactive, const int* del_step, int n, int step_id,
+                                                  const int* cond) {
+  if (cond && *cond == 0) return;
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j < n && inactive[j] == 2 && del_step[j] == step_id) inactive[j] = 0;
+}
+
 __global__ void k_clear_flag(int* flag, int idx) {
   if (threadIdx.x == 0 && blockIdx.x == 0) flag[idx] = 0;
+}
+
+__global__ void k_set_flag(int* flag, int idx, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) flag[idx] = v;
+}
+
+// dst[q][j] = src[q][perm[j]] for every listed array, one pass
+constexpr int PERM_MAX = 16;
+struct PermArgs {
+  const int* perm;
+  int n, nd, ni;
+  const double* dsrc[PERM_MAX];
+  double* ddst[PERM_MAX];
+  const int* isrc[PERM_MAX];
+  int* idst[PERM_MAX];
+};
+
+__global__ void __launch_bounds__(BLK) k_permute_all(PermArgs p) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= p.n) return;
+  int s = p.perm[j];
+#pragma unroll
+  for (int q = 0; q < PERM_MAX; ++q)
+    if (q < p.nd) p.ddst[q][j] = p.dsrc[q][s];
+#pragma unroll
+  for (int q = 0; q < PERM_MAX; ++q)
+    if (q < p.ni) p.idst[q][j] = p.isrc[q][s];
 }
 
 }  // namespace cyl

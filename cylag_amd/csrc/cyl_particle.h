@@ -43,6 +43,7 @@ __device__ inline void delete_part(Part& p, int dim) {
 struct Draws {
   const StreamDev* st;
   const JumpTable* jt;
+  const JumpTable8* jt8;  // byte-window table (null: bit table only)
   const ZigTables* zt;
   int method;
   bool dry;
@@ -53,42 +54,63 @@ struct Draws {
   bool live;
   int err;
   Pcg g;
+  // stream position and generator state of each slot's first draw, prepared
+  // by a separate pass (null: found here)
+  const int64_t* pre_pos;
+  const uint64_t* pre_state;
+  int slot;
 
   __device__ uint32_t next() {
     ++ncalls;
     return pcg_next(g);
   }
 
-  __device__ int64_t zig_position(int64_t u) const {
-    // number of on-chain slow events with normal index < u, then extra positions
-    int64_t lo = 0, hi = st->n_ev;
+  // #events with key < u: bucket start, then a short forward scan (binary
+  // search past the end of the bucket table)
+  __device__ int64_t events_before(int64_t u) const {
+    int64_t b = u >> st->ev_bshift, lo;
+    if (b < st->n_bucket) {
+      lo = st->ev_bucket[b];
+      while (lo < st->n_ev && st->ev_key[lo] < u) ++lo;
+      return lo;
+    }
+    lo = 0;
+    int64_t hi = st->n_ev;
     while (lo < hi) {
       int64_t mid = (lo + hi) >> 1;
       if (st->ev_key[mid] < u) lo = mid + 1;
       else hi = mid;
     }
+    return lo;
+  }
+
+  __device__ int64_t zig_position(int64_t u) const {
+    // number of on-chain slow events with normal index < u, then extra positions
+    int64_t lo = events_before(u);
     return u + (lo > 0 ? st->ev_extra[lo - 1] : 0);
   }
 
   __device__ int64_t polar_position(int64_t q) const {
     // first attempt of call q = q + #{rejected attempts j : r_j - j < q}
-    int64_t lo = 0, hi = st->n_ev;
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if (st->ev_key[mid] < q) lo = mid + 1;
-      else hi = mid;
-    }
-    return 2 * (q + lo);
+    return 2 * (q + events_before(q));
   }
 
   __device__ void start() {
     live = true;
+    if (pre_pos) {
+      pos0 = pre_pos[slot];
+      if (pos0 < 0) err = E_RNG_WINDOW;
+      g.inc = st->inc;
+      g.state = pre_state[slot];
+      ncalls = 0;
+      return;
+    }
     if (method == 1) pos0 = unit0;
     else if (method == 2) pos0 = zig_position(unit0);
     else pos0 = polar_position(unit0);
     if (method != 1 && pos0 >= st->window) err = E_RNG_WINDOW;
     g.inc = st->inc;
-    g.state = pcg_jump(st->base, (uint64_t)pos0, *jt);
+    g.state = jt8 ? pcg_jump8(st->base, (uint64_t)pos0, *jt8, *jt) : pcg_jump(st->base, (uint64_t)pos0, *jt);
     ncalls = 0;
   }
 
@@ -174,6 +196,7 @@ struct BcDraws {
     Draws d;
     d.method = method;
     d.zt = zt;
+    d.pre_pos = nullptr;
     d.dry = false;
     d.live = true;
     d.g = g;

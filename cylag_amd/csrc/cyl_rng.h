@@ -91,6 +91,34 @@ CYL_HD inline uint64_t pcg_jump(uint64_t base, uint64_t delta, const JumpTable& 
   return s;
 }
 
+// Byte-window jump table: entry [l][v] advances by v << (8 l), l = 0..3, so a
+// jump below 2^32 is four affine steps with no branch (entry [l][0] is the identity).
+struct JumpTable8 {
+  uint64_t mult[4][256];
+  uint64_t plus[4][256];
+};
+
+inline void make_jump_table8(uint64_t inc, JumpTable8& t) {
+  for (int l = 0; l < 4; ++l)
+    for (int v = 0; v < 256; ++v) pcg_advance_map(inc, (uint64_t)v << (8 * l), t.mult[l][v], t.plus[l][v]);
+}
+
+CYL_HD inline uint64_t pcg_jump8(uint64_t base, uint64_t delta, const JumpTable8& t8, const JumpTable& jt) {
+  uint64_t s = base;
+  for (int l = 0; l < 4; ++l) {
+    uint32_t v = (uint32_t)(delta >> (8 * l)) & 255u;
+    s = t8.mult[l][v] * s + t8.plus[l][v];
+  }
+  uint64_t hi = delta >> 32;
+  int k = 32;
+  while (hi) {
+    if (hi & 1) s = jt.mult[k] * s + jt.plus[k];
+    hi >>= 1;
+    ++k;
+  }
+  return s;
+}
+
 CYL_HD inline double u32_to_uniform(uint32_t u) { return u * (1.0 / 4294967296.0); }
 
 // rectangular normal approximation, random_utils.pyx:217-219

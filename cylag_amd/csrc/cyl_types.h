@@ -42,6 +42,7 @@ enum ErrCode : int {
   E_MISPREDICT = 8,       // draw count differs from the uniform prediction (host reruns)
   E_METHOD1 = 9,          // neighbourhood search requested (reference has no table for it)
   E_LSM3 = 10,
+  E_RNG_EVENTS = 11,      // a slow Ziggurat start missing from the step's event table (internal)
 };
 
 struct MeshDev {
@@ -119,8 +120,13 @@ struct StreamDev {
   // Ziggurat / polar: compacted slow events in the step window
   const int64_t* ev_key;    // normal index (zig) or key r_j - j (polar) of each event
   const int64_t* ev_extra;  // cumulative extra positions (zig)
+  const double* ev_val;     // value of the slow normal of each event (zig)
   int64_t n_ev;
   int64_t window;           // positions covered by the event scan
+  // bucket index over ev_key: ev_bucket[b] = #events with key < (b << ev_bshift)
+  const int* ev_bucket;
+  int64_t n_bucket;
+  int ev_bshift;
 };
 
 struct StepDev {

@@ -129,6 +129,9 @@ int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int block_size);
  * launches (HIP events on the context's stream), wall time of the call, and
  * the number of timed launches. */
 int cyl_last_timing(cyl_ctx *ctx, double *step_kernel_ms, double *total_ms, int *kernel_launches);
+/* Summed duration of the draw-start pass that precedes the fused 2D LSM-1
+ * step kernel in the same timed steps (0 when the generic kernel ran). */
+int cyl_last_prep_timing(cyl_ctx *ctx, double *prep_ms);
 
 #ifdef __cplusplus
 }

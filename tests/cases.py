@@ -105,7 +105,8 @@ class Case:
         return SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(),
                                tri=ps.tri.copy(), inactive=ps.inactive.copy(),
                                lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy(), rng=rng,
-                               solver=s)
+                               last_position=ps.last_position.copy(),
+                               last_velocity=ps.last_velocity.copy(), solver=s)
 
 
 def square_case(n=101, npart=10000, u=0.1, v=0.0, depth=1.0, centre=50., length=100.):
@@ -137,9 +138,11 @@ def reference_square_case(position, velocity=None, pool=None):
 
 
 def vortex_case(nx=41, ny=11, lx=2000., ly=500., npart=2000, seed=3, ntimes=2, keps=True,
-                wavelength=400.):
+                wavelength=400., dead_zone=False):
     """Small version of the unstructured 2D workload: jittered Delaunay mesh,
-    two frames of a cellular vortex, k-epsilon diffusivity varying in space."""
+    two frames of a cellular vortex, k-epsilon diffusivity varying in space.
+    dead_zone: no turbulence (K = 0) for x < lx/4, where particles draw no
+    normal, so the number of draws per particle is not uniform."""
     x, y, tri, mpl = jittered_delaunay(nx, ny, lx, ly, seed=7)
     k = 2. * np.pi / wavelength
     n = x.size
@@ -153,6 +156,8 @@ def vortex_case(nx=41, ny=11, lx=2000., ly=500., npart=2000, seed=3, ntimes=2, k
     K = 0.1 + 4.9 * (0.5 + 0.5 * np.cos(x / lx * 5.) * np.sin(y / ly * 4.))
     eps = np.full(n, 1e-3)
     tke = np.sqrt(K * eps / (0.09 / 0.72))
+    if dead_zone:
+        tke[x < 0.25 * lx] = 0.
     fields = dict(dim=2, nlayers=1, times=times, elevation=np.tile(depth, (ntimes, 1)),
                   velocity_x=np.array(frames_u), velocity_y=np.array(frames_v))
     if keps:

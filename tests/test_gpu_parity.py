@@ -82,6 +82,29 @@ def test_unstructured_keps_two_frames(rng_method):
     assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
 
 
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+def test_keps_dead_zone_non_uniform_draws(rng_method):
+    """Particles in a zone without turbulence draw nothing: the uniform draw
+    assignment is rejected and the step reruns with pool-order offsets."""
+    case = vortex_case(npart=3000, dead_zone=True)
+    params = dict(final_time=200., time_step=5., model=1, diffusion_model=2,
+                  rng_method=rng_method)
+    r = case.run_oracle(R, params)
+    assert_parity(case.run_gpu(cy, params), r)
+
+
+@pytest.mark.parametrize("rng_method", [2, 3])
+def test_large_pool_two_steps(rng_method):
+    """2.2M particles on a 20k-element mesh for two steps: multi-tile scans of
+    ranks and event tables, many generator chunks, persistent-grid strides."""
+    case = vortex_case(nx=201, ny=51, lx=10000., ly=2500., npart=2_200_000, seed=11,
+                       wavelength=2000.)
+    params = dict(final_time=10., time_step=5., model=1, diffusion_model=2,
+                  rng_method=rng_method)
+    r = case.run_oracle(R, params)
+    assert_parity(case.run_gpu(cy, params), r)
+
+
 @pytest.mark.parametrize("rng_method", [1, 2])
 def test_open_boundary_deletions_and_tags(rng_method):
     """Particles leave through open boundaries: deletions, tag renumbering,
@@ -99,9 +122,19 @@ def test_open_boundary_deletions_and_tags(rng_method):
     np.testing.assert_array_equal(gs, np.array(r.solver.bnd_history))
 
 
-def test_element_resort_does_not_change_results():
-    case = vortex_case(npart=2000)
-    params = dict(final_time=150., time_step=5., model=1, diffusion_model=2)
+@pytest.mark.parametrize("which", ["keps_fused", "keps_rk2_generic", "open_deletions"])
+def test_element_resort_does_not_change_results(which):
+    """Re-sorting the slots by element every 3 steps (state gather, swapped
+    per-slot buffers, rank recompute) leaves every result unchanged."""
+    if which == "open_deletions":
+        case = open_channel_case()
+        params = dict(final_time=70., time_step=1., frozen_eulerian_fields=True, model=1,
+                      diffusion_model=1, horizontal_diffusivity=2., bnd_statistics=True)
+    else:
+        case = vortex_case(npart=2000)
+        params = dict(final_time=150., time_step=5., model=1, diffusion_model=2)
+        if which == "keps_rk2_generic":
+            params["time_scheme"] = 2
     s = case.build_gpu(cy, params)
     s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 256)
     s.solve()
@@ -111,3 +144,6 @@ def test_element_resort_does_not_change_results():
     g = SimpleNamespace(position=ps.position, velocity=ps.velocity, tri=ps.tri,
                         inactive=ps.inactive, lowerlayer=ps.lowerlayer, tag=ps.tag)
     assert_parity(g, r)
+    for name in ("last_position", "last_velocity"):
+        gv, rv = getattr(ps, name), getattr(r, name)
+        assert np.max(np.abs(gv - rv) / np.maximum(np.abs(rv), 1.)) <= TOL, name
