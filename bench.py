@@ -2,14 +2,16 @@
 
 Workload (2D LSM1 on an unstructured mesh): Delaunay triangulation of 1M
 jittered points in a 20 x 5 km channel (~2M elements, seed 7), two velocity
-frames of a cellular vortex (amplitude 1 m/s, wavelength 2 km) interpolated
-in time, depth 2-20 m, k-epsilon diffusivity K(x, y) in [0.1, 5] m2/s with
-its gradient drift, 10M FP64 particles, dt = 5 s, default Ziggurat normals.
-A "step" is one Solver.forward over the whole particle set.  Inputs are in
-HBM before the timed region, which brackets K steps with device syncs.
+frames (t = 0 and 10000 s) of a cellular vortex (amplitude 1 m/s, wavelength
+2 km) interpolated in time, depth 2-20 m, k-epsilon diffusivity K(x, y) in
+[0.1, 5] m2/s with its gradient drift, 10M FP64 particles, dt = 5 s, 2000
+steps in all (warm-up included), default Ziggurat normals, schmidt number
+0.72.  A "step" is one Solver.forward over the whole particle set.  Inputs
+are in HBM before the timed region, which brackets K steps with device syncs.
 
-Prints one JSON line (metric particle-steps/s) with the roofline of the
-fused step kernel and a CPU baseline (the oracle, one core, on a sample).
+Prints one JSON line with the roofline of the fused step kernel (its launch
+durations measured with HIP events on the context's stream) and a CPU
+baseline (the oracle, one core, on a bounded sample of the same workload).
 """
 import argparse
 import json
@@ -23,6 +25,43 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+METRIC = "particle-steps/s plain vs batched (1 GPU); p50/p99 scenario latency at fixed QPS"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lsm1_2d_unstructured_keps_10M_counters.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fused step kernel, from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration); None when no summary is present."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(nbytes=1 << 30):
+    """A plain device-to-device copy of known size (the HIP runtime's blit
+    kernel), profiled in the same run as the step kernel to scale the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("calibration copy: 3 x {} bytes".format(nbytes))
 
 
 def log(*a):
@@ -35,7 +74,7 @@ def c2_workload(npts_x=2000, npts_y=500, npart=10_000_000, seed=7):
     x, y, tri, mpl = jittered_delaunay(npts_x, npts_y, lx, ly, seed=seed)
     n = x.size
     k = 2. * np.pi / 2000.
-    times = np.array([0., 20000.])
+    times = np.array([0., 10000.])
     u = np.stack([np.sin(k * x + ph) * np.cos(k * y) for ph in (0., 0.5)])
     v = np.stack([-np.cos(k * x + ph) * np.sin(k * y) for ph in (0., 0.5)])
     rng = np.random.default_rng(seed)
@@ -48,14 +87,16 @@ def c2_workload(npts_x=2000, npts_y=500, npart=10_000_000, seed=7):
                   opt_fields=np.stack([np.stack([tke, tke]), np.stack([eps, eps])]),
                   opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
     pos = np.stack([rng.uniform(0.01 * lx, 0.99 * lx, npart), rng.uniform(0.01 * ly, 0.99 * ly, npart)], 1)
-    params = dict(final_time=19000., time_step=5., model=1, diffusion_model=2)
+    params = dict(final_time=10000., time_step=5., model=1, diffusion_model=2, schmidt_number=0.72)
     return x, y, tri, fields, pos, params
 
 
 def bytes_per_particle_step(nt, nv, npart):
-    """Algorithmic bytes: evolving state read and written once (x, y f64 +
-    element id i32), plus the mesh and node fields each step must read once
-    (element record 84 B; node x, y, two frames of u, v, k, eps)."""
+    """Algorithmic bytes per particle-step: the evolving state read and written
+    once (x, y f64 + element id i32 = 20 B each way), plus the mesh and node
+    fields a step must read once, shared by the N particles (element record
+    84 B: P1 coefficients 72 B + 3 node ids; node: x, y and two frames of u,
+    v, k, eps)."""
     state = 2 * (16 + 4)
     mesh = nt * 84 + nv * (16 + 2 * 2 * 16)
     return state + mesh / npart, state, mesh
@@ -91,11 +132,13 @@ def cpu_baseline(x, y, tri, fields, pos, params, nsample=100_000, budget_s=15.):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1990)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--npart", type=int, default=10_000_000)
-    ap.add_argument("--sort-interval", type=int, default=0)
+    ap.add_argument("--sort-interval", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="after the timed steps, copy 1 GiB device to device (for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the step runs on one MI355X holding the whole mesh "
@@ -120,16 +163,19 @@ def main():
     solver._run(args.steps)
     wall = time.perf_counter() - t0
     k_ms, tot_ms, nlaunch = solver.ctx.last_timing()
+    prep_ms = solver.ctx.last_prep_ms()
     ps._sync()
     nact = ps.npart_active
-    log("timed: {} steps in {:.3f} s, step kernel {:.3f} ms/launch over {} launches".format(
-        args.steps, wall, k_ms / max(nlaunch, 1), nlaunch))
+    log("timed: {} steps in {:.3f} s, step kernel {:.3f} ms/launch over {} launches, normals {:.3f} ms".format(
+        args.steps, wall, k_ms / max(nlaunch, 1), nlaunch, prep_ms / max(nlaunch, 1)))
+    if args.calibrate_copy:
+        calibrate_copy()
     psteps = nact * args.steps / wall
     b_alg, b_state, b_mesh = bytes_per_particle_step(mesh.nt, mesh.nv, nact)
     avg_ms = k_ms / max(nlaunch, 1)
     achieved = b_alg * nact / (avg_ms * 1e-3)
     line = {
-        "metric": "particle-steps/s (plain step, 1 GPU)",
+        "metric": METRIC,
         "value": psteps,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -146,9 +192,10 @@ def main():
                    "elements": int(mesh.nt), "nodes": int(mesh.nv), "particles": int(nact),
                    "rng_method": 2, "sort_interval": args.sort_interval},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_lsm1_2d<2>", "kernel_avg_ms": avg_ms,
-                     "bytes_per_particle_step": b_alg},
+                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(),
+                     "kernel": "k_lsm1_2d<2,true>", "kernel_avg_ms": avg_ms,
+                     "bytes_per_particle_step": b_alg,
+                     "normals_kernel_avg_ms": prep_ms / max(nlaunch, 1)},
     }
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/cylag_hip.h"
+#include "cyl_events.h"
 #include "cyl_fast.h"
 
 using namespace cyl;
@@ -46,6 +47,9 @@ struct cyl_ctx {
   MeshDev mesh{};
   bool has_mesh = false;
   Bufs mesh_bufs;
+  // Hilbert rank of each element (re-sort key); node renumbering of the
+  // packed records (new -> old, old -> new)
+  int *d_elem_rank = nullptr, *d_node_old = nullptr, *d_node_new = nullptr;
   // packed records of the fused 2D LSM-1 kernel (built on first use)
   bool fast_ready = false;
   Bufs fast_bufs;
@@ -112,9 +116,10 @@ struct cyl_ctx {
   int* d_chunk_count = nullptr;
   int64_t *d_chunk_off = nullptr, *d_ev_pos = nullptr, *d_ev_len = nullptr, *d_reach = nullptr;
   int64_t *d_bmax = nullptr, *d_E = nullptr, *d_C = nullptr, *d_key = nullptr, *d_xtra = nullptr;
-  double* d_ev_val = nullptr;
+  double* d_ev_val = nullptr;                      // compacted slow-normal values
+  double* d_cand_val = nullptr;                    // value per candidate
+  uint64_t* d_ev_state = nullptr;                  // generator state per candidate
   int64_t *d_ev_begin = nullptr, *d_nev = nullptr, *d_ev_bsum = nullptr, *d_xscratch = nullptr;
-  int* d_on = nullptr;
   int* d_bucket = nullptr;          // bucket index of the event keys, all scenarios
   int64_t* d_bucket_base = nullptr;
   int64_t nbucket = 0;
@@ -171,6 +176,25 @@ static hipError_t dalloc(Bufs& b, T** p, size_t count) {
 }
 
 static inline unsigned nblk(int64_t n, int b = BLK) { return (unsigned)((n + b - 1) / b); }
+
+// distance along the Hilbert curve of order 16 of the cell (x, y), x, y < 2^16
+// (the classic rotate-and-accumulate conversion)
+static uint64_t hilbert_index(uint32_t x, uint32_t y) {
+  const uint32_t n = 1u << 16;
+  uint64_t d = 0;
+  for (uint32_t s = n / 2; s > 0; s /= 2) {
+    uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += (uint64_t)s * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = n - 1 - x;
+        y = n - 1 - y;
+      }
+      std::swap(x, y);
+    }
+  }
+  return d;
+}
 
 // ---------------------------------------------------------------------------
 // context
@@ -325,6 +349,37 @@ extern "C" int cyl_mesh_upload(cyl_ctx* ctx, int nv, const double* x, const doub
   }
   HIPCHK(up(&dcen, cen.data(), cen.size()));
   double w = std::max(xmax - xmin, 1e-300), h = std::max(ymax - ymin, 1e-300);
+  // Hilbert-curve ranks of the element centroids and of the nodes: the
+  // element re-sort key, and the node order of the packed node records
+  {
+    double sc = 65535.0 / std::max(w, h);
+    auto hkey = [&](double px, double py) {
+      double fx = std::floor((px - xmin) * sc), fy = std::floor((py - ymin) * sc);
+      uint32_t hx = (uint32_t)std::min(65535.0, std::max(0.0, fx));
+      uint32_t hy = (uint32_t)std::min(65535.0, std::max(0.0, fy));
+      return hilbert_index(hx, hy);
+    };
+    auto ranks = [&](int cnt, auto&& coord, std::vector<int>& order) {
+      std::vector<uint64_t> key(cnt);
+      for (int i = 0; i < cnt; ++i) {
+        double px, py;
+        coord(i, px, py);
+        key[i] = hkey(px, py);
+      }
+      order.resize(cnt);
+      for (int i = 0; i < cnt; ++i) order[i] = i;
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    };
+    std::vector<int> eord, nord;
+    ranks(nt, [&](int k, double& px, double& py) { px = cen[2 * k]; py = cen[2 * k + 1]; }, eord);
+    ranks(nv, [&](int i, double& px, double& py) { px = x[i]; py = y[i]; }, nord);
+    std::vector<int> erank(nt), nnew(nv);
+    for (int i = 0; i < nt; ++i) erank[eord[i]] = i;
+    for (int i = 0; i < nv; ++i) nnew[nord[i]] = i;
+    HIPCHK(up(&ctx->d_elem_rank, erank.data(), erank.size()));
+    HIPCHK(up(&ctx->d_node_new, nnew.data(), nnew.size()));
+    HIPCHK(up(&ctx->d_node_old, nord.data(), nord.size()));
+  }
   double cell = std::sqrt(w * h / nt) * 1.5;
   if (!(cell > 0.)) cell = std::max(w, h);
   int gnx = (int)std::min<double>(std::max(1.0, std::ceil(w / cell)), 1 << 14);
@@ -940,6 +995,8 @@ static int predict_units(const cyl_params* p, int dim, double dt, int& umax) {
 static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   if (method == 1) return CYL_OK;
   int64_t need = units_per_scen_cap;
+  // event counts and extra positions are packed as two 32-bit halves
+  if (need > (1LL << 30)) return ctx->set_err(CYL_E_ARG, -1, "too many draws in one step for the event tables");
   if (ctx->ev_units_cap >= need && ctx->ev_bufs.ptrs.size()) return CYL_OK;
   ctx->ev_bufs.free_all();
   ctx->ev_units_cap = need;
@@ -986,8 +1043,9 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(dalloc(b, &ctx->d_key, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_xtra, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_val, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_cand_val, evcap + 1));
+  HIPCHK(dalloc(b, &ctx->d_ev_state, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_xscratch, evcap + 1));
-  HIPCHK(dalloc(b, &ctx->d_on, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_begin, ns + 1));
   HIPCHK(dalloc(b, &ctx->d_nev, 1));
   return CYL_OK;
@@ -1025,56 +1083,58 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   EventArgs e{};
   e.stream = ctx->d_stream;
   e.jt = ctx->d_jt;
+  e.jt8 = ctx->d_jt8;
   e.zt = ctx->d_zt;
   e.scen_units = ctx->d_scen_units;
   e.chunk_base = ctx->d_chunk_base;
   e.nscen = ctx->nscen;
   e.nchunks = ctx->nchunks;
-  e.method = method;
   e.chunk_count = ctx->d_chunk_count;
   e.chunk_off = ctx->d_chunk_off;
+  e.nev = ctx->d_nev;
+  e.ev_begin = ctx->d_ev_begin;
   e.ev_pos = ctx->d_ev_pos;
-  e.ev_len = ctx->d_ev_len;
+  e.ev_state = ctx->d_ev_state;
   e.ev_cap = ctx->ev_cap;
   e.overflow = ctx->d_flag + 2;
   e.cond = cond;
   hipStream_t s = ctx->stream;
-  k_rng_events<false><<<nblk(ctx->nchunks), BLK, 0, s>>>(e);
+  auto pgrid = [&](const void* fn) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(ctx->nchunks), (int64_t)ctx->ncu * per_cu));
+  };
+  if (method == 2) k_ev_scan<2, false><<<pgrid((const void*)k_ev_scan<2, false>), BLK, 0, s>>>(e);
+  else k_ev_scan<3, false><<<pgrid((const void*)k_ev_scan<3, false>), BLK, 0, s>>>(e);
   enqueue_scan(ctx, ctx->d_chunk_count, ctx->d_chunk_off, ctx->nchunks, ctx->d_ev_bsum, ctx->d_nev,
                (const int64_t*)nullptr, cond, launches);
-  k_rng_events<true><<<nblk(ctx->nchunks), BLK, 0, s>>>(e);
-  k_ev_begin<<<nblk(ctx->nscen + 1), BLK, 0, s>>>(ctx->d_chunk_off, ctx->d_chunk_base, ctx->nscen, ctx->d_nev,
-                                                  ctx->d_ev_begin, cond);
-  launches += 3;
+  if (method == 2) k_ev_scan<2, true><<<pgrid((const void*)k_ev_scan<2, true>), BLK, 0, s>>>(e);
+  else k_ev_scan<3, true><<<pgrid((const void*)k_ev_scan<3, true>), BLK, 0, s>>>(e);
+  launches += 2;
   int64_t cap = ctx->ev_cap;
+  unsigned nb = std::max(nblk(cap), nblk(ctx->nscen));
   if (method == 2) {
-    unsigned nb = nblk(cap);
-    k_reach_key<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, cap,
-                                   ctx->d_xscratch, cond);
+    k_ev_sample<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->d_zt, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, ctx->d_ev_pos,
+                                   ctx->d_ev_state, ctx->d_ev_len, ctx->d_cand_val, ctx->d_xscratch, cond);
     unsigned nt = nblk(cap, SCAN_TILE);
-    k_scan_tiles<int64_t, OpMax><<<nt, BLK, 0, s>>>(ctx->d_xscratch, ctx->d_reach, ctx->d_bmax, cap, nullptr, cond);
+    k_scan_tiles<int64_t, OpMax><<<nt, BLK, 0, s>>>(ctx->d_xscratch, ctx->d_reach, ctx->d_bmax, cap, ctx->d_nev,
+                                                    cond);
     k_scan_top<OpMax><<<1, 1024, 0, s>>>(ctx->d_bmax, nt, nullptr, cond);
-    k_reach_decode<<<nb, BLK, 0, s>>>(ctx->d_reach, ctx->d_bmax, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, cap, cond);
-    k_zig_resolve<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->d_reach, ctx->d_nev,
-                                     ctx->nscen, ctx->d_on, cond);
-    k_zig_extra<<<nblk(cap + 1), BLK, 0, s>>>(ctx->d_on, ctx->d_ev_len, ctx->d_nev, cap + 1, ctx->d_xscratch, cond);
-    enqueue_scan(ctx, ctx->d_xscratch, ctx->d_E, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, (const int64_t*)nullptr,
-                 cond, launches);
-    enqueue_scan(ctx, ctx->d_on, ctx->d_C, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, ctx->d_nev, cond, launches);
-    k_zig_compact<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_on, ctx->d_E, ctx->d_C, ctx->d_ev_begin,
-                                     ctx->nscen, ctx->d_nev, ctx->d_key, ctx->d_xtra, ctx->d_ev_val, ctx->d_stream,
-                                     ctx->d_jt, ctx->d_zt, cond);
-    launches += 7;
+    k_ev_resolve<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->nscen, ctx->d_nev,
+                                    ctx->d_reach, ctx->d_bmax, ctx->d_E, cond);
+    enqueue_scan(ctx, ctx->d_E, ctx->d_C, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, ctx->d_nev, cond, launches);
+    k_ev_compact<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos,
+                                    ctx->d_ev_len, ctx->d_cand_val, ctx->d_E, ctx->d_C, ctx->d_key, ctx->d_xtra,
+                                    ctx->d_ev_val, ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);
+    launches += 5;
   } else {
-    k_polar_keys<<<nblk(cap), BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, ctx->d_key, cond);
+    k_ev_polar<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos, ctx->d_key,
+                                  ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);
     launches += 1;
   }
-  k_stream_events<<<nblk(ctx->nscen), BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_C, method,
-                                                   ctx->d_key, ctx->d_xtra, ctx->d_ev_val, ctx->d_bucket,
-                                                   ctx->d_bucket_base, ctx->bshift, cond);
   k_ev_buckets<<<nblk(ctx->nbucket), BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_bucket_base, ctx->nbucket,
                                                   ctx->d_bucket, cond);
-  launches += 2;
+  launches += 1;
 }
 
 template <int DIM, int MODEL, bool COUNT>
@@ -1144,8 +1204,8 @@ static int ensure_fast(cyl_ctx* ctx) {
   HIPCHK(dalloc(ctx->fast_bufs, &ed, m.ne));
   HIPCHK(dalloc(ctx->fast_bufs, &xy, m.nv));
   HIPCHK(dalloc(ctx->fast_bufs, &ctx->d_nodes, m.nv));
-  k_pack_elems<<<nblk(m.nt), BLK, 0, ctx->stream>>>(m, ef, en, te);
-  k_pack_edges<<<nblk(std::max(m.ne, m.nv)), BLK, 0, ctx->stream>>>(m, ed, xy);
+  k_pack_elems<<<nblk(m.nt), BLK, 0, ctx->stream>>>(m, ctx->d_node_new, ef, en, te);
+  k_pack_edges<<<nblk(std::max(m.ne, m.nv)), BLK, 0, ctx->stream>>>(m, ctx->d_node_new, ctx->d_node_old, ed, xy);
   HIPCHK(hipGetLastError());
   ctx->fm.ef = ef;
   ctx->fm.en = en;
@@ -1344,7 +1404,8 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       bool keps = p->diffusion_model >= 2;
       k_pack_nodes<<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(
           fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], keps ? fd.tmp[fd.fid_turbeng] : nullptr,
-          keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / p->schmidt_number, fd.npoints, ctx->d_nodes);
+          keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / p->schmidt_number, fd.npoints, ctx->d_node_old,
+          ctx->d_nodes);
       ++launches;
     }
     a.A = ctx->pool[ctx->cur];
@@ -1537,7 +1598,7 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
   PoolDev& A = ctx->pool[ctx->cur];
   PoolDev& B = ctx->pool[1 - ctx->cur];
   HIPCHK(hipMemsetAsync(ctx->d_scnt, 0, nbin * sizeof(int), s));
-  k_sort_hist<<<nblk(n), BLK, 0, s>>>(A.tri, A.inactive, n, nt, ctx->d_scnt, ctx->d_skey);
+  k_sort_hist<<<nblk(n), BLK, 0, s>>>(A.tri, A.inactive, ctx->d_elem_rank, n, nt, ctx->d_scnt, ctx->d_skey);
   enqueue_scan(ctx, ctx->d_scnt, ctx->d_cursor, (int64_t)nbin, ctx->d_sort_tiles, ctx->d_total,
                (const int64_t*)nullptr, (const int*)nullptr, launches);
   k_sort_place<<<nblk(n), BLK, 0, s>>>(ctx->d_skey, n, ctx->d_cursor, ctx->d_perm);

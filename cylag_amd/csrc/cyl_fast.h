@@ -9,6 +9,9 @@
 //          {vertex0, vertex1, element0, element1}
 //   NodeF  (32 B): time-interpolated u, v and nodal k-epsilon diffusivity of a
 //          node, rewritten once per step by k_pack_nodes
+// Packed node records and node ids use the nodes' Hilbert-curve order, so the
+// nodes of neighbouring elements share cache lines; element and edge ids are
+// the mesh's own.
 // The arithmetic is the same, operation for operation, as update_state_lsm1.pyx
 // (euler_lsm1 :135-177, diffusion_lsm1 :658-726), xylocalizer.pyx:338-435 and
 // compute_bc.pyx:35-395 for model 1.
@@ -37,22 +40,26 @@ struct FastMesh {
   int nt;
 };
 
-// tmp -> packed node records; K = (CMU/sigc) k^2 / max(eps, EPSILON) per node
-// (eulerian_field_set.pyx:1995-1997), the value every particle would compute
+// tmp -> packed node records (packed node i is mesh node old[i]);
+// K = (CMU/sigc) k^2 / max(eps, EPSILON) per node (eulerian_field_set.pyx:1995-1997),
+// the value every particle would compute
 __global__ void __launch_bounds__(BLK) k_pack_nodes(const double* u, const double* v, const double* kt,
-                                                    const double* ep, double fac, int n, NodeF* out) {
+                                                    const double* ep, double fac, int n, const int* old,
+                                                    NodeF* out) {
   int i = blockIdx.x * BLK + threadIdx.x;
   if (i >= n) return;
+  int o = old[i];
   NodeF r;
-  r.u = u[i];
-  r.v = v[i];
-  r.K = kt ? fac * kt[i] * kt[i] / fmax(ep[i], EPS) : 0.;
+  r.u = u[o];
+  r.v = v[o];
+  r.K = kt ? fac * kt[o] * kt[o] / fmax(ep[o], EPS) : 0.;
   r.pad = 0.;
   out[i] = r;
 }
 
-// element records from the mesh tables (built once per mesh)
-__global__ void __launch_bounds__(BLK) k_pack_elems(MeshDev m, ElemF* ef, int4* en, int4* te) {
+// element records from the mesh tables (built once per mesh); node ids in
+// the packed numbering nnew[old]
+__global__ void __launch_bounds__(BLK) k_pack_elems(MeshDev m, const int* nnew, ElemF* ef, int4* en, int4* te) {
   int k = blockIdx.x * BLK + threadIdx.x;
   if (k >= m.nt) return;
   ElemF r;
@@ -66,14 +73,16 @@ __global__ void __launch_bounds__(BLK) k_pack_elems(MeshDev m, ElemF* ef, int4* 
   }
   r.pad = 0.;
   ef[k] = r;
-  en[k] = make_int4(m.tri[3 * k], m.tri[3 * k + 1], m.tri[3 * k + 2], 0);
+  en[k] = make_int4(nnew[m.tri[3 * k]], nnew[m.tri[3 * k + 1]], nnew[m.tri[3 * k + 2]], 0);
   te[k] = make_int4(m.triedge[3 * k], m.triedge[3 * k + 1], m.triedge[3 * k + 2], 0);
 }
 
-__global__ void __launch_bounds__(BLK) k_pack_edges(MeshDev m, int4* ed, double2* xy) {
+__global__ void __launch_bounds__(BLK) k_pack_edges(MeshDev m, const int* nnew, const int* old, int4* ed,
+                                                    double2* xy) {
   int j = blockIdx.x * BLK + threadIdx.x;
-  if (j < m.ne) ed[j] = make_int4(m.edges[2 * j], m.edges[2 * j + 1], m.edgetri[2 * j], m.edgetri[2 * j + 1]);
-  if (j < m.nv) xy[j] = make_double2(m.x[j], m.y[j]);
+  if (j < m.ne)
+    ed[j] = make_int4(nnew[m.edges[2 * j]], nnew[m.edges[2 * j + 1]], m.edgetri[2 * j], m.edgetri[2 * j + 1]);
+  if (j < m.nv) xy[j] = make_double2(m.x[old[j]], m.y[old[j]]);
 }
 
 __device__ __forceinline__ bool in_elem(const ElemF& e, double x, double y) {
@@ -483,8 +492,9 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
 
 // Persistent launch over the slots (see for_slots_xcd); the Ziggurat tables
 // are staged in LDS once per block.
+// 5 waves per SIMD (<= 96 VGPRs) with pre-generated normals, 4 with the inline sampler
 template <int RNG, bool PREGEN>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_lsm1_2d(FastArgs a) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PREGEN ? 5 : 4))) k_lsm1_2d(FastArgs a) {
   if (a.guarded && a.flag[0] == 0) return;
   __shared__ ZigTables zt;
   if (RNG == 2 && !PREGEN) {

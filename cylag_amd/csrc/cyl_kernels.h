@@ -247,25 +247,34 @@ __global__ void __launch_bounds__(BLK) k_scan_tiles(const T* in, int64_t* out, i
                                                     const int64_t* ndev, const int* cond) {
   if (cond && *cond == 0) return;
   __shared__ int64_t lds[BLK / 64];
+  __shared__ int64_t buf[SCAN_TILE];
   Op op;
   int64_t lim = ndev ? (*ndev < n ? *ndev : n) : n;
-  int64_t i0 = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  // coalesced load of the tile, then each thread takes SCAN_ITEMS consecutive values
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) {
+    int64_t i = base + k * BLK + threadIdx.x;
+    buf[k * BLK + threadIdx.x] = i < lim ? (int64_t)in[i] : Op::ident;
+  }
+  __syncthreads();
   int64_t v[SCAN_ITEMS];
   int64_t run = Op::ident;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = i0 + k;
-    v[k] = i < lim ? (int64_t)in[i] : Op::ident;
-    int64_t x = v[k];
+    int64_t x = buf[threadIdx.x * SCAN_ITEMS + k];
     v[k] = run;  // exclusive within the thread
     run = op(run, x);
   }
   int64_t total;
   int64_t pre = block_excl_scan<Op, BLK>(run, total, lds);
 #pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; ++k) buf[threadIdx.x * SCAN_ITEMS + k] = op(pre, v[k]);
+  __syncthreads();
+#pragma unroll
   for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = i0 + k;
-    if (i < n) out[i] = op(pre, v[k]);
+    int64_t i = base + k * BLK + threadIdx.x;
+    if (i < n) out[i] = buf[k * BLK + threadIdx.x];
   }
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
 }
@@ -328,193 +337,18 @@ __global__ void k_scen_totals(const int64_t* scan, const int64_t* total, const i
   scen_units[s] = end - scan[scen_base[s]];
 }
 
-// ---------------------------------------------------------------------------
-// Random-stream events of one step (per scenario window)
-// ---------------------------------------------------------------------------
-constexpr int ZCHUNK = 64;
-
-struct EventArgs {
-  StreamDev* stream;          // [nscen]
-  const JumpTable* jt;        // [nscen]
-  const ZigTables* zt;
-  const int64_t* scen_units;  // units needed per scenario this step
-  const int64_t* chunk_base;  // first chunk of each scenario (host-sized capacity)
-  int nscen;
-  int64_t nchunks;
-  int method;
-  int* chunk_count;           // events per chunk
-  int64_t* chunk_off;         // scanned
-  int64_t* ev_pos;            // candidate position (relative to window start)
-  int64_t* ev_len;            // positions consumed by the normal starting there (zig)
-  int64_t ev_cap;
-  int* overflow;
-  const int* cond;            // run only when *cond != 0
-};
-
-__device__ __forceinline__ int scen_of_chunk(const EventArgs& e, int64_t ch) {
-  int lo = 0, hi = e.nscen - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (e.chunk_base[mid] <= ch) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// window needed for u units: zig normals take ~1.03 positions on average
-__device__ __forceinline__ int64_t window_for(int method, int64_t units) {
-  if (units <= 0) return 0;
-  if (method == 2) return units + units / 8 + 4096;
-  return 2 * (units + units / 2 + 4096);  // polar: attempts of 2 positions, ~21% rejected
-}
-
-// Ziggurat: positions consumed by a normal that starts at state g (before the first call)
-__device__ inline int64_t zig_length(Pcg g, const ZigTables& z) {
-  Draws d;
-  d.zt = &z;
-  d.pre_pos = nullptr;
-  d.g = g;
-  d.ncalls = 0;
-  d.zig();
-  return d.ncalls;
-}
-
-template <bool WRITE>
-__global__ void __launch_bounds__(BLK) k_rng_events(EventArgs e) {
-  if (e.cond && *e.cond == 0) return;
-  int64_t ch = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (ch >= e.nchunks) return;
-  int s = scen_of_chunk(e, ch);
-  int64_t local = ch - e.chunk_base[s];
-  int64_t W = window_for(e.method, e.scen_units[s]);
-  int64_t cap_chunks = (s + 1 < e.nscen ? e.chunk_base[s + 1] : e.nchunks) - e.chunk_base[s];
-  if (!WRITE && threadIdx.x == 0 && blockIdx.x == 0) {
-    for (int q = 0; q < e.nscen; ++q) e.stream[q].window = window_for(e.method, e.scen_units[q]);
-  }
-  int64_t p0 = local * ZCHUNK;
-  int cnt = 0;
-  int64_t w = 0;
-  if (WRITE) w = e.chunk_off[ch];
-  if (p0 < W) {
-    if (!WRITE && (local + 1) * ZCHUNK < W && local + 1 == cap_chunks) atomicOr(e.overflow, 1);
-    const StreamDev& st = e.stream[s];
-    Pcg g;
-    g.inc = st.inc;
-    if (e.method == 2) {
-      g.state = pcg_jump(st.base, (uint64_t)p0, e.jt[s]);
-      for (int q = 0; q < ZCHUNK; ++q) {
-        Pcg at = g;
-        uint32_t u = pcg_next(g);
-        int32_t hz = (int32_t)u;
-        uint32_t iz = (uint32_t)hz & 127u;
-        uint32_t ahz = hz < 0 ? (uint32_t)(-(int64_t)hz) : (uint32_t)hz;
-        if (ahz < e.zt->kn[iz]) continue;
-        if (WRITE) {
-          if (w < e.ev_cap) {
-            e.ev_pos[w] = p0 + q;
-            e.ev_len[w] = zig_length(at, *e.zt);
-          } else atomicOr(e.overflow, 2);
-          ++w;
-        } else ++cnt;
-      }
-    } else {
-      // polar: attempt a uses positions 2a, 2a+1
-      int64_t a0 = p0 / 2;
-      g.state = pcg_jump(st.base, (uint64_t)(2 * a0), e.jt[s]);
-      for (int q = 0; q < ZCHUNK / 2; ++q) {
-        uint32_t u1 = pcg_next(g), u2 = pcg_next(g);
-        if (polar_accept(u1, u2)) continue;
-        if (WRITE) {
-          if (w < e.ev_cap) e.ev_pos[w] = a0 + q;
-          else atomicOr(e.overflow, 2);
-          ++w;
-        } else ++cnt;
-      }
-    }
-  }
-  if (!WRITE) e.chunk_count[ch] = cnt;
-}
-
-// Ziggurat chain resolution.  Position p is off the chain iff an on-chain slow
-// start s < p has s + len(s) - 1 >= p.  A candidate is surely on-chain when no
-// earlier candidate reaches it; from there the chain is followed forward.
-__global__ void __launch_bounds__(BLK) k_zig_resolve(const int64_t* pos, const int64_t* len, const int64_t* ev_begin,
-                                                     const int64_t* reach_excl, const int64_t* nev, int nscen,
-                                                     int* on_chain, const int* cond) {
-  if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= *nev) return;
-  int lo = 0, hi = nscen - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (ev_begin[mid] <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  int64_t b = ev_begin[lo];
-  int64_t j0 = i;
-  while (j0 > b && reach_excl[j0] >= pos[j0]) --j0;
-  int64_t cur_end = pos[j0] + len[j0] - 1;
-  int on = 1;
-  for (int64_t l = j0 + 1; l <= i; ++l) {
-    if (pos[l] > cur_end) {
-      cur_end = pos[l] + len[l] - 1;
-      on = 1;
-    } else on = 0;
-  }
-  on_chain[i] = (i == j0) ? 1 : on;
-}
-
-// reach key of event i: (scenario << 40) | (pos + len - 1); the exclusive
-// max-scan of the keys, decoded per scenario, is the reach of earlier events
-constexpr int REACH_SHIFT = 40;
-
-__global__ void __launch_bounds__(BLK) k_reach_key(const int64_t* pos, const int64_t* len, const int64_t* ev_begin,
-                                                   int nscen, const int64_t* nev, int64_t cap, int64_t* key,
-                                                   const int* cond) {
-  if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= cap) return;
-  if (i >= *nev) {
-    key[i] = -1;
-    return;
-  }
-  int lo = 0, hi = nscen - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (ev_begin[mid] <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  key[i] = ((int64_t)lo << REACH_SHIFT) | (pos[i] + len[i] - 1);
-}
-
-__global__ void __launch_bounds__(BLK) k_reach_decode(int64_t* out, const int64_t* bmax, const int64_t* ev_begin,
-                                                      int nscen, const int64_t* nev, int64_t cap, const int* cond) {
-  if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= cap || i >= *nev) return;
-  int64_t m = out[i], bm = bmax[i / SCAN_TILE];
-  m = bm > m ? bm : m;
-  int lo = 0, hi = nscen - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (ev_begin[mid] <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  out[i] = (m >= 0 && (m >> REACH_SHIFT) == lo) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
-}
-
 // Element re-sort: counting sort of the slots by a key that groups 2^SORT_SHIFT
-// neighbouring element ids (active rows), then rows without element, then
-// inactive rows.  The input is nearly sorted after the first sort, so equal
+// elements neighbouring along a Hilbert curve through the element centroids
+// (active rows), then rows without element, then inactive rows.  The input is nearly sorted after the first sort, so equal
 // keys come in runs of adjacent lanes: the first lane of a run does one atomic
 // for the whole run.
 constexpr int SORT_SHIFT = 2;
 
 __device__ __forceinline__ int sort_nbins(int nt) { return (nt >> SORT_SHIFT) + 3; }
 
-__device__ __forceinline__ int sort_key(int t, int inactive, int nt) {
+__device__ __forceinline__ int sort_key(int t, int inactive, const int* elem_rank, int nt) {
   int nb = (nt >> SORT_SHIFT) + 1;
-  return inactive ? nb + 1 : (t >= 0 ? t >> SORT_SHIFT : nb);
+  return inactive ? nb + 1 : (t >= 0 ? elem_rank[t] >> SORT_SHIFT : nb);
 }
 
 struct WaveRun {
@@ -537,11 +371,11 @@ __device__ __forceinline__ WaveRun wave_run(int key) {
   return r;
 }
 
-__global__ void __launch_bounds__(BLK) k_sort_hist(const int* tri, const int* inactive, int n, int nt, int* cnt,
-                                                   int* key) {
+__global__ void __launch_bounds__(BLK) k_sort_hist(const int* tri, const int* inactive, const int* elem_rank, int n,
+                                                   int nt, int* cnt, int* key) {
   int j = blockIdx.x * BLK + threadIdx.x;
   bool valid = j < n;
-  int k = valid ? sort_key(tri[j], inactive[j], nt) : -1 - (int)(threadIdx.x & 63);
+  int k = valid ? sort_key(tri[j], inactive[j], elem_rank, nt) : -1 - (int)(threadIdx.x & 63);
   if (valid) key[j] = k;
   WaveRun r = wave_run(k);
   if (valid && r.head) atomicAdd(&cnt[k], r.len);
@@ -567,110 +401,6 @@ __global__ void __launch_bounds__(BLK) k_permute(const T* src, T* dst, const int
 __global__ void __launch_bounds__(BLK) k_slot_of_pool(const int* gpool, int n, int* slot_of_pool) {
   int j = blockIdx.x * BLK + threadIdx.x;
   if (j < n) slot_of_pool[gpool[j]] = j;
-}
-
-__device__ __forceinline__ int scen_of_event(const int64_t* ev_begin, int nscen, int64_t i) {
-  int lo = 0, hi = nscen - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (ev_begin[mid] <= i) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
-// ev_begin[s] = first event of scenario s; ev_begin[nscen] = number of events
-__global__ void k_ev_begin(const int64_t* chunk_off, const int64_t* chunk_base, int nscen, const int64_t* total,
-                           int64_t* ev_begin, const int* cond) {
-  if (cond && *cond == 0) return;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nscen) ev_begin[s] = chunk_off[chunk_base[s]];
-  if (s == 0) ev_begin[nscen] = *total;
-}
-
-__global__ void __launch_bounds__(BLK) k_zig_extra(const int* on, const int64_t* len, const int64_t* nev,
-                                                   int64_t cap, int64_t* extra, const int* cond) {
-  if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= cap) return;
-  extra[i] = (i < *nev && on[i]) ? len[i] - 1 : 0;
-}
-
-// compact on-chain events: normal index, cumulative extra positions (inclusive)
-// and the value of the slow normal (sampled from its start position)
-__global__ void __launch_bounds__(BLK) k_zig_compact(const int64_t* pos, const int64_t* len, const int* on,
-                                                     const int64_t* E, const int64_t* Cidx, const int64_t* ev_begin,
-                                                     int nscen, const int64_t* nev, int64_t* key, int64_t* xtra,
-                                                     double* val, const StreamDev* st, const JumpTable* jt,
-                                                     const ZigTables* zt, const int* cond) {
-  if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= *nev || !on[i]) return;
-  int s = scen_of_event(ev_begin, nscen, i);
-  int64_t eb = E[ev_begin[s]];
-  int64_t c = Cidx[i];
-  key[c] = pos[i] - (E[i] - eb);
-  xtra[c] = (E[i] - eb) + len[i] - 1;
-  Draws d;
-  d.zt = zt;
-  d.pre_pos = nullptr;
-  d.g.inc = st[s].inc;
-  d.g.state = pcg_jump(st[s].base, (uint64_t)pos[i], jt[s]);
-  d.ncalls = 0;
-  val[c] = d.zig();
-}
-
-// polar: key_j = r_j - j (j counted within the scenario)
-__global__ void __launch_bounds__(BLK) k_polar_keys(const int64_t* pos, const int64_t* ev_begin, int nscen,
-                                                    const int64_t* nev, int64_t* key, const int* cond) {
-  if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= *nev) return;
-  int s = scen_of_event(ev_begin, nscen, i);
-  key[i] = pos[i] - (i - ev_begin[s]);
-}
-
-// point each scenario's stream at its slice of the event tables
-__global__ void k_stream_events(StreamDev* st, int nscen, const int64_t* ev_begin, const int64_t* Cidx, int method,
-                                int64_t* key, int64_t* xtra, double* val, const int* bucket,
-                                const int64_t* bucket_base, int bshift, const int* cond) {
-  if (cond && *cond == 0) return;
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nscen) return;
-  st[s].ev_bucket = bucket + bucket_base[s];
-  st[s].n_bucket = bucket_base[s + 1] - bucket_base[s];
-  st[s].ev_bshift = bshift;
-  if (method == 2) {
-    int64_t c0 = Cidx[ev_begin[s]], c1 = Cidx[ev_begin[s + 1]];
-    st[s].ev_key = key + c0;
-    st[s].ev_extra = xtra + c0;
-    st[s].ev_val = val + c0;
-    st[s].n_ev = c1 - c0;
-  } else {
-    st[s].ev_key = key + ev_begin[s];
-    st[s].ev_extra = nullptr;
-    st[s].ev_val = nullptr;
-    st[s].n_ev = ev_begin[s + 1] - ev_begin[s];
-  }
-}
-
-// bucket index of every scenario's event keys: entry b = lower_bound(b << shift)
-__global__ void __launch_bounds__(BLK) k_ev_buckets(const StreamDev* st, int nscen, const int64_t* bucket_base,
-                                                    int64_t ntot, int* bucket, const int* cond) {
-  if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= ntot) return;
-  int s = 0;
-  while (s + 1 < nscen && bucket_base[s + 1] <= i) ++s;
-  int64_t b = i - bucket_base[s], u = b << st[s].ev_bshift;
-  const int64_t* key = st[s].ev_key;
-  int64_t lo = 0, hi = st[s].n_ev;
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (key[mid] < u) lo = mid + 1;
-    else hi = mid;
-  }
-  bucket[i] = (int)lo;
 }
 
 // end of step: advance each scenario's stream past its last draw, reset counters
