@@ -322,7 +322,7 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
           w = x1 * x1 + x2 * x2;
           pos += 2;
         }
-        w = sqrt((-2.0 * log(w)) / w);
+        w = sqrt((-2.0 * cr_log(w)) / w);
         v[0] = x1 * w;
         v[1] = x2 * w;
       }

@@ -4,6 +4,7 @@
 // fluid acceleration.  Each routine restates the expression order of
 // cylag/core/eulerian_field_set.pyx so results are bit-identical.
 #pragma once
+#include "cyl_crmath.h"
 #include "cyl_types.h"
 
 namespace cyl {
@@ -179,12 +180,12 @@ __device__ inline double friction_calc(const FieldsDev& f, double h) {
   double c = f.friction_coefficient;
   switch (f.friction_model) {
     case 0: return 2. * GRAV / (c * c);
-    case 1: return 2. * GRAV / (c * c) / pow(fmax(EPS, h), 1. / 3.);
-    case 2: return 2. * GRAV * (c * c) / pow(fmax(EPS, h), 1. / 3.);
+    case 1: return 2. * GRAV / (c * c) / cr_pow(fmax(EPS, h), 1. / 3.);
+    case 2: return 2. * GRAV * (c * c) / cr_pow(fmax(EPS, h), 1. / 3.);
     case 3: {
       double aux = h * 11.036 / c;
       aux = aux > 1.001 ? aux : 1.001;
-      double q = log(aux) / KAPPA;
+      double q = cr_log(aux) / KAPPA;
       return 2. / (q * q);
     }
   }
@@ -203,8 +204,8 @@ __device__ inline void pseudo3d(const FieldsDev& f, const MeshDev& m, int k, dou
   double h = fmax(EPS, zs - zb);
   double sigma = fmin(1., fmax(0., (z - zb) / h));
   double Cf = friction_calc(f, h);
-  double z0 = fmax(EPS, h * exp(-(1. + KAPPA / sqrt(Cf / 2.))));
-  hor = (z - zb <= z0) ? 0.0 : log((z - zb) / z0) / (log(h / z0) - 1.);
+  double z0 = fmax(EPS, h * cr_exp(-(1. + KAPPA / sqrt(Cf / 2.))));
+  hor = (z - zb <= z0) ? 0.0 : cr_log((z - zb) / z0) / (cr_log(h / z0) - 1.);
   double uc = ux * hor, vc = uy * hor;
   double dzsdx = p1g(zs_n, b, gx), dzsdy = p1g(zs_n, b, gy);
   double dzbdx = p1g(zb_n, b, gx), dzbdy = p1g(zb_n, b, gy);

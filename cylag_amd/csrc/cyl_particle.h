@@ -123,13 +123,13 @@ struct Draws {
       if (ahz < zt->kn[iz]) return hz * zt->wn[iz];
       if (iz == 0) {
         for (;;) {
-          double x = -log(u32_to_uniform(next())) * 0.2904764;
-          double y = -log(u32_to_uniform(next()));
+          double x = -cr_log(u32_to_uniform(next())) * 0.2904764;
+          double y = -cr_log(u32_to_uniform(next()));
           if (y + y >= x * x) return hz > 0 ? ZIG_R + x : -ZIG_R - x;
         }
       }
       double x = hz * zt->wn[iz];
-      if (zt->fn[iz] + u32_to_uniform(next()) * (zt->fn[iz - 1] - zt->fn[iz]) < exp(-0.5 * x * x)) return x;
+      if (zt->fn[iz] + u32_to_uniform(next()) * (zt->fn[iz - 1] - zt->fn[iz]) < cr_exp(-0.5 * x * x)) return x;
     }
   }
 
@@ -141,7 +141,7 @@ struct Draws {
       x2 = 2.0 * u32_to_uniform(next()) - 1.0;
       w = x1 * x1 + x2 * x2;
     }
-    w = sqrt((-2.0 * log(w)) / w);
+    w = sqrt((-2.0 * cr_log(w)) / w);
     a = x1 * w;
     b = x2 * w;
   }
@@ -356,7 +356,7 @@ __device__ inline double buoyancy_velocity(const ParticleConst& pc, double dp) {
   double s = pc.density / pc.rho0, wb;
   if (dp < 1.e-4) wb = fabs(s - 1.) * (GRAV * (dp * dp)) / (18. * NU0);
   else if (dp > 1.e-3) wb = sqrt(fabs(s - 1.) * (4. * dp * GRAV) / (3. * pc.drag_coef));
-  else wb = (10. * NU0 / dp) * (sqrt(1. + fabs(s - 1.) * GRAV * pow(dp, 3.0) / (100. * (NU0 * NU0))) - 1.);
+  else wb = (10. * NU0 / dp) * (sqrt(1. + fabs(s - 1.) * GRAV * cr_pow(dp, 3.0) / (100. * (NU0 * NU0))) - 1.);
   if (s > 1.) wb *= -1.;
   return wb;
 }
@@ -418,13 +418,13 @@ __device__ inline int update_test(Ctx& c, Part& p) {
 __device__ inline double drag_coefficient(const ParticleConst& pc, double U, double dp) {
   double Re = fmax(EPS, U * dp / NU0);
   if (pc.drag_coef_model == 0) return pc.drag_coef;
-  if (pc.drag_coef_model == 1) return Re <= 1000. ? (24. / Re) * (1. + 0.15 * pow(Re, 0.687)) : 0.44;
+  if (pc.drag_coef_model == 1) return Re <= 1000. ? (24. / Re) * (1. + 0.15 * cr_pow(Re, 0.687)) : 0.44;
   if (pc.drag_coef_model == 2) {
-    double phi1 = pow(24. / Re, 10.) + pow(21. * pow(Re, -0.67), 10.) + pow(4. * pow(Re, -0.33), 10.) + pow(0.4, 10.);
-    double phi2 = 1. / (pow(0.148 * pow(Re, 0.11), -10.) + pow(0.5, -10.));
-    double phi3 = pow(1.57 * pow(Re, -1.625) * 1.e8, 10.);
-    double phi4 = 1. / (pow(6e-17 * pow(Re, 2.63), -10.) + pow(0.2, -10.));
-    return pow(1. / (pow(phi1 + phi2, -1.) + pow(phi3, -1.)) + phi4, 1 / 10.);
+    double phi1 = cr_pow(24. / Re, 10.) + cr_pow(21. * cr_pow(Re, -0.67), 10.) + cr_pow(4. * cr_pow(Re, -0.33), 10.) + cr_pow(0.4, 10.);
+    double phi2 = 1. / (cr_pow(0.148 * cr_pow(Re, 0.11), -10.) + cr_pow(0.5, -10.));
+    double phi3 = cr_pow(1.57 * cr_pow(Re, -1.625) * 1.e8, 10.);
+    double phi4 = 1. / (cr_pow(6e-17 * cr_pow(Re, 2.63), -10.) + cr_pow(0.2, -10.));
+    return cr_pow(1. / (cr_pow(phi1 + phi2, -1.) + cr_pow(phi3, -1.)) + phi4, 1 / 10.);
   }
   return 0.44;
 }
@@ -583,7 +583,7 @@ __device__ inline bool lsm2_coefficients(double a1, double dt, double& beta, dou
                                          double& quu, double& qxu, double& qxx) {
   double r = a1 * dt;
   if (!isfinite(a1) || !isfinite(dt) || a1 < 0. || dt < 0. || !isfinite(r)) return false;
-  beta = exp(-r);
+  beta = cr_exp(-r);
   if (r < 0.1) {
     double phi1 = 1., phi2 = 0.5, term = 1.;
     for (int n = 1; n < 13; ++n) {
@@ -595,11 +595,11 @@ __device__ inline bool lsm2_coefficients(double a1, double dt, double& beta, dou
     D2 = dt * r * phi2;
     fxc = dt * dt * phi2;
   } else {
-    D1 = -expm1(-r) / a1;
+    D1 = -cr_expm1(-r) / a1;
     D2 = dt - D1;
     fxc = D2 / a1;
   }
-  quu = (r == 0.) ? dt : dt * (-expm1(-2. * r) / (2. * r));
+  quu = (r == 0.) ? dt : dt * (-cr_expm1(-2. * r) / (2. * r));
   qxu = 0.5 * D1 * D1;
   if (r < 1.) {
     double term = 1. / 6., power2 = 4., q = 1. / 3.;
@@ -633,7 +633,7 @@ __device__ inline void di_lsm2(Ctx& c, Part& p, Draws& dr, int dm, double nux, d
     c.err = E_LSM2_COEF;
     return;
   }
-  double um_beta = -expm1(-a1 * dt);
+  double um_beta = -cr_expm1(-a1 * dt);
   double b2[3] = {0., 0., 0.};
   if (dm == 2 || dm == 3) {
     double tk, ep;

@@ -45,9 +45,10 @@ class Case:
     """mesh + fields + initial rows; `fields` is a dict of EulerianFieldSet kwargs."""
 
     def __init__(self, x, y, tri, fields, position, velocity=None, pool=None, boundary_points=None,
-                 nopen=0, use_trifinder=True, pset_kw=None, mpl_tri=None):
+                 nopen=0, use_trifinder=True, pset_kw=None, mpl_tri=None, pdim=None):
         self.x, self.y, self.tri = x, y, tri
         self.fields = fields
+        self.pdim = pdim if pdim is not None else fields.get("dim", 2)
         self.position = np.asarray(position, dtype=np.float64)
         self.velocity = velocity
         self.pool = pool
@@ -70,7 +71,7 @@ class Case:
         mesh = cy.TriangularMesh(self.x, self.y, self.tri, boundary_points=self.boundary_points,
                                  nopen=self.nopen)
         fs = cy.EulerianFieldSet(mesh, trifinder=self.trifinder(), **self.fields)
-        ps = cy.LagrangianParticleSet(self.position, self.velocity, dim=self.fields.get("dim", 2),
+        ps = cy.LagrangianParticleSet(self.position, self.velocity, dim=self.pdim,
                                       initial_pool_size=self.pool, **self.pset_kw)
         cy.pcg32_seed(*SEED)
         return cy.Solver(fs, ps, params)
@@ -94,7 +95,7 @@ class Case:
         f = dict(self.fields)
         fs = R.RefFields(mesh, **{k: v for k, v in f.items() if k != "trifinder"})
         kw = {k: v for k, v in self.pset_kw.items()}
-        ps = R.RefParticles(self.position, self.velocity, dim=f.get("dim", 2),
+        ps = R.RefParticles(self.position, self.velocity, dim=self.pdim,
                             initial_pool_size=self.pool, **kw)
         rng = R.RefRng(*SEED)
         s = R.RefSolver(fs, ps, params, trifinder=self.trifinder(), rng=rng)
@@ -137,6 +138,50 @@ def reference_square_case(position, velocity=None, pool=None):
     return Case(x, y, tri, fields, position, velocity, pool, boundary_points=bp)
 
 
+def _square_mesh():
+    x = np.loadtxt(os.path.join(GOLDEN, "square_meshx.txt"))
+    y = np.loadtxt(os.path.join(GOLDEN, "square_meshy.txt"))
+    tri = np.loadtxt(os.path.join(GOLDEN, "square_meshtri.txt"), dtype=np.int32)
+    from oracle.refpy import read_cli
+    bp, _ = read_cli(os.path.join(GOLDEN, "square_geo.cli"))
+    return x, y, tri, bp
+
+
+P4_3D = np.array([[5000., 0., 0.5], [5000., 500., 0.9999], [1000., 0., 0.5], [1000., 500., 0.9999]])
+
+
+def reference_square_3d_case(position=P4_3D, pool=4, nlayers=11):
+    """The reference's 3D test field set: 11 uniform sigma planes on [0, 1], the
+    solid-body rotation scaled by z, w = 0
+    (tests/test_21_lsm1_analytical_advection_3d.py:15-84)."""
+    x, y, tri, bp = _square_mesh()
+    om = 2. * np.pi / 3600.
+    r, phi = np.sqrt(x ** 2 + y ** 2), np.arctan2(x, y)
+    n = x.size
+    dz = 1. / max(nlayers - 1, 1)
+    z = np.repeat(np.arange(nlayers) * dz, n)
+    rr, pp = np.tile(r, nlayers), np.tile(phi, nlayers)
+    fields = dict(dim=3, nlayers=nlayers, times=np.zeros(1), elevation=z[None],
+                  velocity_x=(om * rr * np.cos(pp) * z)[None], velocity_y=(-om * rr * np.sin(pp) * z)[None],
+                  velocity_z=np.zeros((1, n * nlayers)))
+    return Case(x, y, tri, fields, position, None, pool, boundary_points=bp)
+
+
+def reference_square_p3d_case(position=P4_3D, pool=4):
+    """Pseudo-3D: the 2D rotation field with a zero BOTTOM field, unit depth and
+    log-law friction (friction_model 1, coefficient 40), 3D particles
+    (tests/test_22_lsm1_analytical_advection_p3d.py:15-72)."""
+    x, y, tri, bp = _square_mesh()
+    om = 2. * np.pi / 3600.
+    r, phi = np.sqrt(x ** 2 + y ** 2), np.arctan2(x, y)
+    n = x.size
+    fields = dict(dim=2, nlayers=1, times=np.zeros(1), elevation=np.ones((1, n)),
+                  velocity_x=(om * r * np.cos(phi))[None], velocity_y=(-om * r * np.sin(phi))[None],
+                  opt_fields=np.zeros((1, 1, n)), opt_fields_names=['BOTTOM'],
+                  friction_model=1, friction_coefficient=40.)
+    return Case(x, y, tri, fields, position, None, pool, boundary_points=bp, pdim=3)
+
+
 def vortex_case(nx=41, ny=11, lx=2000., ly=500., npart=2000, seed=3, ntimes=2, keps=True,
                 wavelength=400., dead_zone=False):
     """Small version of the unstructured 2D workload: jittered Delaunay mesh,
@@ -167,6 +212,43 @@ def vortex_case(nx=41, ny=11, lx=2000., ly=500., npart=2000, seed=3, ntimes=2, k
     pos = np.stack([rng.uniform(0.05 * lx, 0.95 * lx, npart),
                     rng.uniform(0.05 * ly, 0.95 * ly, npart)], 1)
     return Case(x, y, tri, fields, pos, mpl_tri=mpl)
+
+
+def vortex_3d_case(nx=21, ny=7, lx=2000., ly=500., npart=1500, seed=4, nlayers=5, ntimes=2,
+                   wavelength=400.):
+    """3D sigma-layer version of the vortex workload: nlayers planes between a
+    smooth bottom -h(x, y) and the surface 0, horizontal velocity growing with
+    height, w = 0, k-epsilon fields giving a parabolic K(z) (nodal k, eps per
+    plane); particles spread through the water column."""
+    x, y, tri, mpl = jittered_delaunay(nx, ny, lx, ly, seed=7)
+    n = x.size
+    k = 2. * np.pi / wavelength
+    h = 5. + 10. * (0.5 + 0.5 * np.sin(x / lx * 3.1) * np.cos(y / ly * 2.3))
+    sig = np.linspace(0., 1., nlayers)           # 0 bottom .. 1 surface
+    z = np.concatenate([-h + s_ * h for s_ in sig])
+    prof = np.concatenate([np.full(n, 0.3 + 0.7 * s_) for s_ in sig])
+    xx, yy = np.tile(x, nlayers), np.tile(y, nlayers)
+    us, vs = [], []
+    for t in range(ntimes):
+        ph = 0.3 * t
+        us.append(np.sin(k * xx + ph) * np.cos(k * yy) * prof)
+        vs.append(-np.cos(k * xx + ph) * np.sin(k * yy) * prof)
+    K = np.concatenate([1e-2 + 0.2 * s_ * (1.05 - s_) * np.ones(n) for s_ in sig])
+    eps = np.full(n * nlayers, 1e-3)
+    tke = np.sqrt(K * eps / (0.09 / 0.72))
+    fields = dict(dim=3, nlayers=nlayers, times=np.linspace(0., 10000., ntimes),
+                  elevation=np.tile(z, (ntimes, 1)), velocity_x=np.array(us), velocity_y=np.array(vs),
+                  velocity_z=np.zeros((ntimes, n * nlayers)),
+                  opt_fields=np.stack([np.tile(tke, (ntimes, 1)), np.tile(eps, (ntimes, 1))]),
+                  opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
+    rng = np.random.default_rng(seed)
+    px = rng.uniform(0.05 * lx, 0.95 * lx, npart)
+    py = rng.uniform(0.05 * ly, 0.95 * ly, npart)
+    # depth at the particle from the nodal depth (nearest node: only to place it in the column)
+    from scipy.spatial import cKDTree
+    hn = h[cKDTree(np.stack([x, y], 1)).query(np.stack([px, py], 1))[1]]
+    pz = -hn * rng.uniform(0.15, 0.85, npart)
+    return Case(x, y, tri, fields, np.stack([px, py, pz], 1), mpl_tri=mpl)
 
 
 def open_channel_case(n=21, npart=400, u=0.8, seed=5):

@@ -6,7 +6,10 @@ import pytest
 
 import cylag_amd as cy
 from oracle import refpy as R
-from tests.cases import open_channel_case, reference_square_case, square_case, vortex_case
+from tests.cases import (open_channel_case, reference_square_case, square_case, vortex_3d_case,
+                         vortex_case)
+from tests.test_oracle_golden import (GOLDEN_3D, GOLDEN_3D_IDS, LSM2_GOLDEN, P4, golden_3d_setup,
+                                      lsm2_params)
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +75,56 @@ def test_model0_rebound_golden():
     xp = g.position[np.argsort(g.inactive, kind="stable")][:int((g.inactive == 0).sum())]
     from tests.test_oracle_golden import REBOUND_EXPECTED
     np.testing.assert_allclose(xp, REBOUND_EXPECTED, rtol=1e-7)
+
+
+@pytest.mark.parametrize("ts,dt,dm,opt,tf,expected", LSM2_GOLDEN,
+                         ids=["euler", "rk2", "rk4", "di1", "di_opt2", "euler_maruyama"])
+def test_lsm2_2d_golden(ts, dt, dm, opt, tf, expected):
+    """LSM2 (velocity as state) on the reference tests' rotation field."""
+    case = reference_square_case(P4, pool=1000)
+    params = lsm2_params(ts, dt, dm, opt, tf)
+    g = case.run_gpu(cy, params)
+    assert_parity(g, case.run_oracle(R, params))
+    np.testing.assert_allclose(g.position[g.inactive == 0], np.array(expected), rtol=1e-7)
+
+
+@pytest.mark.parametrize("kind,model,ts,dt,dm,opt,expected", GOLDEN_3D, ids=GOLDEN_3D_IDS)
+def test_3d_and_pseudo3d_golden(kind, model, ts, dt, dm, opt, expected):
+    """3D sigma-layer fields (prism interpolation, z-localizer) and 3D particles
+    in 2D fields (log-law pseudo-3D corrections)."""
+    case, params = golden_3d_setup(kind, model, ts, dt, dm, opt)
+    g = case.run_gpu(cy, params)
+    assert_parity(g, case.run_oracle(R, params))
+    np.testing.assert_allclose(g.position[g.inactive == 0], np.array(expected), rtol=1e-7)
+
+
+V3D = [
+    dict(model=1, diffusion_model=1, horizontal_diffusivity=0.1, vertical_diffusivity=1e-3),
+    dict(model=1, diffusion_model=3),
+    dict(model=2, time_scheme=5, diffusion_model=3, diffusion_lsm2_option=1, particle_velocity_init=1),
+    dict(model=2, time_scheme=5, diffusion_model=3, diffusion_lsm2_option=2, particle_velocity_init=1),
+    dict(model=2, time_scheme=1, diffusion_model=1, diffusion_lsm2_option=2, particle_velocity_init=1,
+         horizontal_diffusivity=0.1, vertical_diffusivity=1e-3),
+]
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+@pytest.mark.parametrize("which", range(len(V3D)), ids=["lsm1_const", "lsm1_keps3d", "lsm2_di1", "lsm2_di2",
+                                                        "lsm2_em"])
+def test_3d_stochastic(which, rng_method):
+    """Stochastic 3D steps: K(z) from k-epsilon planes and its gradient,
+    sigma-layer localization, bottom and surface reflections, two frames."""
+    case = vortex_3d_case()
+    params = dict(final_time=300., time_step=5., rng_method=rng_method, **V3D[which])
+    assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
+
+
+@pytest.mark.parametrize("opt", [1, 2, 3])
+def test_lsm2_2d_keps_stochastic(opt):
+    case = vortex_case(npart=2000)
+    params = dict(final_time=200., time_step=5., model=2, time_scheme=5, diffusion_model=2,
+                  diffusion_lsm2_option=opt, particle_velocity_init=1)
+    assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
 
 
 @pytest.mark.parametrize("rng_method", [1, 2, 3])

@@ -125,3 +125,14 @@ def test_host_rng_matches_oracle():
             assert cy.generate_random_pair(method) == g.pair(method)
         assert cy.random_utils.get_state() == g.state()
     cy.pcg32_seed(42, 0xda3e06d11bb2b203)
+
+
+def test_crmath_tables_match_generator():
+    """The double-double constants of the device exp/log are the generator's output."""
+    import subprocess
+    import sys
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "cylag_amd", "csrc")
+    out = subprocess.run([sys.executable, os.path.join(here, "gen_crmath_tables.py")], capture_output=True,
+                         text=True, check=True).stdout
+    with open(os.path.join(here, "cyl_crmath_tables.h")) as f:
+        assert f.read() == out

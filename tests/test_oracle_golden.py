@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import refpy as R
-from tests.cases import reference_square_case
+from tests.cases import reference_square_3d_case, reference_square_case, reference_square_p3d_case
 
 P4 = np.array([[5000., 0.], [5000., 500.], [1000., 0.], [1000., 500.]])
 
@@ -38,6 +38,83 @@ def test_lsm1_euler_maruyama_zero_diffusivity_golden():
         [[5.28176591e+03, 3.36909449e+00], [5.28142900e+03, 5.31545686e+02],
          [1.05635318e+03, 6.73818897e-01], [1.05601627e+03, 5.28850410e+02]]), rtol=1e-7)
     assert out.rng.state() == R.RefRng(42, 0xda3e06d11bb2b203).state()
+
+
+LSM2_GOLDEN = [
+    # tests/test_30_lsm2_analytical_advection_2d.py: (time_scheme, dt, diffusion_model,
+    # diffusion_lsm2_option, final_time, expected positions)
+    (1, 1., 0, 1, 3600., [[5.02240714e+03, 1.59416032e-02], [5.02231846e+03, 5.02247697e+02],
+                          [1.01149457e+03, 3.88291522e-02], [1.01090280e+03, 5.05494818e+02]]),  # :106-110
+    (2, 1., 0, 1, 3600., [[5.04996265e+03, 1.42800005e-01], [5.04986094e+03, 5.05129760e+02],
+                          [1.01703735e+03, 8.91474852e-02], [1.01641878e+03, 5.08313138e+02]]),  # :177-181
+    (4, 1., 0, 1, 3600., [[5.04996154e+03, 1.58905815e-01], [5.04985822e+03, 5.05145755e+02],
+                          [1.01703715e+03, 9.23901032e-02], [1.01641697e+03, 5.08316281e+02]]),  # :248-252
+    (5, 20., 0, 1, 3600., [[5.58482914e+03, 1.46424824e+01], [5.58333674e+03, 5.73120494e+02],
+                           [1.12027542e+03, 3.17155011e+00], [1.11839840e+03, 5.63135678e+02]]),  # :320-324
+    (5, 20., 1, 2, 3600., [[5.58482914e+03, 1.46424824e+01], [5.58333674e+03, 5.73120494e+02],
+                           [1.12027542e+03, 3.17155011e+00], [1.11839840e+03, 5.63135678e+02]]),  # :389-393
+    (1, 0.005, 1, 2, 50., [[4981.651198, -435.835882], [5025.233608, 62.329215],
+                           [996.422963, -87.174511], [1040.00244, 411.033803]]),  # :460-464
+]
+
+
+def lsm2_params(ts, dt, dm, opt, tf):
+    return dict(final_time=tf, time_step=dt, time_scheme=ts, frozen_eulerian_fields=True, model=2,
+                particle_velocity_init=1, diffusion_model=dm, diffusion_lsm2_option=opt,
+                horizontal_diffusivity=0., vertical_diffusivity=0.)
+
+
+@pytest.mark.parametrize("ts,dt,dm,opt,tf,expected", LSM2_GOLDEN,
+                         ids=["euler", "rk2", "rk4", "di1", "di_opt2", "euler_maruyama"])
+def test_lsm2_advection_golden(ts, dt, dm, opt, tf, expected):
+    case = reference_square_case(P4, pool=1000)
+    out = case.run_oracle(R, lsm2_params(ts, dt, dm, opt, tf))
+    np.testing.assert_allclose(out.position[out.inactive == 0], np.array(expected), rtol=1e-7)
+
+
+GOLDEN_3D = [
+    # (case, model, time_scheme, dt, diffusion_model, diffusion_lsm2_option, expected)
+    # tests/test_21_lsm1_analytical_advection_3d.py:126, 200, 274, 349
+    ("3d", 1, 1, 10., 0, 1, [[-5.06900817e+03, -4.04228110e-01, 0.5], [5.28103619e+03, 5.34857127e+02, 0.9999],
+                             [-1.01380163e+03, -8.08456219e-02, 0.5], [1.05567231e+03, 5.29507814e+02, 0.9999]]),
+    ("3d", 1, 2, 200., 0, 1, [[-5.00982318e+03, 7.91738560e+01, 0.5], [5.19393649e+03, -1.16404904e+02, 0.9999],
+                              [-1.00196464e+03, 1.58347712e+01, 0.5], [1.08914758e+03, 3.87197910e+02, 0.9999]]),
+    ("3d", 1, 4, 200., 0, 1, [[-4.99998240e+03, -1.20144922e-01, 0.5], [4.99819692e+03, 5.06745590e+02, 0.9999],
+                              [-9.99996480e+02, -2.40289843e-02, 0.5], [9.99090798e+02, 5.01259730e+02, 0.9999]]),
+    ("3d", 1, 1, 10., 1, 1, [[-5.06900817e+03, -4.04228110e-01, 0.5], [5.28103619e+03, 5.34857127e+02, 0.9999],
+                             [-1.01380163e+03, -8.08456219e-02, 0.5], [1.05567231e+03, 5.29507814e+02, 0.9999]]),
+    # tests/test_31_lsm2_analytical_advection_3d.py:122-128, 206
+    ("3d", 2, 5, 10., 0, 1, [[-5.07600851e+03, -5.30137280e-01, 0.5], [5.29211338e+03, 5.36375306e+02, 0.9999],
+                             [-1.01721737e+03, -1.44511801e-01, 0.5], [1.06172324e+03, 5.32820087e+02, 0.9999]]),
+    ("3d", 2, 5, 10., 1, 2, [[-5.07600851e+03, -5.30137280e-01, 0.5], [5.29211338e+03, 5.36375306e+02, 0.9999],
+                             [-1.01721737e+03, -1.44511801e-01, 0.5], [1.06172324e+03, 5.32820087e+02, 0.9999]]),
+    # tests/test_22_lsm1_analytical_advection_p3d.py:116
+    ("p3d", 1, 1, 10., 0, 1, [[4.96199336e+03, -1.90971737e+03, 0.5], [2.46890626e+03, -4.83763012e+03, 0.9999],
+                              [9.92398673e+02, -3.81943473e+02, 0.5], [8.96515570e+02, -8.10286955e+02, 0.9999]]),
+    # tests/test_32_lsm2_analytical_advection_p3d.py:118, 191
+    ("p3d", 2, 5, 10., 0, 1, [[4.97261675e+03, -1.91332310e+03, 0.5], [2.47449397e+03, -4.84746475e+03, 0.9999],
+                              [9.98973944e+02, -3.84159272e+02, 0.5], [9.02721202e+02, -8.15390229e+02, 0.9999]]),
+    ("p3d", 2, 5, 10., 1, 2, [[4.97261675e+03, -1.91332310e+03, 0.5], [2.47449397e+03, -4.84746475e+03, 0.9999],
+                              [9.98973944e+02, -3.84159272e+02, 0.5], [9.02721202e+02, -8.15390229e+02, 0.9999]]),
+]
+GOLDEN_3D_IDS = ["3d_lsm1_euler", "3d_lsm1_rk2", "3d_lsm1_rk4", "3d_lsm1_em", "3d_lsm2_di1", "3d_lsm2_di2",
+                 "p3d_lsm1_euler", "p3d_lsm2_di1", "p3d_lsm2_di2"]
+
+
+def golden_3d_setup(kind, model, ts, dt, dm, opt):
+    case = reference_square_3d_case() if kind == "3d" else reference_square_p3d_case()
+    params = dict(final_time=3600., time_step=dt, time_scheme=ts, frozen_eulerian_fields=True, model=model,
+                  diffusion_model=dm, horizontal_diffusivity=0., vertical_diffusivity=0.)
+    if model == 2:
+        params.update(particle_velocity_init=1, diffusion_lsm2_option=opt)
+    return case, params
+
+
+@pytest.mark.parametrize("kind,model,ts,dt,dm,opt,expected", GOLDEN_3D, ids=GOLDEN_3D_IDS)
+def test_3d_and_pseudo3d_golden(kind, model, ts, dt, dm, opt, expected):
+    case, params = golden_3d_setup(kind, model, ts, dt, dm, opt)
+    out = case.run_oracle(R, params)
+    np.testing.assert_allclose(out.position[out.inactive == 0], np.array(expected), rtol=1e-7)
 
 
 REBOUND_EXPECTED = np.array([
