@@ -69,6 +69,10 @@ SIGNATURES = {
     "cyl_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "cyl_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
     "cyl_last_prep_timing": (C.c_int, [C.c_void_p, _dp]),
+    "cyl_scenarios_set": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_void_p, C.POINTER(C.c_uint64),
+                                    C.POINTER(C.c_uint64), _ip]),
+    "cyl_scenario_rng_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "cyl_batch_step": (C.c_int, [C.c_void_p, _dp, C.c_double, C.c_int, _ip]),
 }
 
 _lib = None

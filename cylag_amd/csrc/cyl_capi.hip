@@ -80,13 +80,31 @@ struct cyl_ctx {
   Bufs part_bufs;
   int *d_gpool = nullptr, *d_rank = nullptr, *d_init_tri = nullptr, *d_del_step = nullptr;
   double2* d_xi = nullptr;            // the step's normals per active rank (fused 2D step)
+  int64_t xi_cap = 0;
   int64_t* d_off = nullptr;
   double *d_diam = nullptr, *d_vol = nullptr, *d_depth = nullptr;
   ParticleConst pc{};
   bool pending_init = false;   // rows with tri == -2 exist
   bool have_last = false;
 
-  // scenarios (a single one for the plain solver)
+  // scenarios (a single one for the plain solver; several after cyl_scenarios_set)
+  bool batched = false;
+  std::vector<cyl_params> scen_par;     // per scenario (batched)
+  std::vector<int> scen_nsteps;         // steps each scenario runs (batched)
+  int batch_steps = 0;                  // steps run since the scenarios were set
+  ParamsDev* d_pars = nullptr;          // [nscen] device copy of the parameters
+  FastScen* d_fsc = nullptr;            // [nscen] fused-kernel parameters
+  int* d_scen_end = nullptr;            // [nscen] number of steps a scenario runs
+  int* d_pregen = nullptr;              // [nscen] 1: scenario stepped by the fused kernel
+  int64_t* d_gen_base = nullptr;        // [nscen + 1] normal-generator chunks
+  int* d_class_of_scen = nullptr;       // [nscen] kernel class (physical slot group)
+  struct KClass { bool fast; int model; int lo, hi; };
+  std::vector<KClass> classes;          // batched: slot ranges per kernel class
+  std::vector<int> class_of_scen_h;     // batched: class of each scenario
+  std::vector<int> pregen_h;            // last uploaded d_pregen
+  bool classes_fast_of_scen(int s) const {
+    return !classes.empty() && s < (int)class_of_scen_h.size() && classes[class_of_scen_h[s]].fast;
+  }
   int nscen = 1;
   std::vector<int> scen_base_h{0, 0};
   int* d_scen_base = nullptr;
@@ -130,7 +148,8 @@ struct cyl_ctx {
   int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
   int64_t* d_cursor = nullptr;
   int64_t* d_sort_tiles = nullptr;
-  int sort_nt = 0;
+  int sort_nt = 0, sort_ncls = 0;
+  int* sp_scen = nullptr;
   // spare per-slot buffers the sort gathers into (pointers swapped after)
   int *sp_gpool = nullptr, *sp_init_tri = nullptr, *sp_del_step = nullptr, *sp_inactive = nullptr;
   int* sp_rank = nullptr;
@@ -623,6 +642,13 @@ static int ensure_scen_tables(cyl_ctx* ctx) {
   HIPCHK(hipMemset(ctx->d_err_step, 0xff, ns * sizeof(unsigned long long)));
   HIPCHK(dalloc(b, &ctx->d_scen_units, ns));
   HIPCHK(dalloc(b, &ctx->d_scen_active, ns));
+  HIPCHK(dalloc(b, &ctx->d_pars, ns));
+  HIPCHK(dalloc(b, &ctx->d_fsc, ns));
+  HIPCHK(dalloc(b, &ctx->d_scen_end, ns));
+  HIPCHK(dalloc(b, &ctx->d_pregen, ns));
+  HIPCHK(dalloc(b, &ctx->d_class_of_scen, ns));
+  HIPCHK(hipMemset(ctx->d_class_of_scen, 0, ns * sizeof(int)));
+  HIPCHK(dalloc(b, &ctx->d_gen_base, ns + 1));
   HIPCHK(hipMemset(ctx->d_stream_end, 0, ns * sizeof(unsigned long long)));
   HIPCHK(hipMemset(ctx->d_err, 0xff, ns * sizeof(unsigned long long)));
   std::vector<StreamDev> st(ns);
@@ -648,6 +674,12 @@ extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const doub
   ctx->sp_diam = ctx->sp_vol = ctx->sp_depth = nullptr;
   ctx->ev_bufs.free_all();
   ctx->ev_units_cap = 0;
+  ctx->batched = false;
+  ctx->classes.clear();
+  ctx->class_of_scen_h.clear();
+  ctx->pregen_h.clear();
+  ctx->sp_scen = nullptr;
+  ctx->sort_ncls = 0;
   ctx->n = npart;
   ctx->dim = dim;
   ctx->cur = 0;
@@ -670,7 +702,9 @@ extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const doub
   int n = npart;
   HIPCHK(dalloc(bb, &ctx->d_gpool, n));
   HIPCHK(dalloc(bb, &ctx->d_rank, n));
-  HIPCHK(dalloc(bb, &ctx->d_xi, n));
+  // transposed normals: 32 x (chunks) entries, chunks <= n/32 + 1 per scenario
+  HIPCHK(dalloc(bb, &ctx->d_xi, (size_t)n + 32 * 3));
+  ctx->xi_cap = (int64_t)n + 32 * 3;
   HIPCHK(dalloc(bb, &ctx->d_off, n));
   HIPCHK(dalloc(bb, &ctx->d_init_tri, n));
   HIPCHK(dalloc(bb, &ctx->d_del_step, n));
@@ -869,6 +903,109 @@ extern "C" int cyl_rng_get(cyl_ctx* ctx, uint64_t* state, uint64_t* inc) {
   }
   *state = ctx->rng_state[0];
   *inc = ctx->rng_inc[0];
+  return CYL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched scenarios
+// ---------------------------------------------------------------------------
+__global__ void k_scen_of_slot(const int* gpool, const int* base, int nscen, int n, int* scen) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= n) return;
+  int g = gpool[j], lo = 0, hi = nscen - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  scen[j] = lo;
+}
+
+static bool fast_params(const cyl_ctx* ctx, const cyl_params* p);
+static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
+
+extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const cyl_params* par,
+                                 const uint64_t* state, const uint64_t* inc, const int* nsteps) {
+  if (!ctx || nscen < 1 || !base || !par || !state || !inc || !nsteps) return CYL_E_ARG;
+  if (ctx->n == 0 || !ctx->has_mesh || !ctx->has_fields) return ctx->set_err(CYL_E_STATE, -1, "upload mesh, fields and the pool first");
+  if (base[0] != 0 || base[nscen] != ctx->n) return ctx->set_err(CYL_E_ARG, -1, "scenario ranges must cover the pool");
+  for (int s = 0; s < nscen; ++s) {
+    if (base[s + 1] < base[s]) return ctx->set_err(CYL_E_ARG, -1, "scenario ranges must be increasing");
+    if (nsteps[s] < 0) return ctx->set_err(CYL_E_ARG, -1, "negative scenario duration");
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->nscen = nscen;
+  ctx->scen_base_h.assign(base, base + nscen + 1);
+  int e = ensure_scen_tables(ctx);
+  if (e) return e;
+  if (!ctx->d_scen) HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scen, ctx->n));
+  // normals buffer for the per-scenario chunk padding (see k_gen_normals)
+  HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)ctx->n + 32 * ((size_t)nscen + 2)));
+  ctx->xi_cap = (int64_t)ctx->n + 32 * ((int64_t)nscen + 2);
+  k_scen_of_slot<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_gpool, ctx->d_scen_base, nscen, ctx->n, ctx->d_scen);
+  ctx->rng_state.assign(state, state + nscen);
+  ctx->rng_inc.assign(inc, inc + nscen);
+  ctx->rng_dirty = true;
+  ctx->scen_par.assign(par, par + nscen);
+  ctx->scen_nsteps.assign(nsteps, nsteps + nscen);
+  ctx->batch_steps = 0;
+  // kernel classes: the fused kernel for LSM-1 Euler scenarios sharing the
+  // first one's Schmidt number (its packed nodal K), the generic kernel per model
+  ctx->classes.clear();
+  ctx->class_of_scen_h.assign(nscen, 0);
+  double sigc = 0.;
+  bool have_fast = false;
+  for (int s = 0; s < nscen; ++s)
+    if (fast_params(ctx, &par[s]) && (!have_fast || par[s].schmidt_number == sigc)) {
+      if (!have_fast) sigc = par[s].schmidt_number;
+      have_fast = true;
+    }
+  std::vector<int64_t> count;
+  auto class_of = [&](bool fast, int model) {
+    for (size_t c = 0; c < ctx->classes.size(); ++c)
+      if (ctx->classes[c].fast == fast && (fast || ctx->classes[c].model == model)) return (int)c;
+    ctx->classes.push_back({fast, model, 0, 0});
+    count.push_back(0);
+    return (int)ctx->classes.size() - 1;
+  };
+  for (int s = 0; s < nscen; ++s) {
+    bool fast = have_fast && fast_params(ctx, &par[s]) && par[s].schmidt_number == sigc;
+    int c = class_of(fast, par[s].model);
+    ctx->class_of_scen_h[s] = c;
+    count[c] += base[s + 1] - base[s];
+  }
+  int lo = 0;
+  for (size_t c = 0; c < ctx->classes.size(); ++c) {
+    ctx->classes[c].lo = lo;
+    lo += (int)count[c];
+    ctx->classes[c].hi = lo;
+  }
+  HIPCHK(hipMemcpy(ctx->d_class_of_scen, ctx->class_of_scen_h.data(), nscen * sizeof(int), hipMemcpyHostToDevice));
+  ctx->batched = true;
+  ctx->pregen_h.clear();
+  ctx->ev_bufs.free_all();
+  ctx->ev_units_cap = 0;
+  // ranks restart at every scenario; slots are grouped by kernel class now
+  int launches = 0;
+  k_set_flag<<<1, 1, 0, ctx->stream>>>(ctx->d_flag, 1, 1);
+  e = enqueue_sort(ctx, true, launches);
+  if (e) return e;
+  ctx->steps_since_sort = 0;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return CYL_OK;
+}
+
+extern "C" int cyl_scenario_rng_get(cyl_ctx* ctx, int s, uint64_t* state, uint64_t* inc) {
+  if (!ctx || !state || !inc || s < 0 || s >= ctx->nscen) return CYL_E_ARG;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (!ctx->rng_dirty && ctx->d_stream) {
+    StreamDev st;
+    HIPCHK(hipMemcpy(&st, ctx->d_stream + s, sizeof(StreamDev), hipMemcpyDeviceToHost));
+    ctx->rng_state[s] = st.base;
+  }
+  *state = ctx->rng_state[s];
+  *inc = ctx->rng_inc[s];
   return CYL_OK;
 }
 
@@ -1155,7 +1292,7 @@ static void launch_step(cyl_ctx* ctx, const StepArgs& a, int model, bool count) 
   }
 }
 
-static int upload_streams(cyl_ctx* ctx, int units_per_part) {
+static int upload_streams(cyl_ctx* ctx, const std::vector<int>& units_per_part) {
   std::vector<StreamDev> st(ctx->nscen);
   HIPCHK(hipMemcpy(st.data(), ctx->d_stream, ctx->nscen * sizeof(StreamDev), hipMemcpyDeviceToHost));
   for (int s = 0; s < ctx->nscen; ++s) {
@@ -1163,7 +1300,7 @@ static int upload_streams(cyl_ctx* ctx, int units_per_part) {
       st[s].base = ctx->rng_state[s];
       st[s].inc = ctx->rng_inc[s];
     }
-    st[s].units_per_part = units_per_part;
+    st[s].units_per_part = units_per_part[s];
   }
   if (ctx->rng_dirty) {
     std::vector<JumpTable> jt(ctx->nscen);
@@ -1183,13 +1320,18 @@ static int upload_streams(cyl_ctx* ctx, int units_per_part) {
 // The fused 2D LSM-1 kernel covers the Euler scheme with diffusion models
 // 0-3 on 2D fields for one scenario whose rows are all initialized; every
 // other configuration runs the generic k_step.
-static bool fast_eligible(const cyl_ctx* ctx, const cyl_params* p, bool init_only) {
+static bool fast_params(const cyl_ctx* ctx, const cyl_params* p) {
   const FieldsDev& f = ctx->fields;
-  if (init_only || ctx->pending_init || ctx->nscen != 1 || ctx->dim != 2 || f.dim != 2) return false;
+  if (ctx->dim != 2 || f.dim != 2) return false;
   if (p->model != 1 || p->time_scheme != 1 || p->diffusion_model < 0 || p->diffusion_model > 3) return false;
   if (f.npoints != ctx->mesh.nv || f.fid_ux < 0 || f.fid_uy < 0) return false;
   if (p->diffusion_model >= 2 && (f.fid_turbeng < 0 || f.fid_dissip < 0)) return false;
   return true;
+}
+
+static bool fast_eligible(const cyl_ctx* ctx, const cyl_params* p, bool init_only) {
+  if (init_only || ctx->pending_init || ctx->nscen != 1) return false;
+  return fast_params(ctx, p);
 }
 
 static int ensure_fast(cyl_ctx* ctx) {
@@ -1218,17 +1360,21 @@ static int ensure_fast(cyl_ctx* ctx) {
   return CYL_OK;
 }
 
-static int launch_fast(cyl_ctx* ctx, const StepArgs& a, const cyl_params* p, hipEvent_t mid) {
-  if (ctx->start_cap < ctx->n) {
-    if (ctx->d_pos0) HIPCHK(hipFree(ctx->d_pos0));
-    if (ctx->d_state0) HIPCHK(hipFree(ctx->d_state0));
-    ctx->d_pos0 = nullptr;
-    ctx->d_state0 = nullptr;
-    ctx->start_cap = 0;
-    HIPCHK(hipMalloc(&ctx->d_pos0, (size_t)ctx->n * sizeof(int64_t)));
-    HIPCHK(hipMalloc(&ctx->d_state0, (size_t)ctx->n * sizeof(uint64_t)));
-    ctx->start_cap = ctx->n;
-  }
+static int ensure_start_bufs(cyl_ctx* ctx) {
+  if (ctx->start_cap >= ctx->n) return CYL_OK;
+  if (ctx->d_pos0) HIPCHK(hipFree(ctx->d_pos0));
+  if (ctx->d_state0) HIPCHK(hipFree(ctx->d_state0));
+  ctx->d_pos0 = nullptr;
+  ctx->d_state0 = nullptr;
+  ctx->start_cap = 0;
+  HIPCHK(hipMalloc(&ctx->d_pos0, (size_t)ctx->n * sizeof(int64_t)));
+  HIPCHK(hipMalloc(&ctx->d_state0, (size_t)ctx->n * sizeof(uint64_t)));
+  ctx->start_cap = ctx->n;
+  return CYL_OK;
+}
+
+// arguments of the fused kernel over slots [j0, j0 + n)
+static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
   FastArgs f{};
   f.fm = ctx->fm;
   f.m = ctx->mesh;
@@ -1236,6 +1382,10 @@ static int launch_fast(cyl_ctx* ctx, const StepArgs& a, const cyl_params* p, hip
   f.xA = a.A.x;
   f.yA = a.A.y;
   f.triA = a.A.tri;
+  f.upA0 = a.A.up[0];
+  f.upA1 = a.A.up[1];
+  f.usA0 = a.A.us[0];
+  f.usA1 = a.A.us[1];
   f.xB = a.B.x;
   f.yB = a.B.y;
   f.triB = a.B.tri;
@@ -1248,19 +1398,16 @@ static int launch_fast(cyl_ctx* ctx, const StepArgs& a, const cyl_params* p, hip
   f.rank = ctx->d_rank;
   f.off = ctx->d_off;
   f.pool = ctx->d_gpool;
-  f.n = ctx->n;
+  f.scen = ctx->d_scen;
+  f.j0 = j0;
+  f.n = n;
+  f.nscen = ctx->nscen;
   f.dt = a.dt;
   f.sqrtdt = sqrt(a.dt);
-  f.diffusion_model = p->diffusion_model >= 2 ? 2 : p->diffusion_model;
-  f.diff_const = sqrt(2. * p->horizontal_diffusivity) * f.sqrtdt;  // diffusion_lsm1, update_state_lsm1.pyx:700
-  f.sigc = p->schmidt_number;
-  f.rng_method = p->rng_method;
-  f.boundary_conditions = p->boundary_conditions;
-  f.max_wall_rebounds = p->max_wall_rebounds;
-  f.bnd_statistics = p->bnd_statistics;
+  f.fsc = ctx->d_fsc;
+  f.scen_end = a.scen_end;
   f.nopen = ctx->nopen;
-  f.damp = fmin(p->rebound_damping_coef, 1.0 - EPS_REBOUND);
-  f.strand = p->stranding_probability;
+  f.nopen_stride = std::max(ctx->nopen, 1);
   f.general = a.general;
   f.guarded = a.guarded;
   f.flag = a.flag;
@@ -1273,45 +1420,49 @@ static int launch_fast(cyl_ctx* ctx, const StepArgs& a, const cyl_params* p, hip
   f.err = a.err;
   f.stream_end = a.stream_end;
   f.step_id = a.step_id;
+  f.batch_step = a.batch_step;
   f.pos0 = ctx->d_pos0;
   f.state0 = ctx->d_state0;
-  // persistent grids: as many blocks as fit on the device at once, a multiple of 8
-  auto grid = [&](const void* fn) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-    unsigned g = std::min<unsigned>(nblk(ctx->n), (unsigned)(ctx->ncu * per_cu));
-    return std::max<unsigned>(8, g / 8 * 8);
-  };
   f.xi = ctx->d_xi;
+  f.xi_cap = ctx->xi_cap;
   f.nact = ctx->d_scen_active;
-  int m = p->rng_method;
-  if (!a.general) {
-    // uniform draw assignment: the step's normals in pool order, then the step
-    int64_t nch = ((int64_t)ctx->n + GEN_PER - 1) / GEN_PER;
-    auto ggrid = [&](const void* fn) {
-      int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
-      return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(nch), (int64_t)ctx->ncu * per_cu));
-    };
-    if (p->diffusion_model != 0) {
-      if (m == 1) k_gen_normals<1><<<ggrid((const void*)k_gen_normals<1>), BLK, 0, ctx->stream>>>(f);
-      else if (m == 2) k_gen_normals<2><<<ggrid((const void*)k_gen_normals<2>), BLK, 0, ctx->stream>>>(f);
-      else k_gen_normals<3><<<ggrid((const void*)k_gen_normals<3>), BLK, 0, ctx->stream>>>(f);
-    }
-    if (mid) HIPCHK(hipEventRecord(mid, ctx->stream));
-    if (m == 1) k_lsm1_2d<1, true><<<grid((const void*)k_lsm1_2d<1, true>), BLK, 0, ctx->stream>>>(f);
-    else if (m == 2) k_lsm1_2d<2, true><<<grid((const void*)k_lsm1_2d<2, true>), BLK, 0, ctx->stream>>>(f);
-    else k_lsm1_2d<3, true><<<grid((const void*)k_lsm1_2d<3, true>), BLK, 0, ctx->stream>>>(f);
-    return CYL_OK;
+  f.chunk_base = ctx->d_gen_base;
+  return f;
+}
+
+// persistent grid: as many blocks as fit on the device at once (a multiple of 8), at most the work
+static unsigned persistent_grid(cyl_ctx* ctx, const void* fn, int64_t nblocks, bool mult8) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  int64_t g = std::min<int64_t>(std::max<int64_t>(nblocks, 1), (int64_t)ctx->ncu * per_cu);
+  return mult8 ? (unsigned)std::max<int64_t>(8, g / 8 * 8) : (unsigned)g;
+}
+
+// normals of the step for every scenario of the fused kernel (uniform assignment)
+static void launch_gen(cyl_ctx* ctx, const FastArgs& f, int m) {
+  int64_t nb = nblk(((int64_t)ctx->n + GEN_PER - 1) / GEN_PER + ctx->nscen);
+  if (m == 1) k_gen_normals<1><<<persistent_grid(ctx, (const void*)k_gen_normals<1>, nb, false), BLK, 0, ctx->stream>>>(f);
+  else if (m == 2) k_gen_normals<2><<<persistent_grid(ctx, (const void*)k_gen_normals<2>, nb, false), BLK, 0, ctx->stream>>>(f);
+  else k_gen_normals<3><<<persistent_grid(ctx, (const void*)k_gen_normals<3>, nb, false), BLK, 0, ctx->stream>>>(f);
+}
+
+// the fused kernel over f's slot range: pre-generated normals (uniform), or
+// prepared stream positions and the inline sampler (general)
+static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int m, bool general) {
+  int64_t nb = nblk(f.n);
+  hipStream_t s = ctx->stream;
+  if (!general) {
+    if (m == 1) k_lsm1_2d<1, true><<<persistent_grid(ctx, (const void*)k_lsm1_2d<1, true>, nb, true), BLK, 0, s>>>(f);
+    else if (m == 2) k_lsm1_2d<2, true><<<persistent_grid(ctx, (const void*)k_lsm1_2d<2, true>, nb, true), BLK, 0, s>>>(f);
+    else k_lsm1_2d<3, true><<<persistent_grid(ctx, (const void*)k_lsm1_2d<3, true>, nb, true), BLK, 0, s>>>(f);
+    return;
   }
-  if (m == 1) k_draw_start<1><<<grid((const void*)k_draw_start<1>), BLK, 0, ctx->stream>>>(f);
-  else if (m == 2) k_draw_start<2><<<grid((const void*)k_draw_start<2>), BLK, 0, ctx->stream>>>(f);
-  else k_draw_start<3><<<grid((const void*)k_draw_start<3>), BLK, 0, ctx->stream>>>(f);
-  if (mid) HIPCHK(hipEventRecord(mid, ctx->stream));
-  if (m == 1) k_lsm1_2d<1, false><<<grid((const void*)k_lsm1_2d<1, false>), BLK, 0, ctx->stream>>>(f);
-  else if (m == 2) k_lsm1_2d<2, false><<<grid((const void*)k_lsm1_2d<2, false>), BLK, 0, ctx->stream>>>(f);
-  else k_lsm1_2d<3, false><<<grid((const void*)k_lsm1_2d<3, false>), BLK, 0, ctx->stream>>>(f);
-  return CYL_OK;
+  if (m == 1) k_draw_start<1><<<persistent_grid(ctx, (const void*)k_draw_start<1>, nb, true), BLK, 0, s>>>(f);
+  else if (m == 2) k_draw_start<2><<<persistent_grid(ctx, (const void*)k_draw_start<2>, nb, true), BLK, 0, s>>>(f);
+  else k_draw_start<3><<<persistent_grid(ctx, (const void*)k_draw_start<3>, nb, true), BLK, 0, s>>>(f);
+  if (m == 1) k_lsm1_2d<1, false><<<persistent_grid(ctx, (const void*)k_lsm1_2d<1, false>, nb, true), BLK, 0, s>>>(f);
+  else if (m == 2) k_lsm1_2d<2, false><<<persistent_grid(ctx, (const void*)k_lsm1_2d<2, false>, nb, true), BLK, 0, s>>>(f);
+  else k_lsm1_2d<3, false><<<persistent_grid(ctx, (const void*)k_lsm1_2d<3, false>, nb, true), BLK, 0, s>>>(f);
 }
 
 static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
@@ -1319,20 +1470,59 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
 static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, double dt, int nsteps, int* bnd_out,
                      bool init_only) {
   if (!ctx->has_mesh || !ctx->has_fields || ctx->n == 0) return ctx->set_err(CYL_E_STATE, -1, "context not ready");
-  if (p->model < 0 || p->model > 2)
-    return ctx->set_err(CYL_E_ARG, -1, "LSM3 (model 3) is not available in the device step");
-  if (p->rng_method < 1 || p->rng_method > 3) return ctx->set_err(CYL_E_ARG, -1, "Unknown RNG method");
+  const int ns = ctx->nscen;
+  std::vector<cyl_params> P(ns);
+  if (ctx->batched) P = ctx->scen_par;
+  else P[0] = *p;
+  for (int s = 0; s < ns; ++s) {
+    if (P[s].model < 0 || P[s].model > 2)
+      return ctx->set_err(CYL_E_ARG, -1, "LSM3 (model 3) is not available in the device step");
+    if (P[s].rng_method < 1 || P[s].rng_method > 3) return ctx->set_err(CYL_E_ARG, -1, "Unknown RNG method");
+    if (P[s].rng_method != P[0].rng_method)
+      return ctx->set_err(CYL_E_ARG, -1, "the scenarios of a batch share one rng_method");
+  }
+  const int method = P[0].rng_method;
   HIPCHK(hipSetDevice(ctx->device));
   auto t_start = std::chrono::steady_clock::now();
+  // uniform draw prediction per scenario
   int umax = 0;
-  int D = predict_units(p, ctx->dim, dt, umax);
-  bool can_uniform = D >= 0 && !init_only;
+  bool can_uniform = !init_only;
+  std::vector<int> D(ns, 0);
+  for (int s = 0; s < ns; ++s) {
+    int um = 0;
+    D[s] = predict_units(&P[s], ctx->dim, dt, um);
+    umax = std::max(umax, um);
+    if (D[s] < 0) can_uniform = false;
+  }
+  if (!can_uniform) std::fill(D.begin(), D.end(), 0);
   int launches = 0;
-  int e = upload_streams(ctx, can_uniform ? D : 0);
+  int e = upload_streams(ctx, D);
   if (e) return e;
-  e = ensure_events(ctx, (int64_t)umax * ctx->n, p->rng_method);
+  e = ensure_events(ctx, (int64_t)umax * ctx->n, method);
   if (e) return e;
-  int nstat = ctx->nscen * std::max(ctx->nopen, 1);
+  e = ensure_start_bufs(ctx);
+  if (e) return e;
+  // per-scenario device parameters
+  {
+    std::vector<ParamsDev> pd(ns);
+    std::vector<FastScen> fs(ns);
+    std::vector<int> send(ns);
+    for (int s = 0; s < ns; ++s) {
+      pd[s] = to_dev(&P[s]);
+      fs[s].diffusion_model = P[s].diffusion_model >= 2 ? 2 : P[s].diffusion_model;
+      fs[s].diff_const = sqrt(2. * P[s].horizontal_diffusivity) * sqrt(dt);  // diffusion_lsm1, update_state_lsm1.pyx:700
+      fs[s].boundary_conditions = P[s].boundary_conditions;
+      fs[s].max_wall_rebounds = P[s].max_wall_rebounds;
+      fs[s].bnd_statistics = P[s].bnd_statistics;
+      fs[s].damp = fmin(P[s].rebound_damping_coef, 1.0 - EPS_REBOUND);
+      fs[s].strand = P[s].stranding_probability;
+      send[s] = ctx->batched ? ctx->scen_nsteps[s] : 0;
+    }
+    HIPCHK(hipMemcpy(ctx->d_pars, pd.data(), ns * sizeof(ParamsDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_fsc, fs.data(), ns * sizeof(FastScen), hipMemcpyHostToDevice));
+    if (ctx->batched) HIPCHK(hipMemcpy(ctx->d_scen_end, send.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+  }
+  int nstat = ns * std::max(ctx->nopen, 1);
   size_t need_bnd = (size_t)std::max(nsteps, 1) * nstat;
   if (need_bnd > ctx->bnd_cap) {
     if (ctx->d_bnd) HIPCHK(hipFree(ctx->d_bnd));
@@ -1340,16 +1530,16 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     ctx->bnd_cap = need_bnd;
   }
   HIPCHK(hipMemsetAsync(ctx->d_bnd, 0, need_bnd * sizeof(int), ctx->stream));
-  HIPCHK(hipMemsetAsync(ctx->d_err, 0xff, ctx->nscen * sizeof(unsigned long long), ctx->stream));
+  HIPCHK(hipMemsetAsync(ctx->d_err, 0xff, ns * sizeof(unsigned long long), ctx->stream));
   HIPCHK(hipMemsetAsync(ctx->d_flag + 2, 0, sizeof(int), ctx->stream));
+  bool any_stats = false;
+  for (int s = 0; s < ns; ++s) any_stats |= P[s].bnd_statistics != 0;
   StepArgs a{};
   a.m = ctx->mesh;
-  a.par = to_dev(p);
+  a.par = to_dev(&P[0]);
+  a.pars = ctx->d_pars;
+  a.scen_end = ctx->batched ? ctx->d_scen_end : nullptr;
   a.pc = ctx->pc;
-  a.init_tri = ctx->d_init_tri;
-  a.del_step = ctx->d_del_step;
-  a.depth = ctx->dim == 3 ? ctx->d_depth : nullptr;
-  a.n = ctx->n;
   a.dim = ctx->dim;
   a.dt = dt;
   a.stream = ctx->d_stream;
@@ -1363,23 +1553,42 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   a.stream_end = ctx->d_stream_end;
   a.flag = ctx->d_flag;
   a.units_pool = ctx->d_vals;
-  a.gpool = ctx->d_gpool;
   a.init_only = init_only ? 1 : 0;
-  a.slot.pool = ctx->d_gpool;
-  a.slot.rank = ctx->d_rank;
   a.slot.off = ctx->d_off;
   a.slot.scen = ctx->d_scen;
-  a.slot.diameter = ctx->d_diam;
-  a.slot.volume = ctx->d_vol;
   double t = *time_inout;
   int* one = ctx->d_flag + 3;
   float kms = 0.f;
   int n_timed = 0;
   for (int step = 0; step < nsteps; ++step) {
-    bool fast = fast_eligible(ctx, p, init_only);
+    // kernel classes of this step: slot ranges stepped by the fused or the generic kernel
+    std::vector<cyl_ctx::KClass> cls;
+    if (!ctx->batched) {
+      cls.push_back({fast_eligible(ctx, p, init_only), p->model, 0, ctx->n});
+    } else {
+      cls = ctx->classes;
+      if (init_only || ctx->pending_init)
+        for (auto& c : cls) c.fast = false;
+    }
+    bool any_fast = false, all_fast = true;
+    for (auto& c : cls) {
+      any_fast |= c.fast;
+      all_fast &= c.fast;
+    }
+    {
+      std::vector<int> pregen(ns, 0);
+      if (!ctx->batched) pregen[0] = cls[0].fast ? 1 : 0;
+      else
+        for (int s = 0; s < ns; ++s) pregen[s] = ctx->classes_fast_of_scen(s) && any_fast ? 1 : 0;
+      if (pregen != ctx->pregen_h) {  // changes only when arrivals switch the fused kernel off or on
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        HIPCHK(hipMemcpy(ctx->d_pregen, pregen.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+        ctx->pregen_h = pregen;
+      }
+    }
     bool do_sort = !init_only && ctx->sort_interval > 0 && ctx->steps_since_sort >= ctx->sort_interval;
     if (do_sort) {
-      e = enqueue_sort(ctx, !fast, launches);
+      e = enqueue_sort(ctx, !all_fast, launches);
       if (e) return e;
       ctx->steps_since_sort = 0;
     }
@@ -1394,33 +1603,45 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     a.gpool = ctx->d_gpool;
     a.slot.pool = ctx->d_gpool;
     a.slot.rank = ctx->d_rank;
+    a.slot.scen = ctx->d_scen;
     a.slot.diameter = ctx->d_diam;
     a.slot.volume = ctx->d_vol;
     a.f = ctx->fields;
-    if (fast) {
+    if (any_fast) {
       e = ensure_fast(ctx);
       if (e) return e;
       const FieldsDev& fd = ctx->fields;
-      bool keps = p->diffusion_model >= 2;
+      bool keps = false;
+      double sigc = P[0].schmidt_number;
+      for (int s = 0; s < ns; ++s)
+        if (!ctx->batched || ctx->classes_fast_of_scen(s)) {
+          keps |= P[s].diffusion_model >= 2;
+          sigc = P[s].schmidt_number;
+        }
       k_pack_nodes<<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(
           fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], keps ? fd.tmp[fd.fid_turbeng] : nullptr,
-          keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / p->schmidt_number, fd.npoints, ctx->d_node_old,
-          ctx->d_nodes);
+          keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / sigc, fd.npoints, ctx->d_node_old, ctx->d_nodes);
       ++launches;
     }
     a.A = ctx->pool[ctx->cur];
     a.B = ctx->pool[1 - ctx->cur];
     a.step_id = ctx->step_counter;
-    a.bnd_stats = p->bnd_statistics ? ctx->d_bnd + (size_t)step * nstat : nullptr;
+    a.batch_step = ctx->batch_steps;
+    a.bnd_stats = any_stats ? ctx->d_bnd + (size_t)step * nstat : nullptr;
     // active rank when rows were deleted or added since the last scan
     enqueue_rank(ctx, ctx->d_flag + 1, launches);
     bool uniform = can_uniform && !ctx->pending_init;
     if (step == 0) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (uniform) {
-      k_uniform_units<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, ctx->nscen,
-                                                                 ctx->d_scen_units);
+      k_uniform_units<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, ns, a.scen_end,
+                                                         a.batch_step, ctx->d_scen_units);
       ++launches;
-      enqueue_events(ctx, p->rng_method, one, launches);
+      if (any_fast) {
+        k_gen_chunks<<<1, 1, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, a.scen_end, ctx->d_pregen,
+                                               a.batch_step, ns, ctx->d_gen_base);
+        ++launches;
+      }
+      enqueue_events(ctx, method, one, launches);
       a.general = 0;
       a.guarded = 0;
       while (ctx->kev.size() < (size_t)3 * (step + 1)) {
@@ -1429,57 +1650,73 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         ctx->kev.push_back(ev);
       }
       HIPCHK(hipEventRecord(ctx->kev[3 * step], ctx->stream));
-      if (fast) {
-        e = launch_fast(ctx, a, p, ctx->kev[3 * step + 1]);
-        if (e) return e;
-      } else {
-        HIPCHK(hipEventRecord(ctx->kev[3 * step + 1], ctx->stream));
-        launch_step(ctx, a, p->model, false);
+      if (any_fast) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
+      HIPCHK(hipEventRecord(ctx->kev[3 * step + 1], ctx->stream));
+      for (auto& c : cls) {
+        if (c.hi <= c.lo) continue;
+        if (c.fast) {
+          launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), method, false);
+        } else {
+          a.j0 = c.lo;
+          a.n = c.hi - c.lo;
+          launch_step(ctx, a, c.model, false);
+        }
+        ++launches;
       }
       HIPCHK(hipEventRecord(ctx->kev[3 * step + 2], ctx->stream));
-      ++launches;
       ++n_timed;
     } else {
       HIPCHK(hipMemsetAsync(ctx->d_flag, 0xff, sizeof(int), ctx->stream));  // force the general path
     }
     // general mode: exact per-row unit counts, pool-order scan, rerun
     int* g = ctx->d_flag;
-    k_rerun_reset<<<nblk(std::max(nstat, ctx->nscen)), BLK, 0, ctx->stream>>>(
-        ctx->d_stream_end, ctx->d_err_step, a.bnd_stats, nstat, ctx->nscen, ctx->d_n_deleted, g);
+    k_rerun_reset<<<nblk(std::max(nstat, ns)), BLK, 0, ctx->stream>>>(
+        ctx->d_stream_end, ctx->d_err_step, a.bnd_stats, nstat, ns, ctx->d_n_deleted, g);
     k_undelete<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->pool[0].inactive, ctx->d_del_step, ctx->n,
                                                       ctx->step_counter, g);
     a.general = 0;
     a.guarded = 1;
-    launch_step(ctx, a, p->model, true);
+    for (auto& c : cls) {
+      if (c.hi <= c.lo) continue;
+      a.j0 = c.lo;
+      a.n = c.hi - c.lo;
+      launch_step(ctx, a, c.model, true);
+    }
     enqueue_scan(ctx, ctx->d_vals, ctx->d_scan, (int64_t)ctx->n, ctx->d_bsum, ctx->d_total, (const int64_t*)nullptr,
                  g, launches);
     k_gather_rank<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_gpool, ctx->d_scen, ctx->d_scen_base,
                                                          ctx->n, nullptr, ctx->d_off, g);
-    k_scen_totals<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_total, ctx->d_scen_base,
-                                                             ctx->nscen, ctx->n, ctx->d_scen_units, g);
+    k_scen_totals<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_total, ctx->d_scen_base, ns, ctx->n,
+                                                     ctx->d_scen_units, g);
     launches += 4;
-    enqueue_events(ctx, p->rng_method, g, launches);
+    enqueue_events(ctx, method, g, launches);
     a.general = 1;
     a.guarded = 1;
-    if (fast) {
-      e = launch_fast(ctx, a, p, nullptr);
-      if (e) return e;
-    } else {
-      launch_step(ctx, a, p->model, false);
+    for (auto& c : cls) {
+      if (c.hi <= c.lo) continue;
+      if (c.fast) {
+        launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), method, true);
+      } else {
+        a.j0 = c.lo;
+        a.n = c.hi - c.lo;
+        launch_step(ctx, a, c.model, false);
+      }
+      ++launches;
     }
-    ++launches;
     if (step == nsteps - 1) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
-    k_finish<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_stream_end, ctx->nscen,
-                                                        ctx->d_flag, ctx->d_n_deleted, ctx->d_n_deleted_total,
-                                                        ctx->step_counter, ctx->d_step_done, ctx->d_err_step,
-                                                        ctx->d_err);
+    k_finish<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_stream_end, ns, ctx->d_flag,
+                                                ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->step_counter,
+                                                ctx->d_step_done, ctx->d_err_step, ctx->d_err);
     ++launches;
     ctx->cur = 1 - ctx->cur;
     ctx->have_last = !init_only;
     ctx->pending_init = false;
     ++ctx->step_counter;
     ++ctx->steps_since_sort;
-    if (!init_only) t += dt;  // SimuTime.increment, simutime.pyx:81
+    if (!init_only) {
+      t += dt;  // SimuTime.increment, simutime.pyx:81
+      ++ctx->batch_steps;
+    }
   }
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if (nsteps > 0) HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
@@ -1496,11 +1733,14 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   ctx->last_step_kernel_ms = sk;
   ctx->last_prep_ms = sp;
   ctx->last_step_launches = n_timed;
-  if (bnd_out && p->bnd_statistics && ctx->nopen > 0) {
+  if (bnd_out && any_stats && ctx->nopen > 0) {
     std::vector<int> h(need_bnd);
     HIPCHK(hipMemcpy(h.data(), ctx->d_bnd, need_bnd * sizeof(int), hipMemcpyDeviceToHost));
     for (int s = 0; s < nsteps; ++s)
-      for (int q = 0; q < ctx->nopen; ++q) bnd_out[(size_t)s * ctx->nopen + q] = h[(size_t)s * nstat + q];
+      for (int sc = 0; sc < ctx->nscen; ++sc)
+        for (int q = 0; q < ctx->nopen; ++q)
+          bnd_out[((size_t)s * ctx->nscen + sc) * ctx->nopen + q] =
+              h[(size_t)s * nstat + (size_t)sc * std::max(ctx->nopen, 1) + q];
   }
   int flags[4];
   HIPCHK(hipMemcpy(flags, ctx->d_flag, 4 * sizeof(int), hipMemcpyDeviceToHost));
@@ -1537,6 +1777,13 @@ extern "C" int cyl_step(cyl_ctx* ctx, const cyl_params* p, double* time_inout, d
   if (!ctx || !p || !time_inout || nsteps < 0) return CYL_E_ARG;
   ctx->fields.frozen = ctx->frozen;
   return run_steps(ctx, p, time_inout, dt, nsteps, bnd, false);
+}
+
+extern "C" int cyl_batch_step(cyl_ctx* ctx, double* time_inout, double dt, int nsteps, int* bnd) {
+  if (!ctx || !time_inout || nsteps < 0) return CYL_E_ARG;
+  if (!ctx->batched) return ctx->set_err(CYL_E_STATE, -1, "no scenarios set");
+  ctx->fields.frozen = ctx->frozen;
+  return run_steps(ctx, &ctx->scen_par[0], time_inout, dt, nsteps, bnd, false);
 }
 
 extern "C" int cyl_init_particles(cyl_ctx* ctx, const cyl_params* p, double time, double dt) {
@@ -1578,13 +1825,16 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
   int n = ctx->n, nt = ctx->mesh.nt;
   hipStream_t s = ctx->stream;
   Bufs& bb = ctx->part_bufs;
-  int nbin = (nt >> SORT_SHIFT) + 3;
-  if (!ctx->d_scnt || ctx->sort_nt != nt) {
+  int ncls = ctx->batched ? std::max<int>(1, (int)ctx->classes.size()) : 1;
+  int nbin = ((nt >> SORT_SHIFT) + 3) * ncls;
+  if (!ctx->d_scnt || ctx->sort_nt != nt || ctx->sort_ncls != ncls) {
     HIPCHK(dalloc(bb, &ctx->d_scnt, nbin));
     HIPCHK(dalloc(bb, &ctx->d_cursor, nbin));
     HIPCHK(dalloc(bb, &ctx->d_sort_tiles, nblk(nbin, SCAN_TILE) + 1));
     ctx->sort_nt = nt;
+    ctx->sort_ncls = ncls;
   }
+  if (ctx->d_scen && !ctx->sp_scen) HIPCHK(dalloc(bb, &ctx->sp_scen, n));
   if (!ctx->sp_gpool) {
     HIPCHK(dalloc(bb, &ctx->sp_gpool, n));
     HIPCHK(dalloc(bb, &ctx->sp_init_tri, n));
@@ -1598,7 +1848,8 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
   PoolDev& A = ctx->pool[ctx->cur];
   PoolDev& B = ctx->pool[1 - ctx->cur];
   HIPCHK(hipMemsetAsync(ctx->d_scnt, 0, nbin * sizeof(int), s));
-  k_sort_hist<<<nblk(n), BLK, 0, s>>>(A.tri, A.inactive, ctx->d_elem_rank, n, nt, ctx->d_scnt, ctx->d_skey);
+  k_sort_hist<<<nblk(n), BLK, 0, s>>>(A.tri, A.inactive, ctx->d_elem_rank, ctx->batched ? ctx->d_scen : nullptr,
+                                      ctx->d_class_of_scen, n, nt, ctx->d_scnt, ctx->d_skey);
   enqueue_scan(ctx, ctx->d_scnt, ctx->d_cursor, (int64_t)nbin, ctx->d_sort_tiles, ctx->d_total,
                (const int64_t*)nullptr, (const int*)nullptr, launches);
   k_sort_place<<<nblk(n), BLK, 0, s>>>(ctx->d_skey, n, ctx->d_cursor, ctx->d_perm);
@@ -1630,7 +1881,7 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
   pi(ctx->d_del_step, ctx->sp_del_step);
   pi(ctx->pool[0].inactive, ctx->sp_inactive);
   pi(ctx->d_rank, ctx->sp_rank);  // a slot's active rank moves with it
-  if (ctx->d_scen) return ctx->set_err(CYL_E_STATE, -1, "element sort of a batched pool is not supported");
+  if (ctx->d_scen) pi(ctx->d_scen, ctx->sp_scen);
   k_permute_all<<<nblk(n), BLK, 0, s>>>(pa);
   launches += 3;
   HIPCHK(hipGetLastError());
@@ -1643,6 +1894,7 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
   std::swap(ctx->d_depth, ctx->sp_depth);
   std::swap(ctx->pool[0].inactive, ctx->sp_inactive);
   std::swap(ctx->d_rank, ctx->sp_rank);
+  if (ctx->d_scen) std::swap(ctx->d_scen, ctx->sp_scen);
   ctx->pool[1].inactive = ctx->pool[0].inactive;
   ctx->have_last = false;
   return CYL_OK;

@@ -143,6 +143,14 @@ __device__ __forceinline__ int localize_fast(const FastMesh& fm, const MeshDev& 
   return nk;
 }
 
+// Per-scenario parameters of the fused kernel (one entry per scenario)
+struct FastScen {
+  int diffusion_model;     // 0, 1 (constant), 2 (k-epsilon)
+  double diff_const;       // sqrt(2*nux)*sqrtdt for model 1
+  int boundary_conditions, max_wall_rebounds, bnd_statistics;
+  double damp, strand;
+};
+
 struct FastArgs {
   FastMesh fm;
   MeshDev m;               // grid for the fallback search
@@ -150,6 +158,10 @@ struct FastArgs {
   const double* xA;
   const double* yA;
   const int* triA;
+  const double* upA0;
+  const double* upA1;
+  const double* usA0;
+  const double* usA1;
   double* xB;
   double* yB;
   int* triB;
@@ -162,29 +174,31 @@ struct FastArgs {
   const int* rank;
   const int64_t* off;
   const int* pool;
-  int n;
+  const int* scen;         // scenario of each slot (batched runs) or null
+  int j0, n;               // slots [j0, j0 + n)
+  int nscen;
   double dt, sqrtdt;
-  int diffusion_model;     // 0, 1 (constant), 2 (k-epsilon)
-  double diff_const;       // sqrt(2*nux)*sqrtdt for model 1
-  double sigc;
-  int rng_method;
-  int boundary_conditions, max_wall_rebounds, bnd_statistics, nopen;
-  double damp, strand;
+  const FastScen* fsc;     // [nscen]
+  const int* scen_end;     // [nscen] number of steps each scenario runs, or null
+  int batch_step;          // steps run since the scenarios were set
+  int nopen, nopen_stride; // open boundaries; stride of bnd_stats per scenario
   int general, guarded;
   int* flag;
-  StreamDev* stream;
-  const JumpTable* jt;
-  const JumpTable8* jt8;
+  StreamDev* stream;       // [nscen]
+  const JumpTable* jt;     // [nscen]
+  const JumpTable8* jt8;   // [nscen]
   const ZigTables* zt;
   int* bnd_stats;
   int* n_deleted;
-  unsigned long long* err;
-  unsigned long long* stream_end;
+  unsigned long long* err;         // [nscen]
+  unsigned long long* stream_end;  // [nscen]
   int step_id;
   int64_t* pos0;      // per slot: stream position of the first draw
   uint64_t* state0;   // per slot: generator state at that position
-  double2* xi;        // per active rank: the step's two normals (pre-generated)
-  const int64_t* nact;  // active rows (device)
+  double2* xi;        // the step's two normals per (scenario, active rank), see xi_index
+  int64_t xi_cap;     // entries of xi
+  const int64_t* nact;        // [nscen] active rows
+  const int64_t* chunk_base;  // [nscen + 1] first normal-generator chunk of each scenario
 };
 
 
@@ -221,9 +235,11 @@ __global__ void __launch_bounds__(BLK) k_draw_start(FastArgs a) {
   __shared__ JumpTable8 t8;
   load_jump8(t8, a.jt8);
   __syncthreads();
-  const StreamDev& st = a.stream[0];
-  for_slots_xcd(a.n, [&](int j) {
+  for_slots_xcd(a.n, [&](int jl) {
+    int j = a.j0 + jl;
     if (a.inactive[j] != 0) return;
+    int s = a.scen ? a.scen[j] : 0;
+    const StreamDev& st = a.stream[s];
     int64_t u = a.general ? a.off[j] : (int64_t)st.units_per_part * a.rank[j];
     Draws d;
     d.st = &st;
@@ -233,30 +249,46 @@ __global__ void __launch_bounds__(BLK) k_draw_start(FastArgs a) {
     else pos = d.polar_position(u);
     // a position past the scanned window is reported if the slot draws (-1)
     a.pos0[j] = (RNG != 1 && pos >= st.window) ? -1 : pos;
-    a.state0[j] = pcg_jump8(st.base, (uint64_t)pos, t8, *a.jt);
+    a.state0[j] = s == 0 ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[0])
+                         : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
   });
 }
 
-// Normals of the step in pool order: thread c serves the active ranks
-// [GEN_PER c, GEN_PER (c+1)), i.e. a contiguous run of the serial stream.  It
+// Normals of the step in pool order.  The active ranks of scenario s are cut
+// into chunks of GEN_PER ranks, numbered globally from chunk_base[s]; thread c
+// serves one chunk, i.e. a contiguous run of the scenario's serial stream.  It
 // finds the run's first position once (event table), jumps there, then draws
 // sequentially: a Ziggurat normal that starts on a slow event takes the
 // event's precomputed value and skips its positions; every other normal is a
-// one-draw fast normal.  The pair of rank r = GEN_PER c + k is stored at
+// one-draw fast normal.  The pair of rank r = GEN_PER c_local + k is stored at
 // xi[k * nchunk + c] so that each of the thread's stores is coalesced across
-// the wavefront (see xi_index).
+// the wavefront.  Indices reach 32 * nchunk - 1, where nchunk <= (active rows) /
+// 32 + (scenarios): the buffer holds rows + 32 * (scenarios + 2) pairs.
 constexpr int GEN_PER = 32;
 
-__device__ __forceinline__ int64_t xi_index(int64_t r, int64_t nact) {
-  int64_t nchunk = (nact + GEN_PER - 1) / GEN_PER;
-  return (r % GEN_PER) * nchunk + r / GEN_PER;
+__device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int nscen, int s, int64_t r) {
+  return (r % GEN_PER) * chunk_base[nscen] + chunk_base[s] + r / GEN_PER;
+}
+
+// chunk_base[s] = first generator chunk of scenario s: ceil(active / GEN_PER)
+// chunks for scenarios of the fused kernel that draw this step, none otherwise
+__global__ void k_gen_chunks(const StreamDev* st, const int64_t* nact, const int* scen_end, const int* pregen,
+                             int step_id, int nscen, int64_t* chunk_base) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t c = 0;
+  for (int s = 0; s < nscen; ++s) {
+    chunk_base[s] = c;
+    bool ended = scen_end && step_id >= scen_end[s];
+    if ((!pregen || pregen[s]) && !ended && st[s].units_per_part > 0) c += (nact[s] + GEN_PER - 1) / GEN_PER;
+  }
+  chunk_base[nscen] = c;
 }
 
 template <int RNG>
 __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   __shared__ JumpTable8 t8;
   __shared__ ZigTables zt;
-  load_jump8(t8, a.jt8);
+  load_jump8(t8, a.jt8);  // scenario 0's table; other scenarios read theirs from memory
   if (RNG == 2) {
     for (int q = threadIdx.x; q < 128; q += BLK) {
       zt.kn[q] = a.zt->kn[q];
@@ -265,13 +297,17 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
     }
   }
   __syncthreads();
-  const StreamDev& st = a.stream[0];
-  int64_t nact = *a.nact;
-  int64_t nchunk = (nact + GEN_PER - 1) / GEN_PER;
+  const int64_t nchunk = a.chunk_base[a.nscen];
   const int64_t NONE = 0x7fffffffffffffffLL;
   unsigned long long emax = 0;
+  int emax_s = -1;
+  bool mixed = false;
   for (int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x; c < nchunk; c += (int64_t)gridDim.x * BLK) {
-    int64_t r0 = c * GEN_PER, r1 = r0 + GEN_PER < nact ? r0 + GEN_PER : nact;
+    int s = scen_of(a.chunk_base, a.nscen, c);
+    if (a.scen_end && a.batch_step >= a.scen_end[s]) continue;
+    const StreamDev& st = a.stream[s];
+    if (st.units_per_part == 0) continue;
+    int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
     int64_t u = r0 * st.units_per_part;
     Draws d;
     d.st = &st;
@@ -288,7 +324,8 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
     int err = (RNG != 1 && pos >= st.window) ? E_RNG_WINDOW : 0;
     Pcg g;
     g.inc = st.inc;
-    g.state = pcg_jump8(st.base, (uint64_t)pos, t8, *a.jt);
+    g.state = s == 0 ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[0])
+                     : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
     for (int64_t r = r0; r < r1; ++r) {
       double v[2];
       if (RNG == 1) {
@@ -326,17 +363,31 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
         v[0] = x1 * w;
         v[1] = x2 * w;
       }
-      a.xi[(r - r0) * nchunk + c] = make_double2(v[0], v[1]);
+      int64_t xidx = (r - r0) * nchunk + c;
+      if (xidx < a.xi_cap) a.xi[xidx] = make_double2(v[0], v[1]);
     }
     // reported against pool index 0: the rank's slot is not known here
-    if (err) atomicMin(&a.err[0], (unsigned long long)(unsigned)err);
+    if (err) atomicMin(&a.err[s], (unsigned long long)(unsigned)err);
+    if (emax_s >= 0 && emax_s != s) {
+      atomicMax(&a.stream_end[emax_s], emax);  // a thread crossing into another scenario
+      mixed = true;
+      emax = 0;
+    }
+    emax_s = s;
     emax = (unsigned long long)pos > emax ? (unsigned long long)pos : emax;
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long q = __shfl_xor(emax, o);
-    emax = q > emax ? q : emax;
+  // one atomic per wavefront when its threads ended in one scenario
+  int s0 = __shfl(emax_s, 0);
+  bool same = __all(emax_s == s0 || emax_s < 0) && !__any(mixed);
+  if (same) {
+    for (int o = 32; o > 0; o >>= 1) {
+      unsigned long long q = __shfl_xor(emax, o);
+      emax = q > emax ? q : emax;
+    }
+    if ((threadIdx.x & 63) == 0 && emax && s0 >= 0) atomicMax(&a.stream_end[s0], emax);
+  } else if (emax_s >= 0 && emax) {
+    atomicMax(&a.stream_end[emax_s], emax);
   }
-  if ((threadIdx.x & 63) == 0 && emax) atomicMax(&a.stream_end[0], emax);
 }
 
 // One particle of the step; returns the stream position after its last draw
@@ -346,12 +397,25 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
 template <int RNG, bool PREGEN>
 __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, int j, const ZigTables& zt) {
   if (a.inactive[j] != 0) return 0;
-  const StreamDev& st = a.stream[0];
+  const int s = a.scen ? a.scen[j] : 0;
   int tri0 = a.triA[j];
   double x0 = a.xA[j], y0 = a.yA[j];
+  if (a.scen_end && a.batch_step >= a.scen_end[s]) {
+    // the scenario has run all its steps: its rows carry over unchanged
+    a.xB[j] = x0;
+    a.yB[j] = y0;
+    a.triB[j] = tri0;
+    a.upB0[j] = a.upA0[j];
+    a.upB1[j] = a.upA1[j];
+    a.usB0[j] = a.usA0[j];
+    a.usB1[j] = a.usA1[j];
+    return 0;
+  }
+  const StreamDev& st = a.stream[s];
+  const FastScen& q = a.fsc[s];
   if (tri0 < 0) {
     // the particle has no element: the reference raises in the velocity lookup
-    atomicMin(&a.err[0], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)E_NOT_IN_TRI);
+    atomicMin(&a.err[s], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)E_NOT_IN_TRI);
     a.xB[j] = x0;
     a.yB[j] = y0;
     a.triB[j] = tri0;
@@ -359,7 +423,10 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
   }
   // the pair of normals of this slot's active rank (load issued early)
   double2 xi = make_double2(0., 0.);
-  if (PREGEN && a.diffusion_model != 0) xi = a.xi[xi_index(a.rank[j], *a.nact)];
+  if (PREGEN && q.diffusion_model != 0 && st.units_per_part > 0) {
+    int64_t xidx = xi_index(a.chunk_base, a.nscen, s, a.rank[j]);
+    if (xidx < a.xi_cap) xi = a.xi[xidx];
+  }
   const ElemF& ef = a.fm.ef[tri0];
   int4 en = a.fm.en[tri0];
   // P1 factors at the start position (eulerian_field_set.pyx:1103-1105)
@@ -374,7 +441,7 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
   double y = y0 + uy * a.dt;
   Draws dr;
   dr.st = &st;
-  dr.jt = a.jt;
+  dr.jt = &a.jt[s];
   dr.jt8 = nullptr;
   dr.pre_pos = PREGEN ? nullptr : a.pos0;
   dr.pre_state = a.state0;
@@ -389,9 +456,9 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
   dr.err = 0;
   dr.unit0 = 0;
   // diffusion_lsm1 with K and grad K at the start-of-step element
-  if (a.diffusion_model != 0) {
+  if (q.diffusion_model != 0) {
     double diffx, driftx = 0., drifty = 0.;
-    if (a.diffusion_model == 2) {
+    if (q.diffusion_model == 2) {
       double K = n0.K * f0 + n1.K * f1 + n2.K * f2;
       double dkdx = n0.K * ef.gx[0] + n1.K * ef.gx[1] + n2.K * ef.gx[2];
       double dkdy = n0.K * ef.gy[0] + n1.K * ef.gy[1] + n2.K * ef.gy[2];
@@ -399,7 +466,7 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
       driftx = dkdx * a.dt;
       drifty = dkdy * a.dt;
     } else {
-      diffx = a.diff_const;
+      diffx = q.diff_const;
     }
     double xix = 0., xiy = 0.;
     if (diffx != 0.) {
@@ -422,28 +489,28 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
     if (tri == -1 && bnd_j == -1) tri = locate_grid(a.m, x, y);
   }
   bool deleted = false;
-  if (a.boundary_conditions) {
+  if (q.boundary_conditions) {
     if (bnd_j != -1) {
       int lab = a.fm.edge_labels[bnd_j];
       int strand_cond = 0;
-      if (lab == 0 && a.strand > 0.) {
+      if (lab == 0 && q.strand > 0.) {
         BcDraws bd;
         bd.method = RNG;
         bd.zt = &zt;
         bd.g.inc = 0x6c62272e07bb0143ULL | 1u;
         bd.g.state = st.base ^ ((uint64_t)(unsigned)a.pool[j] * 0x9e3779b97f4a7c15ULL);
-        strand_cond = bd.uniform() < a.strand;
+        strand_cond = bd.uniform() < q.strand;
       }
       if (lab == 0 && !strand_cond) {
         int bjj = bnd_j, rc = 0;
-        while (tri == PART_LOC_O && rc < a.max_wall_rebounds) {
+        while (tri == PART_LOC_O && rc < q.max_wall_rebounds) {
           int4 ed = a.fm.ed[bjj];
           double2 v0 = a.fm.xy[ed.x], v1 = a.fm.xy[ed.y];
           // compute_symetric_point2d, intersections.pyx:27-62
           double nx = v0.y - v1.y, ny = v1.x - v0.x;
           double norm2 = nx * nx + ny * ny;
           double dist = (x - v0.x) * nx + (y - v0.y) * ny;
-          double coef = 1. - a.damp / 2.;
+          double coef = 1. - q.damp / 2.;
           double px = x - 2. * ((dist * coef) / norm2) * nx;
           double py = y - 2. * ((dist * coef) / norm2) * ny;
           x = px;
@@ -451,7 +518,7 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
           tri = localize_fast(a.fm, a.m, tri0, x0, y0, px, py, bjj);
           if (tri == PART_LOC_O && bjj != -1 && a.fm.edge_labels[bjj] >= 1) {
             int l2 = a.fm.edge_labels[bjj];
-            if (a.bnd_statistics && a.bnd_stats && l2 - 1 < a.nopen) atomicAdd(&a.bnd_stats[l2 - 1], 1);
+            if (q.bnd_statistics && a.bnd_stats && l2 - 1 < a.nopen) atomicAdd(&a.bnd_stats[s * a.nopen_stride + l2 - 1], 1);
             deleted = true;
             break;
           }
@@ -461,14 +528,14 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
       }
       if (lab >= 1 && !deleted) {
         deleted = true;
-        if (a.bnd_statistics && a.bnd_stats && lab - 1 < a.nopen) atomicAdd(&a.bnd_stats[lab - 1], 1);
+        if (q.bnd_statistics && a.bnd_stats && lab - 1 < a.nopen) atomicAdd(&a.bnd_stats[s * a.nopen_stride + lab - 1], 1);
       }
     }
     if (!deleted && tri == PART_LOC_O) deleted = true;
   } else if (tri == PART_LOC_O) {
     deleted = true;
   }
-  if (dr.err) atomicMin(&a.err[0], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)dr.err);
+  if (dr.err) atomicMin(&a.err[s], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)dr.err);
   if (!a.general && dr.nunits != st.units_per_part) atomicOr(&a.flag[0], 1);
   if (deleted) {
     a.inactive[j] = 2;
@@ -493,6 +560,7 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
 // Persistent launch over the slots (see for_slots_xcd); the Ziggurat tables
 // are staged in LDS once per block.
 // 5 waves per SIMD (<= 96 VGPRs) with pre-generated normals, 4 with the inline sampler
+// 5 waves per SIMD (<= 96 VGPRs) with pre-generated normals, 4 with the inline sampler
 template <int RNG, bool PREGEN>
 __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PREGEN ? 5 : 4))) k_lsm1_2d(FastArgs a) {
   if (a.guarded && a.flag[0] == 0) return;
@@ -506,15 +574,32 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PREGEN
   }
   __syncthreads();
   unsigned long long e = 0;
-  for_slots_xcd(a.n, [&](int j) {
+  int es = -1;
+  for_slots_xcd(a.n, [&](int jl) {
+    int j = a.j0 + jl;
     unsigned long long q = lsm1_particle<RNG, PREGEN>(a, j, zt);
-    e = q > e ? q : e;
+    if (!PREGEN && q) {
+      int s = a.scen ? a.scen[j] : 0;
+      if (es >= 0 && es != s) {
+        atomicMax(&a.stream_end[es], e);  // the thread moved on to another scenario
+        e = 0;
+      }
+      es = s;
+      e = q > e ? q : e;
+    }
   });
-  for (int o = 32; o > 0; o >>= 1) {
-    unsigned long long q = __shfl_xor(e, o);
-    e = q > e ? q : e;
+  if (PREGEN) return;  // the normal generator advanced the streams
+  int s0 = __shfl(es, 0);
+  bool same = __all(es == s0 || es < 0);
+  if (same) {
+    for (int o = 32; o > 0; o >>= 1) {
+      unsigned long long q = __shfl_xor(e, o);
+      e = q > e ? q : e;
+    }
+    if ((threadIdx.x & 63) == 0 && e && s0 >= 0) atomicMax(&a.stream_end[s0], e);
+  } else if (es >= 0 && e) {
+    atomicMax(&a.stream_end[es], e);
   }
-  if ((threadIdx.x & 63) == 0 && e) atomicMax(&a.stream_end[0], e);
 }
 
 }  // namespace cyl

@@ -45,6 +45,10 @@ struct StepArgs {
   int init_only;      // 1: only initialize tri == -2 rows
   int general;        // 1: offsets come from slot.off
   int guarded;        // general-mode launch that runs only if flag[0] is set
+  int j0;             // first slot of the launch (slots [j0, j0 + n))
+  const ParamsDev* pars;  // per-scenario parameters (batched runs) or null: par
+  const int* scen_end;    // per scenario: number of steps it runs, or null
+  int batch_step;         // steps run since the scenarios were set
 };
 
 __device__ __forceinline__ void load_part(const PoolDev& A, const SlotDev& s, int j, int dim, Part& p) {
@@ -90,8 +94,15 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
     if (COUNT) a.units_pool[a.gpool[j]] = 0;
     return 0;
   }
+  if (a.scen_end && a.batch_step >= a.scen_end[scen] && !a.init_only) {
+    // the scenario has run all its steps: its rows carry over unchanged
+    if (COUNT) a.units_pool[a.gpool[j]] = 0;
+    else store_part(a.B, j, DIM, p, 0);
+    return 0;
+  }
   const StreamDev& st = a.stream[scen];
-  Ctx c{a.m, a.f, a.par, a.pc, a.dt, DIM, 0};
+  const ParamsDev& par = a.pars ? a.pars[scen] : a.par;
+  Ctx c{a.m, a.f, par, a.pc, a.dt, DIM, 0};
   Draws dr;
   dr.st = &st;
   dr.jt = &a.jt[scen];
@@ -100,7 +111,7 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
   dr.pre_state = nullptr;
   dr.slot = j;
   dr.zt = a.zt;
-  dr.method = a.par.rng_method;
+  dr.method = par.rng_method;
   dr.dry = COUNT;
   dr.nunits = 0;
   dr.ncalls = 0;
@@ -122,9 +133,9 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
     else bnd_j = update_lsm2(c, p, dr);
     if (!c.err && !COUNT) {
       // draws are counted before the boundary conditions, which use a sub-stream
-      if (a.par.boundary_conditions) {
+      if (par.boundary_conditions) {
         BcDraws bd;
-        bd.method = a.par.rng_method;
+        bd.method = par.rng_method;
         bd.zt = a.zt;
         bd.g.inc = 0x6c62272e07bb0143ULL | 1u;
         bd.g.state = st.base ^ ((uint64_t)(unsigned)pool * 0x9e3779b97f4a7c15ULL);
@@ -134,11 +145,11 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
         if (p.ll == PART_LOC_B) delete_part(p, DIM);
         if (p.ll == PART_LOC_A) delete_part(p, DIM);
       }
-      if (a.par.compute_depth && !p.inactive && DIM == 3 && a.depth)
+      if (par.compute_depth && !p.inactive && DIM == 3 && a.depth)
         a.depth[j] = field2d(a.f, a.m, a.f.fid_zs, p.tri, a.f.nlayers - 1, p.x[0], p.x[1]) - p.x[2];
     }
   }
-  if (was_new && !p.inactive && a.par.compute_depth && DIM == 3 && a.depth && !COUNT && !c.err && a.init_only)
+  if (was_new && !p.inactive && par.compute_depth && DIM == 3 && a.depth && !COUNT && !c.err && a.init_only)
     a.depth[j] = field2d(a.f, a.m, a.f.fid_zs, p.tri, a.f.nlayers - 1, p.x[0], p.x[1]) - p.x[2];
   if (dr.err && !c.err) c.err = dr.err;
   if (c.err) atomicMin(&a.err[scen], ((unsigned long long)(unsigned)pool << 8) | (unsigned)c.err);
@@ -162,10 +173,10 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
 template <int DIM, int MODEL, bool COUNT>
 __global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
   if (a.guarded && a.flag[0] == 0) return;
-  int j = blockIdx.x * BLK + threadIdx.x;
+  int j = a.j0 + blockIdx.x * BLK + threadIdx.x;
   int scen = -1;
   unsigned long long e = 0;
-  if (j < a.n) e = step_slot<DIM, MODEL, COUNT>(a, j, scen);
+  if (j < a.j0 + a.n) e = step_slot<DIM, MODEL, COUNT>(a, j, scen);
   if (COUNT) return;
   // stream end: one atomic per wavefront when its rows share a scenario
   int s0 = __shfl(scen, 0);
@@ -346,9 +357,11 @@ constexpr int SORT_SHIFT = 2;
 
 __device__ __forceinline__ int sort_nbins(int nt) { return (nt >> SORT_SHIFT) + 3; }
 
-__device__ __forceinline__ int sort_key(int t, int inactive, const int* elem_rank, int nt) {
+// batched pools: the kernel class of the slot's scenario is the major key, so
+// each class occupies one contiguous slot range
+__device__ __forceinline__ int sort_key(int t, int inactive, const int* elem_rank, int nt, int cls) {
   int nb = (nt >> SORT_SHIFT) + 1;
-  return inactive ? nb + 1 : (t >= 0 ? elem_rank[t] >> SORT_SHIFT : nb);
+  return cls * (nb + 2) + (inactive ? nb + 1 : (t >= 0 ? elem_rank[t] >> SORT_SHIFT : nb));
 }
 
 struct WaveRun {
@@ -371,11 +384,13 @@ __device__ __forceinline__ WaveRun wave_run(int key) {
   return r;
 }
 
-__global__ void __launch_bounds__(BLK) k_sort_hist(const int* tri, const int* inactive, const int* elem_rank, int n,
-                                                   int nt, int* cnt, int* key) {
+__global__ void __launch_bounds__(BLK) k_sort_hist(const int* tri, const int* inactive, const int* elem_rank,
+                                                   const int* scen, const int* class_of_scen, int n, int nt,
+                                                   int* cnt, int* key) {
   int j = blockIdx.x * BLK + threadIdx.x;
   bool valid = j < n;
-  int k = valid ? sort_key(tri[j], inactive[j], elem_rank, nt) : -1 - (int)(threadIdx.x & 63);
+  int cls = (valid && scen) ? class_of_scen[scen[j]] : 0;
+  int k = valid ? sort_key(tri[j], inactive[j], elem_rank, nt, cls) : -1 - (int)(threadIdx.x & 63);
   if (valid) key[j] = k;
   WaveRun r = wave_run(k);
   if (valid && r.head) atomicAdd(&cnt[k], r.len);
@@ -438,9 +453,13 @@ __global__ void k_rerun_reset(unsigned long long* stream_end, unsigned long long
 }
 
 // units needed this step in uniform mode: units_per_part * active rows of the scenario
-__global__ void k_uniform_units(const StreamDev* st, const int64_t* scen_active, int nscen, int64_t* scen_units) {
+// (scenarios that have run all their steps draw nothing)
+__global__ void k_uniform_units(const StreamDev* st, const int64_t* scen_active, int nscen, const int* scen_end,
+                                int step_id, int64_t* scen_units) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nscen) scen_units[s] = (int64_t)st[s].units_per_part * scen_active[s];
+  if (s >= nscen) return;
+  bool ended = scen_end && step_id >= scen_end[s];
+  scen_units[s] = ended ? 0 : (int64_t)st[s].units_per_part * scen_active[s];
 }
 
 // a general-mode rerun starts from the step's input: undo its deletion marks

@@ -59,6 +59,32 @@ def upload_fields(ctx, f):
         int(f.friction_model), float(f.friction_coefficient)), "cyl_fields_upload")
 
 
+def check_setup(fs, ps, p):
+    """The Solver constructor's sanity checks on a field set, particle set and
+    parameters (solver.pyx:120-177); raises ValueError with the reference's texts."""
+    if fs.dim != ps.dim:
+        if fs.dim == 2 and ps.dim == 3:
+            if fs.fid_zs == -1 or fs.fid_zb == -1:
+                raise ValueError("2D EulerianFieldSet with 3D LagrangianParticleSet : "
+                                 "EulerianFieldSet should contain free surface and bottom fields")
+        else:
+            raise ValueError("EulerianFieldSet dimension ({}) does not match "
+                             "LagrangianParticleSet set dimension ({})".format(fs.dim, ps.dim))
+    if p.frozen_eulerian_fields != 1 and p.final_time > fs.times[fs.ntimes - 1]:
+        raise ValueError("final_time > EulerianFieldSet maximum time")
+    if fs.ntimes == 1 and p.frozen_eulerian_fields != 1:
+        raise ValueError("EulerianFieldSet has only one time step, use frozen_eulerian_fields=True")
+    if ps.additional_velocity and p.model in (2, 3):
+        raise ValueError("Additional velocity only compatible with LSM1")
+    if ps.additional_force and p.model == 1:
+        raise ValueError("Additional force not compatible with LSM1")
+    if p.diffusion_model in (2, 3) and (fs.fid_turbeng == -1 or fs.fid_dissip == -1):
+        raise ValueError(
+            "LSM: diffusion_model={} requires 'TURBULENT ENERG.' and 'DISSIPATION' fields in "
+            "EulerianFieldSet (fid_turbeng={}, fid_dissip={})".format(
+                p.diffusion_model, fs.fid_turbeng, fs.fid_dissip))
+
+
 class Solver:
     def __init__(self, field_set, particle_set, parameters={}, sources=None,
                  control_sections=None, device=0):
@@ -73,29 +99,8 @@ class Solver:
         self.particle_set = particle_set
         self.field_set = field_set
         fs, ps = field_set, particle_set
-        # sanity checks, solver.pyx:120-177
-        if fs.dim != ps.dim:
-            if fs.dim == 2 and ps.dim == 3:
-                if fs.fid_zs == -1 or fs.fid_zb == -1:
-                    raise ValueError("2D EulerianFieldSet with 3D LagrangianParticleSet : "
-                                     "EulerianFieldSet should contain free surface and bottom fields")
-            else:
-                raise ValueError("EulerianFieldSet dimension ({}) does not match "
-                                 "LagrangianParticleSet set dimension ({})".format(fs.dim, ps.dim))
-        if p.frozen_eulerian_fields != 1 and p.final_time > fs.times[fs.ntimes - 1]:
-            raise ValueError("final_time > EulerianFieldSet maximum time")
+        check_setup(fs, ps, p)
         fs.frozen_hydro = bool(p.frozen_eulerian_fields == 1)
-        if fs.ntimes == 1 and not fs.frozen_hydro:
-            raise ValueError("EulerianFieldSet has only one time step, use frozen_eulerian_fields=True")
-        if ps.additional_velocity and p.model in (2, 3):
-            raise ValueError("Additional velocity only compatible with LSM1")
-        if ps.additional_force and p.model == 1:
-            raise ValueError("Additional force not compatible with LSM1")
-        if p.diffusion_model in (2, 3) and (fs.fid_turbeng == -1 or fs.fid_dissip == -1):
-            raise ValueError(
-                "LSM: diffusion_model={} requires 'TURBULENT ENERG.' and 'DISSIPATION' fields in "
-                "EulerianFieldSet (fid_turbeng={}, fid_dissip={})".format(
-                    p.diffusion_model, fs.fid_turbeng, fs.fid_dissip))
         if p.model == 3:
             raise ValueError("LSM3 (model=3) is not available in the MI355X step")
         ps.rho0 = p.water_density

@@ -123,6 +123,21 @@ int cyl_init_particles(cyl_ctx *ctx, const cyl_params *par, double time, double 
  * is advanced exactly like SimuTime.increment (simutime.pyx:77-85). */
 int cyl_step(cyl_ctx *ctx, const cyl_params *par, double *time_inout, double dt, int nsteps,
              int *bnd_stats);
+/* ---- batched scenarios ------------------------------------------------------ *
+ * No counterpart in the reference (the nearest is SolverParallel's split of one
+ * release into pool-index ranges with per-rank streams, solver_parallel.pyx:99-130).
+ * The uploaded pool is split into nscen contiguous pool ranges [base[s], base[s+1]);
+ * scenario s has its own parameters, random stream (state, inc) and number of steps;
+ * all share the mesh, the fields, the time step, the particle constants and one
+ * rng_method.  Each scenario's result is the one a plain run of it alone gives. */
+int cyl_scenarios_set(cyl_ctx *ctx, int nscen, const int *base, const cyl_params *par,
+                      const uint64_t *state, const uint64_t *inc, const int *nsteps);
+/* Random stream of scenario s after the steps run so far. */
+int cyl_scenario_rng_get(cyl_ctx *ctx, int s, uint64_t *state, uint64_t *inc);
+/* nsteps steps of every scenario that has steps left (finished scenarios keep
+ * their rows).  bnd_stats, if not NULL, receives nsteps*nscen*nopen counts. */
+int cyl_batch_step(cyl_ctx *ctx, double *time_inout, double dt, int nsteps, int *bnd_stats);
+
 /* Tuning: element re-sort interval in steps (0 = never) and block size. */
 int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int block_size);
 /* Timing of the last cyl_step: summed duration of the fused step-kernel

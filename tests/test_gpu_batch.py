@@ -1,0 +1,117 @@
+"""Batched scenarios: each scenario's result in the batch equals the result of
+running it alone with a plain Solver (same stream, same steps): element ids,
+flags, tags and random stream identical, positions within 1e-9 relative."""
+import numpy as np
+import pytest
+
+import cylag_amd as cy
+from oracle import refpy as R
+from tests.cases import open_channel_case, vortex_case
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+def streams(n):
+    """Scenario s: the default stream advanced by s << 48 (solver_parallel.pyx:113-115)."""
+    out = []
+    for s in range(n):
+        g = R.RefRng(42, 0xda3e06d11bb2b203)
+        g.advance(s << 48)
+        out.append(g.state())
+    return out
+
+
+def make_pset(case, npart, seed):
+    rng = np.random.default_rng(seed)
+    pos = case.position[rng.choice(case.position.shape[0], npart, replace=False)]
+    return cy.LagrangianParticleSet(pos.copy(), dim=2)
+
+
+def run_alone(case, pset, params, state, nsteps):
+    mesh = cy.TriangularMesh(case.x, case.y, case.tri, boundary_points=case.boundary_points, nopen=case.nopen)
+    fs = cy.EulerianFieldSet(mesh, trifinder=case.trifinder(), **case.fields)
+    cy.random_utils.set_state(*state)
+    s = cy.Solver(fs, pset, params)
+    for _ in range(nsteps):
+        s.forward()
+    ps = s.particle_set
+    out = dict(position=ps.position.copy(), tri=ps.tri.copy(), inactive=ps.inactive.copy(),
+               tag=ps.tag.copy(), velocity=ps.velocity.copy(), rng=cy.random_utils.get_state())
+    s.ctx.close()
+    return out
+
+
+def compare(b, a, rng_b):
+    assert np.array_equal(b.inactive, a["inactive"])
+    assert np.array_equal(b.tri, a["tri"])
+    assert np.array_equal(b.tag, a["tag"])
+    scale = np.maximum(np.abs(a["position"]), 1.)
+    assert np.max(np.abs(b.position - a["position"]) / scale) <= TOL
+    vs = np.maximum(np.abs(a["velocity"]), 1.)
+    assert np.max(np.abs(b.velocity - a["velocity"]) / vs) <= TOL
+    assert tuple(rng_b) == tuple(a["rng"])
+
+
+MIX = [
+    (dict(model=1, diffusion_model=2), 1500, 30),                     # fused kernel, k-eps
+    (dict(model=1, diffusion_model=1, horizontal_diffusivity=0.3), 900, 12),  # fused kernel, constant K
+    (dict(model=2, time_scheme=5, diffusion_model=2, diffusion_lsm2_option=1,
+          particle_velocity_init=1), 700, 25),                         # generic LSM2 DI
+    (dict(model=1, time_scheme=4, diffusion_model=0), 500, 20),        # generic LSM1 RK4
+    (dict(model=1, diffusion_model=2), 1200, 0),                       # no steps at all
+]
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+def test_batch_equals_alone(rng_method):
+    case = vortex_case(npart=4000)
+    base = dict(final_time=10000., time_step=5., rng_method=rng_method)
+    params = [dict(base, **p) for p, _, _ in MIX]
+    nsteps = [k for _, _, k in MIX]
+    st = streams(len(MIX))
+    psets = [make_pset(case, n, 100 + i) for i, (_, n, _) in enumerate(MIX)]
+    mesh = cy.TriangularMesh(case.x, case.y, case.tri)
+    fs = cy.EulerianFieldSet(mesh, trifinder=case.trifinder(), **case.fields)
+    b = cy.BatchSolver(fs, psets, params, st, nsteps=nsteps, sort_interval=3)
+    b.run()
+    out = b.results()
+    for s, (p, n, k) in enumerate(MIX):
+        alone = run_alone(case, make_pset(case, n, 100 + s), params[s], st[s], k)
+        compare(out[s], alone, b.rng_state(s))
+    b.ctx.close()
+
+
+def test_batch_deletions_tags_and_statistics():
+    """Open boundaries: per-scenario deletions, tag renumbering and statistics."""
+    case = open_channel_case()
+    base = dict(final_time=1e9, time_step=1., frozen_eulerian_fields=True, model=1, bnd_statistics=True)
+    mix = [(dict(diffusion_model=1, horizontal_diffusivity=2.), 400, 70),
+           (dict(diffusion_model=1, horizontal_diffusivity=0.5), 300, 40),
+           (dict(time_scheme=2, diffusion_model=1, horizontal_diffusivity=1.), 350, 55)]
+    params = [dict(base, **p) for p, _, _ in mix]
+    st = streams(len(mix))
+    mesh = lambda: cy.TriangularMesh(case.x, case.y, case.tri, boundary_points=case.boundary_points,
+                                     nopen=case.nopen)
+    fs = cy.EulerianFieldSet(mesh(), trifinder=case.trifinder(), **case.fields)
+    psets = [make_pset(case, n, 200 + i) for i, (_, n, _) in enumerate(mix)]
+    b = cy.BatchSolver(fs, psets, params, st, nsteps=[k for _, _, k in mix], sort_interval=5)
+    b.run()
+    out = b.results()
+    stats = np.concatenate(b.bnd_stats)
+    for s, (p, n, k) in enumerate(mix):
+        pset = make_pset(case, n, 200 + s)
+        fs1 = cy.EulerianFieldSet(mesh(), trifinder=case.trifinder(), **case.fields)
+        cy.random_utils.set_state(*st[s])
+        solo = cy.Solver(fs1, pset, params[s])
+        for _ in range(k):
+            solo.forward()
+        ps = solo.particle_set
+        alone = dict(position=ps.position.copy(), tri=ps.tri.copy(), inactive=ps.inactive.copy(),
+                     tag=ps.tag.copy(), velocity=ps.velocity.copy(), rng=cy.random_utils.get_state())
+        assert alone["inactive"].sum() > 0
+        compare(out[s], alone, b.rng_state(s))
+        np.testing.assert_array_equal(stats[:k, s, :], solo.bnd_stats[1][:k])
+        solo.ctx.close()
+    b.ctx.close()
