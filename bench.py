@@ -129,6 +129,129 @@ def cpu_baseline(x, y, tri, fields, pos, params, nsample=100_000, budget_s=15.):
                        "-O2, serial pool-order loop)".format(nsample, nsteps, el))
 
 
+def c4_scenarios(nscen=256, seed=11, lx=20000., ly=5000.):
+    """Release scenarios on the unstructured channel: particle counts
+    log-uniform in [1e3, 1e6], 500-5000 steps, model 1 or 2, a random point
+    release in the interior."""
+    rng = np.random.default_rng(seed)
+    npart = np.exp(rng.uniform(np.log(1e3), np.log(1e6), nscen)).astype(np.int64)
+    nsteps = rng.integers(500, 5001, nscen)
+    model = rng.integers(1, 3, nscen)
+    xy = np.stack([rng.uniform(0.05 * lx, 0.95 * lx, nscen), rng.uniform(0.05 * ly, 0.95 * ly, nscen)], 1)
+    return npart, nsteps, model, xy
+
+
+def c4_params(model):
+    """LSM1 as the plain workload (k-eps K with drift), or 2D LSM2 with the
+    exact Ornstein-Uhlenbeck update of the particle velocity (time_scheme 5,
+    diffusion_lsm2_option 1); dt = 2 s so that 5000 steps stay within the two
+    field frames (t = 0 and 10000 s)."""
+    p = dict(final_time=10000., time_step=2., diffusion_model=2, schmidt_number=0.72)
+    if model == 1:
+        p.update(model=1)
+    else:
+        p.update(model=2, time_scheme=5, diffusion_lsm2_option=1)
+    return p
+
+
+def scenario_streams(cy, nscen):
+    """Scenario s draws from the default stream advanced by s << 48, the split
+    SolverParallel gives its ranks (solver_parallel.pyx:113-115)."""
+    out = []
+    for s in range(nscen):
+        cy.pcg32_seed(42, 0xda3e06d11bb2b203)
+        cy.pcg32_advance(s << 48)
+        out.append(cy.random_utils.get_state())
+    return out
+
+
+def point_release(cy, xy, n):
+    return cy.LagrangianParticleSet(np.repeat(np.asarray(xy, dtype=np.float64)[None], int(n), 0), dim=2)
+
+
+def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250):
+    """The whole batched job: every scenario runs all its steps (or the first
+    max_steps of them); the clock brackets BatchSolver.run calls (inputs
+    resident on the device)."""
+    npart, nsteps, model, xy = c4_scenarios()
+    fs = cy.EulerianFieldSet(mesh, **fields)
+    streams = scenario_streams(cy, len(npart))
+    t0 = time.time()
+    psets = [point_release(cy, xy[s], npart[s]) for s in range(len(npart))]
+    b = cy.BatchSolver(fs, psets, [c4_params(m) for m in model], streams, nsteps=nsteps,
+                       sort_interval=sort_interval)
+    log("batched: {} scenarios, {} particles, {}..{} steps, ready ({:.1f} s)".format(
+        len(npart), int(npart.sum()), int(nsteps.min()), int(nsteps.max()), time.time() - t0))
+    total = int(nsteps.max()) if max_steps is None else min(int(max_steps), int(nsteps.max()))
+    t0 = time.perf_counter()
+    done = 0
+    while done < total:
+        k = min(chunk, total - done)
+        b.run(k)
+        done += k
+        log("batched: {} / {} steps, {:.1f} s".format(done, total, time.perf_counter() - t0))
+    wall = time.perf_counter() - t0
+    k_ms, tot_ms, nlaunch = b.ctx.last_timing()
+    work = float(np.sum(npart * np.minimum(nsteps, total)))
+    b.ctx.close()
+    return dict(scenarios=len(npart), particles=int(npart.sum()), lsm2_scenarios=int(np.sum(model == 2)),
+                particle_steps=work, steps=total, wall_s=wall, value=work / wall, sort_interval=sort_interval)
+
+
+def plain_sequential_leg(cy, mesh, fields, ksteps=40, warm=3, nsteps_cap=None):
+    """The same scenarios, one after another, each with the plain step
+    (cyl_step on a context holding the mesh and fields, the scenario's pool
+    uploaded and initialized as Solver does).  Each scenario is timed over its
+    first `ksteps` steps and its job time taken as T_s times that per-step
+    time."""
+    import ctypes as C
+    from cylag_amd import _hip
+    from cylag_amd.parameters import Parameters
+    from cylag_amd.solver import _params_struct, _pconst_struct, upload_fields, upload_mesh
+    npart, nsteps, model, xy = c4_scenarios()
+    if nsteps_cap is not None:
+        nsteps = np.minimum(nsteps, nsteps_cap)
+    fs = cy.EulerianFieldSet(mesh, **fields)
+    streams = scenario_streams(cy, len(npart))
+    ctx = _hip.Context(0)
+    lib, h, P = ctx.lib, ctx.h, _hip.ptr
+    upload_mesh(ctx, mesh)
+    upload_fields(ctx, fs)
+    t_job = 0.
+    t0 = time.perf_counter()
+    for s in range(len(npart)):
+        p = Parameters(c4_params(model[s]))
+        ps = point_release(cy, xy[s], npart[s])
+        ps.rho0 = p.water_density
+        if p.model == 2:
+            ps._initialize_model_coefficients()
+        par, pc = _params_struct(p, ps), _pconst_struct(ps)
+        n = int(npart[s])
+        h_ = ps._h
+        d = lambda k: np.ascontiguousarray(h_[k], dtype=np.float64)
+        tri = np.full(n, -2, dtype=np.int32)
+        init_tri = np.full(n, -3, dtype=np.int32)
+        ctx.check(lib.cyl_fields_reset(h, 0, 0.), "cyl_fields_reset")
+        ctx.check(lib.cyl_fields_interpolate(h, 0.), "cyl_fields_interpolate")
+        ctx.check(lib.cyl_particles_upload(h, n, 2, P(d("position")), P(d("velocity")),
+                                           P(d("fluid_velocity_seen")), P(np.zeros(n, np.int32), "i"), P(tri, "i"),
+                                           P(np.zeros(n, np.int32), "i"), P(d("diameter")), P(d("volume")),
+                                           P(init_tri, "i"), C.byref(pc)), "cyl_particles_upload")
+        ctx.check(lib.cyl_rng_set(h, *streams[s]), "cyl_rng_set")
+        ctx.check(lib.cyl_init_particles(h, C.byref(par), 0., 2.), "cyl_init_particles")
+        t = C.c_double(0.)
+        ctx.check(lib.cyl_step(h, C.byref(par), C.byref(t), 2., warm, None), "cyl_step")
+        ta = time.perf_counter()
+        ctx.check(lib.cyl_step(h, C.byref(par), C.byref(t), 2., ksteps, None), "cyl_step")
+        per_step = (time.perf_counter() - ta) / ksteps
+        t_job += per_step * int(nsteps[s])
+        if s % 32 == 31:
+            log("plain sequential: {} / {} scenarios ({:.1f} s)".format(s + 1, len(npart), time.perf_counter() - t0))
+    ctx.close()
+    work = float(np.sum(npart * nsteps))
+    return dict(value=work / t_job, job_s_estimated=t_job, steps_timed_per_scenario=ksteps)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +260,11 @@ def main():
     ap.add_argument("--npart", type=int, default=10_000_000)
     ap.add_argument("--sort-interval", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="plain,batched,sequential",
+                    help="comma-separated: plain (the 10M-particle step), batched (256 scenarios in one batch), "
+                         "sequential (the same scenarios one after another with the plain step)")
+    ap.add_argument("--batch-steps", type=int, default=None,
+                    help="run only the first steps of the batched job (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, copy 1 GiB device to device (for counter calibration)")
     args = ap.parse_args()
@@ -144,12 +272,40 @@ def main():
         print("bench.py measures one GPU: the step runs on one MI355X holding the whole mesh "
               "and particle set; --gpus {} is not supported".format(args.gpus))
         sys.exit(2)
+    legs = set(args.legs.split(","))
     import cylag_amd as cy
     t0 = time.time()
-    x, y, tri, fields, pos, params = c2_workload(npart=args.npart)
+    x, y, tri, fields, pos, params = c2_workload(npart=args.npart if "plain" in legs else 1)
     log("workload built: {} nodes, {} triangles, {} particles ({:.1f} s)".format(
         x.size, tri.shape[0], pos.shape[0], time.time() - t0))
     mesh = cy.TriangularMesh(x, y, tri)
+    line = {"metric": METRIC, "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    if "plain" in legs:
+        line.update(plain_leg(cy, args, mesh, fields, pos, params))
+    bt = None
+    if "batched" in legs:
+        bt = batched_leg(cy, mesh, fields, args.sort_interval, args.batch_steps)
+        log("batched: {:.3e} particle-steps/s over {:.1f} s".format(bt["value"], bt["wall_s"]))
+        bt["workload"] = ("256 point releases on the same mesh and fields, N ~ logU(1e3, 1e6) (seed 11), "
+                          "500-5000 steps of 2 s, LSM1 or 2D LSM2 per scenario, streams 42 advanced by s<<48")
+        line["batched"] = bt
+    if "sequential" in legs:
+        seq = plain_sequential_leg(cy, mesh, fields, nsteps_cap=args.batch_steps)
+        log("plain sequential: {:.3e} particle-steps/s (job {:.1f} s estimated)".format(
+            seq["value"], seq["job_s_estimated"]))
+        if bt is not None:
+            bt["plain_sequential"] = seq
+            bt["speedup_vs_plain_sequential"] = bt["value"] / seq["value"]
+        else:
+            line["plain_sequential"] = seq
+    if "plain" in legs and not args.no_cpu:
+        log("cpu baseline ...")
+        line["cpu_baseline"] = cpu_baseline(x, y, tri, fields, pos, params)
+    print(json.dumps(line), flush=True)
+
+
+def plain_leg(cy, args, mesh, fields, pos, params):
     fs = cy.EulerianFieldSet(mesh, **fields)
     ps = cy.LagrangianParticleSet(pos, dim=2)
     cy.pcg32_seed(42, 0xda3e06d11bb2b203)
@@ -170,23 +326,16 @@ def main():
         args.steps, wall, k_ms / max(nlaunch, 1), nlaunch, prep_ms / max(nlaunch, 1)))
     if args.calibrate_copy:
         calibrate_copy()
+    solver.ctx.close()
     psteps = nact * args.steps / wall
     b_alg, b_state, b_mesh = bytes_per_particle_step(mesh.nt, mesh.nv, nact)
     avg_ms = k_ms / max(nlaunch, 1)
     achieved = b_alg * nact / (avg_ms * 1e-3)
-    line = {
-        "metric": METRIC,
+    return {
         "value": psteps,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
         "config": {"workload": "2D LSM1 k-eps, Delaunay 1M pts / {} elements, {} particles, 2 frames, "
                                "dt=5 s, Ziggurat".format(mesh.nt, nact),
                    "elements": int(mesh.nt), "nodes": int(mesh.nv), "particles": int(nact),
@@ -197,10 +346,6 @@ def main():
                      "bytes_per_particle_step": b_alg,
                      "normals_kernel_avg_ms": prep_ms / max(nlaunch, 1)},
     }
-    if not args.no_cpu:
-        log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(x, y, tri, fields, pos, params)
-    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

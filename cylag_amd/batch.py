@@ -100,9 +100,17 @@ class BatchSolver:
         act = inactive == 0
         tri[act] = -2  # initialize_particle_state for every active row (solver.pyx:193-197)
         init_tri = np.full(n, -3, dtype=np.int32)
-        if fs.trifinder is not None and act.any():
-            rows = np.nonzero(act)[0]
-            init_tri[rows] = np.asarray(fs.trifinder(pos[rows, 0], pos[rows, 1]), dtype=np.int32)
+        if fs.trifinder is not None:
+            for s in range(len(sizes)):
+                lo, hi = int(self.base[s]), int(self.base[s + 1])
+                rows = lo + np.nonzero(act[lo:hi])[0]
+                if not len(rows):
+                    continue
+                p = pos[rows]
+                if np.all(p == p[0]):  # a point release: one lookup
+                    init_tri[rows] = int(fs.trifinder(p[0, 0], p[0, 1]))
+                else:
+                    init_tri[rows] = np.asarray(fs.trifinder(p[:, 0], p[:, 1]), dtype=np.int32)
         P = _hip.ptr
         pc = _pconst_struct(self._pc_set)
         self.ctx.check(lib.cyl_particles_upload(h, n, dim, P(pos), P(vel), P(us), P(inactive, "i"), P(tri, "i"),

@@ -950,8 +950,9 @@ extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const
   ctx->scen_par.assign(par, par + nscen);
   ctx->scen_nsteps.assign(nsteps, nsteps + nscen);
   ctx->batch_steps = 0;
-  // kernel classes: the fused kernel for LSM-1 Euler scenarios sharing the
-  // first one's Schmidt number (its packed nodal K), the generic kernel per model
+  // kernel classes: a fused kernel per model for the scenarios it covers that
+  // share the first one's Schmidt number (the packed nodal K), the generic
+  // kernel per model for the others
   ctx->classes.clear();
   ctx->class_of_scen_h.assign(nscen, 0);
   double sigc = 0.;
@@ -964,7 +965,7 @@ extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const
   std::vector<int64_t> count;
   auto class_of = [&](bool fast, int model) {
     for (size_t c = 0; c < ctx->classes.size(); ++c)
-      if (ctx->classes[c].fast == fast && (fast || ctx->classes[c].model == model)) return (int)c;
+      if (ctx->classes[c].fast == fast && ctx->classes[c].model == model) return (int)c;
     ctx->classes.push_back({fast, model, 0, 0});
     count.push_back(0);
     return (int)ctx->classes.size() - 1;
@@ -1274,9 +1275,12 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   launches += 1;
 }
 
+static unsigned persistent_grid(cyl_ctx* ctx, const void* fn, int64_t nblocks, bool mult8);
+
 template <int DIM, int MODEL, bool COUNT>
 static void launch_step_t(cyl_ctx* ctx, const StepArgs& a) {
-  k_step<DIM, MODEL, COUNT><<<nblk(a.n, BLK), BLK, 0, ctx->stream>>>(a);
+  unsigned g = persistent_grid(ctx, (const void*)k_step<DIM, MODEL, COUNT>, nblk(a.n, BLK), false);
+  k_step<DIM, MODEL, COUNT><<<g, BLK, 0, ctx->stream>>>(a);
 }
 
 static void launch_step(cyl_ctx* ctx, const StepArgs& a, int model, bool count) {
@@ -1317,13 +1321,25 @@ static int upload_streams(cyl_ctx* ctx, const std::vector<int>& units_per_part) 
   return CYL_OK;
 }
 
-// The fused 2D LSM-1 kernel covers the Euler scheme with diffusion models
-// 0-3 on 2D fields for one scenario whose rows are all initialized; every
+// The fused 2D kernels cover LSM-1 with the Euler scheme, and LSM-2 with the
+// direct integrator and no velocity noise (diffusion_lsm2_option 1, or
+// diffusion model 0), without added mass and without wall roughness, with
+// diffusion models 0-3 on 2D fields, once every row is initialized; every
 // other configuration runs the generic k_step.
 static bool fast_params(const cyl_ctx* ctx, const cyl_params* p) {
   const FieldsDev& f = ctx->fields;
   if (ctx->dim != 2 || f.dim != 2) return false;
-  if (p->model != 1 || p->time_scheme != 1 || p->diffusion_model < 0 || p->diffusion_model > 3) return false;
+  if (p->diffusion_model < 0 || p->diffusion_model > 3) return false;
+  if (p->model == 1) {
+    if (p->time_scheme != 1) return false;
+  } else if (p->model == 2) {
+    int opt = p->diffusion_lsm2_option;
+    if (p->time_scheme != 5 || !(opt == 1 || (opt >= 2 && p->diffusion_model == 0))) return false;
+    if (ctx->pc.added_mass_force) return false;
+    if (p->boundary_conditions_type == 2 && p->wall_roughness_angle > 0.) return false;
+  } else {
+    return false;
+  }
   if (f.npoints != ctx->mesh.nv || f.fid_ux < 0 || f.fid_uy < 0) return false;
   if (p->diffusion_model >= 2 && (f.fid_turbeng < 0 || f.fid_dissip < 0)) return false;
   return true;
@@ -1395,6 +1411,8 @@ static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
   f.usB1 = a.B.us[1];
   f.inactive = ctx->pool[0].inactive;
   f.del_step = ctx->d_del_step;
+  f.diameter = ctx->d_diam;
+  f.pc = ctx->pc;
   f.rank = ctx->d_rank;
   f.off = ctx->d_off;
   f.pool = ctx->d_gpool;
@@ -1406,6 +1424,7 @@ static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
   f.sqrtdt = sqrt(a.dt);
   f.fsc = ctx->d_fsc;
   f.scen_end = a.scen_end;
+  f.scen_base = a.scen_base;
   f.nopen = ctx->nopen;
   f.nopen_stride = std::max(ctx->nopen, 1);
   f.general = a.general;
@@ -1446,23 +1465,38 @@ static void launch_gen(cyl_ctx* ctx, const FastArgs& f, int m) {
   else k_gen_normals<3><<<persistent_grid(ctx, (const void*)k_gen_normals<3>, nb, false), BLK, 0, ctx->stream>>>(f);
 }
 
-// the fused kernel over f's slot range: pre-generated normals (uniform), or
-// prepared stream positions and the inline sampler (general)
-static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int m, bool general) {
+// the fused kernel of `model` over f's slot range: pre-generated normals
+// (uniform), or prepared stream positions and the inline sampler (general)
+template <int MODEL>
+static void launch_fast_model(cyl_ctx* ctx, const FastArgs& f, int m, bool general) {
   int64_t nb = nblk(f.n);
   hipStream_t s = ctx->stream;
+#define CYL_FAST_K(R, P) (MODEL == 1 ? (const void*)k_lsm1_2d<R, P> : (const void*)k_lsm2_2d<R, P>)
+#define CYL_FAST_LAUNCH(R, P)                                                                           \
+  do {                                                                                                  \
+    unsigned g = persistent_grid(ctx, CYL_FAST_K(R, P), nb, true);                                      \
+    if (MODEL == 1) k_lsm1_2d<R, P><<<g, BLK, 0, s>>>(f);                                               \
+    else k_lsm2_2d<R, P><<<g, BLK, 0, s>>>(f);                                                          \
+  } while (0)
   if (!general) {
-    if (m == 1) k_lsm1_2d<1, true><<<persistent_grid(ctx, (const void*)k_lsm1_2d<1, true>, nb, true), BLK, 0, s>>>(f);
-    else if (m == 2) k_lsm1_2d<2, true><<<persistent_grid(ctx, (const void*)k_lsm1_2d<2, true>, nb, true), BLK, 0, s>>>(f);
-    else k_lsm1_2d<3, true><<<persistent_grid(ctx, (const void*)k_lsm1_2d<3, true>, nb, true), BLK, 0, s>>>(f);
+    if (m == 1) CYL_FAST_LAUNCH(1, true);
+    else if (m == 2) CYL_FAST_LAUNCH(2, true);
+    else CYL_FAST_LAUNCH(3, true);
     return;
   }
   if (m == 1) k_draw_start<1><<<persistent_grid(ctx, (const void*)k_draw_start<1>, nb, true), BLK, 0, s>>>(f);
   else if (m == 2) k_draw_start<2><<<persistent_grid(ctx, (const void*)k_draw_start<2>, nb, true), BLK, 0, s>>>(f);
   else k_draw_start<3><<<persistent_grid(ctx, (const void*)k_draw_start<3>, nb, true), BLK, 0, s>>>(f);
-  if (m == 1) k_lsm1_2d<1, false><<<persistent_grid(ctx, (const void*)k_lsm1_2d<1, false>, nb, true), BLK, 0, s>>>(f);
-  else if (m == 2) k_lsm1_2d<2, false><<<persistent_grid(ctx, (const void*)k_lsm1_2d<2, false>, nb, true), BLK, 0, s>>>(f);
-  else k_lsm1_2d<3, false><<<persistent_grid(ctx, (const void*)k_lsm1_2d<3, false>, nb, true), BLK, 0, s>>>(f);
+  if (m == 1) CYL_FAST_LAUNCH(1, false);
+  else if (m == 2) CYL_FAST_LAUNCH(2, false);
+  else CYL_FAST_LAUNCH(3, false);
+#undef CYL_FAST_LAUNCH
+#undef CYL_FAST_K
+}
+
+static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int model, int m, bool general) {
+  if (model == 1) launch_fast_model<1>(ctx, f, m, general);
+  else launch_fast_model<2>(ctx, f, m, general);
 }
 
 static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
@@ -1539,6 +1573,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   a.par = to_dev(&P[0]);
   a.pars = ctx->d_pars;
   a.scen_end = ctx->batched ? ctx->d_scen_end : nullptr;
+  a.scen_base = ctx->batched ? ctx->d_scen_base : nullptr;
   a.pc = ctx->pc;
   a.dim = ctx->dim;
   a.dt = dt;
@@ -1588,7 +1623,9 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     }
     bool do_sort = !init_only && ctx->sort_interval > 0 && ctx->steps_since_sort >= ctx->sort_interval;
     if (do_sort) {
-      e = enqueue_sort(ctx, !all_fast, launches);
+      // the fused kernels rewrite us for every row they step; rows of finished
+      // scenarios and generic classes carry it over
+      e = enqueue_sort(ctx, !all_fast || ctx->batched, launches);
       if (e) return e;
       ctx->steps_since_sort = 0;
     }
@@ -1655,7 +1692,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       for (auto& c : cls) {
         if (c.hi <= c.lo) continue;
         if (c.fast) {
-          launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), method, false);
+          launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), c.model, method, false);
         } else {
           a.j0 = c.lo;
           a.n = c.hi - c.lo;
@@ -1695,7 +1732,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     for (auto& c : cls) {
       if (c.hi <= c.lo) continue;
       if (c.fast) {
-        launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), method, true);
+        launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), c.model, method, true);
       } else {
         a.j0 = c.lo;
         a.n = c.hi - c.lo;
@@ -1848,11 +1885,11 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
   PoolDev& A = ctx->pool[ctx->cur];
   PoolDev& B = ctx->pool[1 - ctx->cur];
   HIPCHK(hipMemsetAsync(ctx->d_scnt, 0, nbin * sizeof(int), s));
-  k_sort_hist<<<nblk(n), BLK, 0, s>>>(A.tri, A.inactive, ctx->d_elem_rank, ctx->batched ? ctx->d_scen : nullptr,
+  k_sort_hist<<<nblk(n, BLK * SORT_ITEMS), BLK, 0, s>>>(A.tri, A.inactive, ctx->d_elem_rank, ctx->batched ? ctx->d_scen : nullptr,
                                       ctx->d_class_of_scen, n, nt, ctx->d_scnt, ctx->d_skey);
   enqueue_scan(ctx, ctx->d_scnt, ctx->d_cursor, (int64_t)nbin, ctx->d_sort_tiles, ctx->d_total,
                (const int64_t*)nullptr, (const int*)nullptr, launches);
-  k_sort_place<<<nblk(n), BLK, 0, s>>>(ctx->d_skey, n, ctx->d_cursor, ctx->d_perm);
+  k_sort_place<<<nblk(n, BLK * SORT_ITEMS), BLK, 0, s>>>(ctx->d_skey, n, ctx->d_cursor, ctx->d_perm);
   PermArgs pa{};
   pa.perm = ctx->d_perm;
   pa.n = n;

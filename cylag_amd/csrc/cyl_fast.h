@@ -1,6 +1,8 @@
-// Fused 2D LSM-1 step (Euler advection + Euler-Maruyama diffusion + walk +
-// rebounds) on packed, cache-line-sized mesh records.  This is the common
-// production configuration; every other configuration runs k_step.
+// Fused 2D steps on packed, cache-line-sized mesh records: LSM-1 (Euler
+// advection + Euler-Maruyama diffusion + walk + rebounds) and LSM-2 with the
+// direct integrator and position diffusion (the velocity relaxes exactly
+// towards the fluid velocity; rebounds reflect it).  These are the common
+// production configurations; every other configuration runs k_step.
 //
 // Packed layout in HBM:
 //   ElemF  (128 B, one L2 line): P1 coefficients det/gradx/grady and the three
@@ -13,8 +15,9 @@
 // nodes of neighbouring elements share cache lines; element and edge ids are
 // the mesh's own.
 // The arithmetic is the same, operation for operation, as update_state_lsm1.pyx
-// (euler_lsm1 :135-177, diffusion_lsm1 :658-726), xylocalizer.pyx:338-435 and
-// compute_bc.pyx:35-395 for model 1.
+// (euler_lsm1 :135-177, diffusion_lsm1 :658-726), update_state_lsm2.pyx
+// (di_lsm2 :1329-1543, lsm2_direct_coefficients :1240-1327), xylocalizer.pyx:338-435
+// and compute_bc.pyx:35-395.
 #pragma once
 #include "cyl_kernels.h"
 
@@ -171,6 +174,8 @@ struct FastArgs {
   double* usB1;
   int* inactive;
   int* del_step;
+  const double* diameter;  // per slot (LSM-2)
+  ParticleConst pc;        // drag and added-mass constants (LSM-2)
   const int* rank;
   const int64_t* off;
   const int* pool;
@@ -181,6 +186,7 @@ struct FastArgs {
   const FastScen* fsc;     // [nscen]
   const int* scen_end;     // [nscen] number of steps each scenario runs, or null
   int batch_step;          // steps run since the scenarios were set
+  const int* scen_base;    // [nscen] first pool index of each scenario, or null
   int nopen, nopen_stride; // open boundaries; stride of bnd_stats per scenario
   int general, guarded;
   int* flag;
@@ -392,10 +398,13 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
 
 // One particle of the step; returns the stream position after its last draw
 // (0 if it drew nothing or its normals were generated beforehand).
+// MODEL 1: euler_lsm1 + diffusion_lsm1.  MODEL 2: di_lsm2 without added mass
+// and without velocity noise (diffusion_lsm2_option 1), then diffusion_lsm2,
+// which is diffusion_lsm1 (update_state_lsm2.pyx:30-139, 1329-1543).
 // PREGEN: the step's two normals of the slot are xi[rank] (uniform draw
 // assignment); otherwise they are sampled here from the prepared start.
-template <int RNG, bool PREGEN>
-__device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, int j, const ZigTables& zt) {
+template <int MODEL, int RNG, bool PREGEN>
+__device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, int j, const ZigTables& zt) {
   if (a.inactive[j] != 0) return 0;
   const int s = a.scen ? a.scen[j] : 0;
   int tri0 = a.triA[j];
@@ -436,9 +445,29 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
   NodeF n0 = a.nodes[en.x], n1 = a.nodes[en.y], n2 = a.nodes[en.z];
   double ux = n0.u * f0 + n1.u * f1 + n2.u * f2;
   double uy = n0.v * f0 + n1.v * f1 + n2.v * f2;
-  // euler_lsm1
-  double x = x0 + ux * a.dt;
-  double y = y0 + uy * a.dt;
+  double x, y, upx = ux, upy = uy;  // LSM-1: particle velocity = fluid velocity seen = U
+  int perr = 0;
+  if (MODEL == 1) {
+    // euler_lsm1
+    x = x0 + ux * a.dt;
+    y = y0 + uy * a.dt;
+  } else {
+    // di_lsm2: exact Ornstein-Uhlenbeck drift of the particle velocity
+    double u0x = a.upA0[j], u0y = a.upA1[j];
+    double dx = ux - u0x, dy = uy - u0y;
+    double a1 = relax_rate_dp(a.pc, a.diameter[j], sqrt(dx * dx + dy * dy));
+    double beta, D1, D2, fxc;
+    if (!lsm2_drift_coefficients(a1, a.dt, beta, D1, D2, fxc)) {
+      perr = E_LSM2_COEF;
+      beta = D1 = D2 = fxc = 0.;
+    }
+    double um_beta = -cr_expm1(-a1 * a.dt);
+    double Fx = a.pc.a2c * 0., Fy = a.pc.a2c * 0.;  // no added mass: zero fluid acceleration
+    upx = u0x * beta + ux * um_beta + Fx * D1;
+    upy = u0y * beta + uy * um_beta + Fy * D1;
+    x = x0 + u0x * D1 + ux * D2 + Fx * fxc;
+    y = y0 + u0y * D1 + uy * D2 + Fy * fxc;
+  }
   Draws dr;
   dr.st = &st;
   dr.jt = &a.jt[s];
@@ -498,7 +527,8 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
         bd.method = RNG;
         bd.zt = &zt;
         bd.g.inc = 0x6c62272e07bb0143ULL | 1u;
-        bd.g.state = st.base ^ ((uint64_t)(unsigned)a.pool[j] * 0x9e3779b97f4a7c15ULL);
+        int lpool = a.pool[j] - (a.scen_base ? a.scen_base[s] : 0);  // row of the scenario's own pool
+        bd.g.state = st.base ^ ((uint64_t)(unsigned)lpool * 0x9e3779b97f4a7c15ULL);
         strand_cond = bd.uniform() < q.strand;
       }
       if (lab == 0 && !strand_cond) {
@@ -522,6 +552,22 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
             deleted = true;
             break;
           }
+          if (MODEL == 2) {
+            // specular reflection of the particle velocity on the wall (normal2u,
+            // utils.pyx:99-119; compute_bc.pyx rebound branch), damped
+            double ex = v1.x - v0.x, ey = v1.y - v0.y;
+            double nrm = sqrt(ex * ex + ey * ey);
+            double nn0 = (v0.y - v1.y) / nrm, nn1 = (v1.x - v0.x) / nrm;
+            double up0 = upx;
+            upx = up0 * nn0 + upy * nn1;
+            upy = -up0 * nn1 + upy * nn0;
+            upx = -1. * upx;
+            up0 = upx;
+            upx = up0 * nn0 - upy * nn1;
+            upy = up0 * nn1 + upy * nn0;
+            upx *= 1. - q.damp;
+            upy *= 1. - q.damp;
+          }
           ++rc;
           if (tri == PART_LOC_O && bjj == -1) break;
         }
@@ -535,6 +581,7 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
   } else if (tri == PART_LOC_O) {
     deleted = true;
   }
+  if (perr) atomicMin(&a.err[s], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)perr);
   if (dr.err) atomicMin(&a.err[s], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)dr.err);
   if (!a.general && dr.nunits != st.units_per_part) atomicOr(&a.flag[0], 1);
   if (deleted) {
@@ -546,12 +593,14 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
     tri = -1;
     ux = 0.;
     uy = 0.;
+    upx = 0.;
+    upy = 0.;
   }
   a.xB[j] = x;
   a.yB[j] = y;
   a.triB[j] = tri;
-  a.upB0[j] = ux;
-  a.upB1[j] = uy;
+  a.upB0[j] = upx;
+  a.upB1[j] = upy;
   a.usB0[j] = ux;
   a.usB1[j] = uy;
   return dr.live ? (unsigned long long)(dr.pos0 + dr.ncalls) : 0ull;
@@ -559,10 +608,8 @@ __device__ __forceinline__ unsigned long long lsm1_particle(const FastArgs& a, i
 
 // Persistent launch over the slots (see for_slots_xcd); the Ziggurat tables
 // are staged in LDS once per block.
-// 5 waves per SIMD (<= 96 VGPRs) with pre-generated normals, 4 with the inline sampler
-// 5 waves per SIMD (<= 96 VGPRs) with pre-generated normals, 4 with the inline sampler
-template <int RNG, bool PREGEN>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PREGEN ? 5 : 4))) k_lsm1_2d(FastArgs a) {
+template <int MODEL, int RNG, bool PREGEN>
+__device__ __forceinline__ void fast_step_body(const FastArgs& a) {
   if (a.guarded && a.flag[0] == 0) return;
   __shared__ ZigTables zt;
   if (RNG == 2 && !PREGEN) {
@@ -577,7 +624,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PREGEN
   int es = -1;
   for_slots_xcd(a.n, [&](int jl) {
     int j = a.j0 + jl;
-    unsigned long long q = lsm1_particle<RNG, PREGEN>(a, j, zt);
+    unsigned long long q = fast_particle<MODEL, RNG, PREGEN>(a, j, zt);
     if (!PREGEN && q) {
       int s = a.scen ? a.scen[j] : 0;
       if (es >= 0 && es != s) {
@@ -600,6 +647,17 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PREGEN
   } else if (es >= 0 && e) {
     atomicMax(&a.stream_end[es], e);
   }
+}
+
+// 5 waves per SIMD (<= 96 VGPRs) with pre-generated normals, 4 with the inline sampler
+template <int RNG, bool PREGEN>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PREGEN ? 5 : 4))) k_lsm1_2d(FastArgs a) {
+  fast_step_body<1, RNG, PREGEN>(a);
+}
+
+template <int RNG, bool PREGEN>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4))) k_lsm2_2d(FastArgs a) {
+  fast_step_body<2, RNG, PREGEN>(a);
 }
 
 }  // namespace cyl

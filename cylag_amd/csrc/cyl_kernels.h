@@ -49,6 +49,7 @@ struct StepArgs {
   const ParamsDev* pars;  // per-scenario parameters (batched runs) or null: par
   const int* scen_end;    // per scenario: number of steps it runs, or null
   int batch_step;         // steps run since the scenarios were set
+  const int* scen_base;   // per scenario: first pool index, or null (one scenario)
 };
 
 __device__ __forceinline__ void load_part(const PoolDev& A, const SlotDev& s, int j, int dim, Part& p) {
@@ -138,7 +139,8 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
         bd.method = par.rng_method;
         bd.zt = a.zt;
         bd.g.inc = 0x6c62272e07bb0143ULL | 1u;
-        bd.g.state = st.base ^ ((uint64_t)(unsigned)pool * 0x9e3779b97f4a7c15ULL);
+        int lpool = pool - (a.scen_base ? a.scen_base[scen] : 0);  // row of the scenario's own pool
+        bd.g.state = st.base ^ ((uint64_t)(unsigned)lpool * 0x9e3779b97f4a7c15ULL);
         compute_bc(c, p, bnd_j, bd, a.bnd_stats ? a.bnd_stats + scen * a.nopen : nullptr, a.nopen_real);
       } else {
         if (p.tri == PART_LOC_O) delete_part(p, DIM);
@@ -170,25 +172,39 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
   return dr.live ? (unsigned long long)(dr.pos0 + dr.ncalls) : 0ull;
 }
 
+// Grid-stride over the slots of the launch (the grid is at most what fits on
+// the device at once, so a guarded launch that finds nothing to do ends at once).
 template <int DIM, int MODEL, bool COUNT>
 __global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
   if (a.guarded && a.flag[0] == 0) return;
-  int j = a.j0 + blockIdx.x * BLK + threadIdx.x;
-  int scen = -1;
   unsigned long long e = 0;
-  if (j < a.j0 + a.n) e = step_slot<DIM, MODEL, COUNT>(a, j, scen);
+  int es = -1;
+  for (int64_t b = (int64_t)blockIdx.x * BLK; b < a.n; b += (int64_t)gridDim.x * BLK) {
+    int64_t jl = b + threadIdx.x;
+    if (jl >= a.n) continue;
+    int scen = -1;
+    unsigned long long q = step_slot<DIM, MODEL, COUNT>(a, a.j0 + (int)jl, scen);
+    if (!COUNT && q) {
+      if (es >= 0 && es != scen) {
+        atomicMax(&a.stream_end[es], e);  // the thread moved on to another scenario
+        e = 0;
+      }
+      es = scen;
+      e = q > e ? q : e;
+    }
+  }
   if (COUNT) return;
   // stream end: one atomic per wavefront when its rows share a scenario
-  int s0 = __shfl(scen, 0);
-  bool same = __all(scen == s0 || scen < 0);
+  int s0 = __shfl(es, 0);
+  bool same = __all(es == s0 || es < 0);
   if (same) {
     for (int o = 32; o > 0; o >>= 1) {
       unsigned long long q = __shfl_xor(e, o);
       e = q > e ? q : e;
     }
     if ((threadIdx.x & 63) == 0 && e && s0 >= 0) atomicMax(&a.stream_end[s0], e);
-  } else if (e) {
-    atomicMax(&a.stream_end[scen], e);
+  } else if (es >= 0 && e) {
+    atomicMax(&a.stream_end[es], e);
   }
 }
 
@@ -350,9 +366,9 @@ __global__ void k_scen_totals(const int64_t* scan, const int64_t* total, const i
 
 // Element re-sort: counting sort of the slots by a key that groups 2^SORT_SHIFT
 // elements neighbouring along a Hilbert curve through the element centroids
-// (active rows), then rows without element, then inactive rows.  The input is nearly sorted after the first sort, so equal
-// keys come in runs of adjacent lanes: the first lane of a run does one atomic
-// for the whole run.
+// (active rows), then rows without element, then inactive rows.  The input is
+// nearly sorted after the first sort, so equal keys come in long runs of
+// consecutive slots.
 constexpr int SORT_SHIFT = 2;
 
 __device__ __forceinline__ int sort_nbins(int nt) { return (nt >> SORT_SHIFT) + 3; }
@@ -364,47 +380,91 @@ __device__ __forceinline__ int sort_key(int t, int inactive, const int* elem_ran
   return cls * (nb + 2) + (inactive ? nb + 1 : (t >= 0 ? elem_rank[t] >> SORT_SHIFT : nb));
 }
 
-struct WaveRun {
-  int head_lane, len, rank;
-  bool head;
-};
+// Block-local histogram: a block's SORT_ITEMS * BLK slots meet in an LDS hash
+// table of their keys (open addressing, SORT_PROBES probes), so a bin that
+// holds many of the block's slots takes one global atomic per block -- a point
+// release puts a million slots in a handful of bins.  A key that finds no free
+// entry goes to the global counters directly.
+constexpr int SORT_ITEMS = 8;
+constexpr int SORT_TABLE = 4096;  // entries, a power of two >= 2 * SORT_ITEMS * BLK
+constexpr int SORT_PROBES = 8;
 
-__device__ __forceinline__ WaveRun wave_run(int key) {
-  int lane = threadIdx.x & 63;
-  int prev = __shfl_up(key, 1);
-  WaveRun r;
-  r.head = lane == 0 || prev != key;
-  unsigned long long heads = __ballot(r.head);
-  unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-  r.head_lane = 63 - __clzll(heads & upto);
-  unsigned long long above = heads & ~upto;
-  int next = above ? __ffsll((long long)above) - 1 : 64;
-  r.len = next - r.head_lane;
-  r.rank = lane - r.head_lane;
-  return r;
+__device__ __forceinline__ int sort_table_slot(int* tkey, int k) {
+  unsigned h = ((unsigned)k * 2654435761u) >> 20;  // 12 bits: SORT_TABLE entries
+#pragma unroll 1
+  for (int p = 0; p < SORT_PROBES; ++p) {
+    int idx = (int)((h + (unsigned)p) & (SORT_TABLE - 1));
+    int old = atomicCAS(&tkey[idx], -1, k);
+    if (old == -1 || old == k) return idx;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ void sort_table_clear(int* tkey, int* tcnt) {
+  for (int i = threadIdx.x; i < SORT_TABLE; i += BLK) {
+    tkey[i] = -1;
+    tcnt[i] = 0;
+  }
 }
 
 __global__ void __launch_bounds__(BLK) k_sort_hist(const int* tri, const int* inactive, const int* elem_rank,
                                                    const int* scen, const int* class_of_scen, int n, int nt,
                                                    int* cnt, int* key) {
-  int j = blockIdx.x * BLK + threadIdx.x;
-  bool valid = j < n;
-  int cls = (valid && scen) ? class_of_scen[scen[j]] : 0;
-  int k = valid ? sort_key(tri[j], inactive[j], elem_rank, nt, cls) : -1 - (int)(threadIdx.x & 63);
-  if (valid) key[j] = k;
-  WaveRun r = wave_run(k);
-  if (valid && r.head) atomicAdd(&cnt[k], r.len);
+  __shared__ int tkey[SORT_TABLE];
+  __shared__ int tcnt[SORT_TABLE];
+  sort_table_clear(tkey, tcnt);
+  __syncthreads();
+  int64_t b0 = (int64_t)blockIdx.x * BLK * SORT_ITEMS;
+#pragma unroll
+  for (int q = 0; q < SORT_ITEMS; ++q) {
+    int64_t j = b0 + q * BLK + threadIdx.x;
+    if (j >= n) continue;
+    int cls = scen ? class_of_scen[scen[j]] : 0;
+    int k = sort_key(tri[j], inactive[j], elem_rank, nt, cls);
+    key[j] = k;
+    int idx = sort_table_slot(tkey, k);
+    if (idx >= 0) atomicAdd(&tcnt[idx], 1);
+    else atomicAdd(&cnt[k], 1);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < SORT_TABLE; i += BLK)
+    if (tkey[i] >= 0) atomicAdd(&cnt[tkey[i]], tcnt[i]);
 }
 
+// Each block reserves one range per key in the global cursors and places its
+// slots in it (the order of slots within a bin is arbitrary).
 __global__ void __launch_bounds__(BLK) k_sort_place(const int* key, int n, int64_t* cursor, int* perm) {
-  int j = blockIdx.x * BLK + threadIdx.x;
-  bool valid = j < n;
-  int k = valid ? key[j] : -1 - (int)(threadIdx.x & 63);
-  WaveRun r = wave_run(k);
-  unsigned long long base = 0;
-  if (valid && r.head) base = atomicAdd((unsigned long long*)&cursor[k], (unsigned long long)r.len);
-  base = __shfl(base, r.head_lane);
-  if (valid) perm[base + r.rank] = j;
+  __shared__ int tkey[SORT_TABLE];
+  __shared__ int tcnt[SORT_TABLE];
+  __shared__ int tbase[SORT_TABLE];
+  sort_table_clear(tkey, tcnt);
+  __syncthreads();
+  int64_t b0 = (int64_t)blockIdx.x * BLK * SORT_ITEMS;
+  int slot[SORT_ITEMS], lrank[SORT_ITEMS];
+#pragma unroll
+  for (int q = 0; q < SORT_ITEMS; ++q) {
+    int64_t j = b0 + q * BLK + threadIdx.x;
+    slot[q] = -1;
+    lrank[q] = 0;
+    if (j >= n) continue;
+    int k = key[j];
+    int idx = sort_table_slot(tkey, k);
+    if (idx >= 0) {
+      slot[q] = idx;
+      lrank[q] = atomicAdd(&tcnt[idx], 1);
+    } else {
+      perm[atomicAdd((unsigned long long*)&cursor[k], 1ull)] = (int)j;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < SORT_TABLE; i += BLK)
+    if (tkey[i] >= 0) tbase[i] = (int)atomicAdd((unsigned long long*)&cursor[tkey[i]], (unsigned long long)tcnt[i]);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < SORT_ITEMS; ++q) {
+    int64_t j = b0 + q * BLK + threadIdx.x;
+    if (slot[q] >= 0) perm[tbase[slot[q]] + lrank[q]] = (int)j;
+  }
 }
 
 template <class T>

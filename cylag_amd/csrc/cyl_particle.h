@@ -429,10 +429,15 @@ __device__ inline double drag_coefficient(const ParticleConst& pc, double U, dou
   return 0.44;
 }
 
-__device__ inline double relax_rate(const Ctx& c, const Part& p, double unorm) {
+// a1 of a particle of diameter dp with slip velocity unorm (update_state_lsm2.pyx:1346-1350)
+__device__ inline double relax_rate_dp(const ParticleConst& pc, double dp, double unorm) {
   unorm = fmax(EPS_LSM2, unorm);
-  double Cd = drag_coefficient(c.pc, unorm, p.diameter);
-  return c.pc.a1c * (1.5 / p.diameter) * Cd * unorm;
+  double Cd = drag_coefficient(pc, unorm, dp);
+  return pc.a1c * (1.5 / dp) * Cd * unorm;
+}
+
+__device__ inline double relax_rate(const Ctx& c, const Part& p, double unorm) {
+  return relax_rate_dp(c.pc, p.diameter, unorm);
 }
 
 __device__ inline double rel_norm(int d, double ux, double uy, double uz, double a, double b, double e) {
@@ -578,9 +583,10 @@ __device__ inline void keps_at(Ctx& c, const Part& p, double x, double y, double
   }
 }
 
-// lsm2_direct_coefficients, update_state_lsm2.pyx:1240-1327
-__device__ inline bool lsm2_coefficients(double a1, double dt, double& beta, double& D1, double& D2, double& fxc,
-                                         double& quu, double& qxu, double& qxx) {
+// lsm2_direct_coefficients, update_state_lsm2.pyx:1240-1327: the drift part
+// (beta, D1, D2 and the force coefficient); false when a1 or dt is invalid
+__device__ inline bool lsm2_drift_coefficients(double a1, double dt, double& beta, double& D1, double& D2,
+                                               double& fxc) {
   double r = a1 * dt;
   if (!isfinite(a1) || !isfinite(dt) || a1 < 0. || dt < 0. || !isfinite(r)) return false;
   beta = cr_exp(-r);
@@ -599,6 +605,14 @@ __device__ inline bool lsm2_coefficients(double a1, double dt, double& beta, dou
     D2 = dt - D1;
     fxc = D2 / a1;
   }
+  return true;
+}
+
+// lsm2_direct_coefficients with the covariance terms of the noise
+__device__ inline bool lsm2_coefficients(double a1, double dt, double& beta, double& D1, double& D2, double& fxc,
+                                         double& quu, double& qxu, double& qxx) {
+  if (!lsm2_drift_coefficients(a1, dt, beta, D1, D2, fxc)) return false;
+  double r = a1 * dt;
   quu = (r == 0.) ? dt : dt * (-cr_expm1(-2. * r) / (2. * r));
   qxu = 0.5 * D1 * D1;
   if (r < 1.) {

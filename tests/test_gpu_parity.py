@@ -128,6 +128,20 @@ def test_lsm2_2d_keps_stochastic(opt):
 
 
 @pytest.mark.parametrize("rng_method", [1, 2, 3])
+@pytest.mark.parametrize("drag", [0, 1, 2])
+def test_lsm2_2d_direct_rebounds(drag, rng_method):
+    """2D LSM-2, direct integrator without velocity noise (the fused kernel's
+    LSM-2 path): drag models, k-epsilon position diffusion, damped wall
+    rebounds that reflect the particle velocity, two frames."""
+    case = vortex_case(npart=2500)
+    case.pset_kw = dict(drag_coefficient_model=drag, particle_diameter=2e-4, particle_density=1800.)
+    params = dict(final_time=300., time_step=5., model=2, time_scheme=5, diffusion_model=2,
+                  diffusion_lsm2_option=1, particle_velocity_init=1, rng_method=rng_method,
+                  rebound_damping_coef=0.3)
+    assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
 def test_unstructured_keps_two_frames(rng_method):
     case = vortex_case(npart=3000)
     params = dict(final_time=200., time_step=5., model=1, diffusion_model=2,
