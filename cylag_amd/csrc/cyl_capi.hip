@@ -68,6 +68,8 @@ struct cyl_ctx {
   double* d_tmp[4 + MAX_OPT] = {};
   double* d_prev[4] = {};
   int record = 0, interp_done = 0, prev_valid = 0, frozen = 1;
+  double interp_hist[2] = {0., 0.};         // times of the last two interpolations (most recent last)
+  int interp_count = 0;
   const double** d_ti_src = nullptr;        // [ntimes][nti] frame pointers of the interpolated fields
   double** d_ti_dst = nullptr;              // [2][nti] destination per buffer parity
   int nti = 0, tmp_parity = 0;
@@ -547,6 +549,7 @@ extern "C" int cyl_fields_reset(cyl_ctx* ctx, int frozen, double t0) {
   ctx->frozen = frozen ? 1 : 0;
   ctx->fields.frozen = ctx->frozen;
   if (!ctx->frozen) ctx->record = lower_index(ctx->times, t0, 0);
+  ctx->interp_count = 0;
   ctx->interp_done = 0;
   ctx->prev_valid = 0;
   ctx->fields.prev_valid = 0;
@@ -567,6 +570,9 @@ static int enqueue_time_interp(cyl_ctx* ctx, double time, int& launches) {
   if (ctx->frozen) return CYL_OK;
   ctx->prev_valid = ctx->interp_done;
   ctx->interp_done = 1;
+  ctx->interp_hist[0] = ctx->interp_hist[1];
+  ctx->interp_hist[1] = time;
+  ++ctx->interp_count;
   ctx->record = lower_index(ctx->times, time, ctx->record);
   int rec = ctx->record;
   if (rec + 1 >= ctx->ntimes) return ctx->set_err(CYL_E_STEP, -1, "time interpolation beyond the last record");
@@ -658,6 +664,55 @@ static int ensure_scen_tables(cyl_ctx* ctx) {
   return CYL_OK;
 }
 
+// Every per-slot buffer of a pool of n slots (both state copies, slot
+// tables, scans, sort), in ctx->part_bufs; nscen_pad scenarios of padding in
+// the normals buffer.
+static int alloc_slot_bufs(cyl_ctx* ctx, int n, int nscen_pad) {
+  ctx->n = n;
+  int dim = ctx->dim;
+  for (int b = 0; b < 2; ++b) {
+    int e = alloc_pool(ctx, ctx->pool[b]);
+    if (e) return e;
+  }
+  // the pool flags are per slot, not double-buffered: a row's flag is only
+  // changed by its own thread (deletion), in place
+  ctx->pool[1].inactive = ctx->pool[0].inactive;
+  Bufs& bb = ctx->part_bufs;
+  HIPCHK(dalloc(bb, &ctx->d_gpool, n));
+  HIPCHK(dalloc(bb, &ctx->d_rank, n));
+  // transposed normals: 32 x (chunks) entries, chunks <= n/32 + 1 per scenario
+  HIPCHK(dalloc(bb, &ctx->d_xi, (size_t)n + 32 * (size_t)nscen_pad));
+  ctx->xi_cap = (int64_t)n + 32 * (int64_t)nscen_pad;
+  HIPCHK(dalloc(bb, &ctx->d_off, n));
+  HIPCHK(dalloc(bb, &ctx->d_init_tri, n));
+  HIPCHK(dalloc(bb, &ctx->d_del_step, n));
+  HIPCHK(dalloc(bb, &ctx->d_diam, n));
+  HIPCHK(dalloc(bb, &ctx->d_vol, n));
+  HIPCHK(dalloc(bb, &ctx->d_depth, dim == 3 ? n : 1));
+  HIPCHK(dalloc(bb, &ctx->d_vals, n + 1));
+  HIPCHK(dalloc(bb, &ctx->d_scan, n + 1));
+  HIPCHK(dalloc(bb, &ctx->d_bsum, nblk(n + 1) + 1));
+  HIPCHK(dalloc(bb, &ctx->d_total, 2));
+  HIPCHK(dalloc(bb, &ctx->d_perm, n));
+  HIPCHK(dalloc(bb, &ctx->d_skey, n));
+  {
+    double* tb;
+    HIPCHK(dalloc(bb, &tb, (size_t)n));
+    ctx->d_tmpbuf = tb;
+  }
+  // sort buffers and spares are allocated on the first sort
+  ctx->d_scnt = nullptr;
+  ctx->d_cursor = nullptr;
+  ctx->d_sort_tiles = nullptr;
+  ctx->sort_nt = 0;
+  ctx->sort_ncls = 0;
+  ctx->sp_gpool = ctx->sp_init_tri = ctx->sp_del_step = ctx->sp_inactive = ctx->sp_rank = nullptr;
+  ctx->sp_diam = ctx->sp_vol = ctx->sp_depth = nullptr;
+  ctx->sp_scen = nullptr;
+  ctx->d_scen = nullptr;
+  return CYL_OK;
+}
+
 extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const double* pos, const double* vel,
                                     const double* us, const int* inactive, const int* tri, const int* ll,
                                     const double* diameter, const double* volume, const int* init_tri,
@@ -691,37 +746,9 @@ extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const doub
     p.buoyancy_velocity_model = pconst->buoyancy_velocity_model;
     p.a0c = pconst->a0c; p.a1c = pconst->a1c; p.a2c = pconst->a2c; p.a3c = pconst->a3c;
   }
-  for (int b = 0; b < 2; ++b) {
-    int e = alloc_pool(ctx, ctx->pool[b]);
-    if (e) return e;
-  }
-  // the pool flags are per slot, not double-buffered: a row's flag is only
-  // changed by its own thread (deletion), in place
-  ctx->pool[1].inactive = ctx->pool[0].inactive;
-  Bufs& bb = ctx->part_bufs;
+  int e0 = alloc_slot_bufs(ctx, npart, 3);
+  if (e0) return e0;
   int n = npart;
-  HIPCHK(dalloc(bb, &ctx->d_gpool, n));
-  HIPCHK(dalloc(bb, &ctx->d_rank, n));
-  // transposed normals: 32 x (chunks) entries, chunks <= n/32 + 1 per scenario
-  HIPCHK(dalloc(bb, &ctx->d_xi, (size_t)n + 32 * 3));
-  ctx->xi_cap = (int64_t)n + 32 * 3;
-  HIPCHK(dalloc(bb, &ctx->d_off, n));
-  HIPCHK(dalloc(bb, &ctx->d_init_tri, n));
-  HIPCHK(dalloc(bb, &ctx->d_del_step, n));
-  HIPCHK(dalloc(bb, &ctx->d_diam, n));
-  HIPCHK(dalloc(bb, &ctx->d_vol, n));
-  HIPCHK(dalloc(bb, &ctx->d_depth, dim == 3 ? n : 1));
-  HIPCHK(dalloc(bb, &ctx->d_vals, n + 1));
-  HIPCHK(dalloc(bb, &ctx->d_scan, n + 1));
-  HIPCHK(dalloc(bb, &ctx->d_bsum, nblk(n + 1) + 1));
-  HIPCHK(dalloc(bb, &ctx->d_total, 2));
-  HIPCHK(dalloc(bb, &ctx->d_perm, n));
-  HIPCHK(dalloc(bb, &ctx->d_skey, n));
-  {
-    double* tb;
-    HIPCHK(dalloc(bb, &tb, (size_t)n));
-    ctx->d_tmpbuf = tb;
-  }
   ctx->nscen = 1;
   ctx->scen_base_h = {0, n};
   ctx->d_scen = nullptr;
@@ -924,35 +951,14 @@ __global__ void k_scen_of_slot(const int* gpool, const int* base, int nscen, int
 static bool fast_params(const cyl_ctx* ctx, const cyl_params* p);
 static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
 
-extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const cyl_params* par,
-                                 const uint64_t* state, const uint64_t* inc, const int* nsteps) {
-  if (!ctx || nscen < 1 || !base || !par || !state || !inc || !nsteps) return CYL_E_ARG;
-  if (ctx->n == 0 || !ctx->has_mesh || !ctx->has_fields) return ctx->set_err(CYL_E_STATE, -1, "upload mesh, fields and the pool first");
-  if (base[0] != 0 || base[nscen] != ctx->n) return ctx->set_err(CYL_E_ARG, -1, "scenario ranges must cover the pool");
-  for (int s = 0; s < nscen; ++s) {
-    if (base[s + 1] < base[s]) return ctx->set_err(CYL_E_ARG, -1, "scenario ranges must be increasing");
-    if (nsteps[s] < 0) return ctx->set_err(CYL_E_ARG, -1, "negative scenario duration");
-  }
-  HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  ctx->nscen = nscen;
-  ctx->scen_base_h.assign(base, base + nscen + 1);
-  int e = ensure_scen_tables(ctx);
-  if (e) return e;
-  if (!ctx->d_scen) HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scen, ctx->n));
-  // normals buffer for the per-scenario chunk padding (see k_gen_normals)
-  HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)ctx->n + 32 * ((size_t)nscen + 2)));
-  ctx->xi_cap = (int64_t)ctx->n + 32 * ((int64_t)nscen + 2);
-  k_scen_of_slot<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_gpool, ctx->d_scen_base, nscen, ctx->n, ctx->d_scen);
-  ctx->rng_state.assign(state, state + nscen);
-  ctx->rng_inc.assign(inc, inc + nscen);
-  ctx->rng_dirty = true;
-  ctx->scen_par.assign(par, par + nscen);
-  ctx->scen_nsteps.assign(nsteps, nsteps + nscen);
-  ctx->batch_steps = 0;
-  // kernel classes: a fused kernel per model for the scenarios it covers that
-  // share the first one's Schmidt number (the packed nodal K), the generic
-  // kernel per model for the others
+// Kernel classes: a fused kernel per model for the scenarios it covers that
+// share the first one's Schmidt number (the packed nodal K), the generic
+// kernel per model for the others; each class gets the slot range its rows
+// occupy once sorted (class is the major sort key).
+static int set_classes(cyl_ctx* ctx) {
+  int nscen = ctx->nscen;
+  const cyl_params* par = ctx->scen_par.data();
+  const std::vector<int>& base = ctx->scen_base_h;
   ctx->classes.clear();
   ctx->class_of_scen_h.assign(nscen, 0);
   double sigc = 0.;
@@ -983,6 +989,37 @@ extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const
     ctx->classes[c].hi = lo;
   }
   HIPCHK(hipMemcpy(ctx->d_class_of_scen, ctx->class_of_scen_h.data(), nscen * sizeof(int), hipMemcpyHostToDevice));
+  return CYL_OK;
+}
+
+extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const cyl_params* par,
+                                 const uint64_t* state, const uint64_t* inc, const int* nsteps) {
+  if (!ctx || nscen < 1 || !base || !par || !state || !inc || !nsteps) return CYL_E_ARG;
+  if (ctx->n == 0 || !ctx->has_mesh || !ctx->has_fields) return ctx->set_err(CYL_E_STATE, -1, "upload mesh, fields and the pool first");
+  if (base[0] != 0 || base[nscen] != ctx->n) return ctx->set_err(CYL_E_ARG, -1, "scenario ranges must cover the pool");
+  for (int s = 0; s < nscen; ++s) {
+    if (base[s + 1] < base[s]) return ctx->set_err(CYL_E_ARG, -1, "scenario ranges must be increasing");
+    if (nsteps[s] < 0) return ctx->set_err(CYL_E_ARG, -1, "negative scenario duration");
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->nscen = nscen;
+  ctx->scen_base_h.assign(base, base + nscen + 1);
+  int e = ensure_scen_tables(ctx);
+  if (e) return e;
+  if (!ctx->d_scen) HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scen, ctx->n));
+  // normals buffer for the per-scenario chunk padding (see k_gen_normals)
+  HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)ctx->n + 32 * ((size_t)nscen + 2)));
+  ctx->xi_cap = (int64_t)ctx->n + 32 * ((int64_t)nscen + 2);
+  k_scen_of_slot<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_gpool, ctx->d_scen_base, nscen, ctx->n, ctx->d_scen);
+  ctx->rng_state.assign(state, state + nscen);
+  ctx->rng_inc.assign(inc, inc + nscen);
+  ctx->rng_dirty = true;
+  ctx->scen_par.assign(par, par + nscen);
+  ctx->scen_nsteps.assign(nsteps, nsteps + nscen);
+  ctx->batch_steps = 0;
+  e = set_classes(ctx);
+  if (e) return e;
   ctx->batched = true;
   ctx->pregen_h.clear();
   ctx->ev_bufs.free_all();
@@ -1007,6 +1044,346 @@ extern "C" int cyl_scenario_rng_get(cyl_ctx* ctx, int s, uint64_t* state, uint64
   }
   *state = ctx->rng_state[s];
   *inc = ctx->rng_inc[s];
+  return CYL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// continuous batching: scenarios leave and join a running batch on the device
+// ---------------------------------------------------------------------------
+template <class T>
+static void enqueue_scan(cyl_ctx* ctx, const T* in, int64_t* out, int64_t n, int64_t* bsum, int64_t* total,
+                         const int64_t* ndev, const int* cond, int& launches);
+static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, double dt, int nsteps, int* bnd_out,
+                     bool init_only);
+
+struct ReshapeArgs {
+  PoolDev A;                  // current state of the old pool
+  PoolDev N0, N1;             // both state copies of the new pool
+  const int *gpool, *scen, *init_tri, *del_step;
+  const double *diam, *vol, *depth;
+  int *ngpool, *nscen_of, *ninit_tri, *ndel_step;
+  double *ndiam, *nvol, *ndepth;
+  const int* map;             // old scenario -> new scenario, or -1 (leaves)
+  const int* obase;           // old scenario -> first pool index
+  const int* nbase;           // new scenario -> first pool index
+  const int64_t* pos;         // old slot -> new slot (exclusive scan of the kept flags)
+  int n, dim;
+};
+
+__global__ void k_reshape_flags(const int* scen, const int* map, int n, int* keep) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j < n) keep[j] = map[scen[j]] >= 0 ? 1 : 0;
+}
+
+// kept rows move, in their physical order, to the front of the new pool
+__global__ void k_reshape_copy(ReshapeArgs r) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= r.n) return;
+  int s = r.scen[j], s2 = r.map[s];
+  if (s2 < 0) return;
+  int64_t q = r.pos[j];
+  double v;
+  v = r.A.x[j]; r.N0.x[q] = v; r.N1.x[q] = v;
+  v = r.A.y[j]; r.N0.y[q] = v; r.N1.y[q] = v;
+  if (r.dim == 3) { v = r.A.z[j]; r.N0.z[q] = v; r.N1.z[q] = v; }
+  for (int c = 0; c < r.dim; ++c) {
+    v = r.A.up[c][j]; r.N0.up[c][q] = v; r.N1.up[c][q] = v;
+    v = r.A.us[c][j]; r.N0.us[c][q] = v; r.N1.us[c][q] = v;
+  }
+  int t = r.A.tri[j];
+  r.N0.tri[q] = t; r.N1.tri[q] = t;
+  t = r.A.ll[j];
+  r.N0.ll[q] = t; r.N1.ll[q] = t;
+  r.N0.inactive[q] = r.A.inactive[j];
+  r.ngpool[q] = r.nbase[s2] + (r.gpool[j] - r.obase[s]);
+  r.nscen_of[q] = s2;
+  r.ninit_tri[q] = r.init_tri[j];
+  r.ndel_step[q] = r.del_step[j];
+  r.ndiam[q] = r.diam[j];
+  r.nvol[q] = r.vol[j];
+  if (r.dim == 3) r.ndepth[q] = r.depth[j];
+}
+
+struct CollectArgs {
+  PoolDev P;
+  const int *gpool, *scen, *del_step;
+  int s, base, n, dim;
+  double* out_d;   // [2 * dim + dim][npart]: position, velocity, fluid velocity seen
+  int* out_i;      // [4][npart]: tri, lowerlayer, inactive, deletion step
+  int npart;
+};
+
+// rows of scenario s, by their index in the scenario's own pool; reported
+// deletions become plain inactive rows
+__global__ void k_collect(CollectArgs a) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= a.n || a.scen[j] != a.s) return;
+  int i = a.gpool[j] - a.base;
+  if (i < 0 || i >= a.npart) return;
+  const double* px[3] = {a.P.x, a.P.y, a.P.z};
+  int np = a.npart;
+  for (int c = 0; c < a.dim; ++c) {
+    a.out_d[(size_t)c * np + i] = px[c][j];
+    a.out_d[(size_t)(a.dim + c) * np + i] = a.P.up[c][j];
+    a.out_d[(size_t)(2 * a.dim + c) * np + i] = a.P.us[c][j];
+  }
+  int ina = a.P.inactive[j];
+  a.out_i[i] = a.P.tri[j];
+  a.out_i[np + i] = a.P.ll[j];
+  a.out_i[2 * np + i] = ina;
+  a.out_i[3 * np + i] = a.del_step[j];
+  if (ina == 2) a.P.inactive[j] = 1;
+}
+
+extern "C" int cyl_batch_collect(cyl_ctx* ctx, int s, int npart, double* pos, double* vel, double* us, int* inactive,
+                                 int* tri, int* ll, int* deleted, int cap, int* ndeleted) {
+  if (!ctx || !ctx->batched || s < 0 || s >= ctx->nscen || !ndeleted) return CYL_E_ARG;
+  int base = ctx->scen_base_h[s], np = ctx->scen_base_h[s + 1] - base, dim = ctx->dim;
+  if (npart != np) return ctx->set_err(CYL_E_ARG, -1, "npart differs from the scenario's pool size");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *ndeleted = 0;
+  if (np == 0) return CYL_OK;
+  Bufs tmp;
+  CollectArgs a{};
+  HIPCHK(dalloc(tmp, &a.out_d, (size_t)3 * dim * np));
+  HIPCHK(dalloc(tmp, &a.out_i, (size_t)4 * np));
+  a.P = ctx->pool[ctx->cur];
+  a.gpool = ctx->d_gpool;
+  a.scen = ctx->d_scen;
+  a.del_step = ctx->d_del_step;
+  a.s = s;
+  a.base = base;
+  a.n = ctx->n;
+  a.dim = dim;
+  a.npart = np;
+  k_collect<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(a);
+  HIPCHK(hipGetLastError());
+  std::vector<double> d((size_t)3 * dim * np);
+  std::vector<int> iv((size_t)4 * np);
+  HIPCHK(hipMemcpyAsync(d.data(), a.out_d, d.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(iv.data(), a.out_i, iv.size() * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  tmp.free_all();
+  const int* ina = iv.data() + 2 * (size_t)np;
+  for (int c = 0; c < dim; ++c)
+    for (int i = 0; i < np; ++i) {
+      bool z = ina[i] != 0;
+      if (pos) pos[(size_t)i * dim + c] = z ? 0. : d[(size_t)c * np + i];
+      if (vel) vel[(size_t)i * dim + c] = z ? 0. : d[(size_t)(dim + c) * np + i];
+      if (us) us[(size_t)i * dim + c] = z ? 0. : d[(size_t)(2 * dim + c) * np + i];
+    }
+  std::vector<std::pair<int, int>> del;
+  for (int i = 0; i < np; ++i) {
+    bool z = ina[i] != 0;
+    if (tri) tri[i] = z ? -1 : iv[i];
+    if (ll) ll[i] = z ? 0 : iv[(size_t)np + i];
+    if (inactive) inactive[i] = z ? 1 : 0;
+    if (ina[i] == 2) del.push_back({iv[(size_t)3 * np + i], i});
+  }
+  std::sort(del.begin(), del.end());
+  *ndeleted = (int)del.size();
+  for (int q = 0; q < (int)del.size() && q < cap; ++q)
+    if (deleted) deleted[q] = del[q].second;
+  return CYL_OK;
+}
+
+extern "C" int cyl_batch_reshape(cyl_ctx* ctx, int nkeep, const int* keep, int nnew, const int* new_npart,
+                                 const cyl_params* new_par, const uint64_t* new_state, const uint64_t* new_inc,
+                                 const int* new_nsteps, const double* pos, const double* vel, const double* us,
+                                 const int* inactive, const int* tri, const int* ll, const double* diameter,
+                                 const double* volume, const int* init_tri, double time, double dt) {
+  if (!ctx || !ctx->batched || nkeep < 0 || nnew < 0 || (nkeep > 0 && !keep)) return CYL_E_ARG;
+  if (nnew > 0 && (!new_npart || !new_par || !new_state || !new_inc || !new_nsteps)) return CYL_E_ARG;
+  if (nkeep + nnew < 1) return ctx->set_err(CYL_E_ARG, -1, "a batch keeps at least one scenario");
+  const int ons = ctx->nscen, on = ctx->n, dim = ctx->dim;
+  for (int q = 0; q < nkeep; ++q)
+    if (keep[q] < 0 || keep[q] >= ons || (q > 0 && keep[q] <= keep[q - 1]))
+      return ctx->set_err(CYL_E_ARG, -1, "kept scenarios must be increasing scenario ids");
+  int64_t nadd = 0;
+  for (int q = 0; q < nnew; ++q) {
+    if (new_npart[q] < 0 || new_nsteps[q] < 0) return ctx->set_err(CYL_E_ARG, -1, "negative size or duration");
+    if (new_par[q].rng_method != ctx->scen_par[0].rng_method)
+      return ctx->set_err(CYL_E_ARG, -1, "the scenarios of a batch share one rng_method");
+    nadd += new_npart[q];
+  }
+  if (nadd > 0 && (!pos || !inactive || !tri)) return CYL_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // the kept scenarios' random streams as they stand now
+  if (!ctx->rng_dirty) {
+    std::vector<StreamDev> st(ons);
+    HIPCHK(hipMemcpy(st.data(), ctx->d_stream, ons * sizeof(StreamDev), hipMemcpyDeviceToHost));
+    for (int s = 0; s < ons; ++s) ctx->rng_state[s] = st[s].base;
+  }
+  // new scenario table: kept scenarios in order, then the new ones
+  std::vector<int> map(ons, -1), nbase(1, 0);
+  std::vector<cyl_params> par;
+  std::vector<int> nsteps;
+  std::vector<uint64_t> rs, ri;
+  for (int q = 0; q < nkeep; ++q) {
+    int s = keep[q];
+    map[s] = q;
+    nbase.push_back(nbase.back() + ctx->scen_base_h[s + 1] - ctx->scen_base_h[s]);
+    par.push_back(ctx->scen_par[s]);
+    nsteps.push_back(std::max(0, ctx->scen_nsteps[s] - ctx->batch_steps));
+    rs.push_back(ctx->rng_state[s]);
+    ri.push_back(ctx->rng_inc[s]);
+  }
+  const int nkept_rows = nbase.back();
+  for (int q = 0; q < nnew; ++q) {
+    nbase.push_back(nbase.back() + new_npart[q]);
+    par.push_back(new_par[q]);
+    nsteps.push_back(new_nsteps[q]);
+    rs.push_back(new_state[q]);
+    ri.push_back(new_inc[q]);
+  }
+  const int nn = nbase.back(), nns = nkeep + nnew;
+  if (nn < 1) return ctx->set_err(CYL_E_ARG, -1, "the new batch has no rows");
+  // compaction of the kept rows (old buffers stay alive until the copy is done)
+  Bufs oldb, tmp;
+  oldb.ptrs.swap(ctx->part_bufs.ptrs);
+  PoolDev A = ctx->pool[ctx->cur];
+  const int *o_gpool = ctx->d_gpool, *o_scen = ctx->d_scen, *o_init = ctx->d_init_tri, *o_del = ctx->d_del_step;
+  const double *o_diam = ctx->d_diam, *o_vol = ctx->d_vol, *o_depth = ctx->d_depth;
+  int *d_map, *d_obase, *d_nbase, *d_keepf;
+  int64_t *d_pos, *d_bs, *d_tot;
+  HIPCHK(dalloc(tmp, &d_map, ons));
+  HIPCHK(dalloc(tmp, &d_obase, ons + 1));
+  HIPCHK(dalloc(tmp, &d_nbase, nns + 1));
+  HIPCHK(dalloc(tmp, &d_keepf, on));
+  HIPCHK(dalloc(tmp, &d_pos, on + 1));
+  HIPCHK(dalloc(tmp, &d_bs, nblk(on + 1, SCAN_TILE) + 1));
+  HIPCHK(dalloc(tmp, &d_tot, 2));
+  HIPCHK(hipMemcpy(d_map, map.data(), ons * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_obase, ctx->scen_base_h.data(), (ons + 1) * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_nbase, nbase.data(), (nns + 1) * sizeof(int), hipMemcpyHostToDevice));
+  int launches = 0;
+  k_reshape_flags<<<nblk(on), BLK, 0, ctx->stream>>>(o_scen, d_map, on, d_keepf);
+  enqueue_scan(ctx, (const int*)d_keepf, d_pos, (int64_t)on, d_bs, d_tot, (const int64_t*)nullptr, (const int*)nullptr,
+               launches);
+  int e = alloc_slot_bufs(ctx, nn, nns + 2);
+  if (e) return e;
+  HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scen, nn));
+  ReshapeArgs r{};
+  r.A = A;
+  r.N0 = ctx->pool[0];
+  r.N1 = ctx->pool[1];
+  r.gpool = o_gpool; r.scen = o_scen; r.init_tri = o_init; r.del_step = o_del;
+  r.diam = o_diam; r.vol = o_vol; r.depth = o_depth;
+  r.ngpool = ctx->d_gpool; r.nscen_of = ctx->d_scen; r.ninit_tri = ctx->d_init_tri; r.ndel_step = ctx->d_del_step;
+  r.ndiam = ctx->d_diam; r.nvol = ctx->d_vol; r.ndepth = ctx->d_depth;
+  r.map = d_map; r.obase = d_obase; r.nbase = d_nbase; r.pos = d_pos;
+  r.n = on;
+  r.dim = dim;
+  k_reshape_copy<<<nblk(on), BLK, 0, ctx->stream>>>(r);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  oldb.free_all();
+  tmp.free_all();
+  // the new scenarios' rows, in their pool order, after the kept ones
+  bool pend = false;
+  if (nadd > 0) {
+    size_t m = (size_t)nadd, o = (size_t)nkept_rows;
+    std::vector<double> col(m);
+    std::vector<int> icol(m);
+    auto putd = [&](double* d0, double* d1, const double* aos, int c, int stride) -> hipError_t {
+      for (size_t i = 0; i < m; ++i) col[i] = aos ? aos[i * stride + c] : 0.;
+      hipError_t er = hipMemcpy(d0 + o, col.data(), m * sizeof(double), hipMemcpyHostToDevice);
+      if (er == hipSuccess && d1) er = hipMemcpy(d1 + o, col.data(), m * sizeof(double), hipMemcpyHostToDevice);
+      return er;
+    };
+    auto puti = [&](int* d0, int* d1) -> hipError_t {
+      hipError_t er = hipMemcpy(d0 + o, icol.data(), m * sizeof(int), hipMemcpyHostToDevice);
+      if (er == hipSuccess && d1) er = hipMemcpy(d1 + o, icol.data(), m * sizeof(int), hipMemcpyHostToDevice);
+      return er;
+    };
+    PoolDev &P0 = ctx->pool[0], &P1 = ctx->pool[1];
+    HIPCHK(putd(P0.x, P1.x, pos, 0, dim));
+    HIPCHK(putd(P0.y, P1.y, pos, 1, dim));
+    if (dim == 3) HIPCHK(putd(P0.z, P1.z, pos, 2, dim));
+    for (int c = 0; c < dim; ++c) {
+      HIPCHK(putd(P0.up[c], P1.up[c], vel, c, dim));
+      HIPCHK(putd(P0.us[c], P1.us[c], us, c, dim));
+    }
+    HIPCHK(putd(ctx->d_diam, nullptr, diameter, 0, 1));
+    HIPCHK(putd(ctx->d_vol, nullptr, volume, 0, 1));
+    if (dim == 3) HIPCHK(putd(ctx->d_depth, nullptr, nullptr, 0, 1));
+    for (size_t i = 0; i < m; ++i) {
+      icol[i] = tri[i];
+      if (!inactive[i] && tri[i] == -2) pend = true;
+    }
+    HIPCHK(puti(P0.tri, P1.tri));
+    for (size_t i = 0; i < m; ++i) icol[i] = ll ? ll[i] : 0;
+    HIPCHK(puti(P0.ll, P1.ll));
+    for (size_t i = 0; i < m; ++i) icol[i] = inactive[i] ? 1 : 0;
+    HIPCHK(puti(P0.inactive, nullptr));
+    for (size_t i = 0; i < m; ++i) icol[i] = init_tri ? init_tri[i] : -3;
+    HIPCHK(puti(ctx->d_init_tri, nullptr));
+    for (size_t i = 0; i < m; ++i) icol[i] = -1;
+    HIPCHK(puti(ctx->d_del_step, nullptr));
+    size_t i = 0;
+    for (int q = 0; q < nnew; ++q)
+      for (int k = 0; k < new_npart[q]; ++k, ++i) icol[i] = nbase[nkeep + q] + k;
+    HIPCHK(puti(ctx->d_gpool, nullptr));
+    i = 0;
+    for (int q = 0; q < nnew; ++q)
+      for (int k = 0; k < new_npart[q]; ++k, ++i) icol[i] = nkeep + q;
+    HIPCHK(puti(ctx->d_scen, nullptr));
+  }
+  // scenario tables
+  ctx->nscen = nns;
+  ctx->scen_base_h = nbase;
+  e = ensure_scen_tables(ctx);
+  if (e) return e;
+  ctx->rng_state = rs;
+  ctx->rng_inc = ri;
+  ctx->rng_dirty = true;
+  ctx->scen_par = par;
+  ctx->scen_nsteps = nsteps;
+  ctx->batch_steps = 0;
+  ctx->cur = 0;
+  ctx->have_last = false;
+  ctx->pending_init = ctx->pending_init || pend;
+  e = set_classes(ctx);
+  if (e) return e;
+  ctx->pregen_h.clear();
+  ctx->ev_bufs.free_all();
+  ctx->ev_units_cap = 0;
+  HIPCHK(hipMemsetAsync(ctx->d_n_deleted, 0, sizeof(int), ctx->stream));
+  // ranks restart at every scenario; slots are grouped by kernel class again
+  k_set_flag<<<1, 1, 0, ctx->stream>>>(ctx->d_flag, 1, 1);
+  e = enqueue_sort(ctx, true, launches);
+  if (e) return e;
+  ctx->steps_since_sort = 0;
+  if (pend) {
+    // initialize_particle_state of the new rows with the fields at `time`, as a
+    // Solver built at that time does; the interpolation state of the running
+    // batch (the last two steps' fields) is then rebuilt
+    ctx->fields.frozen = ctx->frozen;
+    int nh = ctx->interp_count;
+    double h0 = ctx->interp_hist[0], h1 = ctx->interp_hist[1];
+    if (!ctx->frozen) {
+      ctx->record = 0;
+      e = enqueue_time_interp(ctx, time, launches);
+      if (e) return e;
+    }
+    double t = time;
+    e = run_steps(ctx, &ctx->scen_par[0], &t, dt, 1, nullptr, true);
+    if (e) return e;
+    ctx->have_last = false;
+    if (!ctx->frozen && nh >= 1) {
+      ctx->record = 0;
+      if (nh >= 2) {
+        e = enqueue_time_interp(ctx, h0, launches);
+        if (e) return e;
+      }
+      ctx->record = 0;
+      e = enqueue_time_interp(ctx, h1, launches);
+      if (e) return e;
+    }
+  }
+  HIPCHK(hipStreamSynchronize(ctx->stream));
   return CYL_OK;
 }
 

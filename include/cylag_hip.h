@@ -137,6 +137,26 @@ int cyl_scenario_rng_get(cyl_ctx *ctx, int s, uint64_t *state, uint64_t *inc);
 /* nsteps steps of every scenario that has steps left (finished scenarios keep
  * their rows).  bnd_stats, if not NULL, receives nsteps*nscen*nopen counts. */
 int cyl_batch_step(cyl_ctx *ctx, double *time_inout, double dt, int nsteps, int *bnd_stats);
+/* Continuous batching (no counterpart in the reference; it serves callers that
+ * keep many releases resident on one GPU and let them join and leave).
+ * cyl_batch_collect: the rows of scenario s in the order of its own pool, as
+ * cyl_particles_download gives them, and its deleted rows in deletion order
+ * (lagrangian_particle_set.pyx:786-788; reported rows become plain inactive
+ * rows).  Arrays may be NULL. */
+int cyl_batch_collect(cyl_ctx *ctx, int s, int npart, double *pos, double *vel, double *us, int *inactive,
+                      int *tri, int *lowerlayer, int *deleted, int cap, int *ndeleted);
+/* A new batch on the same device state: the nkeep scenarios keep[] (increasing
+ * ids) keep their rows, random streams and remaining steps and become
+ * scenarios 0..nkeep-1; nnew scenarios follow, with their rows given as in
+ * cyl_particles_upload (concatenated; rows with tri = -2 are initialized at
+ * `time`, as Solver's constructor does, solver.pyx:193-197), parameters,
+ * streams and step counts.  The other scenarios leave the device.  The new
+ * scenarios' particle constants are the context's. */
+int cyl_batch_reshape(cyl_ctx *ctx, int nkeep, const int *keep, int nnew, const int *new_npart,
+                      const cyl_params *new_par, const uint64_t *new_state, const uint64_t *new_inc,
+                      const int *new_nsteps, const double *pos, const double *vel, const double *us,
+                      const int *inactive, const int *tri, const int *lowerlayer, const double *diameter,
+                      const double *volume, const int *init_tri, double time, double dt);
 
 /* Tuning: element re-sort interval in steps (0 = never) and block size. */
 int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int block_size);

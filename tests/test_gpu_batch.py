@@ -99,7 +99,7 @@ def test_batch_deletions_tags_and_statistics():
     b = cy.BatchSolver(fs, psets, params, st, nsteps=[k for _, _, k in mix], sort_interval=5)
     b.run()
     out = b.results()
-    stats = np.concatenate(b.bnd_stats)
+    stats = b.bnd_stats
     for s, (p, n, k) in enumerate(mix):
         pset = make_pset(case, n, 200 + s)
         fs1 = cy.EulerianFieldSet(mesh(), trifinder=case.trifinder(), **case.fields)
@@ -112,6 +112,49 @@ def test_batch_deletions_tags_and_statistics():
                      tag=ps.tag.copy(), velocity=ps.velocity.copy(), rng=cy.random_utils.get_state())
         assert alone["inactive"].sum() > 0
         compare(out[s], alone, b.rng_state(s))
-        np.testing.assert_array_equal(stats[:k, s, :], solo.bnd_stats[1][:k])
+        np.testing.assert_array_equal(stats[s], solo.bnd_stats[1][:k])
         solo.ctx.close()
     b.ctx.close()
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+def test_continuous_batching_equals_alone(rng_method):
+    """Scenarios leave a running batch and others join it (two frames of
+    time-dependent fields): each one's result equals a plain Solver run of that
+    scenario alone, built at the batch time at which it joined."""
+    case = vortex_case(npart=4000)
+    dt = 5.
+    base = dict(final_time=10000., time_step=dt, rng_method=rng_method)
+    spec = [  # (parameters, particles, steps, joins after batch step)
+        (dict(model=1, diffusion_model=2), 1500, 12, 0),
+        (dict(model=2, time_scheme=5, diffusion_model=2, diffusion_lsm2_option=1, particle_velocity_init=1), 700, 30, 0),
+        (dict(model=1, time_scheme=4, diffusion_model=0), 500, 30, 0),
+        (dict(model=1, diffusion_model=1, horizontal_diffusivity=0.3), 900, 15, 12),
+        (dict(model=2, time_scheme=5, diffusion_model=2, diffusion_lsm2_option=2, particle_velocity_init=1), 600, 10, 12),
+    ]
+    st = streams(len(spec))
+    params = [dict(base, **p) for p, _, _, _ in spec]
+    psets = [make_pset(case, n, 300 + i) for i, (_, n, _, _) in enumerate(spec)]
+    mesh = cy.TriangularMesh(case.x, case.y, case.tri)
+    fs = cy.EulerianFieldSet(mesh, trifinder=case.trifinder(), **case.fields)
+    b = cy.BatchSolver(fs, psets[:3], params[:3], st[:3], nsteps=[k for _, _, k, _ in spec[:3]], sort_interval=4)
+    b.run(12)
+    done = {}
+    s0 = b.index(0)
+    done[0] = (b.collect(s0), b.rng_state(s0))
+    b.retire([s0])
+    ids = b.admit(psets[3:], params[3:], st[3:], nsteps=[k for _, _, k, _ in spec[3:]])
+    assert ids == [3, 4]
+    b.run(10)
+    s4 = b.index(4)
+    assert b.steps_left(s4) == 0
+    done[4] = (b.collect(s4), b.rng_state(s4))
+    b.retire([s4])
+    b.run()
+    for sid in (1, 2, 3):
+        s = b.index(sid)
+        done[sid] = (b.collect(s), b.rng_state(s))
+    b.close()
+    for sid, (p, n, k, join) in enumerate(spec):
+        alone = run_alone(case, make_pset(case, n, 300 + sid), dict(params[sid], initial_time=join * dt), st[sid], k)
+        compare(done[sid][0], alone, done[sid][1])
