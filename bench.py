@@ -91,6 +91,38 @@ def c2_workload(npts_x=2000, npts_y=500, npart=10_000_000, seed=7):
     return x, y, tri, fields, pos, params
 
 
+def c3_workload(npts_x=1000, npts_y=250, nlayers=10, seed=7):
+    """The 3D sigma-layer channel: the 20 x 5 km domain with 250k jittered
+    points (~500k triangles), 10 uniform sigma planes between the bottom -h
+    (h = 2-20 m, smooth) and the surface 0; log-law u(z) along x, v = w = 0;
+    k and eps giving a parabolic K(z) = kappa u* z' (1 - z'/h); frozen (one frame)."""
+    from tests.cases import jittered_delaunay
+    lx, ly = 20000., 5000.
+    x, y, tri, mpl = jittered_delaunay(npts_x, npts_y, lx, ly, seed=seed)
+    n = x.size
+    h = 2. + 18. * (0.5 + 0.5 * np.sin(2 * np.pi * x / 7000. + 1.) * np.cos(2 * np.pi * y / 3000.))
+    kappa, ustar, z0, cmu = 0.41, 0.05, 1e-3, 0.09
+    sig = np.linspace(0., 1., nlayers)
+    zp = np.concatenate([np.maximum(s_ * h, z0 * 10.) for s_ in sig])   # height above the bottom
+    hh = np.tile(h, nlayers)
+    z = np.concatenate([-h + s_ * h for s_ in sig])
+    u = (ustar / kappa) * np.log(zp / z0)
+    frac = np.maximum(1. - zp / hh, 0.05)
+    tke = ustar ** 2 / np.sqrt(cmu) * frac
+    eps = ustar ** 3 / (kappa * zp) * frac
+    fields = dict(dim=3, nlayers=nlayers, times=np.zeros(1), elevation=z[None], velocity_x=u[None],
+                  velocity_y=np.zeros((1, n * nlayers)), velocity_z=np.zeros((1, n * nlayers)),
+                  opt_fields=np.stack([tke[None], eps[None]]),
+                  opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
+    return x, y, tri, h, fields
+
+
+def settling_diameters(rng, n, rho_p=2650.):
+    """Stokes diameters of settling velocities w_s ~ LN(median 1 mm/s, sigma 0.5)."""
+    ws = np.exp(np.log(1e-3) + 0.5 * rng.standard_normal(n))
+    return np.sqrt(18. * 1e-6 * ws / ((rho_p / 1000. - 1.) * 9.81))
+
+
 def bytes_per_particle_step(nt, nv, npart):
     """Algorithmic bytes per particle-step: the evolving state read and written
     once (x, y f64 + element id i32 = 20 B each way), plus the mesh and node

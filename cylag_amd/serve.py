@@ -1,0 +1,181 @@
+"""Serving: a request loop over one device-resident BatchSolver.
+
+Requests (a particle set, its parameters, its random stream and its number of
+steps) arrive over time.  The loop admits waiting requests while the device
+has room for their rows (admission by free HBM), merges them into the running
+batch, advances the batch until the next request finishes (or a tick of
+steps), writes finished requests' rows back into their particle sets and drops
+them from the device.  Rows never leave the device while a request runs.
+
+Each request's result is the result of a plain Solver run of that request
+alone with the same stream and steps (with frozen fields, at any admission
+time; with time-dependent fields, for a Solver built at the batch time at
+which it was admitted).
+"""
+import time
+from collections import deque
+
+import numpy as np
+
+from .batch import BatchSolver
+
+# device bytes one particle row holds (both state copies, slot tables, scans,
+# sort spares, normals and event scratch), with margin
+ROW_BYTES = {2: 480, 3: 640}
+
+
+class Request:
+    def __init__(self, particle_set, parameters, rng_state, nsteps, arrival=0.):
+        self.particle_set = particle_set
+        self.parameters = parameters
+        self.rng_state = rng_state
+        self.nsteps = int(nsteps)
+        self.arrival = float(arrival)     # seconds after the loop starts
+        self.id = None                    # batch scenario id once admitted
+        self.admitted = None              # seconds after the loop starts
+        self.finished = None
+        self.final_rng_state = None
+
+    @property
+    def npart(self):
+        return self.particle_set.npart
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+class ServeLoop:
+    """Continuous batching over a field set (one context on one GPU).
+
+    capacity: rows the device may hold (default: hbm_fraction of the free HBM
+    after the mesh and fields are resident, at ROW_BYTES per row; a reshape
+    holds the old and the new pool at once, hence the default fraction).
+    tick_steps: most steps between two admission points."""
+
+    def __init__(self, field_set, device=0, capacity=None, hbm_fraction=0.4, tick_steps=50, sort_interval=8,
+                 clock=time.perf_counter, log=None, batch_factory=BatchSolver):
+        self.field_set = field_set
+        self.batch_factory = batch_factory
+        self.device = device
+        self.capacity = capacity
+        self.hbm_fraction = hbm_fraction
+        self.tick_steps = int(tick_steps)
+        self.sort_interval = sort_interval
+        self.clock = clock
+        self.log = log
+        self.batch = None
+        self.done = []
+        self.steps = 0
+        self.particle_steps = 0
+        self.max_running = 0
+
+    # -- device side --------------------------------------------------------------
+    def _resident(self):
+        return 0 if self.batch is None else self.batch.npart
+
+    def _room(self, dim):
+        if self.capacity is None:
+            free = self.batch.ctx.device_info()[1] if self.batch is not None else None
+            if free is None:
+                return None  # decided once the first batch exists
+            self.capacity = int(self.hbm_fraction * (free + self._resident() * ROW_BYTES[dim]) / ROW_BYTES[dim])
+        return self.capacity - self._resident()
+
+    def _admit(self, reqs, now):
+        psets = [r.particle_set for r in reqs]
+        params = [r.parameters for r in reqs]
+        streams = [r.rng_state for r in reqs]
+        nsteps = [r.nsteps for r in reqs]
+        if self.batch is None:
+            self.batch = self.batch_factory(self.field_set, psets, params, streams, nsteps=nsteps, device=self.device,
+                                            sort_interval=self.sort_interval)
+            ids = self.batch.ids
+        else:
+            finished = [s for s in range(self.batch.nscen) if self.batch.steps_left(s) == 0]
+            keep = [s for s in range(self.batch.nscen) if s not in set(finished)]
+            ids = self.batch.reshape(keep, psets, params, streams, nsteps)
+        for r, sid in zip(reqs, ids):
+            r.id = sid
+            r.admitted = now
+
+    def _finish(self, running, now):
+        """Collect the requests whose steps are done; drop them from the device
+        unless nothing else would remain there."""
+        b = self.batch
+        fin = [s for s in range(b.nscen) if b.steps_left(s) == 0 and b.ids[s] in running]
+        for s in fin:
+            r = running.pop(b.ids[s])
+            b.collect(s)
+            r.final_rng_state = b.rng_state(s)
+            r.finished = now
+            self.done.append(r)
+        gone = [s for s in range(b.nscen) if b.steps_left(s) == 0]
+        if gone and len(gone) < b.nscen:
+            b.retire(gone)
+        return fin
+
+    # -- loop -----------------------------------------------------------------------
+    def serve(self, requests, horizon=None):
+        """Run the loop until every request is done, or `horizon` seconds after
+        the start (requests not done then are left unfinished)."""
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting = deque()
+        running = {}
+        t0 = self.clock()
+        while True:
+            now = self.clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            if self.batch is not None:
+                self._finish(running, now)
+            # admission: first come, first served, while the device has room
+            adm = []
+            dim = waiting[0].particle_set.dim if waiting else 2
+            room = self._room(dim)
+            while waiting and (room is None or waiting[0].npart <= room - sum(r.npart for r in adm)):
+                adm.append(waiting.popleft())
+                if room is None:
+                    break  # the first batch fixes the capacity
+            if waiting and not adm and not running:
+                adm.append(waiting.popleft())  # larger than the device's room: it runs alone
+            if adm:
+                self._admit(adm, now)
+                for r in adm:
+                    running[r.id] = r
+                self.max_running = max(self.max_running, len(running))
+                if self.log:
+                    self.log("serve: t={:.2f} s admitted {} ({} rows resident, {} running, {} waiting)".format(
+                        now, len(adm), self._resident(), len(running), len(waiting)))
+                if room is None:
+                    continue  # the device's room is known now: admit the others before stepping
+            if horizon is not None and now >= horizon:
+                break
+            if running:
+                b = self.batch
+                left = [b.steps_left(s) for s in range(b.nscen) if b.ids[s] in running]
+                k = min(self.tick_steps, min(left))
+                active = sum(b.particle_sets[s].npart for s in range(b.nscen) if b.ids[s] in running)
+                b.run(k)
+                self.steps += k
+                self.particle_steps += k * active
+            elif pending:
+                time.sleep(max(0., min(pending[0].arrival - (self.clock() - t0), 0.01)))
+            elif not waiting:
+                break
+        self.elapsed = self.clock() - t0
+        self.unfinished = list(running.values()) + list(waiting) + list(pending)
+        return self.done
+
+    def close(self):
+        if self.batch is not None:
+            self.batch.close()
+            self.batch = None
+
+
+def latency_stats(reqs):
+    lat = np.array([r.latency for r in reqs if r.latency is not None])
+    if not lat.size:
+        return dict(completed=0, p50_s=None, p99_s=None)
+    return dict(completed=int(lat.size), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                mean_s=float(lat.mean()), max_s=float(lat.max()))
