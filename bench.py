@@ -203,8 +203,9 @@ def point_release(cy, xy, n):
 
 def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250):
     """The whole batched job: every scenario runs all its steps (or the first
-    max_steps of them); the clock brackets BatchSolver.run calls (inputs
-    resident on the device)."""
+    max_steps of them); every 250 steps the finished scenarios' rows come back
+    to the host and the scenarios leave the device.  The clock brackets the
+    stepping, collection and retirement (inputs resident on the device)."""
     npart, nsteps, model, xy = c4_scenarios()
     fs = cy.EulerianFieldSet(mesh, **fields)
     streams = scenario_streams(cy, len(npart))
@@ -216,12 +217,23 @@ def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250):
         len(npart), int(npart.sum()), int(nsteps.min()), int(nsteps.max()), time.time() - t0))
     total = int(nsteps.max()) if max_steps is None else min(int(max_steps), int(nsteps.max()))
     t0 = time.perf_counter()
-    done = 0
+    done, ncollected = 0, 0
     while done < total:
         k = min(chunk, total - done)
         b.run(k)
         done += k
-        log("batched: {} / {} steps, {:.1f} s".format(done, total, time.perf_counter() - t0))
+        # finished scenarios: rows back to the host, then off the device
+        fin = [s for s in range(b.nscen) if b.steps_left(s) == 0]
+        if fin and done < total:
+            for s in fin:
+                b.collect(s)
+            ncollected += len(fin)
+            if len(fin) < b.nscen:
+                b.retire(fin)
+        log("batched: {} / {} steps, {} scenarios on the device, {:.1f} s".format(
+            done, total, b.nscen, time.perf_counter() - t0))
+    for s in range(b.nscen):
+        b.collect(s)
     wall = time.perf_counter() - t0
     k_ms, tot_ms, nlaunch = b.ctx.last_timing()
     work = float(np.sum(npart * np.minimum(nsteps, total)))
@@ -284,6 +296,93 @@ def plain_sequential_leg(cy, mesh, fields, ksteps=40, warm=3, nsteps_cap=None):
     return dict(value=work / t_job, job_s_estimated=t_job, steps_timed_per_scenario=ksteps)
 
 
+def c5_requests(cy, x, y, h, window, lam=20., seed=13, dt=10.):
+    """Poisson arrivals (lam per second) over `window` seconds; each request a
+    point release of N ~ logU(1e3, 2e6) particles at a random interior point
+    and depth, simulated for U[1, 48] h at dt = 10 s, LSM1 or LSM2 (3D,
+    k-epsilon K(z)); settling diameters per particle; request i draws from
+    the default stream advanced by i << 48."""
+    from functools import partial
+    from scipy.spatial import cKDTree
+    lx, ly = 20000., 5000.
+    rng = np.random.default_rng(seed)
+    arr, t = [], 0.
+    while True:
+        t += rng.exponential(1. / lam)
+        if t > window:
+            break
+        arr.append(t)
+    m = len(arr)
+    npart = np.exp(rng.uniform(np.log(1e3), np.log(2e6), m)).astype(np.int64)
+    nsteps = np.round(rng.uniform(1., 48., m) * 3600. / dt).astype(np.int64)
+    model = rng.integers(1, 3, m)
+    px, py = rng.uniform(0.05 * lx, 0.95 * lx, m), rng.uniform(0.05 * ly, 0.95 * ly, m)
+    hn = h[cKDTree(np.stack([x, y], 1)).query(np.stack([px, py], 1))[1]]
+    pz = -hn * rng.uniform(0.1, 0.9, m)
+    streams = scenario_streams(cy, m)
+
+    def make(i):
+        n = int(npart[i])
+        ps = cy.LagrangianParticleSet(np.repeat([[px[i], py[i], pz[i]]], n, 0), dim=3, particle_density=2650.,
+                                      buoyancy_velocity_model=1)
+        d = settling_diameters(np.random.default_rng((seed, i)), n)
+        ps.diameter = d
+        ps.volume = np.pi * d ** 3 / 6.
+        return ps
+
+    def params(mdl):
+        p = dict(final_time=1e9, time_step=dt, frozen_eulerian_fields=True, diffusion_model=3)
+        if mdl == 1:
+            p.update(model=1)
+        else:
+            p.update(model=2, time_scheme=5, diffusion_lsm2_option=1, particle_velocity_init=1)
+        return p
+
+    return [cy.Request(None, params(model[i]), streams[i], int(nsteps[i]), arrival=arr[i], make=partial(make, i),
+                       npart=int(npart[i]), dim=3) for i in range(m)]
+
+
+def served_leg(cy, window=60., horizon=75., lam=20., max_rows=20_000_000, tick_steps=10, sort_interval=8):
+    """The request loop on the 3D channel: arrivals over `window` seconds, the
+    loop stopped `horizon` seconds after the start; latencies are those of the
+    requests completed by then."""
+    from cylag_amd.serve import latency_stats
+    t0 = time.time()
+    x, y, tri, h, fields = c3_workload()
+    mesh = cy.TriangularMesh(x, y, tri)
+    fs = cy.EulerianFieldSet(mesh, **fields)
+    reqs = c5_requests(cy, x, y, h, window, lam=lam)
+    log("served: 3D channel {} elements, {} requests over {:.0f} s ({:.1f} s)".format(
+        tri.shape[0], len(reqs), window, time.time() - t0))
+    last = [0.]
+
+    def progress(msg):
+        now = time.time()
+        if now - last[0] > 10.:
+            last[0] = now
+            log(msg)
+
+    loop = cy.ServeLoop(fs, tick_steps=tick_steps, sort_interval=sort_interval, max_rows=max_rows,
+                        keep_results=False, log=progress)
+    done = loop.serve(reqs, horizon=horizon)
+    loop.close()
+    st = latency_stats(done)
+    offered = sum(r.npart * r.nsteps for r in reqs) / window
+    out = dict(lam_per_s=lam, window_s=window, horizon_s=horizon, arrived=len(reqs),
+               admitted=sum(r.admitted is not None for r in reqs), completed=st["completed"],
+               latency_p50_s=st["p50_s"], latency_p99_s=st["p99_s"],
+               value=loop.particle_steps / loop.elapsed, unit="particle-steps/s",
+               offered_particle_steps_per_s=offered, max_running=loop.max_running, capacity_rows=loop.capacity,
+               tick_steps=tick_steps,
+               workload=("3D sigma-layer channel (~500k elements, 10 planes, frozen); Poisson {} req/s (seed 13); "
+                         "N ~ logU(1e3, 2e6), 1-48 h at dt = 10 s, LSM1 or LSM2; admission by free HBM "
+                         "capped at {} rows; latencies of requests completed within {:.0f} s").format(
+                             lam, max_rows, horizon))
+    log("served: {} arrived, {} admitted, {} completed, p50 {} s, p99 {} s, {:.3e} particle-steps/s".format(
+        out["arrived"], out["admitted"], out["completed"], out["latency_p50_s"], out["latency_p99_s"], out["value"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,9 +391,10 @@ def main():
     ap.add_argument("--npart", type=int, default=10_000_000)
     ap.add_argument("--sort-interval", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="plain,batched,sequential",
+    ap.add_argument("--legs", default="plain,batched,sequential,served",
                     help="comma-separated: plain (the 10M-particle step), batched (256 scenarios in one batch), "
-                         "sequential (the same scenarios one after another with the plain step)")
+                         "sequential (the same scenarios one after another with the plain step), served "
+                         "(Poisson requests on the 3D channel)")
     ap.add_argument("--batch-steps", type=int, default=None,
                     help="run only the first steps of the batched job (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -306,11 +406,12 @@ def main():
         sys.exit(2)
     legs = set(args.legs.split(","))
     import cylag_amd as cy
-    t0 = time.time()
-    x, y, tri, fields, pos, params = c2_workload(npart=args.npart if "plain" in legs else 1)
-    log("workload built: {} nodes, {} triangles, {} particles ({:.1f} s)".format(
-        x.size, tri.shape[0], pos.shape[0], time.time() - t0))
-    mesh = cy.TriangularMesh(x, y, tri)
+    if legs & {"plain", "batched", "sequential"}:
+        t0 = time.time()
+        x, y, tri, fields, pos, params = c2_workload(npart=args.npart if "plain" in legs else 1)
+        log("workload built: {} nodes, {} triangles, {} particles ({:.1f} s)".format(
+            x.size, tri.shape[0], pos.shape[0], time.time() - t0))
+        mesh = cy.TriangularMesh(x, y, tri)
     line = {"metric": METRIC, "unit": "particle-steps/s", "n_gpus": 1, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
     if "plain" in legs:
@@ -331,6 +432,8 @@ def main():
             bt["speedup_vs_plain_sequential"] = bt["value"] / seq["value"]
         else:
             line["plain_sequential"] = seq
+    if "served" in legs:
+        line["served"] = served_leg(cy)
     if "plain" in legs and not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(x, y, tri, fields, pos, params)

@@ -61,11 +61,8 @@ class LagrangianParticleSet:
         else:
             self.diameter_distribution = 'monodisperse'
             self.diameter_distribution_param = 0.0
-        h["diameter"] = np.zeros(n)
-        h["volume"] = np.zeros(n)
-        for i in range(n):
-            h["diameter"][i] = self._draw_diameter()
-        h["volume"][:] = np.pi * (h["diameter"] ** 3) / 6.
+        h["diameter"] = np.full(n, self._draw_diameter()) if n else np.zeros(0)
+        h["volume"] = np.pi * (h["diameter"] ** 3) / 6.
         h["position"] = np.zeros((n, dim))
         h["velocity"] = np.zeros((n, dim))
         h["fluid_velocity_seen"] = np.zeros((n, dim))

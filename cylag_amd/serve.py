@@ -25,8 +25,16 @@ ROW_BYTES = {2: 480, 3: 640}
 
 
 class Request:
-    def __init__(self, particle_set, parameters, rng_state, nsteps, arrival=0.):
+    """particle_set may be None with `make` a callable that builds it at
+    admission (`npart` and `dim` its size), so that queued requests hold no rows."""
+
+    def __init__(self, particle_set, parameters, rng_state, nsteps, arrival=0., make=None, npart=None, dim=None):
+        if particle_set is None and (make is None or npart is None or dim is None):
+            raise ValueError("a request without a particle set needs make, npart and dim")
         self.particle_set = particle_set
+        self.make = make
+        self._npart = npart
+        self._dim = dim
         self.parameters = parameters
         self.rng_state = rng_state
         self.nsteps = int(nsteps)
@@ -38,7 +46,11 @@ class Request:
 
     @property
     def npart(self):
-        return self.particle_set.npart
+        return self.particle_set.npart if self.particle_set is not None else int(self._npart)
+
+    @property
+    def dim(self):
+        return self.particle_set.dim if self.particle_set is not None else int(self._dim)
 
     @property
     def latency(self):
@@ -50,12 +62,16 @@ class ServeLoop:
 
     capacity: rows the device may hold (default: hbm_fraction of the free HBM
     after the mesh and fields are resident, at ROW_BYTES per row; a reshape
-    holds the old and the new pool at once, hence the default fraction).
-    tick_steps: most steps between two admission points."""
+    holds the old and the new pool at once, hence the default fraction),
+    capped at max_rows if given (a bound on the step time).
+    tick_steps: most steps between two admission points.
+    keep_results: False drops a finished request's rows after collecting them."""
 
     def __init__(self, field_set, device=0, capacity=None, hbm_fraction=0.4, tick_steps=50, sort_interval=8,
-                 clock=time.perf_counter, log=None, batch_factory=BatchSolver):
+                 clock=time.perf_counter, log=None, batch_factory=BatchSolver, keep_results=True, max_rows=None):
         self.field_set = field_set
+        self.keep_results = keep_results
+        self.max_rows = max_rows
         self.batch_factory = batch_factory
         self.device = device
         self.capacity = capacity
@@ -80,9 +96,14 @@ class ServeLoop:
             if free is None:
                 return None  # decided once the first batch exists
             self.capacity = int(self.hbm_fraction * (free + self._resident() * ROW_BYTES[dim]) / ROW_BYTES[dim])
+            if self.max_rows is not None:
+                self.capacity = min(self.capacity, int(self.max_rows))
         return self.capacity - self._resident()
 
     def _admit(self, reqs, now):
+        for r in reqs:
+            if r.particle_set is None:
+                r.particle_set = r.make()
         psets = [r.particle_set for r in reqs]
         params = [r.parameters for r in reqs]
         streams = [r.rng_state for r in reqs]
@@ -109,6 +130,8 @@ class ServeLoop:
             b.collect(s)
             r.final_rng_state = b.rng_state(s)
             r.finished = now
+            if not self.keep_results:
+                r.particle_set = None
             self.done.append(r)
         gone = [s for s in range(b.nscen) if b.steps_left(s) == 0]
         if gone and len(gone) < b.nscen:
@@ -131,7 +154,7 @@ class ServeLoop:
                 self._finish(running, now)
             # admission: first come, first served, while the device has room
             adm = []
-            dim = waiting[0].particle_set.dim if waiting else 2
+            dim = waiting[0].dim if waiting else 2
             room = self._room(dim)
             while waiting and (room is None or waiting[0].npart <= room - sum(r.npart for r in adm)):
                 adm.append(waiting.popleft())
