@@ -201,7 +201,7 @@ def point_release(cy, xy, n):
     return cy.LagrangianParticleSet(np.repeat(np.asarray(xy, dtype=np.float64)[None], int(n), 0), dim=2)
 
 
-def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250):
+def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250, tune=True):
     """The whole batched job: every scenario runs all its steps (or the first
     max_steps of them); every 250 steps the finished scenarios' rows come back
     to the host and the scenarios leave the device.  The clock brackets the
@@ -217,9 +217,18 @@ def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250):
         len(npart), int(npart.sum()), int(nsteps.min()), int(nsteps.max()), time.time() - t0))
     total = int(nsteps.max()) if max_steps is None else min(int(max_steps), int(nsteps.max()))
     t0 = time.perf_counter()
-    done, ncollected = 0, 0
+    done, ncollected, tuned = 0, 0, []
+    tune_at = [0, total // 2] if tune else []
     while done < total:
-        k = min(chunk, total - done)
+        if tune_at and done >= tune_at[0] and total - done > 400:
+            tune_at.pop(0)
+            # the autotuner's trial steps are steps of the job
+            s0 = max(b.steps_left())
+            si, bpc, _ = cy.autotune(b, trial_steps=24, log=log)
+            done += s0 - max(b.steps_left())
+            tuned.append(dict(step=done, sort_interval=si, blocks_per_cu=bpc))
+            continue
+        k = min(chunk - done % chunk, total - done)
         b.run(k)
         done += k
         # finished scenarios: rows back to the host, then off the device
@@ -239,7 +248,8 @@ def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250):
     work = float(np.sum(npart * np.minimum(nsteps, total)))
     b.ctx.close()
     return dict(scenarios=len(npart), particles=int(npart.sum()), lsm2_scenarios=int(np.sum(model == 2)),
-                particle_steps=work, steps=total, wall_s=wall, value=work / wall, sort_interval=sort_interval)
+                particle_steps=work, steps=total, wall_s=wall, value=work / wall,
+                tuning=tuned if tune else [dict(step=0, sort_interval=sort_interval, blocks_per_cu=0)])
 
 
 def plain_sequential_leg(cy, mesh, fields, ksteps=40, warm=3, nsteps_cap=None):
@@ -446,7 +456,7 @@ def plain_leg(cy, args, mesh, fields, pos, params):
     cy.pcg32_seed(42, 0xda3e06d11bb2b203)
     t0 = time.time()
     solver = cy.Solver(fs, ps, params)
-    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 256)
+    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0)
     log("solver ready ({:.1f} s): {}".format(time.time() - t0, solver.ctx.device_info()[0]))
     solver._run(args.warmup)
     log("warmup done")

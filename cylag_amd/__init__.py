@@ -17,6 +17,7 @@ from .particles import LagrangianParticleSet  # noqa: F401
 from .solver import Solver  # noqa: F401
 from .batch import BatchSolver  # noqa: F401
 from .serve import Request, ServeLoop, latency_stats  # noqa: F401
+from .tune import autotune  # noqa: F401
 from .random_utils import (pcg32_seed, pcg32_advance, random_uniform, random_gaussian,  # noqa: F401
                            random_gaussian_pair, random_gaussian_ziggurat,
                            random_gaussian_ziggurat_pair, generate_random_pair,

@@ -97,8 +97,15 @@ class BatchSolver:
         ns = np.ascontiguousarray(nsteps, dtype=np.int32)
         self.ctx.check(lib.cyl_scenarios_set(h, len(particle_sets), P(base, "i"), C.cast(pars, C.c_void_p), st, inc,
                                              P(ns, "i")), "cyl_scenarios_set")
-        lib.cyl_set_tuning(h, int(sort_interval), 256)
+        self.set_tuning(sort_interval, 0)
         self.ctx.check(lib.cyl_init_particles(h, C.byref(pars[0]), self.time, self.dt), "cyl_init_particles")
+
+    def set_tuning(self, sort_interval, blocks_per_cu=0):
+        """Element re-sort interval (steps, 0 = never) and resident blocks per CU
+        of the persistent kernels (0 = as many as fit); results do not depend on them."""
+        self.ctx.check(self.ctx.lib.cyl_set_tuning(self.ctx.h, int(sort_interval), int(blocks_per_cu)),
+                       "cyl_set_tuning")
+        self.tuning = (int(sort_interval), int(blocks_per_cu))
 
     # -- scenario table -----------------------------------------------------------
     def _new_scenario(self, pset, params, nsteps):

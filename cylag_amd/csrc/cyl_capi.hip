@@ -146,7 +146,8 @@ struct cyl_ctx {
   int bshift = 6;
 
   // element re-sort
-  int sort_interval = 0, block = BLK, steps_since_sort = 0;
+  int sort_interval = 0, steps_since_sort = 0;
+  int blocks_per_cu = 0;  // persistent grids: resident blocks per CU (0 = as many as fit)
   int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
   int64_t* d_cursor = nullptr;
   int64_t* d_sort_tiles = nullptr;
@@ -1830,6 +1831,7 @@ static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
 static unsigned persistent_grid(cyl_ctx* ctx, const void* fn, int64_t nblocks, bool mult8) {
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (ctx->blocks_per_cu > 0) per_cu = std::min(per_cu, ctx->blocks_per_cu);
   int64_t g = std::min<int64_t>(std::max<int64_t>(nblocks, 1), (int64_t)ctx->ncu * per_cu);
   return mult8 ? (unsigned)std::max<int64_t>(8, g / 8 * 8) : (unsigned)g;
 }
@@ -2209,10 +2211,10 @@ extern "C" int cyl_init_particles(cyl_ctx* ctx, const cyl_params* p, double time
   return e;
 }
 
-extern "C" int cyl_set_tuning(cyl_ctx* ctx, int sort_interval, int block_size) {
-  if (!ctx || sort_interval < 0) return CYL_E_ARG;
+extern "C" int cyl_set_tuning(cyl_ctx* ctx, int sort_interval, int blocks_per_cu) {
+  if (!ctx || sort_interval < 0 || blocks_per_cu < 0) return CYL_E_ARG;
   ctx->sort_interval = sort_interval;
-  if (block_size > 0) ctx->block = block_size;
+  ctx->blocks_per_cu = blocks_per_cu;
   return CYL_OK;
 }
 

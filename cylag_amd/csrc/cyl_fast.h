@@ -294,7 +294,7 @@ template <int RNG>
 __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   __shared__ JumpTable8 t8;
   __shared__ ZigTables zt;
-  load_jump8(t8, a.jt8);  // scenario 0's table; other scenarios read theirs from memory
+  __shared__ int t8_scen;
   if (RNG == 2) {
     for (int q = threadIdx.x; q < 128; q += BLK) {
       zt.kn[q] = a.zt->kn[q];
@@ -302,14 +302,28 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
       zt.fn[q] = a.zt->fn[q];
     }
   }
+  if (threadIdx.x == 0) t8_scen = -1;
   __syncthreads();
   const int64_t nchunk = a.chunk_base[a.nscen];
   const int64_t NONE = 0x7fffffffffffffffLL;
   unsigned long long emax = 0;
   int emax_s = -1;
   bool mixed = false;
-  for (int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x; c < nchunk; c += (int64_t)gridDim.x * BLK) {
-    int s = scen_of(a.chunk_base, a.nscen, c);
+  // the block's BLK chunks of an iteration are consecutive, so they mostly
+  // belong to one scenario: its byte-window jump table is staged in LDS
+  // (reloaded when the block moves to another scenario); threads on chunks
+  // of another scenario read that scenario's table from memory
+  for (int64_t cb = (int64_t)blockIdx.x * BLK; cb < nchunk; cb += (int64_t)gridDim.x * BLK) {
+    int sb = scen_of(a.chunk_base, a.nscen, cb);
+    if (sb != t8_scen) {  // block-uniform
+      __syncthreads();
+      load_jump8(t8, a.jt8 + sb);
+      if (threadIdx.x == 0) t8_scen = sb;
+      __syncthreads();
+    }
+    int64_t c = cb + threadIdx.x;
+    if (c >= nchunk) continue;
+    int s = (c < a.chunk_base[sb + 1]) ? sb : scen_of(a.chunk_base, a.nscen, c);
     if (a.scen_end && a.batch_step >= a.scen_end[s]) continue;
     const StreamDev& st = a.stream[s];
     if (st.units_per_part == 0) continue;
@@ -330,8 +344,8 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
     int err = (RNG != 1 && pos >= st.window) ? E_RNG_WINDOW : 0;
     Pcg g;
     g.inc = st.inc;
-    g.state = s == 0 ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[0])
-                     : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
+    g.state = s == sb ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
+                      : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
     for (int64_t r = r0; r < r1; ++r) {
       double v[2];
       if (RNG == 1) {

@@ -203,7 +203,7 @@ def test_element_resort_does_not_change_results(which):
         if which == "keps_rk2_generic":
             params["time_scheme"] = 2
     s = case.build_gpu(cy, params)
-    s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 256)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 0)
     s.solve()
     r = case.run_oracle(R, params)
     ps = s.particle_set
