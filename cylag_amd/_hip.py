@@ -69,6 +69,7 @@ SIGNATURES = {
     "cyl_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "cyl_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
     "cyl_last_prep_timing": (C.c_int, [C.c_void_p, _dp]),
+    "cyl_last_path": (C.c_int, [C.c_void_p, _ip, _ip]),
     "cyl_scenarios_set": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), _ip]),
     "cyl_scenario_rng_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -153,6 +154,12 @@ class Context:
         k, t, n = C.c_double(), C.c_double(), C.c_int()
         self.lib.cyl_last_timing(self.h, C.byref(k), C.byref(t), C.byref(n))
         return k.value, t.value, n.value
+
+    def last_path(self):
+        """(fused, generic) main-kernel launches of the last step call."""
+        f, g = C.c_int(), C.c_int()
+        self.check(self.lib.cyl_last_path(self.h, C.byref(f), C.byref(g)), "cyl_last_path")
+        return f.value, g.value
 
     def last_prep_ms(self):
         q = C.c_double()

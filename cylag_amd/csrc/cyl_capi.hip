@@ -22,6 +22,7 @@
 #include "../../include/cylag_hip.h"
 #include "cyl_events.h"
 #include "cyl_fast.h"
+#include "cyl_fast3d.h"
 
 using namespace cyl;
 
@@ -55,6 +56,7 @@ struct cyl_ctx {
   Bufs fast_bufs;
   FastMesh fm{};
   NodeF* d_nodes = nullptr;
+  NodeL* d_nodes3 = nullptr;  // 3D: per node and sigma plane
 
   // fields
   FieldsDev fields{};
@@ -171,6 +173,7 @@ struct cyl_ctx {
   int last_step_launches = 0;
   std::vector<hipEvent_t> kev;              // start/stop pairs around the fused step kernel
   int last_launches = 0;
+  int last_fast_launches = 0, last_generic_launches = 0;  // main-kernel launches that did the step
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   int set_err(int code, int particle, const std::string& msg) {
@@ -465,6 +468,10 @@ extern "C" int cyl_fields_upload(cyl_ctx* ctx, int dim, int nlayers, int npoints
   HIPCHK(hipSetDevice(ctx->device));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   ctx->field_bufs.free_all();
+  // the packed records depend on the fields' planes: rebuilt on the next step
+  ctx->fast_bufs.free_all();
+  ctx->fast_ready = false;
+  ctx->d_nodes3 = nullptr;
   int64_t size = (int64_t)npoints * nlayers;
   ctx->fsize = size;
   ctx->ntimes = ntimes;
@@ -1699,14 +1706,16 @@ static int upload_streams(cyl_ctx* ctx, const std::vector<int>& units_per_part) 
   return CYL_OK;
 }
 
-// The fused 2D kernels cover LSM-1 with the Euler scheme, and LSM-2 with the
+// The fused kernels cover LSM-1 with the Euler scheme, and LSM-2 with the
 // direct integrator and no velocity noise (diffusion_lsm2_option 1, or
 // diffusion model 0), without added mass and without wall roughness, with
-// diffusion models 0-3 on 2D fields, once every row is initialized; every
-// other configuration runs the generic k_step.
+// diffusion models 0-3, on 2D fields with 2D particles or on sigma-layer 3D
+// fields with 3D particles (without depth output), once every row is
+// initialized; every other configuration runs the generic k_step.
 static bool fast_params(const cyl_ctx* ctx, const cyl_params* p) {
   const FieldsDev& f = ctx->fields;
-  if (ctx->dim != 2 || f.dim != 2) return false;
+  if (ctx->dim != f.dim) return false;  // pseudo-3D runs the generic kernel
+  if (f.dim == 3 && (p->compute_depth || f.fid_zs < 0 || f.fid_uz < 0 || f.nlayers < 2)) return false;
   if (p->diffusion_model < 0 || p->diffusion_model > 3) return false;
   if (p->model == 1) {
     if (p->time_scheme != 1) return false;
@@ -1740,6 +1749,7 @@ static int ensure_fast(cyl_ctx* ctx) {
   HIPCHK(dalloc(ctx->fast_bufs, &ed, m.ne));
   HIPCHK(dalloc(ctx->fast_bufs, &xy, m.nv));
   HIPCHK(dalloc(ctx->fast_bufs, &ctx->d_nodes, m.nv));
+  if (ctx->fields.dim == 3) HIPCHK(dalloc(ctx->fast_bufs, &ctx->d_nodes3, (size_t)m.nv * ctx->fields.nlayers));
   k_pack_elems<<<nblk(m.nt), BLK, 0, ctx->stream>>>(m, ctx->d_node_new, ef, en, te);
   k_pack_edges<<<nblk(std::max(m.ne, m.nv)), BLK, 0, ctx->stream>>>(m, ctx->d_node_new, ctx->d_node_old, ed, xy);
   HIPCHK(hipGetLastError());
@@ -1873,7 +1883,46 @@ static void launch_fast_model(cyl_ctx* ctx, const FastArgs& f, int m, bool gener
 #undef CYL_FAST_K
 }
 
-static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int model, int m, bool general) {
+// the fused 3D kernel: stream positions of every slot's first draw (uniform or
+// general assignment), then the step with the inline sampler
+template <int MODEL>
+static void launch_fast3(cyl_ctx* ctx, const Fast3Args& f3, int m) {
+  int64_t nb = nblk(f3.f.n);
+  hipStream_t s = ctx->stream;
+  const FastArgs& f = f3.f;
+  if (m == 1) k_draw_start<1><<<persistent_grid(ctx, (const void*)k_draw_start<1>, nb, true), BLK, 0, s>>>(f);
+  else if (m == 2) k_draw_start<2><<<persistent_grid(ctx, (const void*)k_draw_start<2>, nb, true), BLK, 0, s>>>(f);
+  else k_draw_start<3><<<persistent_grid(ctx, (const void*)k_draw_start<3>, nb, true), BLK, 0, s>>>(f);
+  if (m == 1) k_fast_3d<MODEL, 1><<<persistent_grid(ctx, (const void*)k_fast_3d<MODEL, 1>, nb, true), BLK, 0, s>>>(f3);
+  else if (m == 2) k_fast_3d<MODEL, 2><<<persistent_grid(ctx, (const void*)k_fast_3d<MODEL, 2>, nb, true), BLK, 0, s>>>(f3);
+  else k_fast_3d<MODEL, 3><<<persistent_grid(ctx, (const void*)k_fast_3d<MODEL, 3>, nb, true), BLK, 0, s>>>(f3);
+}
+
+static Fast3Args make_fast3_args(cyl_ctx* ctx, const FastArgs& f, const StepArgs& a) {
+  Fast3Args r{};
+  r.f = f;
+  r.nodes3 = ctx->d_nodes3;
+  r.nv = ctx->fields.npoints;
+  r.nl = ctx->fields.nlayers;
+  r.zA = a.A.z;
+  r.zB = a.B.z;
+  r.llA = a.A.ll;
+  r.llB = a.B.ll;
+  r.upA2 = a.A.up[2];
+  r.upB2 = a.B.up[2];
+  r.usA2 = a.A.us[2];
+  r.usB2 = a.B.us[2];
+  return r;
+}
+
+static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int model, int m, bool general,
+                             const StepArgs* a = nullptr) {
+  if (ctx->dim == 3) {
+    Fast3Args f3 = make_fast3_args(ctx, f, *a);
+    if (model == 1) launch_fast3<1>(ctx, f3, m);
+    else launch_fast3<2>(ctx, f3, m);
+    return;
+  }
   if (model == 1) launch_fast_model<1>(ctx, f, m, general);
   else launch_fast_model<2>(ctx, f, m, general);
 }
@@ -1922,8 +1971,9 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     std::vector<int> send(ns);
     for (int s = 0; s < ns; ++s) {
       pd[s] = to_dev(&P[s]);
-      fs[s].diffusion_model = P[s].diffusion_model >= 2 ? 2 : P[s].diffusion_model;
+      fs[s].diffusion_model = P[s].diffusion_model;
       fs[s].diff_const = sqrt(2. * P[s].horizontal_diffusivity) * sqrt(dt);  // diffusion_lsm1, update_state_lsm1.pyx:700
+      fs[s].diff_const_z = sqrt(2. * P[s].vertical_diffusivity) * sqrt(dt);
       fs[s].boundary_conditions = P[s].boundary_conditions;
       fs[s].max_wall_rebounds = P[s].max_wall_rebounds;
       fs[s].bnd_statistics = P[s].bnd_statistics;
@@ -1973,7 +2023,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   double t = *time_inout;
   int* one = ctx->d_flag + 3;
   float kms = 0.f;
-  int n_timed = 0;
+  int n_timed = 0, nfast = 0, ngeneric = 0;
   for (int step = 0; step < nsteps; ++step) {
     // kernel classes of this step: slot ranges stepped by the fused or the generic kernel
     std::vector<cyl_ctx::KClass> cls;
@@ -1991,9 +2041,11 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     }
     {
       std::vector<int> pregen(ns, 0);
-      if (!ctx->batched) pregen[0] = cls[0].fast ? 1 : 0;
+      // the normal generator serves the 2D fused kernels (the 3D one samples inline)
+      bool gen2d = ctx->dim == 2;
+      if (!ctx->batched) pregen[0] = cls[0].fast && gen2d ? 1 : 0;
       else
-        for (int s = 0; s < ns; ++s) pregen[s] = ctx->classes_fast_of_scen(s) && any_fast ? 1 : 0;
+        for (int s = 0; s < ns; ++s) pregen[s] = ctx->classes_fast_of_scen(s) && any_fast && gen2d ? 1 : 0;
       if (pregen != ctx->pregen_h) {  // changes only when arrivals switch the fused kernel off or on
         HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(hipMemcpy(ctx->d_pregen, pregen.data(), ns * sizeof(int), hipMemcpyHostToDevice));
@@ -2034,9 +2086,17 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
           keps |= P[s].diffusion_model >= 2;
           sigc = P[s].schmidt_number;
         }
-      k_pack_nodes<<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(
-          fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], keps ? fd.tmp[fd.fid_turbeng] : nullptr,
-          keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / sigc, fd.npoints, ctx->d_node_old, ctx->d_nodes);
+      if (ctx->dim == 3) {
+        int64_t nrec = (int64_t)fd.npoints * fd.nlayers;
+        k_pack_nodes3<<<nblk(nrec), BLK, 0, ctx->stream>>>(
+            fd.tmp[fd.fid_zs], fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], fd.tmp[fd.fid_uz],
+            keps ? fd.tmp[fd.fid_turbeng] : nullptr, keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / sigc,
+            fd.npoints, fd.nlayers, ctx->d_node_old, ctx->d_nodes3);
+      } else {
+        k_pack_nodes<<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(
+            fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], keps ? fd.tmp[fd.fid_turbeng] : nullptr,
+            keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / sigc, fd.npoints, ctx->d_node_old, ctx->d_nodes);
+      }
       ++launches;
     }
     a.A = ctx->pool[ctx->cur];
@@ -2066,16 +2126,18 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         ctx->kev.push_back(ev);
       }
       HIPCHK(hipEventRecord(ctx->kev[3 * step], ctx->stream));
-      if (any_fast) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
+      if (any_fast && ctx->dim == 2) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
       HIPCHK(hipEventRecord(ctx->kev[3 * step + 1], ctx->stream));
       for (auto& c : cls) {
         if (c.hi <= c.lo) continue;
         if (c.fast) {
-          launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), c.model, method, false);
+          launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), c.model, method, false, &a);
+          ++nfast;
         } else {
           a.j0 = c.lo;
           a.n = c.hi - c.lo;
           launch_step(ctx, a, c.model, false);
+          ++ngeneric;
         }
         ++launches;
       }
@@ -2111,11 +2173,13 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     for (auto& c : cls) {
       if (c.hi <= c.lo) continue;
       if (c.fast) {
-        launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), c.model, method, true);
+        launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), c.model, method, true, &a);
+        if (!uniform) ++nfast;
       } else {
         a.j0 = c.lo;
         a.n = c.hi - c.lo;
         launch_step(ctx, a, c.model, false);
+        if (!uniform) ++ngeneric;
       }
       ++launches;
     }
@@ -2138,6 +2202,8 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   if (nsteps > 0) HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
   ctx->last_kernel_ms = kms;
   ctx->last_launches = launches;
+  ctx->last_fast_launches = nfast;
+  ctx->last_generic_launches = ngeneric;
   double sk = 0., sp = 0.;
   for (int q = 0; q < n_timed; ++q) {
     float ms = 0.f, mp = 0.f;
@@ -2223,6 +2289,13 @@ extern "C" int cyl_last_timing(cyl_ctx* ctx, double* kms, double* tms, int* laun
   if (kms) *kms = ctx->last_step_kernel_ms;
   if (tms) *tms = ctx->last_total_ms;
   if (launches) *launches = ctx->last_step_launches;
+  return CYL_OK;
+}
+
+extern "C" int cyl_last_path(cyl_ctx* ctx, int* fused_launches, int* generic_launches) {
+  if (!ctx) return CYL_E_ARG;
+  if (fused_launches) *fused_launches = ctx->last_fast_launches;
+  if (generic_launches) *generic_launches = ctx->last_generic_launches;
   return CYL_OK;
 }
 

@@ -148,8 +148,9 @@ __device__ __forceinline__ int localize_fast(const FastMesh& fm, const MeshDev& 
 
 // Per-scenario parameters of the fused kernel (one entry per scenario)
 struct FastScen {
-  int diffusion_model;     // 0, 1 (constant), 2 (k-epsilon)
-  double diff_const;       // sqrt(2*nux)*sqrtdt for model 1
+  int diffusion_model;     // 0, 1 (constant), 2 (k-epsilon K, constant vertical), 3 (k-epsilon K in 3D)
+  double diff_const;       // sqrt(2*nux)*sqrtdt (model 1)
+  double diff_const_z;     // sqrt(2*nuz)*sqrtdt (3D, models 1 and 2)
   int boundary_conditions, max_wall_rebounds, bnd_statistics;
   double damp, strand;
 };
@@ -501,7 +502,7 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   // diffusion_lsm1 with K and grad K at the start-of-step element
   if (q.diffusion_model != 0) {
     double diffx, driftx = 0., drifty = 0.;
-    if (q.diffusion_model == 2) {
+    if (q.diffusion_model >= 2) {
       double K = n0.K * f0 + n1.K * f1 + n2.K * f2;
       double dkdx = n0.K * ef.gx[0] + n1.K * ef.gx[1] + n2.K * ef.gx[2];
       double dkdy = n0.K * ef.gy[0] + n1.K * ef.gy[1] + n2.K * ef.gy[2];

@@ -166,6 +166,11 @@ int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int blocks_per_cu);
  * launches (HIP events on the context's stream), wall time of the call, and
  * the number of timed launches. */
 int cyl_last_timing(cyl_ctx *ctx, double *step_kernel_ms, double *total_ms, int *kernel_launches);
+/* Which kernels advanced the particles in the last cyl_step / cyl_batch_step
+ * call (no counterpart in the reference): launches of the fused kernels and of
+ * the generic kernel that did the step (guarded reruns that found nothing to
+ * do are not counted). */
+int cyl_last_path(cyl_ctx *ctx, int *fused_launches, int *generic_launches);
 /* Summed duration of the draw-start pass that precedes the fused 2D LSM-1
  * step kernel in the same timed steps (0 when the generic kernel ran). */
 int cyl_last_prep_timing(cyl_ctx *ctx, double *prep_ms);

@@ -214,3 +214,25 @@ def test_element_resort_does_not_change_results(which):
     for name in ("last_position", "last_velocity"):
         gv, rv = getattr(ps, name), getattr(r, name)
         assert np.max(np.abs(gv - rv) / np.maximum(np.abs(rv), 1.)) <= TOL, name
+
+
+@pytest.mark.parametrize("which,fused", [(0, True), (1, True), (2, True), (3, False), (4, False)],
+                         ids=["lsm1_const", "lsm1_keps3d", "lsm2_di1", "lsm2_di2", "lsm2_em"])
+def test_3d_kernel_path(which, fused):
+    """The fused 3D kernel steps the configurations it covers (and parity holds
+    there, test_3d_stochastic); the others run the generic kernel."""
+    case = vortex_3d_case()
+    params = dict(final_time=50., time_step=5., **V3D[which])
+    g = case.run_gpu(cy, params)
+    f, gen = g.solver.ctx.last_path()
+    assert (f > 0 and gen == 0) if fused else (f == 0 and gen > 0), (f, gen)
+
+
+@pytest.mark.parametrize("model", [1, 2])
+def test_2d_kernel_path(model):
+    case = vortex_case(npart=500)
+    p = dict(model=1, diffusion_model=2) if model == 1 else dict(
+        model=2, time_scheme=5, diffusion_model=2, diffusion_lsm2_option=1, particle_velocity_init=1)
+    g = case.run_gpu(cy, dict(final_time=50., time_step=5., **p))
+    f, gen = g.solver.ctx.last_path()
+    assert f > 0 and gen == 0, (f, gen)
