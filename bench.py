@@ -224,9 +224,9 @@ def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250, tune
             tune_at.pop(0)
             # the autotuner's trial steps are steps of the job
             s0 = max(b.steps_left())
-            si, bpc, _ = cy.autotune(b, trial_steps=24, log=log)
+            si, bpc, inl, _ = cy.autotune(b, trial_steps=24, log=log)
             done += s0 - max(b.steps_left())
-            tuned.append(dict(step=done, sort_interval=si, blocks_per_cu=bpc))
+            tuned.append(dict(step=done, sort_interval=si, blocks_per_cu=bpc, inline_normals=inl))
             continue
         k = min(chunk - done % chunk, total - done)
         b.run(k)
@@ -401,6 +401,8 @@ def main():
     ap.add_argument("--npart", type=int, default=10_000_000)
     ap.add_argument("--sort-interval", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--inline-normals", action="store_true",
+                    help="plain leg: sample normals inside the fused kernel instead of the pool-order generator")
     ap.add_argument("--legs", default="plain,batched,sequential,served",
                     help="comma-separated: plain (the 10M-particle step), batched (256 scenarios in one batch), "
                          "sequential (the same scenarios one after another with the plain step), served "
@@ -456,7 +458,7 @@ def plain_leg(cy, args, mesh, fields, pos, params):
     cy.pcg32_seed(42, 0xda3e06d11bb2b203)
     t0 = time.time()
     solver = cy.Solver(fs, ps, params)
-    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0)
+    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0, int(args.inline_normals))
     log("solver ready ({:.1f} s): {}".format(time.time() - t0, solver.ctx.device_info()[0]))
     solver._run(args.warmup)
     log("warmup done")

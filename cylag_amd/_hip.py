@@ -66,7 +66,7 @@ SIGNATURES = {
     "cyl_random_pair": (None, [_u64p, C.c_uint64, C.c_int, _dp, _dp]),
     "cyl_init_particles": (C.c_int, [C.c_void_p, C.POINTER(CylParams), C.c_double, C.c_double]),
     "cyl_step": (C.c_int, [C.c_void_p, C.POINTER(CylParams), _dp, C.c_double, C.c_int, _ip]),
-    "cyl_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "cyl_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "cyl_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
     "cyl_last_prep_timing": (C.c_int, [C.c_void_p, _dp]),
     "cyl_last_path": (C.c_int, [C.c_void_p, _ip, _ip]),

@@ -100,12 +100,13 @@ class BatchSolver:
         self.set_tuning(sort_interval, 0)
         self.ctx.check(lib.cyl_init_particles(h, C.byref(pars[0]), self.time, self.dt), "cyl_init_particles")
 
-    def set_tuning(self, sort_interval, blocks_per_cu=0):
-        """Element re-sort interval (steps, 0 = never) and resident blocks per CU
-        of the persistent kernels (0 = as many as fit); results do not depend on them."""
-        self.ctx.check(self.ctx.lib.cyl_set_tuning(self.ctx.h, int(sort_interval), int(blocks_per_cu)),
-                       "cyl_set_tuning")
-        self.tuning = (int(sort_interval), int(blocks_per_cu))
+    def set_tuning(self, sort_interval, blocks_per_cu=0, inline_normals=0):
+        """Element re-sort interval (steps, 0 = never), resident blocks per CU of
+        the persistent kernels (0 = as many as fit) and inline sampling of the
+        2D fused kernels' normals; results do not depend on them."""
+        self.ctx.check(self.ctx.lib.cyl_set_tuning(self.ctx.h, int(sort_interval), int(blocks_per_cu),
+                                                   int(inline_normals)), "cyl_set_tuning")
+        self.tuning = (int(sort_interval), int(blocks_per_cu), int(inline_normals))
 
     # -- scenario table -----------------------------------------------------------
     def _new_scenario(self, pset, params, nsteps):

@@ -150,6 +150,7 @@ struct cyl_ctx {
   // element re-sort
   int sort_interval = 0, steps_since_sort = 0;
   int blocks_per_cu = 0;  // persistent grids: resident blocks per CU (0 = as many as fit)
+  int inline_normals = 0; // 2D fused kernels: sample normals inline instead of the pool-order generator
   int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
   int64_t* d_cursor = nullptr;
   int64_t* d_sort_tiles = nullptr;
@@ -1923,8 +1924,11 @@ static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int model, int m, 
     else launch_fast3<2>(ctx, f3, m);
     return;
   }
-  if (model == 1) launch_fast_model<1>(ctx, f, m, general);
-  else launch_fast_model<2>(ctx, f, m, general);
+  // the inline variant (draw start + sampling in the kernel) also serves the
+  // uniform assignment when the generator is switched off
+  bool inl = general || ctx->inline_normals;
+  if (model == 1) launch_fast_model<1>(ctx, f, m, inl);
+  else launch_fast_model<2>(ctx, f, m, inl);
 }
 
 static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
@@ -2042,7 +2046,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     {
       std::vector<int> pregen(ns, 0);
       // the normal generator serves the 2D fused kernels (the 3D one samples inline)
-      bool gen2d = ctx->dim == 2;
+      bool gen2d = ctx->dim == 2 && !ctx->inline_normals;
       if (!ctx->batched) pregen[0] = cls[0].fast && gen2d ? 1 : 0;
       else
         for (int s = 0; s < ns; ++s) pregen[s] = ctx->classes_fast_of_scen(s) && any_fast && gen2d ? 1 : 0;
@@ -2126,7 +2130,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         ctx->kev.push_back(ev);
       }
       HIPCHK(hipEventRecord(ctx->kev[3 * step], ctx->stream));
-      if (any_fast && ctx->dim == 2) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
+      if (any_fast && ctx->dim == 2 && !ctx->inline_normals) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
       HIPCHK(hipEventRecord(ctx->kev[3 * step + 1], ctx->stream));
       for (auto& c : cls) {
         if (c.hi <= c.lo) continue;
@@ -2277,10 +2281,12 @@ extern "C" int cyl_init_particles(cyl_ctx* ctx, const cyl_params* p, double time
   return e;
 }
 
-extern "C" int cyl_set_tuning(cyl_ctx* ctx, int sort_interval, int blocks_per_cu) {
+extern "C" int cyl_set_tuning(cyl_ctx* ctx, int sort_interval, int blocks_per_cu, int inline_normals) {
   if (!ctx || sort_interval < 0 || blocks_per_cu < 0) return CYL_E_ARG;
   ctx->sort_interval = sort_interval;
   ctx->blocks_per_cu = blocks_per_cu;
+  if (inline_normals != ctx->inline_normals) ctx->pregen_h.clear();  // re-upload the generator flags
+  ctx->inline_normals = inline_normals ? 1 : 0;
   return CYL_OK;
 }
 

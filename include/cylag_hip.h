@@ -159,9 +159,11 @@ int cyl_batch_reshape(cyl_ctx *ctx, int nkeep, const int *keep, int nnew, const 
                       const double *volume, const int *init_tri, double time, double dt);
 
 /* Tuning (no counterpart in the reference): element re-sort interval in steps
- * (0 = never) and the resident 256-thread blocks per CU of the persistent
- * grids (0 = as many as fit).  Results do not depend on either. */
-int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int blocks_per_cu);
+ * (0 = never), the resident 256-thread blocks per CU of the persistent grids
+ * (0 = as many as fit), and whether the 2D fused kernels sample their normals
+ * inline (1) or read them from the pool-order generator (0).  Results do not
+ * depend on any of them. */
+int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int blocks_per_cu, int inline_normals);
 /* Timing of the last cyl_step: summed duration of the fused step-kernel
  * launches (HIP events on the context's stream), wall time of the call, and
  * the number of timed launches. */

@@ -1,6 +1,8 @@
 """HIP step (through the C ABI) against the oracle on the same seeded inputs:
 identical element ids, pool flags, tags and random-stream state; positions
 within 1e-9 relative (FP64)."""
+from types import SimpleNamespace
+
 import numpy as np
 import pytest
 
@@ -203,11 +205,10 @@ def test_element_resort_does_not_change_results(which):
         if which == "keps_rk2_generic":
             params["time_scheme"] = 2
     s = case.build_gpu(cy, params)
-    s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 0)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 0, 0)
     s.solve()
     r = case.run_oracle(R, params)
     ps = s.particle_set
-    from types import SimpleNamespace
     g = SimpleNamespace(position=ps.position, velocity=ps.velocity, tri=ps.tri,
                         inactive=ps.inactive, lowerlayer=ps.lowerlayer, tag=ps.tag)
     assert_parity(g, r)
@@ -236,3 +237,19 @@ def test_2d_kernel_path(model):
     g = case.run_gpu(cy, dict(final_time=50., time_step=5., **p))
     f, gen = g.solver.ctx.last_path()
     assert f > 0 and gen == 0, (f, gen)
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+def test_inline_normals_keps_two_frames(rng_method):
+    """The 2D fused kernel with its normals sampled inline from each slot's
+    prepared stream position (uniform assignment, generator off)."""
+    case = vortex_case(npart=3000)
+    params = dict(final_time=200., time_step=5., model=1, diffusion_model=2, rng_method=rng_method)
+    s = case.build_gpu(cy, params)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 4, 0, 1)
+    s.solve()
+    ps = s.particle_set
+    g = SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(), tri=ps.tri.copy(),
+                        inactive=ps.inactive.copy(), lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy())
+    assert s.ctx.last_path()[0] > 0
+    assert_parity(g, case.run_oracle(R, params))
