@@ -407,6 +407,8 @@ def main():
                     help="comma-separated: plain (the 10M-particle step), batched (256 scenarios in one batch), "
                          "sequential (the same scenarios one after another with the plain step), served "
                          "(Poisson requests on the 3D channel)")
+    ap.add_argument("--served-window", type=float, default=60.,
+                    help="served leg: seconds of arrivals (the loop stops 15 s later)")
     ap.add_argument("--batch-steps", type=int, default=None,
                     help="run only the first steps of the batched job (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -445,7 +447,7 @@ def main():
         else:
             line["plain_sequential"] = seq
     if "served" in legs:
-        line["served"] = served_leg(cy)
+        line["served"] = served_leg(cy, window=args.served_window, horizon=args.served_window + 15.)
     if "plain" in legs and not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(x, y, tri, fields, pos, params)
