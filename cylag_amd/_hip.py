@@ -66,10 +66,10 @@ SIGNATURES = {
     "cyl_random_pair": (None, [_u64p, C.c_uint64, C.c_int, _dp, _dp]),
     "cyl_init_particles": (C.c_int, [C.c_void_p, C.POINTER(CylParams), C.c_double, C.c_double]),
     "cyl_step": (C.c_int, [C.c_void_p, C.POINTER(CylParams), _dp, C.c_double, C.c_int, _ip]),
-    "cyl_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "cyl_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cyl_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
     "cyl_last_prep_timing": (C.c_int, [C.c_void_p, _dp]),
-    "cyl_last_path": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "cyl_last_path": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     "cyl_scenarios_set": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), _ip]),
     "cyl_scenario_rng_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -156,10 +156,16 @@ class Context:
         return k.value, t.value, n.value
 
     def last_path(self):
-        """(fused, generic) main-kernel launches of the last step call."""
-        f, g = C.c_int(), C.c_int()
-        self.check(self.lib.cyl_last_path(self.h, C.byref(f), C.byref(g)), "cyl_last_path")
+        """(fused, generic) main-kernel launches of the last step call (eager steps)."""
+        f, g, q = C.c_int(), C.c_int(), C.c_int()
+        self.check(self.lib.cyl_last_path(self.h, C.byref(f), C.byref(g), C.byref(q)), "cyl_last_path")
         return f.value, g.value
+
+    def last_graph_steps(self):
+        """Steps of the last step call run by graph replays."""
+        q = C.c_int()
+        self.check(self.lib.cyl_last_path(self.h, None, None, C.byref(q)), "cyl_last_path")
+        return q.value
 
     def last_prep_ms(self):
         q = C.c_double()

@@ -57,7 +57,7 @@ class _Scenario:
 
 class BatchSolver:
     def __init__(self, field_set, particle_sets, parameters, rng_states, nsteps=None, device=0,
-                 sort_interval=8):
+                 sort_interval=8, use_graph=True):
         if not (len(particle_sets) == len(parameters) == len(rng_states)) or not particle_sets:
             raise ValueError("one parameter dict and one (state, inc) per particle set")
         self.field_set = field_set
@@ -97,15 +97,19 @@ class BatchSolver:
         ns = np.ascontiguousarray(nsteps, dtype=np.int32)
         self.ctx.check(lib.cyl_scenarios_set(h, len(particle_sets), P(base, "i"), C.cast(pars, C.c_void_p), st, inc,
                                              P(ns, "i")), "cyl_scenarios_set")
-        self.set_tuning(sort_interval, 0)
+        self.set_tuning(sort_interval, 0, 0, use_graph)
         self.ctx.check(lib.cyl_init_particles(h, C.byref(pars[0]), self.time, self.dt), "cyl_init_particles")
 
-    def set_tuning(self, sort_interval, blocks_per_cu=0, inline_normals=0):
+    def set_tuning(self, sort_interval, blocks_per_cu=0, inline_normals=0, use_graph=None):
         """Element re-sort interval (steps, 0 = never), resident blocks per CU of
-        the persistent kernels (0 = as many as fit) and inline sampling of the
-        2D fused kernels' normals; results do not depend on them."""
+        the persistent kernels (0 = as many as fit), inline sampling of the 2D
+        fused kernels' normals, and graph replay of multi-step calls (None: keep);
+        results do not depend on them."""
+        if use_graph is not None:
+            self.use_graph = int(use_graph)
         self.ctx.check(self.ctx.lib.cyl_set_tuning(self.ctx.h, int(sort_interval), int(blocks_per_cu),
-                                                   int(inline_normals)), "cyl_set_tuning")
+                                                   int(inline_normals), getattr(self, "use_graph", 0)),
+                       "cyl_set_tuning")
         self.tuning = (int(sort_interval), int(blocks_per_cu), int(inline_normals))
 
     # -- scenario table -----------------------------------------------------------

@@ -151,6 +151,9 @@ struct cyl_ctx {
   int sort_interval = 0, steps_since_sort = 0;
   int blocks_per_cu = 0;  // persistent grids: resident blocks per CU (0 = as many as fit)
   int inline_normals = 0; // 2D fused kernels: sample normals inline instead of the pool-order generator
+  int use_graph = 0;      // replay multi-step loops from a captured HIP graph
+  int* d_clock = nullptr; // graph replays: {step id, batch step} on the device
+  int last_graph_steps = 0;
   int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
   int64_t* d_cursor = nullptr;
   int64_t* d_sort_tiles = nullptr;
@@ -262,6 +265,7 @@ extern "C" int cyl_ctx_create(int device, int flags, cyl_ctx** out) {
   HIPCHK(hipMalloc(&ctx->d_n_deleted, sizeof(int)));
   HIPCHK(hipMalloc(&ctx->d_n_deleted_total, sizeof(long long)));
   HIPCHK(hipMalloc(&ctx->d_step_done, sizeof(int)));
+  HIPCHK(hipMalloc(&ctx->d_clock, 2 * sizeof(int)));
   HIPCHK(hipMemset(ctx->d_n_deleted, 0, sizeof(int)));
   HIPCHK(hipMemset(ctx->d_n_deleted_total, 0, sizeof(long long)));
   ctx->rng_state.assign(1, 0);
@@ -288,7 +292,8 @@ extern "C" int cyl_ctx_destroy(cyl_ctx* ctx) {
   ctx->part_bufs.free_all();
   ctx->ev_bufs.free_all();
   for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
-  void* singles[] = {ctx->d_zt, ctx->d_flag, ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->d_step_done, ctx->d_bnd};
+  void* singles[] = {ctx->d_zt,          ctx->d_flag,      ctx->d_n_deleted, ctx->d_n_deleted_total,
+                     ctx->d_step_done, ctx->d_bnd, ctx->d_clock};
   for (void* p : singles)
     if (p) (void)hipFree(p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -1814,6 +1819,7 @@ static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
   f.fsc = ctx->d_fsc;
   f.scen_end = a.scen_end;
   f.scen_base = a.scen_base;
+  f.clock = a.clock;
   f.nopen = ctx->nopen;
   f.nopen_stride = std::max(ctx->nopen, 1);
   f.general = a.general;
@@ -1932,6 +1938,7 @@ static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int model, int m, 
 }
 
 static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
+static int ensure_sort_bufs(cyl_ctx* ctx);
 
 static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, double dt, int nsteps, int* bnd_out,
                      bool init_only) {
@@ -2028,7 +2035,9 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   int* one = ctx->d_flag + 3;
   float kms = 0.f;
   int n_timed = 0, nfast = 0, ngeneric = 0;
-  for (int step = 0; step < nsteps; ++step) {
+  // Enqueue one step.  cap: the stream is being captured into a graph (no
+  // event records, no host synchronisation, step counters read on the device).
+  auto one_step = [&](int step, bool cap) -> int {
     // kernel classes of this step: slot ranges stepped by the fused or the generic kernel
     std::vector<cyl_ctx::KClass> cls;
     if (!ctx->batched) {
@@ -2051,6 +2060,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       else
         for (int s = 0; s < ns; ++s) pregen[s] = ctx->classes_fast_of_scen(s) && any_fast && gen2d ? 1 : 0;
       if (pregen != ctx->pregen_h) {  // changes only when arrivals switch the fused kernel off or on
+        if (cap) return ctx->set_err(CYL_E_STATE, -1, "generator flags changed inside a captured step");
         HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(hipMemcpy(ctx->d_pregen, pregen.data(), ns * sizeof(int), hipMemcpyHostToDevice));
         ctx->pregen_h = pregen;
@@ -2111,27 +2121,30 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     // active rank when rows were deleted or added since the last scan
     enqueue_rank(ctx, ctx->d_flag + 1, launches);
     bool uniform = can_uniform && !ctx->pending_init;
-    if (step == 0) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (step == 0 && !cap) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (uniform) {
       k_uniform_units<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, ns, a.scen_end,
-                                                         a.batch_step, ctx->d_scen_units);
+                                                         a.batch_step, ctx->d_scen_units, a.clock);
       ++launches;
       if (any_fast) {
         k_gen_chunks<<<1, 1, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, a.scen_end, ctx->d_pregen,
-                                               a.batch_step, ns, ctx->d_gen_base);
+                                               a.batch_step, ns, ctx->d_gen_base, a.clock);
         ++launches;
       }
       enqueue_events(ctx, method, one, launches);
       a.general = 0;
       a.guarded = 0;
-      while (ctx->kev.size() < (size_t)3 * (step + 1)) {
-        hipEvent_t ev;
-        HIPCHK(hipEventCreate(&ev));
-        ctx->kev.push_back(ev);
+      const int ti = n_timed;  // event slots of the timed (eager) steps
+      if (!cap) {
+        while (ctx->kev.size() < (size_t)3 * (ti + 1)) {
+          hipEvent_t ev;
+          HIPCHK(hipEventCreate(&ev));
+          ctx->kev.push_back(ev);
+        }
+        HIPCHK(hipEventRecord(ctx->kev[3 * ti], ctx->stream));
       }
-      HIPCHK(hipEventRecord(ctx->kev[3 * step], ctx->stream));
       if (any_fast && ctx->dim == 2 && !ctx->inline_normals) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
-      HIPCHK(hipEventRecord(ctx->kev[3 * step + 1], ctx->stream));
+      if (!cap) HIPCHK(hipEventRecord(ctx->kev[3 * ti + 1], ctx->stream));
       for (auto& c : cls) {
         if (c.hi <= c.lo) continue;
         if (c.fast) {
@@ -2145,8 +2158,10 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         }
         ++launches;
       }
-      HIPCHK(hipEventRecord(ctx->kev[3 * step + 2], ctx->stream));
-      ++n_timed;
+      if (!cap) {
+        HIPCHK(hipEventRecord(ctx->kev[3 * ti + 2], ctx->stream));
+        ++n_timed;
+      }
     } else {
       HIPCHK(hipMemsetAsync(ctx->d_flag, 0xff, sizeof(int), ctx->stream));  // force the general path
     }
@@ -2155,7 +2170,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     k_rerun_reset<<<nblk(std::max(nstat, ns)), BLK, 0, ctx->stream>>>(
         ctx->d_stream_end, ctx->d_err_step, a.bnd_stats, nstat, ns, ctx->d_n_deleted, g);
     k_undelete<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->pool[0].inactive, ctx->d_del_step, ctx->n,
-                                                      ctx->step_counter, g);
+                                                      ctx->step_counter, g, a.clock);
     a.general = 0;
     a.guarded = 1;
     for (auto& c : cls) {
@@ -2187,10 +2202,10 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       }
       ++launches;
     }
-    if (step == nsteps - 1) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     k_finish<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_stream_end, ns, ctx->d_flag,
                                                 ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->step_counter,
-                                                ctx->d_step_done, ctx->d_err_step, ctx->d_err);
+                                                ctx->d_step_done, ctx->d_err_step, ctx->d_err,
+                                                a.clock ? ctx->d_clock : nullptr);
     ++launches;
     ctx->cur = 1 - ctx->cur;
     ctx->have_last = !init_only;
@@ -2201,7 +2216,74 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       t += dt;  // SimuTime.increment, simutime.pyx:81
       ++ctx->batch_steps;
     }
+    return CYL_OK;
+  };
+  int step = 0;
+  ctx->last_graph_steps = 0;
+  // Graph replay of the loop: a period of 2 * sort_interval steps returns the
+  // state copies, the spare buffers and the sort phase to where they were, so
+  // one captured period is launched repeatedly; the step counters live on the
+  // device.  Needs the uniform draw assignment, frozen fields (the captured
+  // interpolation would be the first step's) and no per-step statistics.
+  const int period = 2 * std::max(1, ctx->sort_interval);
+  if (ctx->use_graph && !init_only && can_uniform && ctx->frozen && !any_stats && nsteps >= 1 + 2 * period) {
+    // one eager step first: arrivals initialized, generator flags and packed records in place
+    e = one_step(step++, false);
+    if (e) return e;
+    e = ensure_sort_bufs(ctx);
+    if (e) return e;
+    e = ensure_fast(ctx);  // no allocation or packing inside the capture
+    if (e) return e;
+    if (!ctx->pending_init) {
+      const int nrep = (nsteps - step) / period;
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      int clk[2] = {ctx->step_counter, ctx->batch_steps};
+      HIPCHK(hipMemcpy(ctx->d_clock, clk, sizeof(clk), hipMemcpyHostToDevice));
+      a.clock = ctx->d_clock;
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      int ec = CYL_OK;
+      for (int q = 0; q < period && !ec; ++q) ec = one_step(step + q, true);
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &graph);
+      a.clock = nullptr;
+      if (ec) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return ec;
+      }
+      HIPCHK(ce);
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (ie != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        HIPCHK(ie);
+      }
+      // the capture advanced the host-side state by one period; each further
+      // replay advances the counters and the clock by another
+      for (int r = 0; r < nrep; ++r) {
+        hipError_t le = hipGraphLaunch(exec, ctx->stream);
+        if (le != hipSuccess) {
+          (void)hipGraphExecDestroy(exec);
+          (void)hipGraphDestroy(graph);
+          HIPCHK(le);
+        }
+        if (r > 0) {
+          ctx->step_counter += period;
+          ctx->batch_steps += period;
+          for (int q = 0; q < period; ++q) t += dt;
+        }
+      }
+      step += nrep * period;
+      ctx->last_graph_steps = nrep * period;
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+    }
   }
+  for (; step < nsteps; ++step) {
+    e = one_step(step, false);
+    if (e) return e;
+  }
+  if (nsteps > 0) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   if (nsteps > 0) HIPCHK(hipEventElapsedTime(&kms, ctx->ev0, ctx->ev1));
   ctx->last_kernel_ms = kms;
@@ -2281,8 +2363,10 @@ extern "C" int cyl_init_particles(cyl_ctx* ctx, const cyl_params* p, double time
   return e;
 }
 
-extern "C" int cyl_set_tuning(cyl_ctx* ctx, int sort_interval, int blocks_per_cu, int inline_normals) {
+extern "C" int cyl_set_tuning(cyl_ctx* ctx, int sort_interval, int blocks_per_cu, int inline_normals,
+                              int use_graph) {
   if (!ctx || sort_interval < 0 || blocks_per_cu < 0) return CYL_E_ARG;
+  ctx->use_graph = use_graph ? 1 : 0;
   ctx->sort_interval = sort_interval;
   ctx->blocks_per_cu = blocks_per_cu;
   if (inline_normals != ctx->inline_normals) ctx->pregen_h.clear();  // re-upload the generator flags
@@ -2298,10 +2382,11 @@ extern "C" int cyl_last_timing(cyl_ctx* ctx, double* kms, double* tms, int* laun
   return CYL_OK;
 }
 
-extern "C" int cyl_last_path(cyl_ctx* ctx, int* fused_launches, int* generic_launches) {
+extern "C" int cyl_last_path(cyl_ctx* ctx, int* fused_launches, int* generic_launches, int* graph_steps) {
   if (!ctx) return CYL_E_ARG;
   if (fused_launches) *fused_launches = ctx->last_fast_launches;
   if (generic_launches) *generic_launches = ctx->last_generic_launches;
+  if (graph_steps) *graph_steps = ctx->last_graph_steps;
   return CYL_OK;
 }
 
@@ -2316,9 +2401,9 @@ extern "C" int cyl_last_prep_timing(cyl_ctx* ctx, double* prep_ms) {
 // pointers are then swapped in.  The fluid velocity seen is carried only when
 // the next step reads it (the fused 2D LSM-1 kernel rewrites it).  Active ranks
 // are pool-order quantities and move with their slots.
-static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
+// counters, cursors and spare per-slot buffers of the sort (allocated once per pool)
+static int ensure_sort_bufs(cyl_ctx* ctx) {
   int n = ctx->n, nt = ctx->mesh.nt;
-  hipStream_t s = ctx->stream;
   Bufs& bb = ctx->part_bufs;
   int ncls = ctx->batched ? std::max<int>(1, (int)ctx->classes.size()) : 1;
   int nbin = ((nt >> SORT_SHIFT) + 3) * ncls;
@@ -2340,9 +2425,19 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
     HIPCHK(dalloc(bb, &ctx->sp_vol, n));
     HIPCHK(dalloc(bb, &ctx->sp_depth, ctx->dim == 3 ? n : 1));
   }
+  return CYL_OK;
+}
+
+static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
+  int e = ensure_sort_bufs(ctx);
+  if (e) return e;
+  int n = ctx->n, nt = ctx->mesh.nt;
+  hipStream_t s = ctx->stream;
+  int ncls = ctx->batched ? std::max<int>(1, (int)ctx->classes.size()) : 1;
+  int nbin = ((nt >> SORT_SHIFT) + 3) * ncls;
   PoolDev& A = ctx->pool[ctx->cur];
   PoolDev& B = ctx->pool[1 - ctx->cur];
-  HIPCHK(hipMemsetAsync(ctx->d_scnt, 0, nbin * sizeof(int), s));
+  k_fill_int<<<nblk(nbin), BLK, 0, s>>>(ctx->d_scnt, nbin, 0);  // a kernel, not a memset: graph-capturable
   k_sort_hist<<<nblk(n, BLK * SORT_ITEMS), BLK, 0, s>>>(A.tri, A.inactive, ctx->d_elem_rank, ctx->batched ? ctx->d_scen : nullptr,
                                       ctx->d_class_of_scen, n, nt, ctx->d_scnt, ctx->d_skey);
   enqueue_scan(ctx, ctx->d_scnt, ctx->d_cursor, (int64_t)nbin, ctx->d_sort_tiles, ctx->d_total,

@@ -188,6 +188,7 @@ struct FastArgs {
   const int* scen_end;     // [nscen] number of steps each scenario runs, or null
   int batch_step;          // steps run since the scenarios were set
   const int* scen_base;    // [nscen] first pool index of each scenario, or null
+  const int* clock;        // graph replays: {step id, batch step} in device memory, or null
   int nopen, nopen_stride; // open boundaries; stride of bnd_stats per scenario
   int general, guarded;
   int* flag;
@@ -208,6 +209,9 @@ struct FastArgs {
   const int64_t* chunk_base;  // [nscen + 1] first normal-generator chunk of each scenario
 };
 
+
+__device__ __forceinline__ int step_of(const FastArgs& a) { return a.clock ? a.clock[0] : a.step_id; }
+__device__ __forceinline__ int batch_step_of(const FastArgs& a) { return a.clock ? a.clock[1] : a.batch_step; }
 
 // Grid-stride walk over the slots for a persistent grid (a multiple of 8
 // blocks).  Blocks are dispatched round-robin over the 8 XCDs, so block b
@@ -280,8 +284,9 @@ __device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int nscen
 // chunk_base[s] = first generator chunk of scenario s: ceil(active / GEN_PER)
 // chunks for scenarios of the fused kernel that draw this step, none otherwise
 __global__ void k_gen_chunks(const StreamDev* st, const int64_t* nact, const int* scen_end, const int* pregen,
-                             int step_id, int nscen, int64_t* chunk_base) {
+                             int step_id, int nscen, int64_t* chunk_base, const int* clock) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (clock) step_id = clock[1];
   int64_t c = 0;
   for (int s = 0; s < nscen; ++s) {
     chunk_base[s] = c;
@@ -325,7 +330,7 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
     int64_t c = cb + threadIdx.x;
     if (c >= nchunk) continue;
     int s = (c < a.chunk_base[sb + 1]) ? sb : scen_of(a.chunk_base, a.nscen, c);
-    if (a.scen_end && a.batch_step >= a.scen_end[s]) continue;
+    if (a.scen_end && batch_step_of(a) >= a.scen_end[s]) continue;
     const StreamDev& st = a.stream[s];
     if (st.units_per_part == 0) continue;
     int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
@@ -424,7 +429,7 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   const int s = a.scen ? a.scen[j] : 0;
   int tri0 = a.triA[j];
   double x0 = a.xA[j], y0 = a.yA[j];
-  if (a.scen_end && a.batch_step >= a.scen_end[s]) {
+  if (a.scen_end && batch_step_of(a) >= a.scen_end[s]) {
     // the scenario has run all its steps: its rows carry over unchanged
     a.xB[j] = x0;
     a.yB[j] = y0;
@@ -601,7 +606,7 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   if (!a.general && dr.nunits != st.units_per_part) atomicOr(&a.flag[0], 1);
   if (deleted) {
     a.inactive[j] = 2;
-    a.del_step[j] = a.step_id;
+    a.del_step[j] = step_of(a);
     atomicAdd(a.n_deleted, 1);
     x = 0.;
     y = 0.;

@@ -96,7 +96,7 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
   const int s = a.scen ? a.scen[j] : 0;
   int tri0 = a.triA[j], ll0 = a3.llA[j];
   double x0 = a.xA[j], y0 = a.yA[j], z0 = a3.zA[j];
-  if (a.scen_end && a.batch_step >= a.scen_end[s]) {
+  if (a.scen_end && batch_step_of(a) >= a.scen_end[s]) {
     // the scenario has run all its steps: its rows carry over unchanged
     a.xB[j] = x0;
     a.yB[j] = y0;
@@ -330,7 +330,7 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
   if (!a.general && dr.nunits != st.units_per_part) atomicOr(&a.flag[0], 1);
   if (deleted) {
     a.inactive[j] = 2;
-    a.del_step[j] = a.step_id;
+    a.del_step[j] = step_of(a);
     atomicAdd(a.n_deleted, 1);
     x = y = z = 0.;
     tri = -1;

@@ -50,7 +50,11 @@ struct StepArgs {
   const int* scen_end;    // per scenario: number of steps it runs, or null
   int batch_step;         // steps run since the scenarios were set
   const int* scen_base;   // per scenario: first pool index, or null (one scenario)
+  const int* clock;       // graph replays: {step id, batch step} in device memory, or null
 };
+
+__device__ __forceinline__ int step_of(const StepArgs& a) { return a.clock ? a.clock[0] : a.step_id; }
+__device__ __forceinline__ int batch_step_of(const StepArgs& a) { return a.clock ? a.clock[1] : a.batch_step; }
 
 __device__ __forceinline__ void load_part(const PoolDev& A, const SlotDev& s, int j, int dim, Part& p) {
   p.x[0] = A.x[j];
@@ -95,7 +99,7 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
     if (COUNT) a.units_pool[a.gpool[j]] = 0;
     return 0;
   }
-  if (a.scen_end && a.batch_step >= a.scen_end[scen] && !a.init_only) {
+  if (a.scen_end && batch_step_of(a) >= a.scen_end[scen] && !a.init_only) {
     // the scenario has run all its steps: its rows carry over unchanged
     if (COUNT) a.units_pool[a.gpool[j]] = 0;
     else store_part(a.B, j, DIM, p, 0);
@@ -164,7 +168,7 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
   if (p.deleted) {
     code = 2;
     atomicAdd(a.n_deleted, 1);
-    if (a.del_step) a.del_step[j] = a.step_id;
+    if (a.del_step) a.del_step[j] = step_of(a);
   } else if (p.inactive) {
     code = 1;
   }
@@ -479,9 +483,10 @@ __global__ void __launch_bounds__(BLK) k_slot_of_pool(const int* gpool, int n, i
 }
 
 // end of step: advance each scenario's stream past its last draw, reset counters
+// clock (graph replays): the device-side step counters, advanced here
 __global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long* stream_end, int nscen, int* flag,
                          int* n_deleted, long long* n_deleted_total, int step_id, int* step_done,
-                         unsigned long long* err_step, unsigned long long* err) {
+                         unsigned long long* err_step, unsigned long long* err, int* clock) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nscen) {
     unsigned long long e = stream_end[s];
@@ -495,7 +500,12 @@ __global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long*
     *n_deleted_total += *n_deleted;
     *n_deleted = 0;
     flag[0] = 0;
-    *step_done = step_id + 1;
+    int sid = clock ? clock[0] : step_id;
+    *step_done = sid + 1;
+    if (clock) {
+      clock[0] = sid + 1;
+      clock[1] += 1;
+    }
   }
 }
 
@@ -515,19 +525,27 @@ __global__ void k_rerun_reset(unsigned long long* stream_end, unsigned long long
 // units needed this step in uniform mode: units_per_part * active rows of the scenario
 // (scenarios that have run all their steps draw nothing)
 __global__ void k_uniform_units(const StreamDev* st, const int64_t* scen_active, int nscen, const int* scen_end,
-                                int step_id, int64_t* scen_units) {
+                                int batch_step, int64_t* scen_units, const int* clock) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nscen) return;
-  bool ended = scen_end && step_id >= scen_end[s];
+  if (clock) batch_step = clock[1];
+  bool ended = scen_end && batch_step >= scen_end[s];
   scen_units[s] = ended ? 0 : (int64_t)st[s].units_per_part * scen_active[s];
 }
 
 // a general-mode rerun starts from the step's input: undo its deletion marks
 __global__ void __launch_bounds__(BLK) k_undelete(int* inactive, const int* del_step, int n, int step_id,
-                                                  const int* cond) {
+                                                  const int* cond, const int* clock) {
   if (cond && *cond == 0) return;
+  if (clock) step_id = clock[0];
   int j = blockIdx.x * BLK + threadIdx.x;
   if (j < n && inactive[j] == 2 && del_step[j] == step_id) inactive[j] = 0;
+}
+
+// fill a device int array (graph-capturable replacement of a memset)
+__global__ void __launch_bounds__(BLK) k_fill_int(int* p, int n, int v) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j < n) p[j] = v;
 }
 
 __global__ void k_clear_flag(int* flag, int idx) {

@@ -160,10 +160,12 @@ int cyl_batch_reshape(cyl_ctx *ctx, int nkeep, const int *keep, int nnew, const 
 
 /* Tuning (no counterpart in the reference): element re-sort interval in steps
  * (0 = never), the resident 256-thread blocks per CU of the persistent grids
- * (0 = as many as fit), and whether the 2D fused kernels sample their normals
- * inline (1) or read them from the pool-order generator (0).  Results do not
- * depend on any of them. */
-int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int blocks_per_cu, int inline_normals);
+ * (0 = as many as fit), whether the 2D fused kernels sample their normals
+ * inline (1) or read them from the pool-order generator (0), and whether
+ * multi-step calls replay a captured HIP graph of 2 * sort_interval steps
+ * (frozen fields, uniform draw assignment, no boundary statistics; otherwise
+ * the steps are enqueued one by one).  Results do not depend on any of them. */
+int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int blocks_per_cu, int inline_normals, int use_graph);
 /* Timing of the last cyl_step: summed duration of the fused step-kernel
  * launches (HIP events on the context's stream), wall time of the call, and
  * the number of timed launches. */
@@ -171,8 +173,9 @@ int cyl_last_timing(cyl_ctx *ctx, double *step_kernel_ms, double *total_ms, int 
 /* Which kernels advanced the particles in the last cyl_step / cyl_batch_step
  * call (no counterpart in the reference): launches of the fused kernels and of
  * the generic kernel that did the step (guarded reruns that found nothing to
- * do are not counted). */
-int cyl_last_path(cyl_ctx *ctx, int *fused_launches, int *generic_launches);
+ * do are not counted; steps inside graph replays are not counted), and the
+ * number of steps run by graph replays. */
+int cyl_last_path(cyl_ctx *ctx, int *fused_launches, int *generic_launches, int *graph_steps);
 /* Summed duration of the draw-start pass that precedes the fused 2D LSM-1
  * step kernel in the same timed steps (0 when the generic kernel ran). */
 int cyl_last_prep_timing(cyl_ctx *ctx, double *prep_ms);

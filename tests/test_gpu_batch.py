@@ -158,3 +158,24 @@ def test_continuous_batching_equals_alone(rng_method):
     for sid, (p, n, k, join) in enumerate(spec):
         alone = run_alone(case, make_pset(case, n, 300 + sid), dict(params[sid], initial_time=join * dt), st[sid], k)
         compare(done[sid][0], alone, done[sid][1])
+
+
+def test_batch_graph_replay_equals_alone():
+    """Frozen fields: the batch replays captured graphs of its step loop; every
+    scenario still equals its plain run alone."""
+    case = vortex_case(npart=4000)
+    base = dict(final_time=1e9, time_step=5., rng_method=2, frozen_eulerian_fields=True)
+    params = [dict(base, **p) for p, _, _ in MIX]
+    nsteps = [k + 20 for _, _, k in MIX]
+    st = streams(len(MIX))
+    psets = [make_pset(case, n, 100 + i) for i, (_, n, _) in enumerate(MIX)]
+    mesh = cy.TriangularMesh(case.x, case.y, case.tri)
+    fs = cy.EulerianFieldSet(mesh, trifinder=case.trifinder(), **case.fields)
+    b = cy.BatchSolver(fs, psets, params, st, nsteps=nsteps, sort_interval=3, use_graph=True)
+    b.run()
+    assert b.ctx.last_graph_steps() > 0
+    out = b.results()
+    for s, (p, n, k) in enumerate(MIX):
+        alone = run_alone(case, make_pset(case, n, 100 + s), params[s], st[s], nsteps[s])
+        compare(out[s], alone, b.rng_state(s))
+    b.ctx.close()

@@ -205,7 +205,7 @@ def test_element_resort_does_not_change_results(which):
         if which == "keps_rk2_generic":
             params["time_scheme"] = 2
     s = case.build_gpu(cy, params)
-    s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 0, 0)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 0, 0, 0)
     s.solve()
     r = case.run_oracle(R, params)
     ps = s.particle_set
@@ -246,10 +246,38 @@ def test_inline_normals_keps_two_frames(rng_method):
     case = vortex_case(npart=3000)
     params = dict(final_time=200., time_step=5., model=1, diffusion_model=2, rng_method=rng_method)
     s = case.build_gpu(cy, params)
-    s.ctx.lib.cyl_set_tuning(s.ctx.h, 4, 0, 1)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 4, 0, 1, 0)
     s.solve()
     ps = s.particle_set
     g = SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(), tri=ps.tri.copy(),
                         inactive=ps.inactive.copy(), lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy())
     assert s.ctx.last_path()[0] > 0
+    assert_parity(g, case.run_oracle(R, params))
+
+
+GRAPH_CASES = [
+    ("2d_lsm1", lambda: vortex_case(npart=3000), dict(model=1, diffusion_model=2)),
+    ("2d_lsm2", lambda: vortex_case(npart=2000), dict(model=2, time_scheme=5, diffusion_model=2,
+                                                      diffusion_lsm2_option=1, particle_velocity_init=1)),
+    ("3d_lsm1", lambda: vortex_3d_case(), dict(model=1, diffusion_model=3)),
+    ("3d_lsm2", lambda: vortex_3d_case(), dict(model=2, time_scheme=5, diffusion_model=3,
+                                               diffusion_lsm2_option=1, particle_velocity_init=1)),
+]
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+@pytest.mark.parametrize("which", range(len(GRAPH_CASES)), ids=[c[0] for c in GRAPH_CASES])
+def test_graph_replay(which, rng_method):
+    """Multi-step calls replayed from a captured HIP graph (device-side step
+    counters, a period of two re-sorts) give the oracle's results."""
+    name, mk, p = GRAPH_CASES[which]
+    case = mk()
+    params = dict(final_time=300., time_step=5., frozen_eulerian_fields=True, rng_method=rng_method, **p)
+    s = case.build_gpu(cy, params)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 0, 0, 1)
+    s._run(s._steps_left())
+    assert s.ctx.last_graph_steps() >= 2 * 6
+    ps = s.particle_set
+    g = SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(), tri=ps.tri.copy(),
+                        inactive=ps.inactive.copy(), lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy())
     assert_parity(g, case.run_oracle(R, params))
