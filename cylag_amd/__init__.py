@@ -18,6 +18,11 @@ from .solver import Solver  # noqa: F401
 from .batch import BatchSolver  # noqa: F401
 from .serve import Request, ServeLoop, latency_stats  # noqa: F401
 from .tune import autotune  # noqa: F401
+from .sources import (AlwaysScheduler, CountScheduler, IterationDeltaScheduler, IterationScheduler,  # noqa: F401
+                      ParticleSource, ParticleSourceFromField, Scheduler, TimeDeltaScheduler, TimeScheduler,
+                      circle, init_positions_2d, init_positions_3d, polygon_area, schedules)
+from .telemac import SelafinFile  # noqa: F401
+from .particles_io import ParticlesIO  # noqa: F401
 from .random_utils import (pcg32_seed, pcg32_advance, random_uniform, random_gaussian,  # noqa: F401
                            random_gaussian_pair, random_gaussian_ziggurat,
                            random_gaussian_ziggurat_pair, generate_random_pair,

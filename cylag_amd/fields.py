@@ -70,6 +70,48 @@ class EulerianFieldSet:
         self.friction_model = friction_model
         self.friction_coefficient = friction_coefficient
 
+    @staticmethod
+    def from_telemac2d(file_name, main_var_names=['FREE SURFACE', 'VELOCITY U', 'VELOCITY V'], bnd_file=None,
+                       optional_fields=None, init_trifinder=True, friction_model=1, friction_coefficient=40.):
+        """Field set of a TELEMAC-2D result file, times relative to its first
+        record (eulerian_field_set.pyx:317-398)."""
+        return EulerianFieldSet._from_telemac(file_name, 2, main_var_names, bnd_file, optional_fields,
+                                              init_trifinder, friction_model, friction_coefficient)
+
+    @staticmethod
+    def from_telemac3d(file_name, main_var_names=['ELEVATION Z', 'VELOCITY U', 'VELOCITY V', 'VELOCITY W'],
+                       bnd_file=None, optional_fields=None, init_trifinder=True, friction_model=1,
+                       friction_coefficient=40.):
+        """Field set of a TELEMAC-3D result file (plane-major fields, the
+        bottom plane's triangulation; eulerian_field_set.pyx:400-483)."""
+        return EulerianFieldSet._from_telemac(file_name, 3, main_var_names, bnd_file, optional_fields,
+                                              init_trifinder, friction_model, friction_coefficient)
+
+    @staticmethod
+    def _from_telemac(file_name, dim, names, bnd_file, optional_fields, init_trifinder, friction_model,
+                      friction_coefficient):
+        from .mesh import TriangularMesh, read_cli
+        from .telemac import SelafinFile
+        res = SelafinFile(file_name)
+        if bnd_file is not None:
+            bnd_pts, nopen = read_cli(bnd_file)
+        else:
+            bnd_pts, nopen = None, 0
+        tri = res.tri
+        mesh = TriangularMesh.from_matplotlib(tri, boundary_points=bnd_pts, nopen=nopen)
+        trifinder = tri.get_trifinder() if init_trifinder else None
+        times = res.times - res.times[0]
+        niter = res.ntimestep
+        get = lambda name: np.stack([res.get_data_value(name, k) for k in range(niter)])
+        elev, velx, vely = get(names[0]), get(names[1]), get(names[2])
+        velz = get(names[3]) if dim == 3 else None
+        opt = None
+        if optional_fields is not None:
+            opt = np.stack([get(name) for name in optional_fields])
+        return EulerianFieldSet(mesh, dim, res.nplan if dim == 3 else 1, times, elev, velx, vely, velz, opt,
+                                optional_fields, trifinder, friction_model=friction_model,
+                                friction_coefficient=friction_coefficient)
+
     def check_dim(self, field):
         ref_shape = (self.ntimes, self.npoints) if self.dim == 2 else \
             (self.ntimes, self.npoints * self.nlayers)

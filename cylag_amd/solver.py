@@ -12,6 +12,7 @@ import ctypes as C
 import numpy as np
 
 from . import _hip, random_utils
+from .particles_io import prepare_output, write_particles
 from .parameters import Parameters
 from .simutime import SimuTime
 
@@ -124,6 +125,9 @@ class Solver:
         ps._solver = self
         self._push(init=True)
         self._deleted_rows_to_tags()
+        if p.output_file:  # output directory, header and the initial record (solver.pyx:200-225)
+            prepare_output(ps, p.output_rep, p.output_file_format, p.output_file_name, p.model)
+            write_particles(ps, self.simutime.time, 0, p)
         nopen = fs.triangular_mesh.nopen
         nt_buf = self.simutime.nt
         self._bnd_stats_times = np.zeros(nt_buf)
@@ -146,7 +150,10 @@ class Solver:
         return it
 
     def _push(self, init=False):
-        """Upload the host rows (and initialize tri == -2 rows on the device)."""
+        """Upload the host rows.  init: initialize the tri == -2 rows now (the
+        constructor, initialize_particle_state); otherwise the next step
+        initializes each of them right before updating it, in pool order, as
+        forward() does for arrivals (solver.pyx:458-467)."""
         ps = self.particle_set
         h = ps._h
         P = _hip.ptr
@@ -163,7 +170,7 @@ class Solver:
             "cyl_particles_upload")
         self._n_deleted_seen = 0
         ps._host_dirty = False
-        if len(new):
+        if init and len(new):
             self._rng_to_device()
             self.ctx.check(self.ctx.lib.cyl_init_particles(
                 self.ctx.h, C.byref(self._par), float(self.simutime.time),
@@ -227,7 +234,7 @@ class Solver:
         ps = self.particle_set
         ps._sync()
         ps._h["tri"][i] = -2
-        self._push()
+        self._push(init=True)
 
     def _run(self, nsteps):
         ps, st, p = self.particle_set, self.simutime, self.parameters
@@ -262,18 +269,67 @@ class Solver:
 
     def forward(self, time_step=None):
         """One time step (solver.pyx:413-585); time_step is unused, as in the reference."""
-        self._release_sources()
+        deferred = self._release_sources()
         self._run(1)
+        self._activate(deferred)
+        self._output_if_due()
+
+    def _activate(self, deferred):
+        """Arrivals that grew the pool join from the next step on."""
+        if deferred is None:
+            return
+        rows, keep = deferred
+        ps = self.particle_set
+        ps._sync()
+        h = ps._h
+        for k, v in keep.items():
+            h[k][rows] = v
+        h["inactive"][rows] = 0
+        h["tri"][rows] = -2
+        ps.npart_active += len(rows)
+        ps._host_dirty = True
+
+    def _output_if_due(self):
+        """Particle output every output_printout_period iterations (solver.pyx:556-565)."""
+        p, st = self.parameters, self.simutime
+        if p.output_file and st.iteration % p.output_printout_period == 0:
+            write_particles(self.particle_set, st.time, st.iteration, p)
 
     def _release_sources(self):
+        """Release the sources due at this step.  forward() loops over the rows
+        the pool had when the step began (solver.pyx:432,458), so arrivals that
+        grow the pool are neither initialized nor stepped before the next step:
+        their rows are returned and kept inactive for this one."""
         if self.sources is None:
-            return
+            return None
+        ps = self.particle_set
+        n0 = ps.npart
+        fs = self.field_set
+        if fs.ntimes > 1:  # current time record, for sources that read the fields
+            t, rec = fs.times, 0
+            for i in range(fs.ntimes - 1):
+                if t[i] <= self.simutime.time < t[i + 1]:
+                    rec = i
+                    break
+            else:
+                rec = fs.ntimes - 2 if self.simutime.time >= t[-1] else 0
+            fs.record = rec
         for isrc, source in enumerate(self.sources, start=1):
             if source.release_condition(self.simutime, self.field_set):
                 new = source.init_positions(self.field_set, self.particle_set.dim)
                 self.particle_set.add(new)
                 if self.parameters.listing:
                     print(' ~~> Adding {} particles from source {}'.format(np.shape(new)[0], isrc))
+        if ps.npart > n0:
+            h = ps._h
+            rows = n0 + np.nonzero(h["inactive"][n0:] == 0)[0]
+            keep = {k: h[k][rows].copy() for k in ("position", "velocity", "fluid_velocity_seen", "diameter",
+                                                   "volume")}
+            h["inactive"][rows] = 1
+            ps.npart_active -= len(rows)
+            ps._host_dirty = True
+            return rows, keep
+        return None
 
     def _steps_left(self):
         """Number of forward() calls solve() makes: its loop runs nt times and
@@ -294,13 +350,19 @@ class Solver:
         """All remaining steps (solver.pyx:382-411), in device-resident batches."""
         st = self.simutime
         remaining = self._steps_left()
+        p = self.parameters
         while remaining > 0:
+            deferred = None
             if self.sources is not None:
-                self._release_sources()
+                deferred = self._release_sources()
                 n = 1
             else:
                 n = remaining if batch is None else min(batch, remaining)
+            if p.output_file:  # stop at the next output iteration
+                n = min(n, p.output_printout_period - st.iteration % p.output_printout_period)
             self._run(n)
+            self._activate(deferred)
+            self._output_if_due()
             remaining -= n
         self.flush_statistics()
 

@@ -8,7 +8,7 @@ import pytest
 
 import cylag_amd as cy
 from oracle import refpy as R
-from tests.cases import (open_channel_case, reference_square_case, square_case, vortex_3d_case,
+from tests.cases import (SEED, open_channel_case, reference_square_case, square_case, vortex_3d_case,
                          vortex_case)
 from tests.test_oracle_golden import (GOLDEN_3D, GOLDEN_3D_IDS, LSM2_GOLDEN, P4, golden_3d_setup,
                                       lsm2_params)
@@ -281,3 +281,41 @@ def test_graph_replay(which, rng_method):
     g = SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(), tri=ps.tri.copy(),
                         inactive=ps.inactive.copy(), lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy())
     assert_parity(g, case.run_oracle(R, params))
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+@pytest.mark.parametrize("model", [1, 2])
+def test_arrivals_midrun(model, rng_method):
+    """Rows added mid-run (first-fit into inactive rows) are initialized by the
+    next step right before their own update, in pool order, as forward() does
+    (solver.pyx:458-467): with LSM-2's initial velocity fluctuations the
+    initialization draws interleave with the other rows' step draws."""
+    case = vortex_case(npart=1200)
+    case.pool = 1800
+    p = dict(model=1, diffusion_model=2) if model == 1 else dict(
+        model=2, time_scheme=5, diffusion_model=2, diffusion_lsm2_option=2, particle_velocity_init=1)
+    params = dict(final_time=300., time_step=5., rng_method=rng_method, **p)
+    s = case.build_gpu(cy, params)
+    mesh = R.RefMesh(case.x, case.y, case.tri, boundary_points=case.boundary_points, nopen=case.nopen)
+    fs = R.RefFields(mesh, **{k: v for k, v in case.fields.items() if k != "trifinder"})
+    ps = R.RefParticles(case.position, case.velocity, dim=2, initial_pool_size=case.pool, **case.pset_kw)
+    rng = R.RefRng(*SEED)
+    o = R.RefSolver(fs, ps, params, trifinder=case.trifinder(), rng=rng)
+    rs = np.random.default_rng(9)
+    for k in range(3):
+        for _ in range(5):
+            s.forward()
+        o.forward(5)
+        new = np.stack([rs.uniform(100., 1900., 150), rs.uniform(50., 450., 150)], 1)
+        s.particle_set.add(new)
+        o.add(new)
+    for _ in range(5):
+        s.forward()
+    o.forward(5)
+    g = s.particle_set
+    out = SimpleNamespace(position=g.position.copy(), velocity=g.velocity.copy(), tri=g.tri.copy(),
+                          inactive=g.inactive.copy(), lowerlayer=g.lowerlayer.copy(), tag=g.tag.copy())
+    ref = SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(), tri=ps.tri.copy(),
+                          inactive=ps.inactive.copy(), lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy(),
+                          rng=rng)
+    assert_parity(out, ref)
