@@ -460,7 +460,7 @@ def plain_leg(cy, args, mesh, fields, pos, params):
     cy.pcg32_seed(42, 0xda3e06d11bb2b203)
     t0 = time.time()
     solver = cy.Solver(fs, ps, params)
-    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0, int(args.inline_normals))
+    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0, int(args.inline_normals), 0)
     log("solver ready ({:.1f} s): {}".format(time.time() - t0, solver.ctx.device_info()[0]))
     solver._run(args.warmup)
     log("warmup done")

@@ -23,6 +23,8 @@ from .sources import (AlwaysScheduler, CountScheduler, IterationDeltaScheduler, 
                       circle, init_positions_2d, init_positions_3d, polygon_area, schedules)
 from .telemac import SelafinFile  # noqa: F401
 from .particles_io import ParticlesIO  # noqa: F401
+from .control_shapes import ControlSection  # noqa: F401
+from .write_statistics import prepare_secstats_output, write_secstats_buffer  # noqa: F401
 from .random_utils import (pcg32_seed, pcg32_advance, random_uniform, random_gaussian,  # noqa: F401
                            random_gaussian_pair, random_gaussian_ziggurat,
                            random_gaussian_ziggurat_pair, generate_random_pair,

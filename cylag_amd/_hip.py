@@ -70,6 +70,8 @@ SIGNATURES = {
     "cyl_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
     "cyl_last_prep_timing": (C.c_int, [C.c_void_p, _dp]),
     "cyl_last_path": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    "cyl_control_sections_set": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
+    "cyl_control_counts": (C.c_int, [C.c_void_p, _ip, C.c_int, _ip]),
     "cyl_scenarios_set": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), _ip]),
     "cyl_scenario_rng_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),

@@ -154,6 +154,13 @@ struct cyl_ctx {
   int use_graph = 0;      // replay multi-step loops from a captured HIP graph
   int* d_clock = nullptr; // graph replays: {step id, batch step} on the device
   int last_graph_steps = 0;
+  // control sections: segments (sections ordered), per-step counts of the last call
+  int nsec = 0, nsecseg = 0;
+  double4* d_sec_seg = nullptr;
+  int* d_sec_of = nullptr;
+  int* d_sec_counts = nullptr;
+  size_t sec_cap = 0;
+  std::vector<int> last_sec_counts;
   int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
   int64_t* d_cursor = nullptr;
   int64_t* d_sort_tiles = nullptr;
@@ -292,8 +299,8 @@ extern "C" int cyl_ctx_destroy(cyl_ctx* ctx) {
   ctx->part_bufs.free_all();
   ctx->ev_bufs.free_all();
   for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
-  void* singles[] = {ctx->d_zt,          ctx->d_flag,      ctx->d_n_deleted, ctx->d_n_deleted_total,
-                     ctx->d_step_done, ctx->d_bnd, ctx->d_clock};
+  void* singles[] = {ctx->d_zt,        ctx->d_flag, ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->d_step_done,
+                     ctx->d_bnd,       ctx->d_clock, ctx->d_sec_seg, ctx->d_sec_of,       ctx->d_sec_counts};
   for (void* p : singles)
     if (p) (void)hipFree(p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -2006,7 +2013,16 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   HIPCHK(hipMemsetAsync(ctx->d_bnd, 0, need_bnd * sizeof(int), ctx->stream));
   HIPCHK(hipMemsetAsync(ctx->d_err, 0xff, ns * sizeof(unsigned long long), ctx->stream));
   HIPCHK(hipMemsetAsync(ctx->d_flag + 2, 0, sizeof(int), ctx->stream));
-  bool any_stats = false;
+  if (ctx->nsecseg > 0) {
+    size_t need = (size_t)std::max(nsteps, 1) * ctx->nsec;
+    if (need > ctx->sec_cap) {
+      if (ctx->d_sec_counts) HIPCHK(hipFree(ctx->d_sec_counts));
+      HIPCHK(hipMalloc(&ctx->d_sec_counts, need * sizeof(int)));
+      ctx->sec_cap = need;
+    }
+    HIPCHK(hipMemsetAsync(ctx->d_sec_counts, 0, need * sizeof(int), ctx->stream));
+  }
+  bool any_stats = ctx->nsecseg > 0;  // per-step counts: no graph replay
   for (int s = 0; s < ns; ++s) any_stats |= P[s].bnd_statistics != 0;
   StepArgs a{};
   a.m = ctx->mesh;
@@ -2202,6 +2218,14 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       }
       ++launches;
     }
+    if (ctx->nsecseg > 0 && !init_only) {
+      const PoolDev& PA = ctx->pool[ctx->cur];
+      const PoolDev& PB = ctx->pool[1 - ctx->cur];
+      k_control_sections<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(
+          PA.x, PA.y, PB.x, PB.y, ctx->pool[0].inactive, ctx->d_del_step, ctx->step_counter, ctx->n, ctx->d_sec_seg,
+          ctx->d_sec_of, ctx->nsecseg, ctx->d_sec_counts + (size_t)step * ctx->nsec);
+      ++launches;
+    }
     k_finish<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_stream_end, ns, ctx->d_flag,
                                                 ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->step_counter,
                                                 ctx->d_step_done, ctx->d_err_step, ctx->d_err,
@@ -2310,6 +2334,12 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
           bnd_out[((size_t)s * ctx->nscen + sc) * ctx->nopen + q] =
               h[(size_t)s * nstat + (size_t)sc * std::max(ctx->nopen, 1) + q];
   }
+  if (ctx->nsecseg > 0 && !init_only) {
+    ctx->last_sec_counts.assign((size_t)nsteps * ctx->nsec, 0);
+    if (nsteps > 0)
+      HIPCHK(hipMemcpy(ctx->last_sec_counts.data(), ctx->d_sec_counts,
+                       (size_t)nsteps * ctx->nsec * sizeof(int), hipMemcpyDeviceToHost));
+  }
   int flags[4];
   HIPCHK(hipMemcpy(flags, ctx->d_flag, 4 * sizeof(int), hipMemcpyDeviceToHost));
   std::vector<unsigned long long> errs(ctx->nscen);
@@ -2379,6 +2409,43 @@ extern "C" int cyl_last_timing(cyl_ctx* ctx, double* kms, double* tms, int* laun
   if (kms) *kms = ctx->last_step_kernel_ms;
   if (tms) *tms = ctx->last_total_ms;
   if (launches) *launches = ctx->last_step_launches;
+  return CYL_OK;
+}
+
+extern "C" int cyl_control_sections_set(cyl_ctx* ctx, int nsec, const int* nseg_of, const double* seg) {
+  if (!ctx || nsec < 0 || (nsec > 0 && (!nseg_of || !seg))) return CYL_E_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  std::vector<double4> sg;
+  std::vector<int> of;
+  int q = 0;
+  for (int s = 0; s < nsec; ++s) {
+    if (nseg_of[s] < 0) return CYL_E_ARG;
+    for (int k = 0; k < nseg_of[s]; ++k, ++q) {
+      sg.push_back(make_double4(seg[4 * q], seg[4 * q + 1], seg[4 * q + 2], seg[4 * q + 3]));
+      of.push_back(s);
+    }
+  }
+  if (ctx->d_sec_seg) HIPCHK(hipFree(ctx->d_sec_seg));
+  if (ctx->d_sec_of) HIPCHK(hipFree(ctx->d_sec_of));
+  ctx->d_sec_seg = nullptr;
+  ctx->d_sec_of = nullptr;
+  ctx->nsec = nsec;
+  ctx->nsecseg = (int)sg.size();
+  ctx->last_sec_counts.clear();
+  if (ctx->nsecseg > 0) {
+    HIPCHK(hipMalloc(&ctx->d_sec_seg, sg.size() * sizeof(double4)));
+    HIPCHK(hipMalloc(&ctx->d_sec_of, of.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(ctx->d_sec_seg, sg.data(), sg.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->d_sec_of, of.data(), of.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  return CYL_OK;
+}
+
+extern "C" int cyl_control_counts(cyl_ctx* ctx, int* out, int cap, int* nout) {
+  if (!ctx || !nout) return CYL_E_ARG;
+  *nout = (int)ctx->last_sec_counts.size();
+  for (int i = 0; i < *nout && i < cap && out; ++i) out[i] = ctx->last_sec_counts[i];
   return CYL_OK;
 }
 

@@ -542,6 +542,34 @@ __global__ void __launch_bounds__(BLK) k_undelete(int* inactive, const int* del_
   if (j < n && inactive[j] == 2 && del_step[j] == step_id) inactive[j] = 0;
 }
 
+// Control sections (solver.pyx:540-549, control_shapes.pyx:56-85): a row that
+// was active at the start of the step is counted once per section any of
+// whose segments its step path [last position, position] meets.  A row deleted
+// during the step has both positions zeroed by delete_single, so its path is
+// the origin.  seg[q] = (x0, y0, x1, y1) of segment q, sec_of[q] its section.
+__global__ void __launch_bounds__(BLK) k_control_sections(const double* ax, const double* ay, const double* bx,
+                                                          const double* by, const int* inactive, const int* del_step,
+                                                          int step_id, int n, const double4* seg, const int* sec_of,
+                                                          int nseg, int* counts) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= n) return;
+  int ina = inactive[j];
+  bool deleted_now = ina == 2 && del_step[j] == step_id;
+  if (ina != 0 && !deleted_now) return;
+  double x0 = deleted_now ? 0. : ax[j], y0 = deleted_now ? 0. : ay[j];
+  double x1 = deleted_now ? 0. : bx[j], y1 = deleted_now ? 0. : by[j];
+  int last = -1;
+  for (int q = 0; q < nseg; ++q) {
+    int sc = sec_of[q];
+    if (sc == last) continue;  // this section already counted the row
+    double4 g = seg[q];
+    if (seg_intersect(x0, y0, x1, y1, g.x, g.y, g.z, g.w)) {
+      atomicAdd(&counts[sc], 1);
+      last = sc;
+    }
+  }
+}
+
 // fill a device int array (graph-capturable replacement of a memset)
 __global__ void __launch_bounds__(BLK) k_fill_int(int* p, int n, int v) {
   int j = blockIdx.x * BLK + threadIdx.x;

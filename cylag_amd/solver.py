@@ -12,7 +12,9 @@ import ctypes as C
 import numpy as np
 
 from . import _hip, random_utils
+from .control_shapes import ControlSection, section_segments
 from .particles_io import prepare_output, write_particles
+from .write_statistics import prepare_secstats_output, write_secstats_buffer
 from .parameters import Parameters
 from .simutime import SimuTime
 
@@ -94,6 +96,9 @@ class Solver:
         p = self.parameters
         self.simutime = SimuTime(p.listing, p.initial_time, p.final_time, p.time_step)
         self.sources = sources
+        if control_sections is not None:
+            control_sections = [c if isinstance(c, ControlSection) else ControlSection(c)
+                                for c in control_sections]
         self.control_sections = control_sections
         p.control_sections = control_sections is not None
         p.ncsec = len(control_sections) if control_sections is not None else 0
@@ -111,6 +116,11 @@ class Solver:
         self.ctx = _hip.Context(device)
         upload_mesh(self.ctx, fs.triangular_mesh)
         upload_fields(self.ctx, fs)
+        if control_sections:
+            nseg_of, seg = section_segments(control_sections)
+            self.ctx.check(self.ctx.lib.cyl_control_sections_set(self.ctx.h, len(control_sections),
+                                                                 _hip.ptr(nseg_of, "i"), _hip.ptr(seg)),
+                           "cyl_control_sections_set")
         self.ctx.check(self.ctx.lib.cyl_fields_reset(self.ctx.h, int(fs.frozen_hydro),
                                                      float(p.initial_time)), "cyl_fields_reset")
         if not fs.frozen_hydro:
@@ -127,6 +137,11 @@ class Solver:
         self._deleted_rows_to_tags()
         if p.output_file:  # output directory, header and the initial record (solver.pyx:200-225)
             prepare_output(ps, p.output_rep, p.output_file_format, p.output_file_name, p.model)
+        if p.bnd_statistics or p.control_sections:
+            prepare_secstats_output(p.bnd_statistics, fs.triangular_mesh.nopen, p.output_rep,
+                                    p.bnd_statistics_file_name, p.control_sections, p.ncsec,
+                                    p.control_sections_file_name)
+        if p.output_file:
             write_particles(ps, self.simutime.time, 0, p)
         nopen = fs.triangular_mesh.nopen
         nt_buf = self.simutime.nt
@@ -137,6 +152,7 @@ class Solver:
         self._csec_stats_times = np.zeros(nt_buf)
         self._csec_stats_data = np.zeros((nt_buf, max(1, p.ncsec)), dtype=np.int32)
         self._csec_stats_count = 0
+        self._csec_flushed = []
 
     # -- host <-> device ----------------------------------------------------------
     def _init_tri(self, rows):
@@ -261,6 +277,16 @@ class Solver:
                 self._bnd_stats_times[self._bnd_stats_count] = times[s]
                 self._bnd_stats_data[self._bnd_stats_count, :nopen] = stats[s * nopen:(s + 1) * nopen]
                 self._bnd_stats_count += 1
+        if self.control_sections:
+            nc = p.ncsec
+            counts = np.zeros(max(1, nsteps * nc), dtype=np.int32)
+            nout = C.c_int()
+            self.ctx.check(self.ctx.lib.cyl_control_counts(self.ctx.h, _hip.ptr(counts, "i"), counts.size,
+                                                           C.byref(nout)), "cyl_control_counts")
+            for s in range(nsteps):
+                self._csec_stats_times[self._csec_stats_count] = times[s]
+                self._csec_stats_data[self._csec_stats_count, :nc] = counts[s * nc:(s + 1) * nc]
+                self._csec_stats_count += 1
         if st.print_progress:
             for it in range(st.iteration - nsteps + 1, st.iteration + 1):
                 if p.listing_printout_period and it % p.listing_printout_period == 0:
@@ -367,14 +393,34 @@ class Solver:
         self.flush_statistics()
 
     def flush_statistics(self):
-        """Move the buffered per-step statistics to the flushed record
-        (solver.pyx:587-630 writes them to files and resets the buffers)."""
+        """Write the buffered per-step statistics to their files and reset the
+        buffers (solver.pyx:587-630); the flushed rows stay readable through
+        bnd_stats and csec_stats."""
+        p = self.parameters
         n = self._bnd_stats_count
         if n:
+            if p.bnd_statistics:
+                write_secstats_buffer(self._bnd_stats_times, self._bnd_stats_data, n,
+                                      self.field_set.triangular_mesh.nopen, p.output_rep,
+                                      p.bnd_statistics_file_name)
             self._bnd_flushed.append((self._bnd_stats_times[:n].copy(),
                                       self._bnd_stats_data[:n].copy()))
         self._bnd_stats_count = 0
+        n = self._csec_stats_count
+        if n:
+            write_secstats_buffer(self._csec_stats_times, self._csec_stats_data, n, p.ncsec, p.output_rep,
+                                  p.control_sections_file_name)
+            self._csec_flushed.append((self._csec_stats_times[:n].copy(),
+                                       self._csec_stats_data[:n].copy()))
         self._csec_stats_count = 0
+
+    @property
+    def csec_stats(self):
+        """(times, counts[nstep, ncsec]) of every step run so far."""
+        parts = self._csec_flushed + [(self._csec_stats_times[:self._csec_stats_count],
+                                       self._csec_stats_data[:self._csec_stats_count])]
+        return (np.concatenate([t for t, _ in parts]),
+                np.concatenate([d for _, d in parts])[:, :max(self.parameters.ncsec, 1)])
 
     @property
     def bnd_stats(self):

@@ -170,6 +170,13 @@ int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int blocks_per_cu, int inlin
  * launches (HIP events on the context's stream), wall time of the call, and
  * the number of timed launches. */
 int cyl_last_timing(cyl_ctx *ctx, double *step_kernel_ms, double *total_ms, int *kernel_launches);
+/* Control sections (solver.pyx:540-549, control_shapes.pyx:56-85): section s
+ * owns nseg_of[s] consecutive segments (x0, y0, x1, y1) of seg[].  Each step
+ * counts, per section, the rows active at its start whose path [last
+ * position, position] meets one of the section's segments (nsec = 0 clears). */
+int cyl_control_sections_set(cyl_ctx *ctx, int nsec, const int *nseg_of, const double *seg);
+/* Counts of the last cyl_step call, [nsteps][nsec] (nout = nsteps * nsec). */
+int cyl_control_counts(cyl_ctx *ctx, int *out, int cap, int *nout);
 /* Which kernels advanced the particles in the last cyl_step / cyl_batch_step
  * call (no counterpart in the reference): launches of the fused kernels and of
  * the generic kernel that did the step (guarded reruns that found nothing to

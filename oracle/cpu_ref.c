@@ -1438,6 +1438,17 @@ int ref_initialize_particle_state(ref_solver *s, int i)
     return REF_OK;
 }
 
+static int csec_cross(const ref_solver *s, int c, double x0, double y0, double x1, double y1)
+{   /* ControlSection.c_cross, control_shapes.pyx:56-85 */
+    const double *g = s->csec_seg;
+    for (int q = 0; q < c; q++) g += 4 * s->csec_nseg[q];
+    for (int k = 0; k < s->csec_nseg[c]; k++, g += 4) {
+        double p0[2] = {x0, y0}, p1[2] = {x1, y1}, s0[2] = {g[0], g[1]}, s1[2] = {g[2], g[3]};
+        if (ref_intersec2d(p0, p1, s0, s1)) return 1;
+    }
+    return 0;
+}
+
 int ref_forward(ref_solver *s)
 {   /* forward, solver.pyx:413-585 (time interpolation + particle loop + increment) */
     ref_pset *p = s->pset;
@@ -1445,6 +1456,8 @@ int ref_forward(ref_solver *s)
     int npart = p->npart, i, bnd_j, e;
     if (s->bnd_stats)
         for (i = 0; i < s->mesh->nopen; i++) s->bnd_stats[i] = 0;
+    if (s->csec_stats)
+        for (i = 0; i < s->ncsec; i++) s->csec_stats[i] = 0;
     e = ref_time_interpolation(s->fset, s->time);
     if (e) { s->err_code = e; s->err_particle = -1; return e; }
     for (i = 0; i < npart; i++) {
@@ -1474,6 +1487,12 @@ int ref_forward(ref_solver *s)
                                     p->position[i * 3 + 1], &zs);
             if (e) { s->err_code = e; s->err_particle = i; return e; }
             p->depth[i] = zs - p->position[i * 3 + 2];
+        }
+        /* control sections, solver.pyx:540-549 (a row deleted above has both
+         * positions zeroed by delete_single) */
+        for (int c = 0; c < s->ncsec && s->csec_stats; c++) {
+            const double *a = p->last_position + i * p->dim, *b = p->position + i * p->dim;
+            if (csec_cross(s, c, a[0], a[1], b[0], b[1])) s->csec_stats[c] += 1;
         }
     }
     /* SimuTime.increment, simutime.pyx:77-85 */

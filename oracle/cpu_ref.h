@@ -173,6 +173,13 @@ typedef struct {
     const int *init_tri;      /* per-particle trifinder result for tri==-2 particles, or NULL */
     int has_trifinder;
     int err_code, err_particle;
+    /* control sections (control_shapes.pyx): ncsec sections, section c owns
+     * csec_nseg[c] consecutive segments (x0, y0, x1, y1) of csec_seg; per-step
+     * crossing counts in csec_stats (ncsec, zeroed per forward) */
+    int ncsec;
+    const int *csec_nseg;
+    const double *csec_seg;
+    int *csec_stats;
 } ref_solver;
 
 int ref_initialize_particle_state(ref_solver *s, int i);

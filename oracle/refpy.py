@@ -98,7 +98,8 @@ class SolverS(C.Structure):
                 ("pset", C.POINTER(PsetS)), ("par", C.POINTER(ParS)), ("rng", Pcg),
                 ("time", C.c_double), ("time_step", C.c_double), ("iteration", C.c_int),
                 ("bnd_stats", _ip), ("init_tri", _ip), ("has_trifinder", C.c_int),
-                ("err_code", C.c_int), ("err_particle", C.c_int)]
+                ("err_code", C.c_int), ("err_particle", C.c_int), ("ncsec", C.c_int),
+                ("csec_nseg", _ip), ("csec_seg", _dp), ("csec_stats", _ip)]
 
 
 def _sig(name, res, args):
@@ -403,7 +404,8 @@ def resolve_params(user):
 class RefSolver:
     """Serial solver over the C restatement; rng is a RefRng (default: module RNG)."""
 
-    def __init__(self, fields, particles, parameters=None, trifinder=None, rng=None):
+    def __init__(self, fields, particles, parameters=None, trifinder=None, rng=None,
+                 control_sections=None):
         self.f, self.p, self.mesh = fields, particles, fields.mesh
         self.params = resolve_params(parameters or {})
         pr = self.params
@@ -438,6 +440,20 @@ class RefSolver:
                          C.pointer(self.par), Pcg(0, 0), pr["initial_time"], pr["time_step"], 0,
                          P(self.bnd_stats, "i"), P(self.init_tri, "i"), int(trifinder is not None),
                          0, 0)
+        # control sections: polygons (npoint, 2); ControlSection counts
+        # poly.shape[1] - 1 segments [poly[k], poly[k + 1]] (control_shapes.pyx:17-19)
+        self.csec_history = []
+        if control_sections is not None:
+            polys = [np.asarray(q, dtype=np.float64) for q in control_sections]
+            self._csec_nseg = np.array([q.shape[1] - 1 for q in polys], np.int32)
+            segs = [[q[k, 0], q[k, 1], q[k + 1, 0], q[k + 1, 1]]
+                    for q in polys for k in range(q.shape[1] - 1)]
+            self._csec_seg = np.array(segs, np.float64).reshape(-1)
+            self.csec_stats = np.zeros(max(1, len(polys)), np.int32)
+            self.s.ncsec = len(polys)
+            self.s.csec_nseg = P(self._csec_nseg, "i")
+            self.s.csec_seg = P(self._csec_seg, "d")
+            self.s.csec_stats = P(self.csec_stats, "i")
         if not frozen:
             e = lib.ref_time_interpolation(C.byref(fs), pr["initial_time"])
             if e:
@@ -479,6 +495,8 @@ class RefSolver:
                     raise ValueError("oracle forward failed: code {} particle {}".format(
                         self.s.err_code, self.s.err_particle))
                 self.bnd_history.append(self.bnd_stats[:self.mesh.nopen].copy())
+                if self.s.ncsec:
+                    self.csec_history.append(self.csec_stats[:self.s.ncsec].copy())
         finally:
             self._sync_out()
 

@@ -67,17 +67,17 @@ class Case:
         return self._mpl.get_trifinder()
 
     # -- product -------------------------------------------------------------------
-    def build_gpu(self, cy, params):
+    def build_gpu(self, cy, params, **solver_kw):
         mesh = cy.TriangularMesh(self.x, self.y, self.tri, boundary_points=self.boundary_points,
                                  nopen=self.nopen)
         fs = cy.EulerianFieldSet(mesh, trifinder=self.trifinder(), **self.fields)
         ps = cy.LagrangianParticleSet(self.position, self.velocity, dim=self.pdim,
                                       initial_pool_size=self.pool, **self.pset_kw)
         cy.pcg32_seed(*SEED)
-        return cy.Solver(fs, ps, params)
+        return cy.Solver(fs, ps, params, **solver_kw)
 
-    def run_gpu(self, cy, params, nsteps=None):
-        s = self.build_gpu(cy, params)
+    def run_gpu(self, cy, params, nsteps=None, **solver_kw):
+        s = self.build_gpu(cy, params, **solver_kw)
         if nsteps is None:
             s.solve()
         else:
@@ -89,7 +89,7 @@ class Case:
                                lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy(), solver=s)
 
     # -- oracle -----------------------------------------------------------------------
-    def run_oracle(self, R, params, nsteps=None):
+    def run_oracle(self, R, params, nsteps=None, **solver_kw):
         mesh = R.RefMesh(self.x, self.y, self.tri, boundary_points=self.boundary_points,
                          nopen=self.nopen)
         f = dict(self.fields)
@@ -98,7 +98,7 @@ class Case:
         ps = R.RefParticles(self.position, self.velocity, dim=self.pdim,
                             initial_pool_size=self.pool, **kw)
         rng = R.RefRng(*SEED)
-        s = R.RefSolver(fs, ps, params, trifinder=self.trifinder(), rng=rng)
+        s = R.RefSolver(fs, ps, params, trifinder=self.trifinder(), rng=rng, **solver_kw)
         if nsteps is None:
             s.solve()
         else:

@@ -83,10 +83,11 @@ def test_batch_equals_alone(rng_method):
     b.ctx.close()
 
 
-def test_batch_deletions_tags_and_statistics():
+def test_batch_deletions_tags_and_statistics(tmp_path):
     """Open boundaries: per-scenario deletions, tag renumbering and statistics."""
     case = open_channel_case()
-    base = dict(final_time=1e9, time_step=1., frozen_eulerian_fields=True, model=1, bnd_statistics=True)
+    base = dict(final_time=1e9, time_step=1., frozen_eulerian_fields=True, model=1, bnd_statistics=True,
+                output_rep=str(tmp_path))
     mix = [(dict(diffusion_model=1, horizontal_diffusivity=2.), 400, 70),
            (dict(diffusion_model=1, horizontal_diffusivity=0.5, stranding_probability=0.3), 300, 40),
            (dict(time_scheme=2, diffusion_model=1, horizontal_diffusivity=1.), 350, 55)]

@@ -175,13 +175,13 @@ def test_large_pool_two_steps(rng_method):
 
 
 @pytest.mark.parametrize("rng_method", [1, 2])
-def test_open_boundary_deletions_and_tags(rng_method):
+def test_open_boundary_deletions_and_tags(rng_method, tmp_path):
     """Particles leave through open boundaries: deletions, tag renumbering,
     boundary statistics and the pool-order random stream after them."""
     case = open_channel_case()
     params = dict(final_time=70., time_step=1., frozen_eulerian_fields=True, model=1,
                   diffusion_model=1, horizontal_diffusivity=2., rng_method=rng_method,
-                  bnd_statistics=True)
+                  bnd_statistics=True, output_rep=str(tmp_path))
     g = case.run_gpu(cy, params)
     r = case.run_oracle(R, params)
     assert (r.inactive[:400] == 1).sum() > 50, "the case must delete particles"
@@ -192,13 +192,14 @@ def test_open_boundary_deletions_and_tags(rng_method):
 
 
 @pytest.mark.parametrize("which", ["keps_fused", "keps_rk2_generic", "open_deletions"])
-def test_element_resort_does_not_change_results(which):
+def test_element_resort_does_not_change_results(which, tmp_path):
     """Re-sorting the slots by element every 3 steps (state gather, swapped
     per-slot buffers, rank recompute) leaves every result unchanged."""
     if which == "open_deletions":
         case = open_channel_case()
         params = dict(final_time=70., time_step=1., frozen_eulerian_fields=True, model=1,
-                      diffusion_model=1, horizontal_diffusivity=2., bnd_statistics=True)
+                      diffusion_model=1, horizontal_diffusivity=2., bnd_statistics=True,
+                      output_rep=str(tmp_path))
     else:
         case = vortex_case(npart=2000)
         params = dict(final_time=150., time_step=5., model=1, diffusion_model=2)
@@ -319,3 +320,69 @@ def test_arrivals_midrun(model, rng_method):
                           inactive=ps.inactive.copy(), lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy(),
                           rng=rng)
     assert_parity(out, ref)
+
+
+CSEC_CHANNEL = [np.array([[50.3, -1.], [50.3, 101.]]),      # across the channel
+                np.array([[-1., -1.], [1., 1.]]),            # through the origin: rows deleted in the step
+                np.array([[20., 10.], [80., 90.], [0., 0.]])]  # diagonal (its 2nd segment is not tested)
+
+
+@pytest.mark.parametrize("rng_method", [1, 2])
+@pytest.mark.parametrize("sort_interval", [0, 3])
+def test_control_sections_channel(rng_method, sort_interval, tmp_path):
+    """Per-step control-section counts (solver.pyx:540-549) equal the oracle's:
+    diffusing particles crossing a line, a diagonal, and a section through the
+    origin, where rows deleted at the open boundary count with their zeroed
+    positions.  The counts reach the statistics files unchanged."""
+    case = open_channel_case()
+    params = dict(final_time=70., time_step=1., frozen_eulerian_fields=True, model=1, diffusion_model=1,
+                  horizontal_diffusivity=2., rng_method=rng_method, bnd_statistics=True,
+                  output_rep=str(tmp_path))
+    s = case.build_gpu(cy, params, control_sections=[cy.ControlSection(q) for q in CSEC_CHANNEL])
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, sort_interval, 0, 0, 1)
+    s.solve()
+    assert s.ctx.last_graph_steps() == 0  # per-step counts: eager steps
+    r = case.run_oracle(R, params, control_sections=CSEC_CHANNEL)
+    times, counts = s.csec_stats
+    ref = np.array(r.solver.csec_history)
+    assert counts.shape == ref.shape == (70, 3)
+    np.testing.assert_array_equal(counts, ref)
+    assert ref[:, 0].sum() > 100 and ref[:, 1].sum() > 20 and ref[:, 2].sum() > 20
+    # every open-boundary exit is a deletion (walls delete too, after max_wall_rebounds)
+    assert np.all(ref[:, 1] >= np.array(r.solver.bnd_history).sum(1))
+    for k in range(3):
+        d = np.loadtxt(tmp_path / "cs_stats_{}.txt".format(k + 1), delimiter=',', skiprows=2, ndmin=2)
+        np.testing.assert_array_equal(d[:, 1].astype(int), counts[:, k])
+        np.testing.assert_allclose(d[:, 0], times, rtol=0, atol=1e-8)
+
+
+@pytest.mark.parametrize("model", [1, 2])
+def test_control_sections_vortex(model):
+    """Counts on the fused 2D kernels (LSM-1 and LSM-2, k-epsilon diffusion)
+    with mid-run arrivals, step by step against the oracle."""
+    case = vortex_case(npart=1500)
+    case.pool = 1700
+    p = dict(model=1, diffusion_model=2) if model == 1 else dict(
+        model=2, time_scheme=5, diffusion_model=2, diffusion_lsm2_option=1)
+    params = dict(final_time=200., time_step=5., **p)
+    secs = [np.array([[1000., -10.], [1000., 510.]]), np.array([[0., 250.], [2000., 250.]])]
+    s = case.build_gpu(cy, params, control_sections=secs)
+    mesh = R.RefMesh(case.x, case.y, case.tri, boundary_points=case.boundary_points, nopen=case.nopen)
+    fs = R.RefFields(mesh, **{k: v for k, v in case.fields.items() if k != "trifinder"})
+    ps = R.RefParticles(case.position, case.velocity, dim=2, initial_pool_size=case.pool, **case.pset_kw)
+    rng = R.RefRng(*SEED)
+    o = R.RefSolver(fs, ps, params, trifinder=case.trifinder(), rng=rng, control_sections=secs)
+    for _ in range(10):
+        s.forward()
+    o.forward(10)
+    new = np.stack(np.meshgrid(np.linspace(900., 1100., 10), np.linspace(200., 300., 10)), -1).reshape(-1, 2)
+    s.particle_set.add(new)
+    o.add(new)
+    for _ in range(20):
+        s.forward()
+    o.forward(20)
+    assert s.ctx.last_path()[0] > 0  # the fused kernel ran
+    counts = s.csec_stats[1]
+    ref = np.array(o.csec_history)
+    assert ref.sum() > 100
+    np.testing.assert_array_equal(counts, ref)
