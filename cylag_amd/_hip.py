@@ -25,7 +25,8 @@ class CylParams(C.Structure):
                 ("max_wall_rebounds", C.c_int), ("bnd_statistics", C.c_int),
                 ("rebound_damping_coef", C.c_double), ("wall_roughness_angle", C.c_double),
                 ("stranding_probability", C.c_double), ("compute_depth", C.c_int),
-                ("frozen", C.c_int)]
+                ("frozen", C.c_int), ("lsm3_edge_opt", C.c_int),
+                ("diffusion_lsm3_tl_horizontal", C.c_double), ("diffusion_lsm3_tl_vertical", C.c_double)]
 
 
 class CylPconst(C.Structure):

@@ -121,8 +121,6 @@ class BatchSolver:
     def _check(self, particle_sets, params, initial=False):
         for p in params:
             p.check_values()
-            if p.model == 3:
-                raise ValueError("LSM3 (model=3) is not available in the MI355X step")
             for k, v in self._shared.items():
                 if k == "initial_time" and not initial:
                     continue

@@ -1499,6 +1499,8 @@ static ParamsDev to_dev(const cyl_params* p) {
   d.max_wall_rebounds = p->max_wall_rebounds; d.bnd_statistics = p->bnd_statistics;
   d.rebound_damping_coef = p->rebound_damping_coef; d.wall_roughness_angle = p->wall_roughness_angle;
   d.stranding_probability = p->stranding_probability; d.compute_depth = p->compute_depth;
+  d.lsm3_edge_opt = p->lsm3_edge_opt; d.lsm3_tl_horizontal = p->diffusion_lsm3_tl_horizontal;
+  d.lsm3_tl_vertical = p->diffusion_lsm3_tl_vertical;
   return d;
 }
 
@@ -1524,6 +1526,13 @@ static int predict_units(const cyl_params* p, int dim, double dt, int& umax) {
   if (p->model == 2) {
     if (p->time_scheme == 1) return pair + (dim == 3 ? single : 0);
     return dim * pair;  // direct integrator: one pair per component with nonzero amplitude
+  }
+  if (p->model == 3) {
+    // seen-velocity fluctuations at arrival (pair + single) and the direct
+    // integrator's pair + single per component
+    umax = 4 * pair + 4 * single;
+    if (p->time_scheme == 1) return pair + (dim == 3 ? single : 0);
+    return -1;  // direct integrator: draws depend on the covariance factor
   }
   return -1;
 }
@@ -1686,11 +1695,13 @@ static void launch_step(cyl_ctx* ctx, const StepArgs& a, int model, bool count) 
   if (d == 2) {
     if (model == 0) count ? launch_step_t<2, 0, true>(ctx, a) : launch_step_t<2, 0, false>(ctx, a);
     else if (model == 1) count ? launch_step_t<2, 1, true>(ctx, a) : launch_step_t<2, 1, false>(ctx, a);
-    else count ? launch_step_t<2, 2, true>(ctx, a) : launch_step_t<2, 2, false>(ctx, a);
+    else if (model == 2) count ? launch_step_t<2, 2, true>(ctx, a) : launch_step_t<2, 2, false>(ctx, a);
+    else count ? launch_step_t<2, 3, true>(ctx, a) : launch_step_t<2, 3, false>(ctx, a);
   } else {
     if (model == 0) count ? launch_step_t<3, 0, true>(ctx, a) : launch_step_t<3, 0, false>(ctx, a);
     else if (model == 1) count ? launch_step_t<3, 1, true>(ctx, a) : launch_step_t<3, 1, false>(ctx, a);
-    else count ? launch_step_t<3, 2, true>(ctx, a) : launch_step_t<3, 2, false>(ctx, a);
+    else if (model == 2) count ? launch_step_t<3, 2, true>(ctx, a) : launch_step_t<3, 2, false>(ctx, a);
+    else count ? launch_step_t<3, 3, true>(ctx, a) : launch_step_t<3, 3, false>(ctx, a);
   }
 }
 
@@ -1955,8 +1966,9 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   if (ctx->batched) P = ctx->scen_par;
   else P[0] = *p;
   for (int s = 0; s < ns; ++s) {
-    if (P[s].model < 0 || P[s].model > 2)
-      return ctx->set_err(CYL_E_ARG, -1, "LSM3 (model 3) is not available in the device step");
+    if (P[s].model < 0 || P[s].model > 3) return ctx->set_err(CYL_E_ARG, -1, "Wrong model");
+    if (P[s].model == 3 && (P[s].lsm3_edge_opt < 1 || P[s].lsm3_edge_opt > 2))
+      return ctx->set_err(CYL_E_ARG, -1, "lsm3_edge_opt must be 1 or 2");
     if (P[s].rng_method < 1 || P[s].rng_method > 3) return ctx->set_err(CYL_E_ARG, -1, "Unknown RNG method");
     if (P[s].rng_method != P[0].rng_method)
       return ctx->set_err(CYL_E_ARG, -1, "the scenarios of a batch share one rng_method");
@@ -2362,8 +2374,10 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
                                    "draw-count misprediction",
                                    "neighbourhood search needs triangle_adjacency",
                                    "LSM3 is not available in the device step",
-                                   "random-stream event table inconsistent"};
-      std::string m = code > 0 && code <= 11 ? msgs[code] : "device error";
+                                   "random-stream event table inconsistent",
+                                   "LSM3 requires finite a1 >= 0, A2, TL > 0 and dt >= 0",
+                                   "LSM3 covariance has a nonzero singular row or a significantly negative pivot"};
+      std::string m = code > 0 && code <= 13 ? msgs[code] : "device error";
       ctx->errcode = code;
       return ctx->set_err(CYL_E_STEP, pidx, m);
     }

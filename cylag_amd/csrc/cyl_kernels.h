@@ -135,7 +135,8 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
   if (!p.inactive && !c.err && !a.init_only) {
     if (MODEL == 0) bnd_j = update_test(c, p);
     else if (MODEL == 1) bnd_j = update_lsm1(c, p, dr);
-    else bnd_j = update_lsm2(c, p, dr);
+    else if (MODEL == 2) bnd_j = update_lsm2(c, p, dr);
+    else bnd_j = update_lsm3(c, p, dr);
     if (!c.err && !COUNT) {
       // draws are counted before the boundary conditions, which use a sub-stream
       if (par.boundary_conditions) {

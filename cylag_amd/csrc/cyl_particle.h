@@ -798,6 +798,480 @@ __device__ inline void lsm2_fluctuations(Ctx& c, Part& p, Draws& dr, double& f1,
 }
 
 // ---------------------------------------------------------------------------
+// LSM-3 (update_state_lsm3.pyx): the seen-fluid velocity Us is an
+// Ornstein-Uhlenbeck process relaxing to the mean flow over TL; the particle
+// velocity relaxes to Us at the drag rate a1 (plus added mass A2 dUs).  Over a
+// frozen step the (Us, Ur = Up - A2 Us, Xp) system of each component is linear
+// with lower-triangular transition exp(dt A), A = [[-b,0,0],[a(1-A2),-a,0],[A2,1,0]].
+// ---------------------------------------------------------------------------
+constexpr double EPS_LSM3 = 1.e-8;        // constants.pyx:8
+constexpr double LSM3_CLIP_STEP = 1.e-4;  // constants.pyx:11
+constexpr double DBL_EPS = 2.220446049250313e-16;
+
+struct Lsm3Coef {
+  double phi[9], q[9], force[3], flow[3];
+};
+
+// input checks of the coefficient routines (update_state_lsm3.pyx:74-81)
+__device__ inline bool lsm3_rates_ok(double a, double a2, double tl, double dt) {
+  if (!isfinite(a) || !isfinite(a2) || !isfinite(tl) || !isfinite(dt) || a < 0. || tl <= 0. || dt < 0.)
+    return false;
+  double b = 1. / tl;
+  return isfinite(b) && isfinite(a * dt) && isfinite(b * dt) && isfinite(a * (1. - a2) * dt);
+}
+
+__device__ inline void lsm3_identity(Lsm3Coef& C) {
+  for (int k = 0; k < 9; ++k) {
+    C.phi[k] = 0.;
+    C.q[k] = 0.;
+  }
+  C.phi[0] = C.phi[4] = C.phi[8] = 1.;
+  for (int k = 0; k < 3; ++k) C.force[k] = C.flow[k] = 0.;
+}
+
+__device__ inline bool lsm3_finite(const Lsm3Coef& C) {
+  bool ok = isfinite(C.force[1]) && isfinite(C.force[2]);
+  for (int k = 0; k < 9; ++k) ok = ok && isfinite(C.phi[k]) && isfinite(C.q[k]);
+  for (int k = 0; k < 3; ++k) ok = ok && isfinite(C.flow[k]);
+  return ok;
+}
+
+// _lsm3_direct_coefficients, update_state_lsm3.pyx:32-217: 24-term Taylor
+// series of the normalized system at max(a h, b h) <= 1/2, then step doublings
+__device__ __noinline__ bool lsm3_series_coefficients(double a, double a2, double tl, double dt, Lsm3Coef& C) {
+  if (!lsm3_rates_ok(a, a2, tl, dt)) return false;
+  double b = 1. / tl;
+  lsm3_identity(C);
+  if (dt == 0.) return true;
+  double h = dt;
+  int nd = 0;
+  while (h > 0. && fmax(a * h, b * h) > 0.5) {
+    h *= 0.5;
+    ++nd;
+  }
+  double ah = a * h, bh = b * h, kh = a * (1. - a2) * h;
+  // second moments of (s, p, x, si): ss sp sx pp px xx si pi xi ii
+  double t[10] = {1., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  double qb[10] = {1., 0., 0., 0., 0., 0., 0., 0., 0., 0.};
+  double s = 1., pv = 0., xv = 0., si = 0.;
+  double sum_p = 0., sum_x = 0., sum_si = 0., int_p = 0., int_x = 0., int_si = 0.;
+  double bterm = 1., phi1 = 1., phi2 = 0.5;
+  for (int n = 1; n < 25; ++n) {
+    double dv = n + 1., u[10];
+    u[0] = -2. * bh * t[0] / dv;
+    u[1] = (t[0] - (ah + bh) * t[1]) / dv;
+    u[2] = (t[1] - bh * t[2]) / dv;
+    u[3] = (2. * t[1] - 2. * ah * t[3]) / dv;
+    u[4] = (t[2] + t[3] - ah * t[4]) / dv;
+    u[5] = 2. * t[4] / dv;
+    u[6] = (t[0] - bh * t[6]) / dv;
+    u[7] = (t[6] + t[1] - ah * t[7]) / dv;
+    u[8] = (t[7] + t[2]) / dv;
+    u[9] = 2. * t[6] / dv;
+    for (int k = 0; k < 10; ++k) {
+      t[k] = u[k];
+      qb[k] += t[k];
+    }
+    double sn = -bh * s / n, pn = (s - ah * pv) / n, xn = pv / n, sin_ = s / n;
+    s = sn;
+    pv = pn;
+    xv = xn;
+    si = sin_;
+    sum_p += pv;
+    sum_x += xv;
+    sum_si += si;
+    int_p += pv / (n + 1.);
+    int_x += xv / (n + 1.);
+    int_si += si / (n + 1.);
+    bterm *= -ah / n;
+    phi1 += bterm / (n + 1.);
+    phi2 += bterm / ((n + 1.) * (n + 2.));
+  }
+  double* phi = C.phi;
+  double* q = C.q;
+  q[0] = h * qb[0];
+  q[1] = q[3] = h * kh * qb[1];
+  q[2] = q[6] = h * h * (kh * qb[2] + a2 * qb[6]);
+  q[4] = h * kh * kh * qb[3];
+  q[5] = q[7] = h * h * kh * (kh * qb[4] + a2 * qb[7]);
+  q[8] = h * h * h * (kh * kh * qb[5] + 2. * kh * a2 * qb[8] + a2 * a2 * qb[9]);
+  phi[0] = cr_exp(-bh);
+  phi[3] = kh * sum_p;
+  phi[4] = cr_exp(-ah);
+  phi[6] = h * (kh * sum_x + a2 * sum_si);
+  phi[7] = h * phi1;
+  C.force[1] = phi[7];
+  C.force[2] = h * h * phi2;
+  C.flow[0] = -cr_expm1(-bh);
+  C.flow[1] = kh * bh * int_p;
+  C.flow[2] = h * bh * (kh * int_x + a2 * int_si);
+  for (int n = 0; n < nd; ++n) {
+    // Q(2h) = Q(h) + Phi(h) Q(h) Phi(h)^T, Phi lower triangular
+    double w[9], qn[9];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        double v = 0.;
+        for (int k = 0; k <= r; ++k) v += phi[3 * r + k] * q[3 * k + c];
+        w[3 * r + c] = v;
+      }
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c <= r; ++c) {
+        double v = q[3 * r + c];
+        for (int k = 0; k <= c; ++k) v += w[3 * r + k] * phi[3 * c + k];
+        qn[3 * r + c] = qn[3 * c + r] = v;
+      }
+    for (int k = 0; k < 9; ++k) q[k] = qn[k];
+    C.flow[2] = 2. * C.flow[2] + phi[6] * C.flow[0] + phi[7] * C.flow[1];
+    C.flow[1] = (1. + phi[4]) * C.flow[1] + phi[3] * C.flow[0];
+    C.flow[0] = -cr_expm1(-2. * b * h);
+    phi[6] = phi[6] * (1. + phi[0]) + phi[7] * phi[3];
+    phi[3] *= phi[0] + phi[4];
+    C.force[2] = 2. * C.force[2] + phi[7] * phi[7];
+    phi[7] *= 1. + phi[4];
+    C.force[1] = phi[7];
+    h *= 2.;
+    phi[0] = cr_exp(-b * h);
+    phi[4] = cr_exp(-a * h);
+  }
+  return lsm3_finite(C);
+}
+
+__device__ inline double lsm3_phi1(double z) {  // (exp(z) - 1)/z, update_state_lsm3.pyx:219-223
+  return z == 0. ? 1. : cr_expm1(z) / z;
+}
+
+__device__ inline double lsm3_exp_divdiff(double x, double y) {  // update_state_lsm3.pyx:225-227
+  return cr_exp(-fmin(x, y)) * lsm3_phi1(-fabs(x - y));
+}
+
+__device__ inline double lsm3_phi2(double z) {  // (exp(z) - 1 - z)/z^2, update_state_lsm3.pyx:229-234
+  if (fabs(z) < 0.05)
+    return 0.5 + z * (1. / 6. + z * (1. / 24. + z * (1. / 120. + z * (1. / 720. +
+           z * (1. / 5040. + z * (1. / 40320. + z / 362880.))))));
+  return (cr_expm1(z) - z) / (z * z);
+}
+
+// _lsm3_closed_covariance, update_state_lsm3.pyx:236-266
+__device__ inline void lsm3_closed_covariance(double a, double b, double a2, double h, double* q) {
+  double d = a - b, c = a * (1. - a2) / d, bb = (1. - a2) / d, aa = (a2 + c) / b;
+  double sb = h * lsm3_phi1(-b * h), sa = h * lsm3_phi1(-a * h);
+  double s2b = h * lsm3_phi1(-2. * b * h), s2a = h * lsm3_phi1(-2. * a * h);
+  double sab = h * lsm3_phi1(-(a + b) * h);
+  q[0] = s2b;
+  q[1] = q[3] = c * (s2b - sab);
+  q[2] = q[6] = aa * (sb - s2b) - bb * (sb - sab);
+  q[4] = c * c * (s2b - 2. * sab + s2a);
+  q[5] = q[7] = c * (aa * (sb - s2b - sa + sab) - bb * (sb - sab - sa + s2a));
+  q[8] = (aa * aa * (h - 2. * sb + s2b) - 2. * aa * bb * (h - sb - sa + sab) + bb * bb * (h - 2. * sa + s2a));
+}
+
+// _lsm3_clipped_covariance, update_state_lsm3.pyx:268-310
+__device__ __noinline__ void lsm3_clipped_covariance(double a, double b, double a2, double h, double* q) {
+  double d = a - b, m = 0.5 * (a + b), x = m * h, delta;
+  if (x > 1.) delta = fmin(0.5, LSM3_CLIP_STEP * sqrt(x)) / h;
+  else delta = fmin(0.5, LSM3_CLIP_STEP / sqrt(x)) * m;
+  if (fabs(d) >= delta) {
+    lsm3_closed_covariance(a, b, a2, h, q);
+  } else {
+    double qp[9], qm[9];
+    lsm3_closed_covariance(m + 0.5 * delta, m - 0.5 * delta, a2, h, qp);
+    lsm3_closed_covariance(m - 0.5 * delta, m + 0.5 * delta, a2, h, qm);
+    double w = 0.5 * (1. + d / delta);
+    for (int k = 0; k < 9; ++k) q[k] = qm[k] + w * (qp[k] - qm[k]);
+  }
+  for (int k = 0; k < 3; ++k) q[4 * k] = fmax(q[4 * k], 0.);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < r; ++c) {
+      double bound = sqrt(q[4 * r] * q[4 * c]);
+      q[3 * r + c] = q[3 * c + r] = fmax(-bound, fmin(bound, q[3 * r + c]));
+    }
+}
+
+// _lsm3_clipped_coefficients, update_state_lsm3.pyx:312-374
+__device__ __noinline__ bool lsm3_clipped_coefficients(double a, double a2, double tl, double dt, Lsm3Coef& C) {
+  if (!lsm3_rates_ok(a, a2, tl, dt)) return false;
+  double b = 1. / tl;
+  lsm3_identity(C);
+  if (dt == 0.) return true;
+  double ah = a * dt, bh = b * dt, k = a * (1. - a2);
+  double alpha = cr_exp(-bh), beta = cr_exp(-ah), divdiff = lsm3_exp_divdiff(ah, bh);
+  double ib = dt * lsm3_phi1(-bh), ia = dt * lsm3_phi1(-ah);
+  C.phi[0] = alpha;
+  C.phi[3] = k * dt * divdiff;
+  C.phi[4] = beta;
+  C.phi[6] = ib - (1. - a2) * dt * divdiff;
+  C.phi[7] = ia;
+  C.force[1] = ia;
+  C.force[2] = dt * dt * lsm3_phi2(-ah);
+  C.flow[0] = -cr_expm1(-bh);
+  C.flow[1] = k * dt * (lsm3_phi1(-ah) - divdiff);
+  C.flow[2] = (a2 * (dt - ib) + (1. - a2) * (dt - ia - ib + dt * divdiff));
+  lsm3_clipped_covariance(a, b, a2, dt, C.q);
+  return lsm3_finite(C);
+}
+
+__device__ inline bool lsm3_coefficients(double a, double a2, double tl, double dt, int edge_opt, Lsm3Coef& C) {
+  return edge_opt == 2 ? lsm3_clipped_coefficients(a, a2, tl, dt, C) : lsm3_series_coefficients(a, a2, tl, dt, C);
+}
+
+// _lsm3_cholesky, update_state_lsm3.pyx:520-553 (false: not factorable)
+__device__ inline bool lsm3_cholesky(const double* q, double* L) {
+  for (int k = 0; k < 9; ++k) L[k] = 0.;
+  for (int r = 0; r < 3; ++r) {
+    double pivot = q[3 * r + r];
+    for (int c = 0; c < r; ++c) {
+      double res = q[3 * r + c];
+      for (int k = 0; k < c; ++k) res -= L[3 * r + k] * L[3 * c + k];
+      if (L[3 * c + c] > 0.) {
+        L[3 * r + c] = res / L[3 * c + c];
+      } else {
+        double tol = (128. * DBL_EPS * sqrt(fabs(q[3 * r + r])) * sqrt(fabs(q[3 * c + c])));
+        if (fabs(res) > tol) return false;
+      }
+      pivot -= L[3 * r + c] * L[3 * r + c];
+    }
+    if (pivot < 0.) {
+      if (-pivot > 128. * DBL_EPS * fabs(q[3 * r + r])) return false;
+      pivot = 0.;
+    }
+    L[3 * r + r] = sqrt(pivot);
+  }
+  return true;
+}
+
+// _lsm3_cholesky_clipped, update_state_lsm3.pyx:555-585
+__device__ inline void lsm3_cholesky_clipped(const double* q, double* L) {
+  for (int k = 0; k < 9; ++k) L[k] = 0.;
+  for (int r = 0; r < 3; ++r) {
+    double rem = fmax(q[3 * r + r], 0.);
+    for (int c = 0; c < r; ++c) {
+      if (L[3 * c + c] > 0.) {
+        double v = q[3 * r + c];
+        for (int k = 0; k < c; ++k) v -= L[3 * r + k] * L[3 * c + k];
+        v /= L[3 * c + c];
+        if (v * v > rem) v = copysign(sqrt(rem), v);
+        L[3 * r + c] = v;
+        rem -= v * v;
+      }
+    }
+    L[3 * r + r] = sqrt(fmax(rem, 0.));
+  }
+}
+
+// mean velocity (pseudo-3D corrected) and mean fluid acceleration, which LSM-3
+// always evaluates (update_state_lsm3.pyx:702-727, 876-909)
+__device__ inline void lsm3_mean_flow(Ctx& c, const Part& p, const double* x, double* u, double* acc) {
+  const int d = c.pdim;
+  int ll = d == 3 ? p.ll : 0;
+  double z = d == 3 ? x[2] : 0.;
+  mean_velocity(c, p.tri, ll, x[0], x[1], z, u[0], u[1], u[2]);
+  CYL_RET_ERR();
+  acc[0] = acc[1] = acc[2] = 0.;
+  if (c.f.dim == 2) accel2d(c.f, c.m, p.tri, x[0], x[1], c.dt, u[0], u[1], acc[0], acc[1]);
+  else accel3d(c.f, c.m, p.tri, ll, x[0], x[1], x[2], c.dt, u[0], u[1], u[2], acc[0], acc[1], acc[2], c.err);
+}
+
+// a1 = a1c (1.5/dp) unorm Cd with the slip floored (update_state_lsm3.pyx:729-735)
+__device__ inline double lsm3_relax_rate(const Ctx& c, const Part& p, double unorm) {
+  unorm = fmax(EPS_LSM2, unorm);
+  double Cd = drag_coefficient(c.pc, unorm, p.diameter);
+  return c.pc.a1c * (1.5 / p.diameter) * unorm * Cd;
+}
+
+// euler_maruyama_lsm3, update_state_lsm3.pyx:664-820
+__device__ inline void em_lsm3(Ctx& c, Part& p, Draws& dr) {
+  const int d = c.pdim;
+  int dm = c.par.diffusion_model;
+  double dt = c.dt, sqrtdt = sqrt(dt), u[3], acc[3];
+  double tlh = c.par.lsm3_tl_horizontal, tlv = c.par.lsm3_tl_vertical;
+  lsm3_mean_flow(c, p, p.x, u, acc);
+  CYL_RET_ERR();
+  double du0 = p.us[0] - p.up[0], du1 = p.us[1] - p.up[1], unorm;
+  if (d == 2) {
+    unorm = sqrt(du0 * du0 + du1 * du1);
+  } else {
+    double du2 = p.us[2] - p.up[2];
+    unorm = sqrt(du0 * du0 + du1 * du1 + du2 * du2);
+  }
+  double a1 = lsm3_relax_rate(c, p, unorm);
+  double xix, xiy, xiz = 0., tk = 0., ep = 0., tml, unstml, diffx, diffz;
+  dr.pair(xix, xiy);
+  if (d == 3) xiz = dr.single();
+  if (dm == 2 || dm == 3) {
+    keps_at(c, p, p.x[0], p.x[1], d == 3 ? p.x[2] : 0., tk, ep);
+    CYL_RET_ERR();
+    ep = fmax(ep, EPS);
+    tml = (tk / ep) / (0.5 + (3. / 4.) * C0KOLM);
+    unstml = 1. / fmax(EPS_LSM3, tml);
+    diffx = sqrt(C0KOLM * ep) * sqrtdt;
+  } else {
+    unstml = 1. / tlh;
+    diffx = sqrt(2. * c.par.horizontal_diffusivity) * sqrtdt / tlh;
+  }
+  double a2c = c.pc.a2c;
+  double dusx = (acc[0] + (u[0] - p.us[0]) * unstml) * dt + diffx * xix;
+  double dusy = (acc[1] + (u[1] - p.us[1]) * unstml) * dt + diffx * xiy;
+  p.up[0] += (p.us[0] - p.up[0]) * dt * a1;
+  p.up[1] += (p.us[1] - p.up[1]) * dt * a1;
+  p.us[0] += dusx;
+  p.us[1] += dusy;
+  if (c.pc.added_mass_force) {
+    p.up[0] += a2c * dusx;
+    p.up[1] += a2c * dusy;
+  }
+  p.x[0] += p.up[0] * dt;
+  p.x[1] += p.up[1] * dt;
+  if (d == 3) {
+    if (dm == 1 || dm == 2) {
+      unstml = 1. / tlv;
+      diffz = sqrt(2. * c.par.vertical_diffusivity) * sqrtdt / tlv;
+    } else {
+      tml = (tk / ep) / (0.5 + (3. / 4.) * C0KOLM);
+      unstml = 1. / fmax(EPS_LSM3, tml);
+      diffz = sqrt(C0KOLM * ep) * sqrtdt;
+    }
+    double dusz = (acc[2] + (u[2] - p.us[2]) * unstml) * dt + diffz * xiz;
+    p.up[2] += (p.us[2] - p.up[2]) * dt * a1 - GRAV * c.pc.a0c * dt;
+    p.us[2] += dusz;
+    if (c.pc.added_mass_force) p.up[2] += a2c * dusz;
+    p.x[2] += p.up[2] * dt;
+  }
+}
+
+// di_lsm3, update_state_lsm3.pyx:822-995
+__device__ __noinline__ void di_lsm3(Ctx& c, Part& p, Draws& dr) {
+  const int d = c.pdim;
+  int dm = c.par.diffusion_model, eo = c.par.lsm3_edge_opt;
+  double dt = c.dt;
+  if (dt == 0.) return;
+  double x0[3] = {0., 0., 0.}, u0[3] = {0., 0., 0.}, s0[3] = {0., 0., 0.}, u[3], acc[3], L[9];
+  for (int k = 0; k < d; ++k) {
+    x0[k] = p.x[k];
+    u0[k] = p.up[k];
+    s0[k] = p.us[k];
+  }
+  double a2 = c.pc.added_mass_force ? c.pc.a2c : 0.;
+  double tlh = c.par.lsm3_tl_horizontal, tlv = c.par.lsm3_tl_vertical, tk = 0., ep = 0., sigh, sigv = 0.;
+  lsm3_mean_flow(c, p, x0, u, acc);
+  CYL_RET_ERR();
+  if (dm == 2 || dm == 3) {
+    keps_at(c, p, x0[0], x0[1], x0[2], tk, ep);
+    CYL_RET_ERR();
+  }
+  double unorm = (s0[0] - u0[0]) * (s0[0] - u0[0]) + (s0[1] - u0[1]) * (s0[1] - u0[1]);
+  if (d == 3) unorm += (s0[2] - u0[2]) * (s0[2] - u0[2]);
+  double a1 = lsm3_relax_rate(c, p, sqrt(unorm));
+  if (dm == 2 || dm == 3) {
+    ep = fmax(ep, EPS);
+    tlh = fmax(EPS_LSM3, (tk / ep) / (0.5 + (3. / 4.) * C0KOLM));
+    sigh = sqrt(C0KOLM * ep);
+  } else {
+    sigh = sqrt(2. * c.par.horizontal_diffusivity) / tlh;
+  }
+  if (d == 3) {
+    if (dm == 3) {
+      tlv = tlh;
+      sigv = sigh;
+    } else {
+      sigv = sqrt(2. * c.par.vertical_diffusivity) / tlv;
+    }
+  }
+  Lsm3Coef C;
+  if (!lsm3_coefficients(a1, a2, tlh, dt, eo, C)) {
+    c.err = E_LSM3_COEF;
+    return;
+  }
+  bool factored = false;
+  double sig = sigh;
+  for (int k = 0; k < d; ++k) {
+    double tlk = tlh;
+    if (k == 2) {
+      tlk = tlv;
+      sig = sigv;
+      if (tlv != tlh) {
+        if (!lsm3_coefficients(a1, a2, tlv, dt, eo, C)) {
+          c.err = E_LSM3_COEF;
+          return;
+        }
+        factored = false;
+      }
+    }
+    double uk = u[k], Fp = k == 2 ? -GRAV * c.pc.a0c : 0.;
+    double ur0 = u0[k] - a2 * s0[k];
+    double us_m = (s0[k] * C.phi[0] + uk * C.flow[0] + acc[k] * tlk * C.flow[0]);
+    double ur_m = (ur0 * C.phi[4] + s0[k] * C.phi[3] + uk * C.flow[1] + acc[k] * tlk * C.flow[1] + Fp * C.force[1]);
+    double xp_m = (x0[k] + ur0 * C.force[1] + s0[k] * C.phi[6] + uk * C.flow[2] + acc[k] * tlk * C.flow[2] +
+                   Fp * C.force[2]);
+    if (sig > 0. && C.q[0] > 0.) {
+      if (!factored) {
+        if (eo == 2) {
+          lsm3_cholesky_clipped(C.q, L);
+        } else if (!lsm3_cholesky(C.q, L)) {
+          c.err = E_LSM3_CHOL;
+          return;
+        }
+        factored = true;
+      }
+      double xix, xiy = 0., xiz = 0.;
+      if (L[4] > 0.) dr.pair(xix, xiy);
+      else xix = dr.single();
+      if (L[8] > 0.) xiz = dr.single();
+      us_m += sig * L[0] * xix;
+      ur_m += sig * (L[3] * xix + L[4] * xiy);
+      xp_m += sig * (L[6] * xix + L[7] * xiy + L[8] * xiz);
+    }
+    p.us[k] = us_m;
+    p.up[k] = ur_m + a2 * us_m;
+    p.x[k] = xp_m;
+  }
+}
+
+// update_state_lsm3, update_state_lsm3.pyx:587-662
+__device__ inline int update_lsm3(Ctx& c, Part& p, Draws& dr) {
+  const int d = c.pdim;
+  int bnd_j = -1;
+  for (int q = 0; q < d; ++q) {
+    p.lx[q] = p.x[q];
+    p.lup[q] = p.up[q];
+  }
+  if (c.par.time_scheme == 1) em_lsm3(c, p, dr);
+  else if (c.par.time_scheme == 5) di_lsm3(c, p, dr);
+  else c.err = E_SCHEME;
+  if (c.err) return -1;
+  p.last_tri = p.tri;
+  p.tri = xy_localize(c.m, p.last_tri, p.lx[0], p.lx[1], p.x[0], p.x[1], 3, bnd_j, c.err);
+  if (d == 3) p.ll = z_localize(c.f, c.m, p.tri, p.x[0], p.x[1], p.x[2]);
+  return bnd_j;
+}
+
+// update_lsm3_fluctuations, init_lsm3_fluctuations.pyx:28-90
+__device__ inline void lsm3_fluctuations(Ctx& c, Part& p, Draws& dr, double& f1, double& f2, double& f3) {
+  f1 = f2 = f3 = 0.;
+  int dm = c.par.diffusion_model;
+  double sigh;
+  if (dm == 2 || dm == 3) {
+    double tk, ep;
+    keps_at(c, p, p.x[0], p.x[1], c.pdim == 3 ? p.x[2] : 0., tk, ep);
+    CYL_RET_ERR();
+    ep = fmax(ep, EPS);
+    double tml = fmax(EPS_LSM3, (tk / ep) / (0.5 + (3. / 4.) * C0KOLM));
+    sigh = sqrt(0.5 * C0KOLM * ep * tml);
+  } else {
+    sigh = sqrt(c.par.horizontal_diffusivity / c.par.lsm3_tl_horizontal);
+  }
+  if (sigh > 0.) {
+    double xix, xiy;
+    dr.pair(xix, xiy);
+    f1 = sigh * xix;
+    f2 = sigh * xiy;
+  }
+  if (c.pdim == 3) {
+    double sigv = dm == 3 ? sigh : sqrt(c.par.vertical_diffusivity / c.par.lsm3_tl_vertical);
+    if (sigv > 0.) f3 = sigv * dr.single();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Arrival: initialize_particle_state, solver.pyx:238-380.  `tri_found` is the
 // element found on the host (trifinder) or -3 to search on the device.
 // ---------------------------------------------------------------------------
@@ -834,7 +1308,14 @@ __device__ inline void init_particle(Ctx& c, Part& p, Draws& dr, int tri_found) 
       if (d == 3) p.up[2] += f3;
     }
     for (int q = 0; q < d; ++q) p.us[q] = p.up[q];
-    if (c.par.model == 3) c.err = E_LSM3;
+    if (c.par.model == 3) {
+      double f1, f2, f3;
+      lsm3_fluctuations(c, p, dr, f1, f2, f3);
+      CYL_RET_ERR();
+      p.us[0] += f1;
+      p.us[1] += f2;
+      if (d == 3) p.us[2] += f3;
+    }
   }
 }
 

@@ -43,6 +43,8 @@ enum ErrCode : int {
   E_METHOD1 = 9,          // neighbourhood search requested (reference has no table for it)
   E_LSM3 = 10,
   E_RNG_EVENTS = 11,      // a slow Ziggurat start missing from the step's event table (internal)
+  E_LSM3_COEF = 12,       // "LSM3 requires finite a1 >= 0, A2, TL > 0 and dt >= 0"
+  E_LSM3_CHOL = 13,       // "LSM3 covariance has a nonzero singular row / negative pivot"
 };
 
 struct MeshDev {
@@ -110,6 +112,8 @@ struct ParamsDev {
   int boundary_conditions, boundary_conditions_type, max_wall_rebounds, bnd_statistics;
   double rebound_damping_coef, wall_roughness_angle, stranding_probability;
   int compute_depth;
+  int lsm3_edge_opt;
+  double lsm3_tl_horizontal, lsm3_tl_vertical;
 };
 
 // Random stream of one scenario for one step.

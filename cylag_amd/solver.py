@@ -27,7 +27,8 @@ def _params_struct(p, pset):
         int(p.boundary_conditions_type), int(p.max_wall_rebounds), int(bool(p.bnd_statistics)),
         float(p.rebound_damping_coef), float(p.wall_roughness_angle),
         float(p.stranding_probability), int(bool(pset.compute_depth)),
-        int(bool(p.frozen_eulerian_fields)))
+        int(bool(p.frozen_eulerian_fields)), int(p.lsm3_edge_opt), float(p.diffusion_lsm3_tl_horizontal),
+        float(p.diffusion_lsm3_tl_vertical))
 
 
 def _pconst_struct(ps):
@@ -107,8 +108,6 @@ class Solver:
         fs, ps = field_set, particle_set
         check_setup(fs, ps, p)
         fs.frozen_hydro = bool(p.frozen_eulerian_fields == 1)
-        if p.model == 3:
-            raise ValueError("LSM3 (model=3) is not available in the MI355X step")
         ps.rho0 = p.water_density
         if p.model in (2, 3):
             ps._initialize_model_coefficients()

@@ -39,6 +39,8 @@ typedef struct {
   double rebound_damping_coef, wall_roughness_angle, stranding_probability;
   int compute_depth;
   int frozen;
+  int lsm3_edge_opt;                                                  /* 1 series, 2 clipped closed form */
+  double diffusion_lsm3_tl_horizontal, diffusion_lsm3_tl_vertical;  /* TL of the constant-K models */
 } cyl_params;
 
 /* Particle-set constants (lagrangian_particle_set.pyx:158-261,349-375). */

@@ -35,6 +35,8 @@ extern "C" {
 #define REF_ERR_METHOD1 -8           /* neighbourhood search needs triangle_adjacency (absent) */
 #define REF_ERR_LSM3_INIT -9
 #define REF_ERR_ADJ_OVERFLOW -10
+#define REF_ERR_LSM3_COEF -11         /* LSM3 rates or coefficients not finite (update_state_lsm3.pyx:74-81) */
+#define REF_ERR_LSM3_CHOL -12         /* LSM3 covariance not factorable (update_state_lsm3.pyx:540-551) */
 
 typedef struct { uint64_t state, inc; } ref_pcg32;
 
@@ -182,6 +184,10 @@ typedef struct {
     int *csec_stats;
 } ref_solver;
 
+/* lsm3_direct_coefficients (update_state_lsm3.pyx:449-518): 3x3 transition and
+ * unit-noise covariance in the order (Us, Up, Xp), force response (3) */
+int ref_lsm3_direct_coefficients(double a1, double tl, double dt, double a2, int edge_opt, double *transition,
+                                 double *covariance, double *response);
 int ref_initialize_particle_state(ref_solver *s, int i);
 int ref_forward(ref_solver *s);
 /* nsteps forwards in a row (no sources / output), stops on error */

@@ -1,5 +1,5 @@
 /*
- * oracle/cpu_ref_lsm23.c -- LSM-2 (and the LSM-3 entry points) of the CPU
+ * oracle/cpu_ref_lsm23.c -- LSM-2 of the CPU
  * restatement.  TEST INFRASTRUCTURE ONLY; see cpu_ref.h.
  * Restates cylag/lsm/update_state_lsm2.pyx and init_lsm2_fluctuations.pyx.
  */
@@ -481,17 +481,4 @@ int ref_lsm2_fluctuations(ref_solver *s, int i, double *f1, double *f2, double *
         *f3 = sqrt(0.5 * bz) * xiz;
     }
     return REF_OK;
-}
-
-/* LSM-3 is not restated yet: the entry points report an unsupported scheme. */
-int ref_update_state_lsm3(ref_solver *s, int i, int *bnd_j)
-{
-    (void)s; (void)i; *bnd_j = -1;
-    return REF_ERR_BAD_SCHEME;
-}
-
-int ref_lsm3_fluctuations(ref_solver *s, int i, double *f1, double *f2, double *f3)
-{
-    (void)s; (void)i; *f1 = *f2 = *f3 = 0.;
-    return REF_ERR_LSM3_INIT;
 }

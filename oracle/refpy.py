@@ -141,6 +141,7 @@ _sig("ref_time_interpolation", C.c_int, [C.POINTER(FsetS), C.c_double])
 _sig("ref_diffusivity_gradient", C.c_int, [C.POINTER(FsetS), _MP, C.c_int, C.c_int, C.c_double,
                                            C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp])
 _sig("ref_lsm2_direct_coefficients", C.c_int, [C.c_double, C.c_double] + [_dp] * 7)
+_sig("ref_lsm3_direct_coefficients", C.c_int, [C.c_double] * 4 + [C.c_int] + [_dp] * 3)
 _SP = C.POINTER(SolverS)
 _sig("ref_initialize_particle_state", C.c_int, [_SP, C.c_int])
 _sig("ref_forward", C.c_int, [_SP])
@@ -205,6 +206,16 @@ class RefRng:
 
 
 RNG = RefRng()
+
+
+def lsm3_direct_coefficients(a1, tl, dt, a2=0., edge_opt=1):
+    """(transition (3, 3), unit-noise covariance (3, 3), force response (3,)) in
+    the order (Us, Up, Xp) (update_state_lsm3.pyx:449-518)."""
+    tr, cov, resp = np.zeros(9), np.zeros(9), np.zeros(3)
+    e = lib.ref_lsm3_direct_coefficients(a1, tl, dt, a2, edge_opt, P(tr, "d"), P(cov, "d"), P(resp, "d"))
+    if e:
+        raise ValueError("LSM3 coefficients: error {}".format(e))
+    return tr.reshape(3, 3), cov.reshape(3, 3), resp
 
 
 def ziggurat_tables():
