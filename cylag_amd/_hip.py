@@ -26,7 +26,8 @@ class CylParams(C.Structure):
                 ("rebound_damping_coef", C.c_double), ("wall_roughness_angle", C.c_double),
                 ("stranding_probability", C.c_double), ("compute_depth", C.c_int),
                 ("frozen", C.c_int), ("lsm3_edge_opt", C.c_int),
-                ("diffusion_lsm3_tl_horizontal", C.c_double), ("diffusion_lsm3_tl_vertical", C.c_double)]
+                ("diffusion_lsm3_tl_horizontal", C.c_double), ("diffusion_lsm3_tl_vertical", C.c_double),
+                ("stranding_output", C.c_int)]
 
 
 class CylPconst(C.Structure):
@@ -73,6 +74,7 @@ SIGNATURES = {
     "cyl_last_path": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     "cyl_control_sections_set": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
     "cyl_control_counts": (C.c_int, [C.c_void_p, _ip, C.c_int, _ip]),
+    "cyl_stranded_get": (C.c_int, [C.c_void_p, _dp, C.c_int, _ip]),
     "cyl_scenarios_set": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), _ip]),
     "cyl_scenario_rng_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),

@@ -161,6 +161,12 @@ struct cyl_ctx {
   int* d_sec_counts = nullptr;
   size_t sec_cap = 0;
   std::vector<int> last_sec_counts;
+  // stranded-particle records of the last call (stranding_output)
+  double* d_strand_xyz = nullptr;
+  int2* d_strand_key = nullptr;
+  int* d_strand_count = nullptr;
+  int strand_cap = 0;
+  std::vector<double> last_strand;  // 3 per record, in (step, pool index) order
   int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
   int64_t* d_cursor = nullptr;
   int64_t* d_sort_tiles = nullptr;
@@ -300,7 +306,8 @@ extern "C" int cyl_ctx_destroy(cyl_ctx* ctx) {
   ctx->ev_bufs.free_all();
   for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
   void* singles[] = {ctx->d_zt,        ctx->d_flag, ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->d_step_done,
-                     ctx->d_bnd,       ctx->d_clock, ctx->d_sec_seg, ctx->d_sec_of,       ctx->d_sec_counts};
+                     ctx->d_bnd,       ctx->d_clock, ctx->d_sec_seg, ctx->d_sec_of,       ctx->d_sec_counts,
+                     ctx->d_strand_xyz, ctx->d_strand_key, ctx->d_strand_count};
   for (void* p : singles)
     if (p) (void)hipFree(p);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
@@ -1501,6 +1508,7 @@ static ParamsDev to_dev(const cyl_params* p) {
   d.stranding_probability = p->stranding_probability; d.compute_depth = p->compute_depth;
   d.lsm3_edge_opt = p->lsm3_edge_opt; d.lsm3_tl_horizontal = p->diffusion_lsm3_tl_horizontal;
   d.lsm3_tl_vertical = p->diffusion_lsm3_tl_vertical;
+  d.stranding_output = p->stranding_output;
   return d;
 }
 
@@ -1741,6 +1749,7 @@ static bool fast_params(const cyl_ctx* ctx, const cyl_params* p) {
   if (ctx->dim != f.dim) return false;  // pseudo-3D runs the generic kernel
   if (f.dim == 3 && (p->compute_depth || f.fid_zs < 0 || f.fid_uz < 0 || f.nlayers < 2)) return false;
   if (p->diffusion_model < 0 || p->diffusion_model > 3) return false;
+  if (p->stranding_output && p->stranding_probability > 0.) return false;  // the generic kernel records them
   if (p->model == 1) {
     if (p->time_scheme != 1) return false;
   } else if (p->model == 2) {
@@ -2034,8 +2043,26 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     }
     HIPCHK(hipMemsetAsync(ctx->d_sec_counts, 0, need * sizeof(int), ctx->stream));
   }
-  bool any_stats = ctx->nsecseg > 0;  // per-step counts: no graph replay
+  // stranded positions (one record per row at most: a stranded row is deleted)
+  bool strand_rec = !init_only && !ctx->batched && P[0].stranding_output && P[0].stranding_probability > 0.;
+  ctx->last_strand.clear();
+  if (strand_rec) {
+    if (ctx->strand_cap < ctx->n) {
+      if (ctx->d_strand_xyz) HIPCHK(hipFree(ctx->d_strand_xyz));
+      if (ctx->d_strand_key) HIPCHK(hipFree(ctx->d_strand_key));
+      ctx->d_strand_xyz = nullptr;
+      ctx->d_strand_key = nullptr;
+      HIPCHK(hipMalloc(&ctx->d_strand_xyz, (size_t)ctx->n * 3 * sizeof(double)));
+      HIPCHK(hipMalloc(&ctx->d_strand_key, (size_t)ctx->n * sizeof(int2)));
+      ctx->strand_cap = ctx->n;
+    }
+    if (!ctx->d_strand_count) HIPCHK(hipMalloc(&ctx->d_strand_count, 2 * sizeof(int)));
+    HIPCHK(hipMemsetAsync(ctx->d_strand_count, 0, 2 * sizeof(int), ctx->stream));
+  }
+  bool any_stats = false;
   for (int s = 0; s < ns; ++s) any_stats |= P[s].bnd_statistics != 0;
+  // per-step records (statistics, control sections, stranded rows): eager steps only
+  const bool per_step_records = any_stats || ctx->nsecseg > 0 || strand_rec;
   StepArgs a{};
   a.m = ctx->mesh;
   a.par = to_dev(&P[0]);
@@ -2146,6 +2173,10 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     a.step_id = ctx->step_counter;
     a.batch_step = ctx->batch_steps;
     a.bnd_stats = any_stats ? ctx->d_bnd + (size_t)step * nstat : nullptr;
+    a.strand_xyz = strand_rec ? ctx->d_strand_xyz : nullptr;
+    a.strand_key = strand_rec ? ctx->d_strand_key : nullptr;
+    a.strand_count = strand_rec ? ctx->d_strand_count : nullptr;
+    a.strand_cap = ctx->strand_cap;
     // active rank when rows were deleted or added since the last scan
     enqueue_rank(ctx, ctx->d_flag + 1, launches);
     bool uniform = can_uniform && !ctx->pending_init;
@@ -2196,7 +2227,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     // general mode: exact per-row unit counts, pool-order scan, rerun
     int* g = ctx->d_flag;
     k_rerun_reset<<<nblk(std::max(nstat, ns)), BLK, 0, ctx->stream>>>(
-        ctx->d_stream_end, ctx->d_err_step, a.bnd_stats, nstat, ns, ctx->d_n_deleted, g);
+        ctx->d_stream_end, ctx->d_err_step, a.bnd_stats, nstat, ns, ctx->d_n_deleted, g, a.strand_count);
     k_undelete<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->pool[0].inactive, ctx->d_del_step, ctx->n,
                                                       ctx->step_counter, g, a.clock);
     a.general = 0;
@@ -2241,7 +2272,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     k_finish<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_stream_end, ns, ctx->d_flag,
                                                 ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->step_counter,
                                                 ctx->d_step_done, ctx->d_err_step, ctx->d_err,
-                                                a.clock ? ctx->d_clock : nullptr);
+                                                a.clock ? ctx->d_clock : nullptr, a.strand_count);
     ++launches;
     ctx->cur = 1 - ctx->cur;
     ctx->have_last = !init_only;
@@ -2262,7 +2293,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   // device.  Needs the uniform draw assignment, frozen fields (the captured
   // interpolation would be the first step's) and no per-step statistics.
   const int period = 2 * std::max(1, ctx->sort_interval);
-  if (ctx->use_graph && !init_only && can_uniform && ctx->frozen && !any_stats && nsteps >= 1 + 2 * period) {
+  if (ctx->use_graph && !init_only && can_uniform && ctx->frozen && !per_step_records && nsteps >= 1 + 2 * period) {
     // one eager step first: arrivals initialized, generator flags and packed records in place
     e = one_step(step++, false);
     if (e) return e;
@@ -2345,6 +2376,26 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         for (int q = 0; q < ctx->nopen; ++q)
           bnd_out[((size_t)s * ctx->nscen + sc) * ctx->nopen + q] =
               h[(size_t)s * nstat + (size_t)sc * std::max(ctx->nopen, 1) + q];
+  }
+  if (strand_rec) {
+    int cnt[2];
+    HIPCHK(hipMemcpy(cnt, ctx->d_strand_count, sizeof(cnt), hipMemcpyDeviceToHost));
+    int nr = std::min(cnt[0], ctx->strand_cap);
+    std::vector<double> xyz((size_t)nr * 3);
+    std::vector<int2> key(nr);
+    if (nr > 0) {
+      HIPCHK(hipMemcpy(xyz.data(), ctx->d_strand_xyz, xyz.size() * sizeof(double), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(key.data(), ctx->d_strand_key, key.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    }
+    // serial order of the reference's loop: steps in turn, rows in pool order
+    std::vector<int> ord(nr);
+    for (int q = 0; q < nr; ++q) ord[q] = q;
+    std::sort(ord.begin(), ord.end(), [&](int u, int v) {
+      return key[u].x != key[v].x ? key[u].x < key[v].x : key[u].y < key[v].y;
+    });
+    ctx->last_strand.resize((size_t)nr * 3);
+    for (int q = 0; q < nr; ++q)
+      for (int c = 0; c < 3; ++c) ctx->last_strand[3 * q + c] = xyz[3 * ord[q] + c];
   }
   if (ctx->nsecseg > 0 && !init_only) {
     ctx->last_sec_counts.assign((size_t)nsteps * ctx->nsec, 0);
@@ -2460,6 +2511,14 @@ extern "C" int cyl_control_counts(cyl_ctx* ctx, int* out, int cap, int* nout) {
   if (!ctx || !nout) return CYL_E_ARG;
   *nout = (int)ctx->last_sec_counts.size();
   for (int i = 0; i < *nout && i < cap && out; ++i) out[i] = ctx->last_sec_counts[i];
+  return CYL_OK;
+}
+
+extern "C" int cyl_stranded_get(cyl_ctx* ctx, double* xyz, int cap, int* n) {
+  if (!ctx || !n) return CYL_E_ARG;
+  *n = (int)(ctx->last_strand.size() / 3);
+  for (int q = 0; q < *n && q < cap && xyz; ++q)
+    for (int c = 0; c < 3; ++c) xyz[3 * q + c] = ctx->last_strand[3 * q + c];
   return CYL_OK;
 }
 

@@ -51,6 +51,10 @@ struct StepArgs {
   int batch_step;         // steps run since the scenarios were set
   const int* scen_base;   // per scenario: first pool index, or null (one scenario)
   const int* clock;       // graph replays: {step id, batch step} in device memory, or null
+  double* strand_xyz;     // stranding_output: stranded positions, or null
+  int2* strand_key;       // (step, pool index) of each stranded position
+  int* strand_count;      // [0] records, [1] records at the start of the step
+  int strand_cap;
 };
 
 __device__ __forceinline__ int step_of(const StepArgs& a) { return a.clock ? a.clock[0] : a.step_id; }
@@ -140,6 +144,8 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
     if (!c.err && !COUNT) {
       // draws are counted before the boundary conditions, which use a sub-stream
       if (par.boundary_conditions) {
+        StrandRec sr{a.strand_xyz, a.strand_key, a.strand_count, a.strand_cap, step_of(a), pool};
+        if (a.strand_xyz && !a.init_only) c.sr = &sr;
         BcDraws bd;
         bd.method = par.rng_method;
         bd.zt = a.zt;
@@ -487,7 +493,7 @@ __global__ void __launch_bounds__(BLK) k_slot_of_pool(const int* gpool, int n, i
 // clock (graph replays): the device-side step counters, advanced here
 __global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long* stream_end, int nscen, int* flag,
                          int* n_deleted, long long* n_deleted_total, int step_id, int* step_done,
-                         unsigned long long* err_step, unsigned long long* err, int* clock) {
+                         unsigned long long* err_step, unsigned long long* err, int* clock, int* strand_count) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nscen) {
     unsigned long long e = stream_end[s];
@@ -503,6 +509,7 @@ __global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long*
     flag[0] = 0;
     int sid = clock ? clock[0] : step_id;
     *step_done = sid + 1;
+    if (strand_count) strand_count[1] = strand_count[0];
     if (clock) {
       clock[0] = sid + 1;
       clock[1] += 1;
@@ -512,8 +519,9 @@ __global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long*
 
 // reset the per-step outputs before a general-mode rerun of the step
 __global__ void k_rerun_reset(unsigned long long* stream_end, unsigned long long* err, int* bnd_stats, int nstat,
-                              int nscen, int* n_deleted, const int* cond) {
+                              int nscen, int* n_deleted, const int* cond, int* strand_count) {
   if (cond && *cond == 0) return;
+  if (strand_count && blockIdx.x == 0 && threadIdx.x == 0) strand_count[0] = strand_count[1];
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nscen) {
     stream_end[i] = 0;

@@ -208,6 +208,16 @@ struct BcDraws {
   }
 };
 
+// Stranded-particle records (compute_bc.pyx:107-122): position at stranding,
+// keyed by (step, pool index) so that the host can restore the serial order.
+struct StrandRec {
+  double* xyz;  // cap * 3
+  int2* key;    // cap: (step, pool index)
+  int* count;   // [0] records so far, [1] records at the start of the step
+  int cap;
+  int step, pool;
+};
+
 struct Ctx {
   const MeshDev& m;
   const FieldsDev& f;
@@ -216,6 +226,7 @@ struct Ctx {
   double dt;
   int pdim;  // particle-set dimension
   int err;
+  StrandRec* sr = nullptr;  // stranding_output: where stranded positions go
 };
 
 #define CYL_CHK_TRI(k)              \
@@ -1358,6 +1369,15 @@ __device__ inline void compute_bc(Ctx& c, Part& p, int bnd_j, BcDraws& bd, int* 
   if (bnd_j != -1) {
     int lab = m.edge_labels[bnd_j];
     if (lab == 0 && strand > 0.) strand_cond = bd.uniform() < strand;
+    if (strand_cond && c.sr) {
+      int q = atomicAdd(c.sr->count, 1);
+      if (q < c.sr->cap) {
+        c.sr->xyz[3 * q] = p.x[0];
+        c.sr->xyz[3 * q + 1] = p.x[1];
+        c.sr->xyz[3 * q + 2] = d == 2 ? 0.0 : p.x[2];
+        c.sr->key[q] = make_int2(c.sr->step, c.sr->pool);
+      }
+    }
     if (lab == 0 && !strand_cond) {
       int bjj = bnd_j, rc = 0;
       while (p.tri == PART_LOC_O && rc < c.par.max_wall_rebounds) {

@@ -114,6 +114,7 @@ struct ParamsDev {
   int compute_depth;
   int lsm3_edge_opt;
   double lsm3_tl_horizontal, lsm3_tl_vertical;
+  int stranding_output;
 };
 
 // Random stream of one scenario for one step.

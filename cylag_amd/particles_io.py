@@ -1,7 +1,8 @@
-"""Particle result files: the TXT writer the Solver calls every
-output_printout_period steps, and ParticlesIO, which reads them back.
+"""Particle result files: the TXT and VTK writers the Solver calls every
+output_printout_period steps, the stranded-particle file, and ParticlesIO,
+which reads TXT results back.
 
-File layout (cylag/io/write_particles.pyx:18-50, cylag/io/writer_txt.pyx:25-180):
+TXT layout (cylag/io/write_particles.pyx:18-50, cylag/io/writer_txt.pyx:25-180):
   cylag particles result file
   Variables: Tags, Xp, Yp, Zp, Up, Vp, Wp, [Us, Vs, Ws, ] [dp, ] [depth, ]
 then per record a header line "Time = %.8f, Iter = %16d, Npart = %16d" and one
@@ -35,10 +36,58 @@ def prepare_output(pset, outputdir, output_file_format, output_file_name, model=
 
 
 def write_particles(pset, time, iteration, params):
-    """Append one record of the active particles (write_particles.pyx:52-91)."""
-    if params.output_file_format not in ('txt', 'all'):
-        raise ValueError("output_file_format {!r}: only the TXT writer is provided".format(
-            params.output_file_format))
+    """One output record of the active particles (write_particles.pyx:52-91):
+    a VTK file per record ('vtk', 'all') and a record appended to the TXT file
+    ('txt', 'all')."""
+    fmt = params.output_file_format
+    if fmt not in ('txt', 'vtk', 'all'):
+        raise ValueError("output_file_format {!r}: the writers are 'txt', 'vtk' and 'all'".format(fmt))
+    if fmt in ('vtk', 'all'):
+        write_vtk(pset, time, iteration, params.output_rep, params.output_file_name, params.model)
+    if fmt in ('txt', 'all'):
+        write_txt(pset, time, iteration, params)
+
+
+def write_vtk(pset, time, iteration, outputdir, file_name, model):
+    """Legacy-VTK unstructured grid of vertex cells, <file_name>_<iteration>.vtk
+    (writer_vtk.pyx:7-304): time and cycle field data, points (z = 0 in 2D),
+    then per point the pool index, tag, elevation (3D), velocity, diameter
+    (non-monodisperse), fluid velocity seen (model >= 2) and depth (3D, when
+    computed).  Values are written with Python's shortest round-trip repr."""
+    dim = pset.dim
+    act = np.nonzero(np.asarray(pset.inactive) == 0)[0]
+    npa = act.size
+    pos, vel = np.asarray(pset.position), np.asarray(pset.velocity)
+    tag = np.asarray(pset.tag)
+
+    def vec(a):
+        return "".join("{} {} {}\n".format(a[j, 0], a[j, 1], a[j, 2] if dim == 3 else 0.) for j in act)
+
+    def sca(a):
+        return "".join("{}\n".format(a[j]) for j in act)
+
+    out = ["# vtk DataFile Version 2.0\n", "particle positions (Points {}D)\n".format(dim), "ASCII\n",
+           "DATASET UNSTRUCTURED_GRID\n", "FIELD FIELDDATA 2\n", "TIME 1 1 DOUBLE\n", "{}\n".format(time),
+           "CYCLE 1 1 INT\n", "{}\n".format(iteration), "POINTS {} FLOAT\n".format(npa), vec(pos),
+           "CELLS {} {}\n".format(npa, 2 * npa), "".join("1 {}\n".format(c) for c in range(npa)),
+           "CELL_TYPES {}\n".format(npa), "1\n" * npa, "POINT_DATA {}\n".format(npa),
+           "SCALARS Index INT\n", "LOOKUP_TABLE default\n", "".join("{}\n".format(j) for j in act),
+           "SCALARS tag INT\n", "LOOKUP_TABLE default\n", "".join("{}\n".format(int(tag[j])) for j in act)]
+    if dim == 3:
+        out += ["SCALARS elevation FLOAT\n", "LOOKUP_TABLE default\n", sca(pos[:, 2])]
+    out += ["VECTORS velocity FLOAT\n", vec(vel)]
+    if pset.diameter_distribution != 'monodisperse':
+        out += ["SCALARS diameter FLOAT\n", "LOOKUP_TABLE default\n", sca(np.asarray(pset.diameter))]
+    if model >= 2:
+        out += ["VECTORS fluid_velocity_seen FLOAT\n", vec(np.asarray(pset.fluid_velocity_seen))]
+    if dim == 3 and pset.compute_depth:
+        out += ["SCALARS depth FLOAT\n", "LOOKUP_TABLE default\n", sca(np.asarray(pset.depth))]
+    with open(os.path.join(outputdir, file_name + "_{}.vtk".format(iteration)), "w") as f:
+        f.write("".join(out))
+
+
+def write_txt(pset, time, iteration, params):
+    """Append one TXT record (writer_txt.pyx:25-180)."""
     dim = pset.dim
     inactive = np.asarray(pset.inactive)
     act = np.nonzero(inactive == 0)[0]
@@ -67,6 +116,18 @@ def write_particles(pset, time, iteration, params):
         out.append(''.join(row))
     with open(os.path.join(params.output_rep, params.output_file_name + '.txt'), 'a') as f:
         f.write(''.join(out))
+
+
+def write_stranded_particles_txt(outputdir, file_name, stranded_positions, stranded_count):
+    """Stranded positions recorded since the last flush (write_particles.pyx:339-388):
+    the file is rewritten at each flush that has records."""
+    if stranded_count <= 0:
+        return
+    os.makedirs(outputdir, exist_ok=True)
+    rows = np.asarray(stranded_positions)[:stranded_count]
+    with open(os.path.join(outputdir, file_name + '.txt'), 'w') as f:
+        f.write('cylag stranded particles file\nVariables: Xp, Yp, Zp\n')
+        f.write(''.join('{:.16e}, {:.16e}, {:.16e}\n'.format(r[0], r[1], r[2]) for r in rows))
 
 
 class ParticlesIO:

@@ -13,7 +13,7 @@ import numpy as np
 
 from . import _hip, random_utils
 from .control_shapes import ControlSection, section_segments
-from .particles_io import prepare_output, write_particles
+from .particles_io import prepare_output, write_particles, write_stranded_particles_txt
 from .write_statistics import prepare_secstats_output, write_secstats_buffer
 from .parameters import Parameters
 from .simutime import SimuTime
@@ -28,7 +28,7 @@ def _params_struct(p, pset):
         float(p.rebound_damping_coef), float(p.wall_roughness_angle),
         float(p.stranding_probability), int(bool(pset.compute_depth)),
         int(bool(p.frozen_eulerian_fields)), int(p.lsm3_edge_opt), float(p.diffusion_lsm3_tl_horizontal),
-        float(p.diffusion_lsm3_tl_vertical))
+        float(p.diffusion_lsm3_tl_vertical), int(bool(p.stranding_output)))
 
 
 def _pconst_struct(ps):
@@ -276,6 +276,8 @@ class Solver:
                 self._bnd_stats_times[self._bnd_stats_count] = times[s]
                 self._bnd_stats_data[self._bnd_stats_count, :nopen] = stats[s * nopen:(s + 1) * nopen]
                 self._bnd_stats_count += 1
+        if p.stranding_output and p.stranding_probability > 0.:
+            self._collect_stranded()
         if self.control_sections:
             nc = p.ncsec
             counts = np.zeros(max(1, nsteps * nc), dtype=np.int32)
@@ -291,6 +293,24 @@ class Solver:
                 if p.listing_printout_period and it % p.listing_printout_period == 0:
                     st.progress()
         return t0
+
+    def _collect_stranded(self):
+        """Append the positions of the rows stranded by the last call to the
+        particle set's buffer, in the reference's order (compute_bc.pyx:107-122)."""
+        ps = self.particle_set
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.cyl_stranded_get(self.ctx.h, None, 0, C.byref(n)), "cyl_stranded_get")
+        if n.value == 0:
+            return
+        xyz = np.zeros((n.value, 3))
+        self.ctx.check(self.ctx.lib.cyl_stranded_get(self.ctx.h, _hip.ptr(xyz), n.value, C.byref(n)),
+                       "cyl_stranded_get")
+        for r in xyz:
+            if ps.stranded_count < ps.max_stranded:
+                ps.stranded_positions[ps.stranded_count] = r
+                ps.stranded_count += 1
+            else:
+                print("Warning: stranded particle buffer is full. Increase max_stranded in LagrangianParticleSet")
 
     def forward(self, time_step=None):
         """One time step (solver.pyx:413-585); time_step is unused, as in the reference."""
@@ -405,6 +425,11 @@ class Solver:
             self._bnd_flushed.append((self._bnd_stats_times[:n].copy(),
                                       self._bnd_stats_data[:n].copy()))
         self._bnd_stats_count = 0
+        if p.stranding_output:
+            ps = self.particle_set
+            write_stranded_particles_txt(p.output_rep, p.stranding_output_file_name, ps.stranded_positions,
+                                         ps.stranded_count)
+            ps.stranded_count = 0
         n = self._csec_stats_count
         if n:
             write_secstats_buffer(self._csec_stats_times, self._csec_stats_data, n, p.ncsec, p.output_rep,

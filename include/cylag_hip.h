@@ -41,6 +41,7 @@ typedef struct {
   int frozen;
   int lsm3_edge_opt;                                                  /* 1 series, 2 clipped closed form */
   double diffusion_lsm3_tl_horizontal, diffusion_lsm3_tl_vertical;  /* TL of the constant-K models */
+  int stranding_output;                                               /* record stranded positions */
 } cyl_params;
 
 /* Particle-set constants (lagrangian_particle_set.pyx:158-261,349-375). */
@@ -179,6 +180,9 @@ int cyl_last_timing(cyl_ctx *ctx, double *step_kernel_ms, double *total_ms, int 
 int cyl_control_sections_set(cyl_ctx *ctx, int nsec, const int *nseg_of, const double *seg);
 /* Counts of the last cyl_step call, [nsteps][nsec] (nout = nsteps * nsec). */
 int cyl_control_counts(cyl_ctx *ctx, int *out, int cap, int *nout);
+/* Stranded positions of the last cyl_step call (stranding_output; compute_bc.pyx:107-122):
+ * n records of (x, y, z) in the reference's order (steps in turn, rows in pool order). */
+int cyl_stranded_get(cyl_ctx *ctx, double *xyz, int cap, int *n);
 /* Which kernels advanced the particles in the last cyl_step / cyl_batch_step
  * call (no counterpart in the reference): launches of the fused kernels and of
  * the generic kernel that did the step (guarded reruns that found nothing to

@@ -151,3 +151,46 @@ def test_txt_writer_round_trip(tmp_path):
     np.testing.assert_array_equal(part.zp[1], [0., 0.])
     np.testing.assert_array_equal(part.vp[0], [0.2, 0.6])
     np.testing.assert_array_equal(part.get_trajectory(2), [[5.5, 6.0, 0.], [5.5, 6.0, 0.]])
+
+
+def test_vtk_writer_layout(tmp_path):
+    """writer_vtk.pyx:7-304: legacy-VTK vertex cells of the active rows, with
+    pool index, tag, elevation (3D), velocity, fluid velocity seen (LSM-2/3)."""
+    pos = np.array([[1.25, 2.5, -0.5], [3.0, 4.125, -1.0], [5.5, 6.0, -2.0]])
+    ps = cy.LagrangianParticleSet(pos, dim=3)
+    ps.velocity = np.array([[0.1, 0.2, 0.3], [0.3, 0.4, 0.5], [0.5, 0.6, 0.7]])
+    ps.inactive = np.array([0, 1, 0], dtype=np.int32)
+    ps.npart_active = 2
+    params = cy.Parameters(dict(output_rep=str(tmp_path), output_file_name="p", output_file=True,
+                                output_file_format='all', model=2))
+    prepare_output(ps, params.output_rep, params.output_file_format, params.output_file_name, params.model)
+    write_particles(ps, 12.5, 10, params)
+    lines = (tmp_path / "p_10.vtk").read_text().splitlines()
+    assert lines[:10] == ['# vtk DataFile Version 2.0', 'particle positions (Points 3D)', 'ASCII',
+                          'DATASET UNSTRUCTURED_GRID', 'FIELD FIELDDATA 2', 'TIME 1 1 DOUBLE', '12.5',
+                          'CYCLE 1 1 INT', '10', 'POINTS 2 FLOAT']
+    assert lines[10:12] == ['1.25 2.5 -0.5', '5.5 6.0 -2.0']
+    assert lines[12:17] == ['CELLS 2 4', '1 0', '1 1', 'CELL_TYPES 2', '1']
+    i = lines.index('SCALARS Index INT')
+    assert lines[i + 2:i + 4] == ['0', '2']
+    i = lines.index('SCALARS tag INT')
+    assert lines[i + 2:i + 4] == ['0', '2']
+    i = lines.index('SCALARS elevation FLOAT')
+    assert lines[i + 2:i + 4] == ['-0.5', '-2.0']
+    i = lines.index('VECTORS velocity FLOAT')
+    assert lines[i + 1:i + 3] == ['0.1 0.2 0.3', '0.5 0.6 0.7']
+    assert 'VECTORS fluid_velocity_seen FLOAT' in lines and 'SCALARS diameter FLOAT' not in lines
+    assert (tmp_path / "p.txt").exists()  # 'all' writes both
+
+
+def test_stranded_file_layout(tmp_path):
+    """write_particles.pyx:339-388: header, then x, y, z per stranded row."""
+    from cylag_amd.particles_io import write_stranded_particles_txt
+    buf = np.array([[1.5, 2.25, 0.], [3., 4., 0.], [9., 9., 9.]])
+    write_stranded_particles_txt(str(tmp_path), "stranded", buf, 2)
+    lines = (tmp_path / "stranded.txt").read_text().splitlines()
+    assert lines == ['cylag stranded particles file', 'Variables: Xp, Yp, Zp',
+                     '1.5000000000000000e+00, 2.2500000000000000e+00, 0.0000000000000000e+00',
+                     '3.0000000000000000e+00, 4.0000000000000000e+00, 0.0000000000000000e+00']
+    write_stranded_particles_txt(str(tmp_path), "none", buf, 0)
+    assert not (tmp_path / "none.txt").exists()

@@ -386,3 +386,33 @@ def test_control_sections_vortex(model):
     ref = np.array(o.csec_history)
     assert ref.sum() > 100
     np.testing.assert_array_equal(counts, ref)
+
+
+@pytest.mark.parametrize("scheme", [1, 4])
+def test_stranded_positions_output(scheme, tmp_path):
+    """stranding_output (compute_bc.pyx:107-122, write_particles.pyx:339-388):
+    flow toward the top wall with stranding probability 1, so every wall hit
+    strands whatever the draw; the recorded positions, in the reference's
+    order, equal the oracle's, and the flush writes them to the file."""
+    case = open_channel_case(u=0.3)
+    top, bot = np.nonzero(np.isclose(case.y, 100.))[0], np.nonzero(np.isclose(case.y, 0.))[0]
+    walls = np.concatenate([top, bot])
+    case.boundary_points = np.concatenate([case.boundary_points, np.stack([walls, 0 * walls], 1)]).astype(np.int32)
+    case.fields = dict(case.fields, velocity_y=np.full_like(case.fields["velocity_x"], 0.8))
+    params = dict(final_time=90., time_step=1., frozen_eulerian_fields=True, model=1, diffusion_model=0,
+                  time_scheme=scheme, stranding_probability=1.0, stranding_output=True, output_rep=str(tmp_path))
+    s = case.build_gpu(cy, params)
+    for _ in range(30):  # forward() steps, then a batched solve of the rest
+        s.forward()
+    s.solve()
+    r = case.run_oracle(R, params)
+    ps = s.particle_set
+    np.testing.assert_array_equal(ps.inactive, r.inactive)
+    np.testing.assert_array_equal(ps.tag, r.tag)
+    nr = r.solver.p.s.stranded_count
+    assert nr > 200
+    ref = r.solver.p.stranded[:nr]
+    got = np.loadtxt(tmp_path / "stranded_particles.txt", delimiter=',', skiprows=2, ndmin=2)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.)) <= TOL
+    assert ps.stranded_count == 0  # reset by the flush
