@@ -82,6 +82,9 @@ def check_setup(fs, ps, p):
         raise ValueError("Additional velocity only compatible with LSM1")
     if ps.additional_force and p.model == 1:
         raise ValueError("Additional force not compatible with LSM1")
+    if ps.additional_velocity or ps.additional_force:
+        # add_velocity_i / add_force_i of particle-set subclasses (lagrangian_particle_set.pyx:892-900)
+        raise ValueError("additional velocity / force fields are not supported by the device step")
     if p.diffusion_model in (2, 3) and (fs.fid_turbeng == -1 or fs.fid_dissip == -1):
         raise ValueError(
             "LSM: diffusion_model={} requires 'TURBULENT ENERG.' and 'DISSIPATION' fields in "
