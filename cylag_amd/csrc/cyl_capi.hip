@@ -1781,12 +1781,16 @@ static int ensure_fast(cyl_ctx* ctx) {
   HIPCHK(dalloc(ctx->fast_bufs, &te, m.nt));
   HIPCHK(dalloc(ctx->fast_bufs, &ed, m.ne));
   HIPCHK(dalloc(ctx->fast_bufs, &xy, m.nv));
+  WalkRec* wr;
+  HIPCHK(dalloc(ctx->fast_bufs, &wr, m.nt));
   HIPCHK(dalloc(ctx->fast_bufs, &ctx->d_nodes, m.nv));
   if (ctx->fields.dim == 3) HIPCHK(dalloc(ctx->fast_bufs, &ctx->d_nodes3, (size_t)m.nv * ctx->fields.nlayers));
   k_pack_elems<<<nblk(m.nt), BLK, 0, ctx->stream>>>(m, ctx->d_node_new, ef, en, te);
   k_pack_edges<<<nblk(std::max(m.ne, m.nv)), BLK, 0, ctx->stream>>>(m, ctx->d_node_new, ctx->d_node_old, ed, xy);
+  k_pack_walk<<<nblk(m.nt), BLK, 0, ctx->stream>>>(m, wr);
   HIPCHK(hipGetLastError());
   ctx->fm.ef = ef;
+  ctx->fm.wr = wr;
   ctx->fm.en = en;
   ctx->fm.te = te;
   ctx->fm.ed = ed;

@@ -33,8 +33,22 @@ struct __align__(32) NodeF {
   double u, v, K, pad;
 };
 
+// Walk record of an element (32 B): its edges in increasing id order (the
+// reference's triedge order), for each edge the element-local indices of the
+// edge's two vertices in the edge's own (v0, v1) order, and the element across
+// the edge (-1 on the boundary).  The walk then tests an edge with the vertex
+// coordinates already loaded from the current element's ElemF, in exactly the
+// operand order of the edge table.
+struct __align__(32) WalkRec {
+  int e[3];
+  int vv;     // bits 4i..4i+1: local index of edge i's v0; 4i+2..4i+3: of its v1
+  int nb[3];
+  int pad;
+};
+
 struct FastMesh {
   const ElemF* ef;
+  const WalkRec* wr;
   const int4* en;    // node ids
   const int4* te;    // edge ids of the element, increasing
   const int4* ed;    // edge: v0, v1, t0, t1
@@ -80,6 +94,28 @@ __global__ void __launch_bounds__(BLK) k_pack_elems(MeshDev m, const int* nnew, 
   te[k] = make_int4(m.triedge[3 * k], m.triedge[3 * k + 1], m.triedge[3 * k + 2], 0);
 }
 
+__global__ void __launch_bounds__(BLK) k_pack_walk(MeshDev m, WalkRec* wr) {
+  int k = blockIdx.x * BLK + threadIdx.x;
+  if (k >= m.nt) return;
+  WalkRec r;
+  r.vv = 0;
+  r.pad = 0;
+  for (int i = 0; i < 3; ++i) {
+    int j = m.triedge[3 * k + i];
+    int v0 = m.edges[2 * j], v1 = m.edges[2 * j + 1];
+    int a = 0, b = 0;
+    for (int q = 0; q < 3; ++q) {
+      if (m.tri[3 * k + q] == v0) a = q;
+      if (m.tri[3 * k + q] == v1) b = q;
+    }
+    r.e[i] = j;
+    r.vv |= (a | (b << 2)) << (4 * i);
+    int t0 = m.edgetri[2 * j], t1 = m.edgetri[2 * j + 1];
+    r.nb[i] = (t0 == k) ? t1 : t0;
+  }
+  wr[k] = r;
+}
+
 __global__ void __launch_bounds__(BLK) k_pack_edges(MeshDev m, const int* nnew, const int* old, int4* ed,
                                                     double2* xy) {
   int j = blockIdx.x * BLK + threadIdx.x;
@@ -100,36 +136,55 @@ __device__ __forceinline__ int in_tri_fast(const FastMesh& fm, int k, double x, 
   return in_elem(fm.ef[kk], x, y) ? k : -1;
 }
 
-// xy_localize_point_from_path on the packed tables (xylocalizer.pyx:338-435)
-__device__ inline int walk_fast(const FastMesh& fm, int k, double p0x, double p0y, double x, double y,
-                                int& bnd_j) {
+__device__ __forceinline__ double pick3(int i, double a0, double a1, double a2) {
+  return i == 0 ? a0 : (i == 1 ? a1 : a2);
+}
+
+// xy_localize_point_from_path on the packed tables (xylocalizer.pyx:338-435),
+// starting in element k (k >= 0) whose record is e0.  Each step needs the
+// current element's walk record only: the edge end points are its own
+// vertices, and the record of the next element is loaded for its
+// point-in-triangle test and kept for the next step's edges.
+__device__ inline int walk_fast(const FastMesh& fm, int k, const ElemF& e0, double p0x, double p0y, double x,
+                                double y, int& bnd_j) {
   int new_k = -1, niter = 0, last_edge = -1;
   bnd_j = -1;
+  double vx0 = e0.vx[0], vx1 = e0.vx[1], vx2 = e0.vx[2];
+  double vy0 = e0.vy[0], vy1 = e0.vy[1], vy2 = e0.vy[2];
   while (new_k == -1 && niter <= 50) {
     bool hit = false;
     int kk = k < 0 ? k + fm.nt : k;
-    int4 te = fm.te[kk];
-    int tev[3] = {te.x, te.y, te.z};
+    const WalkRec w = fm.wr[kk];
     for (int i = 0; i < 3; ++i) {
-      int j = tev[i];
+      int j = w.e[i];
       if (j == last_edge) continue;
-      int4 ed = fm.ed[j];
-      double2 a = fm.xy[ed.x], b = fm.xy[ed.y];
-      if (seg_intersect(p0x, p0y, x, y, a.x, a.y, b.x, b.y)) {
+      int ia = (w.vv >> (4 * i)) & 3, ib = (w.vv >> (4 * i + 2)) & 3;
+      double ax = pick3(ia, vx0, vx1, vx2), ay = pick3(ia, vy0, vy1, vy2);
+      double bx = pick3(ib, vx0, vx1, vx2), by = pick3(ib, vy0, vy1, vy2);
+      if (seg_intersect(p0x, p0y, x, y, ax, ay, bx, by)) {
         hit = true;
-        int kn = (ed.z == k) ? ed.w : ed.z;
+        int kn = w.nb[i];
         if (kn == -1) {
           bnd_j = j;
           return -1;
         }
-        new_k = in_tri_fast(fm, kn, x, y);
-        if (new_k != -1) return new_k;
+        const ElemF& en = fm.ef[kn];
+        vx0 = en.vx[0]; vx1 = en.vx[1]; vx2 = en.vx[2];
+        vy0 = en.vy[0]; vy1 = en.vy[1]; vy2 = en.vy[2];
+        // point_in_triangle_dotp on the new element (in_elem)
+        bool in = pintri(x, y, vx0, vy0, vx1, vy1, vx2, vy2) >= 0 && pintri(x, y, vx1, vy1, vx2, vy2, vx0, vy0) >= 0 &&
+                  pintri(x, y, vx2, vy2, vx0, vy0, vx1, vy1) >= 0;
+        if (in) return kn;
         k = kn;
         last_edge = j;
         break;
       }
     }
-    if (!hit) return in_tri_fast(fm, k, x, y) == -1 ? -1 : k;
+    if (!hit) {
+      bool in = pintri(x, y, vx0, vy0, vx1, vy1, vx2, vy2) >= 0 && pintri(x, y, vx1, vy1, vx2, vy2, vx0, vy0) >= 0 &&
+                pintri(x, y, vx2, vy2, vx0, vy0, vx1, vy1) >= 0;
+      return in ? k : -1;
+    }
     ++niter;
   }
   return new_k;
@@ -141,7 +196,7 @@ __device__ __forceinline__ int localize_fast(const FastMesh& fm, const MeshDev& 
   bnd_j = -1;
   int nk = in_tri_fast(fm, k, x, y);
   if (nk != -1) return nk;
-  nk = walk_fast(fm, k, p0x, p0y, x, y, bnd_j);
+  nk = walk_fast(fm, k, fm.ef[k < 0 ? k + fm.nt : k], p0x, p0y, x, y, bnd_j);
   if (nk == -1 && bnd_j == -1) nk = locate_grid(m, x, y);
   return nk;
 }
@@ -534,7 +589,7 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   // xy_localize method 3 after its point-in-triangle test of tri0
   int bnd_j = -1, tri = tri0;
   if (!in_elem(ef, x, y)) {
-    tri = walk_fast(a.fm, tri0, x0, y0, x, y, bnd_j);
+    tri = walk_fast(a.fm, tri0, ef, x0, y0, x, y, bnd_j);
     if (tri == -1 && bnd_j == -1) tri = locate_grid(a.m, x, y);
   }
   bool deleted = false;

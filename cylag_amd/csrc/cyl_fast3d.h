@@ -236,7 +236,7 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
   // localization from (tri0, x0) along the path, then the layer
   int bnd_j = -1, tri = tri0;
   if (!in_elem(ef, x, y)) {
-    tri = walk_fast(a.fm, tri0, x0, y0, x, y, bnd_j);
+    tri = walk_fast(a.fm, tri0, ef, x0, y0, x, y, bnd_j);
     if (tri == -1 && bnd_j == -1) tri = locate_grid(a.m, x, y);
   }
   int ll = tri >= 0 ? z_localize3(a3, a.fm.ef[tri], a.fm.en[tri], tri, x, y, z) : PART_LOC_O;
