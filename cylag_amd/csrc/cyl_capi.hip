@@ -1619,10 +1619,11 @@ static void enqueue_scan(cyl_ctx* ctx, const T* in, int64_t* out, int64_t n, int
 static void enqueue_rank(cyl_ctx* ctx, const int* cond, int& launches) {
   int n = ctx->n;
   const PoolDev& A = ctx->pool[ctx->cur];
-  k_scatter_active<<<nblk(n), BLK, 0, ctx->stream>>>(A.inactive, ctx->d_gpool, n, ctx->d_vals, cond);
+  const unsigned gg = std::min<unsigned>(nblk(n), GUARD_GRID);
+  k_scatter_active<<<gg, BLK, 0, ctx->stream>>>(A.inactive, ctx->d_gpool, n, ctx->d_vals, cond);
   enqueue_scan(ctx, ctx->d_vals, ctx->d_scan, (int64_t)n, ctx->d_bsum, ctx->d_total, (const int64_t*)nullptr, cond,
                launches);
-  k_gather_rank<<<nblk(n), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_gpool, ctx->d_scen, ctx->d_scen_base, n,
+  k_gather_rank<<<gg, BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_gpool, ctx->d_scen, ctx->d_scen_base, n,
                                                   ctx->d_rank, nullptr, cond);
   k_scen_totals<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_total, ctx->d_scen_base, ctx->nscen,
                                                            n, ctx->d_scen_active, cond);
@@ -2232,7 +2233,8 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     int* g = ctx->d_flag;
     k_rerun_reset<<<nblk(std::max(nstat, ns)), BLK, 0, ctx->stream>>>(
         ctx->d_stream_end, ctx->d_err_step, a.bnd_stats, nstat, ns, ctx->d_n_deleted, g, a.strand_count);
-    k_undelete<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->pool[0].inactive, ctx->d_del_step, ctx->n,
+    const unsigned gg = std::min<unsigned>(nblk(ctx->n), GUARD_GRID);
+    k_undelete<<<gg, BLK, 0, ctx->stream>>>(ctx->pool[0].inactive, ctx->d_del_step, ctx->n,
                                                       ctx->step_counter, g, a.clock);
     a.general = 0;
     a.guarded = 1;
@@ -2244,7 +2246,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     }
     enqueue_scan(ctx, ctx->d_vals, ctx->d_scan, (int64_t)ctx->n, ctx->d_bsum, ctx->d_total, (const int64_t*)nullptr,
                  g, launches);
-    k_gather_rank<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_gpool, ctx->d_scen, ctx->d_scen_base,
+    k_gather_rank<<<gg, BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_gpool, ctx->d_scen, ctx->d_scen_base,
                                                          ctx->n, nullptr, ctx->d_off, g);
     k_scen_totals<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_total, ctx->d_scen_base, ns, ctx->n,
                                                      ctx->d_scen_units, g);

@@ -231,11 +231,15 @@ __global__ void __launch_bounds__(BLK) k_time_interp(const double* const* f0, co
 // ---------------------------------------------------------------------------
 // Pool-order scans
 // ---------------------------------------------------------------------------
+// The per-slot kernels guarded by a device flag run grid-stride on a capped
+// grid (GUARD_GRID blocks): when the flag is clear, they end as fast as an
+// empty launch instead of retiring one block per 256 slots.
+constexpr int GUARD_GRID = 2048;
+
 __global__ void __launch_bounds__(BLK) k_scatter_active(const int* inactive, const int* gpool, int n,
                                                         int* vals, const int* cond) {
   if (cond && *cond == 0) return;
-  int j = blockIdx.x * BLK + threadIdx.x;
-  if (j < n) vals[gpool[j]] = inactive[j] == 0 ? 1 : 0;
+  for (int j = blockIdx.x * BLK + threadIdx.x; j < n; j += gridDim.x * BLK) vals[gpool[j]] = inactive[j] == 0 ? 1 : 0;
 }
 
 // block-local exclusive scan into int64, block totals out; elements at or past
@@ -356,12 +360,12 @@ __global__ void __launch_bounds__(BLK) k_gather_rank(const int64_t* scan, const 
                                                      const int* scen_base, int n, int* rank, int64_t* off,
                                                      const int* cond) {
   if (cond && *cond == 0) return;
-  int j = blockIdx.x * BLK + threadIdx.x;
-  if (j >= n) return;
-  int s = scen ? scen[j] : 0;
-  int64_t v = scan[gpool[j]] - scan[scen_base[s]];
-  if (rank) rank[j] = (int)v;
-  if (off) off[j] = v;
+  for (int j = blockIdx.x * BLK + threadIdx.x; j < n; j += gridDim.x * BLK) {
+    int s = scen ? scen[j] : 0;
+    int64_t v = scan[gpool[j]] - scan[scen_base[s]];
+    if (rank) rank[j] = (int)v;
+    if (off) off[j] = v;
+  }
 }
 
 // total units of each scenario after a scan: scan[end] - scan[base]
@@ -547,8 +551,8 @@ __global__ void __launch_bounds__(BLK) k_undelete(int* inactive, const int* del_
                                                   const int* cond, const int* clock) {
   if (cond && *cond == 0) return;
   if (clock) step_id = clock[0];
-  int j = blockIdx.x * BLK + threadIdx.x;
-  if (j < n && inactive[j] == 2 && del_step[j] == step_id) inactive[j] = 0;
+  for (int j = blockIdx.x * BLK + threadIdx.x; j < n; j += gridDim.x * BLK)
+    if (inactive[j] == 2 && del_step[j] == step_id) inactive[j] = 0;
 }
 
 // Control sections (solver.pyx:540-549, control_shapes.pyx:56-85): a row that
