@@ -497,7 +497,8 @@ __global__ void __launch_bounds__(BLK) k_slot_of_pool(const int* gpool, int n, i
 // clock (graph replays): the device-side step counters, advanced here
 __global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long* stream_end, int nscen, int* flag,
                          int* n_deleted, long long* n_deleted_total, int step_id, int* step_done,
-                         unsigned long long* err_step, unsigned long long* err, int* clock, int* strand_count) {
+                         unsigned long long* err_step, unsigned long long* err, int* clock, int* strand_count,
+                         int* unchecked) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nscen) {
     unsigned long long e = stream_end[s];
@@ -510,6 +511,7 @@ __global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long*
     if (*n_deleted) flag[1] = 1;
     *n_deleted_total += *n_deleted;
     *n_deleted = 0;
+    if (unchecked && flag[0]) *unchecked = 1;  // mispredicted in a step run without the rerun
     flag[0] = 0;
     int sid = clock ? clock[0] : step_id;
     *step_done = sid + 1;
