@@ -2234,7 +2234,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
                                                          a.batch_step, ctx->d_scen_units, a.clock);
       ++launches;
       if (any_fast) {
-        k_gen_chunks<<<1, 1, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, a.scen_end, ctx->d_pregen,
+        k_gen_chunks<<<1, BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, a.scen_end, ctx->d_pregen,
                                                a.batch_step, ns, ctx->d_gen_base, a.clock);
         ++launches;
       }

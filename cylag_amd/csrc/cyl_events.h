@@ -80,8 +80,10 @@ __global__ void __launch_bounds__(BLK) k_ev_scan(EventArgs e) {
   __shared__ JumpTable8 t8;
   __shared__ uint32_t kn[128];
   const int s0 = scen_of(e.chunk_base, e.nscen, (int64_t)blockIdx.x * BLK);
-  // the tables of the first scenario of the block (all scenarios share inc
-  // only in single-scenario runs, so other scenarios use their own table)
+  const uint64_t inc0 = e.stream[s0].inc;
+  // the byte-window table of the block's first scenario; it depends on the
+  // stream increment only, so it serves every scenario of that increment (one
+  // stream advanced per scenario), the others jump with their own bit table
   {
     const uint64_t* src = reinterpret_cast<const uint64_t*>(&e.jt8[s0]);
     uint64_t* dst = reinterpret_cast<uint64_t*>(&t8);
@@ -111,7 +113,7 @@ __global__ void __launch_bounds__(BLK) k_ev_scan(EventArgs e) {
       Pcg g;
       g.inc = st.inc;
       if (METHOD == 2) {
-        g.state = s == s0 ? pcg_jump8(st.base, (uint64_t)p0, t8, e.jt[s]) : pcg_jump(st.base, (uint64_t)p0, e.jt[s]);
+        g.state = st.inc == inc0 ? pcg_jump8(st.base, (uint64_t)p0, t8, e.jt[s]) : pcg_jump(st.base, (uint64_t)p0, e.jt[s]);
         for (int q = 0; q < ZCHUNK; ++q) {
           uint64_t at = g.state;
           if (zig_fast(pcg_next(g), kn)) continue;
@@ -131,7 +133,7 @@ __global__ void __launch_bounds__(BLK) k_ev_scan(EventArgs e) {
         // polar: attempt a uses positions 2a, 2a+1
         int64_t a0 = p0 / 2;
         uint64_t d = (uint64_t)(2 * a0);
-        g.state = s == s0 ? pcg_jump8(st.base, d, t8, e.jt[s]) : pcg_jump(st.base, d, e.jt[s]);
+        g.state = st.inc == inc0 ? pcg_jump8(st.base, d, t8, e.jt[s]) : pcg_jump(st.base, d, e.jt[s]);
         for (int q = 0; q < ZCHUNK / 2; ++q) {
           uint32_t u1 = pcg_next(g), u2 = pcg_next(g);
           if (polar_accept(u1, u2)) continue;

@@ -315,8 +315,11 @@ __global__ void __launch_bounds__(BLK) k_draw_start(FastArgs a) {
     else pos = d.polar_position(u);
     // a position past the scanned window is reported if the slot draws (-1)
     a.pos0[j] = (RNG != 1 && pos >= st.window) ? -1 : pos;
-    a.state0[j] = s == 0 ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[0])
-                         : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
+    // the byte-window table depends on the stream increment only: scenarios on
+    // streams of scenario 0's increment (one stream advanced per scenario) use
+    // the staged copy
+    a.state0[j] = st.inc == a.stream[0].inc ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
+                                            : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
   });
 }
 
@@ -338,24 +341,44 @@ __device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int nscen
 
 // chunk_base[s] = first generator chunk of scenario s: ceil(active / GEN_PER)
 // chunks for scenarios of the fused kernel that draw this step, none otherwise
-__global__ void k_gen_chunks(const StreamDev* st, const int64_t* nact, const int* scen_end, const int* pregen,
-                             int step_id, int nscen, int64_t* chunk_base, const int* clock) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ void __launch_bounds__(BLK) k_gen_chunks(const StreamDev* st, const int64_t* nact, const int* scen_end,
+                                                    const int* pregen, int step_id, int nscen, int64_t* chunk_base,
+                                                    const int* clock) {
+  // one block: exclusive scan of the scenarios' chunk counts, BLK at a time
+  __shared__ int64_t part[BLK];
+  __shared__ int64_t carry;
   if (clock) step_id = clock[1];
-  int64_t c = 0;
-  for (int s = 0; s < nscen; ++s) {
-    chunk_base[s] = c;
-    bool ended = scen_end && step_id >= scen_end[s];
-    if ((!pregen || pregen[s]) && !ended && st[s].units_per_part > 0) c += (nact[s] + GEN_PER - 1) / GEN_PER;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < nscen; base += BLK) {
+    const int s = base + threadIdx.x;
+    int64_t v = 0;
+    if (s < nscen) {
+      bool ended = scen_end && step_id >= scen_end[s];
+      if ((!pregen || pregen[s]) && !ended && st[s].units_per_part > 0) v = (nact[s] + GEN_PER - 1) / GEN_PER;
+    }
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < BLK; o <<= 1) {
+      int64_t t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (s < nscen) chunk_base[s] = carry + part[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == BLK - 1) carry += part[BLK - 1];
+    __syncthreads();
   }
-  chunk_base[nscen] = c;
+  if (threadIdx.x == 0) chunk_base[nscen] = carry;
 }
 
 template <int RNG>
 __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   __shared__ JumpTable8 t8;
   __shared__ ZigTables zt;
-  __shared__ int t8_scen;
+  __shared__ uint64_t t8_inc;
+  __shared__ int t8_set;
   if (RNG == 2) {
     for (int q = threadIdx.x; q < 128; q += BLK) {
       zt.kn[q] = a.zt->kn[q];
@@ -363,7 +386,7 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
       zt.fn[q] = a.zt->fn[q];
     }
   }
-  if (threadIdx.x == 0) t8_scen = -1;
+  if (threadIdx.x == 0) t8_set = 0;
   __syncthreads();
   const int64_t nchunk = a.chunk_base[a.nscen];
   const int64_t NONE = 0x7fffffffffffffffLL;
@@ -371,15 +394,19 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   int emax_s = -1;
   bool mixed = false;
   // the block's BLK chunks of an iteration are consecutive, so they mostly
-  // belong to one scenario: its byte-window jump table is staged in LDS
-  // (reloaded when the block moves to another scenario); threads on chunks
-  // of another scenario read that scenario's table from memory
+  // belong to one scenario: its byte-window jump table, which depends on the
+  // stream increment only, is staged in LDS (reloaded when the block reaches a
+  // scenario of another increment); threads on chunks of a scenario with
+  // another increment read its table from memory
   for (int64_t cb = (int64_t)blockIdx.x * BLK; cb < nchunk; cb += (int64_t)gridDim.x * BLK) {
     int sb = scen_of(a.chunk_base, a.nscen, cb);
-    if (sb != t8_scen) {  // block-uniform
+    if (!t8_set || a.stream[sb].inc != t8_inc) {  // block-uniform
       __syncthreads();
       load_jump8(t8, a.jt8 + sb);
-      if (threadIdx.x == 0) t8_scen = sb;
+      if (threadIdx.x == 0) {
+        t8_inc = a.stream[sb].inc;
+        t8_set = 1;
+      }
       __syncthreads();
     }
     int64_t c = cb + threadIdx.x;
@@ -405,8 +432,8 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
     int err = (RNG != 1 && pos >= st.window) ? E_RNG_WINDOW : 0;
     Pcg g;
     g.inc = st.inc;
-    g.state = s == sb ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
-                      : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
+    g.state = st.inc == t8_inc ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
+                               : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
     for (int64_t r = r0; r < r1; ++r) {
       double v[2];
       if (RNG == 1) {
