@@ -735,9 +735,10 @@ static int alloc_slot_bufs(cyl_ctx* ctx, int n, int nscen_pad) {
   Bufs& bb = ctx->part_bufs;
   HIPCHK(dalloc(bb, &ctx->d_gpool, n));
   HIPCHK(dalloc(bb, &ctx->d_rank, n));
-  // transposed normals: 32 x (chunks) entries, chunks <= n/32 + 1 per scenario
-  HIPCHK(dalloc(bb, &ctx->d_xi, (size_t)n + 32 * (size_t)nscen_pad));
-  ctx->xi_cap = (int64_t)n + 32 * (int64_t)nscen_pad;
+  // normals in tiles of 64 chunks x 32: chunks <= n/32 + 1 per scenario,
+  // rounded up to a whole tile
+  HIPCHK(dalloc(bb, &ctx->d_xi, (size_t)n + 32 * (size_t)nscen_pad + 64 * 32));
+  ctx->xi_cap = (int64_t)n + 32 * (int64_t)nscen_pad + 64 * 32;
   HIPCHK(dalloc(bb, &ctx->d_off, n));
   HIPCHK(dalloc(bb, &ctx->d_init_tri, n));
   HIPCHK(dalloc(bb, &ctx->d_del_step, n));
@@ -1064,8 +1065,8 @@ extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const
   if (e) return e;
   if (!ctx->d_scen) HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scen, ctx->n));
   // normals buffer for the per-scenario chunk padding (see k_gen_normals)
-  HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)ctx->n + 32 * ((size_t)nscen + 2)));
-  ctx->xi_cap = (int64_t)ctx->n + 32 * ((int64_t)nscen + 2);
+  HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)ctx->n + 32 * ((size_t)nscen + 2) + 64 * 32));
+  ctx->xi_cap = (int64_t)ctx->n + 32 * ((int64_t)nscen + 2) + 64 * 32;
   k_scen_of_slot<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_gpool, ctx->d_scen_base, nscen, ctx->n, ctx->d_scen);
   ctx->rng_state.assign(state, state + nscen);
   ctx->rng_inc.assign(inc, inc + nscen);

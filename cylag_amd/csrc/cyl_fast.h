@@ -329,14 +329,21 @@ __global__ void __launch_bounds__(BLK) k_draw_start(FastArgs a) {
 // finds the run's first position once (event table), jumps there, then draws
 // sequentially: a Ziggurat normal that starts on a slow event takes the
 // event's precomputed value and skips its positions; every other normal is a
-// one-draw fast normal.  The pair of rank r = GEN_PER c_local + k is stored at
-// xi[k * nchunk + c] so that each of the thread's stores is coalesced across
-// the wavefront.  Indices reach 32 * nchunk - 1, where nchunk <= (active rows) /
-// 32 + (scenarios): the buffer holds rows + 32 * (scenarios + 2) pairs.
+// one-draw fast normal.  The pair of rank r = GEN_PER c_local + k (chunk c =
+// chunk_base[s] + c_local) is stored in tiles of 64 chunks: xi[((c / 64) *
+// GEN_PER + k) * 64 + c % 64], so that each of a wave's stores is one coalesced
+// 1 KB row and its 32 stores stay within 32 KB.  Indices stay below nchunk
+// rounded up to 64, times 32, where nchunk <= (active rows) / 32 + (scenarios):
+// the buffer holds rows + 32 * (scenarios + 2) + 64 * 32 pairs.
 constexpr int GEN_PER = 32;
 
+__device__ __forceinline__ int64_t xi_tile_index(int64_t c, int64_t k) {
+  return ((c >> 6) * GEN_PER + k) * 64 + (c & 63);
+}
+
 __device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int nscen, int s, int64_t r) {
-  return (r % GEN_PER) * chunk_base[nscen] + chunk_base[s] + r / GEN_PER;
+  (void)nscen;
+  return xi_tile_index(chunk_base[s] + r / GEN_PER, r % GEN_PER);
 }
 
 // chunk_base[s] = first generator chunk of scenario s: ceil(active / GEN_PER)
@@ -471,7 +478,7 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
         v[0] = x1 * w;
         v[1] = x2 * w;
       }
-      int64_t xidx = (r - r0) * nchunk + c;
+      int64_t xidx = xi_tile_index(c, r - r0);
       if (xidx < a.xi_cap) a.xi[xidx] = make_double2(v[0], v[1]);
     }
     // reported against pool index 0: the rank's slot is not known here
