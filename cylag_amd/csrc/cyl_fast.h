@@ -346,6 +346,27 @@ __device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int nscen
   return xi_tile_index(chunk_base[s] + r / GEN_PER, r % GEN_PER);
 }
 
+// Max of v into base[key] for the lanes of a wave (lanes with key < 0 or
+// v == 0 take no part): one atomic per distinct key in the wave.  Called by
+// every lane of the wave.
+__device__ __forceinline__ void wave_max_by_key(int key, unsigned long long v, unsigned long long* base) {
+  bool pending = key >= 0 && v != 0;
+  for (;;) {
+    const unsigned long long m = __ballot(pending);
+    if (m == 0) break;
+    const int leader = __ffsll((long long)m) - 1;
+    const int k0 = __shfl(key, leader);
+    const bool mine = pending && key == k0;
+    unsigned long long x = mine ? v : 0ull;
+    for (int o = 32; o > 0; o >>= 1) {
+      unsigned long long y = __shfl_xor(x, o);
+      x = y > x ? y : x;
+    }
+    if ((int)(threadIdx.x & 63) == leader) atomicMax(&base[k0], x);
+    if (mine) pending = false;
+  }
+}
+
 // chunk_base[s] = first generator chunk of scenario s: ceil(active / GEN_PER)
 // chunks for scenarios of the fused kernel that draw this step, none otherwise
 __global__ void __launch_bounds__(BLK) k_gen_chunks(const StreamDev* st, const int64_t* nact, const int* scen_end,
@@ -380,6 +401,79 @@ __global__ void __launch_bounds__(BLK) k_gen_chunks(const StreamDev* st, const i
   if (threadIdx.x == 0) chunk_base[nscen] = carry;
 }
 
+// Normals of generator chunk c (scenario s): the pairs of its ranks, in the
+// serial stream order from the chunk's first position.  Returns the stream
+// position after its last draw (0 if the scenario draws nothing this step).
+template <int RNG>
+__device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64_t c, int s, const JumpTable8& t8,
+                                                        uint64_t t8_inc, const ZigTables& zt) {
+  const int64_t NONE = 0x7fffffffffffffffLL;
+  const StreamDev& st = a.stream[s];
+  if (st.units_per_part == 0) return 0;
+  int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
+  int64_t u = r0 * st.units_per_part;
+  Draws d;
+  d.st = &st;
+  int64_t pos, ev = 0, next_key = NONE;
+  if (RNG == 1) {
+    pos = u;
+  } else if (RNG == 2) {
+    ev = d.events_before(u);
+    pos = u + (ev > 0 ? st.ev_extra[ev - 1] : 0);
+    next_key = ev < st.n_ev ? st.ev_key[ev] : NONE;
+  } else {
+    pos = 2 * (u + d.events_before(u));
+  }
+  int err = (RNG != 1 && pos >= st.window) ? E_RNG_WINDOW : 0;
+  Pcg g;
+  g.inc = st.inc;
+  g.state = st.inc == t8_inc ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
+                             : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
+  for (int64_t r = r0; r < r1; ++r) {
+    double v[2];
+    if (RNG == 1) {
+      v[0] = rect_normal(g);
+      v[1] = rect_normal(g);
+      pos += 2;
+    } else if (RNG == 2) {
+      for (int k = 0; k < 2; ++k, ++u) {
+        if (pos >= st.window) err = E_RNG_WINDOW;
+        if (u == next_key) {
+          int64_t len = st.ev_extra[ev] - (ev > 0 ? st.ev_extra[ev - 1] : 0) + 1;
+          v[k] = st.ev_val[ev];
+          for (int64_t q = 0; q < len; ++q) pcg_next(g);
+          pos += len;
+          ++ev;
+          next_key = ev < st.n_ev ? st.ev_key[ev] : NONE;
+        } else {
+          int32_t hz = (int32_t)pcg_next(g);
+          uint32_t iz = (uint32_t)hz & 127u;
+          uint32_t ahz = hz < 0 ? (uint32_t)(-(int64_t)hz) : (uint32_t)hz;
+          if (ahz >= zt.kn[iz]) err = E_RNG_EVENTS;  // a slow start missing from the table
+          v[k] = hz * zt.wn[iz];
+          pos += 1;
+        }
+      }
+    } else {
+      double x1 = 0., x2 = 0., w = 2.0;
+      while (w >= 1.0 || w == 0.0) {
+        x1 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
+        x2 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
+        w = x1 * x1 + x2 * x2;
+        pos += 2;
+      }
+      w = sqrt((-2.0 * cr_log(w)) / w);
+      v[0] = x1 * w;
+      v[1] = x2 * w;
+    }
+    int64_t xidx = xi_tile_index(c, r - r0);
+    if (xidx < a.xi_cap) a.xi[xidx] = make_double2(v[0], v[1]);
+  }
+  // reported against pool index 0: the rank's slot is not known here
+  if (err) atomicMin(&a.err[s], (unsigned long long)(unsigned)err);
+  return (unsigned long long)pos;
+}
+
 template <int RNG>
 __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   __shared__ JumpTable8 t8;
@@ -396,10 +490,6 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   if (threadIdx.x == 0) t8_set = 0;
   __syncthreads();
   const int64_t nchunk = a.chunk_base[a.nscen];
-  const int64_t NONE = 0x7fffffffffffffffLL;
-  unsigned long long emax = 0;
-  int emax_s = -1;
-  bool mixed = false;
   // the block's BLK chunks of an iteration are consecutive, so they mostly
   // belong to one scenario: its byte-window jump table, which depends on the
   // stream increment only, is staged in LDS (reloaded when the block reaches a
@@ -417,91 +507,15 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
       __syncthreads();
     }
     int64_t c = cb + threadIdx.x;
-    if (c >= nchunk) continue;
-    int s = (c < a.chunk_base[sb + 1]) ? sb : scen_of(a.chunk_base, a.nscen, c);
-    if (a.scen_end && batch_step_of(a) >= a.scen_end[s]) continue;
-    const StreamDev& st = a.stream[s];
-    if (st.units_per_part == 0) continue;
-    int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
-    int64_t u = r0 * st.units_per_part;
-    Draws d;
-    d.st = &st;
-    int64_t pos, ev = 0, next_key = NONE;
-    if (RNG == 1) {
-      pos = u;
-    } else if (RNG == 2) {
-      ev = d.events_before(u);
-      pos = u + (ev > 0 ? st.ev_extra[ev - 1] : 0);
-      next_key = ev < st.n_ev ? st.ev_key[ev] : NONE;
-    } else {
-      pos = 2 * (u + d.events_before(u));
+    int s = -1;
+    unsigned long long pend = 0;
+    if (c < nchunk) {
+      s = (c < a.chunk_base[sb + 1]) ? sb : scen_of(a.chunk_base, a.nscen, c);
+      if (!(a.scen_end && batch_step_of(a) >= a.scen_end[s])) pend = gen_chunk<RNG>(a, c, s, t8, t8_inc, zt);
     }
-    int err = (RNG != 1 && pos >= st.window) ? E_RNG_WINDOW : 0;
-    Pcg g;
-    g.inc = st.inc;
-    g.state = st.inc == t8_inc ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
-                               : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
-    for (int64_t r = r0; r < r1; ++r) {
-      double v[2];
-      if (RNG == 1) {
-        v[0] = rect_normal(g);
-        v[1] = rect_normal(g);
-        pos += 2;
-      } else if (RNG == 2) {
-        for (int k = 0; k < 2; ++k, ++u) {
-          if (pos >= st.window) err = E_RNG_WINDOW;
-          if (u == next_key) {
-            int64_t len = st.ev_extra[ev] - (ev > 0 ? st.ev_extra[ev - 1] : 0) + 1;
-            v[k] = st.ev_val[ev];
-            for (int64_t q = 0; q < len; ++q) pcg_next(g);
-            pos += len;
-            ++ev;
-            next_key = ev < st.n_ev ? st.ev_key[ev] : NONE;
-          } else {
-            int32_t hz = (int32_t)pcg_next(g);
-            uint32_t iz = (uint32_t)hz & 127u;
-            uint32_t ahz = hz < 0 ? (uint32_t)(-(int64_t)hz) : (uint32_t)hz;
-            if (ahz >= zt.kn[iz]) err = E_RNG_EVENTS;  // a slow start missing from the table
-            v[k] = hz * zt.wn[iz];
-            pos += 1;
-          }
-        }
-      } else {
-        double x1 = 0., x2 = 0., w = 2.0;
-        while (w >= 1.0 || w == 0.0) {
-          x1 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
-          x2 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
-          w = x1 * x1 + x2 * x2;
-          pos += 2;
-        }
-        w = sqrt((-2.0 * cr_log(w)) / w);
-        v[0] = x1 * w;
-        v[1] = x2 * w;
-      }
-      int64_t xidx = xi_tile_index(c, r - r0);
-      if (xidx < a.xi_cap) a.xi[xidx] = make_double2(v[0], v[1]);
-    }
-    // reported against pool index 0: the rank's slot is not known here
-    if (err) atomicMin(&a.err[s], (unsigned long long)(unsigned)err);
-    if (emax_s >= 0 && emax_s != s) {
-      atomicMax(&a.stream_end[emax_s], emax);  // a thread crossing into another scenario
-      mixed = true;
-      emax = 0;
-    }
-    emax_s = s;
-    emax = (unsigned long long)pos > emax ? (unsigned long long)pos : emax;
-  }
-  // one atomic per wavefront when its threads ended in one scenario
-  int s0 = __shfl(emax_s, 0);
-  bool same = __all(emax_s == s0 || emax_s < 0) && !__any(mixed);
-  if (same) {
-    for (int o = 32; o > 0; o >>= 1) {
-      unsigned long long q = __shfl_xor(emax, o);
-      emax = q > emax ? q : emax;
-    }
-    if ((threadIdx.x & 63) == 0 && emax && s0 >= 0) atomicMax(&a.stream_end[s0], emax);
-  } else if (emax_s >= 0 && emax) {
-    atomicMax(&a.stream_end[emax_s], emax);
+    // the stream of each scenario advances past its last draw: one atomic per
+    // scenario present in the wave
+    wave_max_by_key(s, pend, a.stream_end);
   }
 }
 
