@@ -272,8 +272,8 @@ __device__ __forceinline__ int batch_step_of(const FastArgs& a) { return a.clock
 // blocks).  Blocks are dispatched round-robin over the 8 XCDs, so block b
 // takes the chunks of the (b % 8)-th eighth of the slots: neighbouring slots
 // (neighbouring elements once sorted) share one XCD's L2.
-template <class F>
-__device__ __forceinline__ void for_slots_xcd(int n, F&& f) {
+template <class F, class G>
+__device__ __forceinline__ void for_slots_xcd(int n, F&& f, G&& g) {
   const int nx = 8;
   int xcd = blockIdx.x % nx, lb = blockIdx.x / nx, per = gridDim.x / nx;
   int64_t nch = ((int64_t)n + BLK - 1) / BLK;
@@ -282,7 +282,13 @@ __device__ __forceinline__ void for_slots_xcd(int n, F&& f) {
     int64_t chunk = xcd * cpx + c;
     int64_t j = chunk * BLK + threadIdx.x;
     if (chunk < nch && j < n) f((int)j);
+    g();  // every lane, once per iteration (the trip count is block-uniform)
   }
+}
+
+template <class F>
+__device__ __forceinline__ void for_slots_xcd(int n, F&& f) {
+  for_slots_xcd(n, f, [] {});
 }
 
 __device__ __forceinline__ void load_jump8(JumpTable8& t8, const JumpTable8* src) {
@@ -344,27 +350,6 @@ __device__ __forceinline__ int64_t xi_tile_index(int64_t c, int64_t k) {
 __device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int nscen, int s, int64_t r) {
   (void)nscen;
   return xi_tile_index(chunk_base[s] + r / GEN_PER, r % GEN_PER);
-}
-
-// Max of v into base[key] for the lanes of a wave (lanes with key < 0 or
-// v == 0 take no part): one atomic per distinct key in the wave.  Called by
-// every lane of the wave.
-__device__ __forceinline__ void wave_max_by_key(int key, unsigned long long v, unsigned long long* base) {
-  bool pending = key >= 0 && v != 0;
-  for (;;) {
-    const unsigned long long m = __ballot(pending);
-    if (m == 0) break;
-    const int leader = __ffsll((long long)m) - 1;
-    const int k0 = __shfl(key, leader);
-    const bool mine = pending && key == k0;
-    unsigned long long x = mine ? v : 0ull;
-    for (int o = 32; o > 0; o >>= 1) {
-      unsigned long long y = __shfl_xor(x, o);
-      x = y > x ? y : x;
-    }
-    if ((int)(threadIdx.x & 63) == leader) atomicMax(&base[k0], x);
-    if (mine) pending = false;
-  }
 }
 
 // chunk_base[s] = first generator chunk of scenario s: ceil(active / GEN_PER)
@@ -743,33 +728,19 @@ __device__ __forceinline__ void fast_step_body(const FastArgs& a) {
     }
   }
   __syncthreads();
-  unsigned long long e = 0;
-  int es = -1;
-  for_slots_xcd(a.n, [&](int jl) {
-    int j = a.j0 + jl;
-    unsigned long long q = fast_particle<MODEL, RNG, PREGEN>(a, j, zt);
-    if (!PREGEN && q) {
-      int s = a.scen ? a.scen[j] : 0;
-      if (es >= 0 && es != s) {
-        atomicMax(&a.stream_end[es], e);  // the thread moved on to another scenario
-        e = 0;
-      }
-      es = s;
-      e = q > e ? q : e;
-    }
-  });
+  StreamEndAcc acc;
+  for_slots_xcd(
+      a.n,
+      [&](int jl) {
+        int j = a.j0 + jl;
+        unsigned long long q = fast_particle<MODEL, RNG, PREGEN>(a, j, zt);
+        if (!PREGEN && q) acc.add(a.scen ? a.scen[j] : 0, q);
+      },
+      [&] {
+        if (!PREGEN) acc.flush(a.stream_end);
+      });
   if (PREGEN) return;  // the normal generator advanced the streams
-  int s0 = __shfl(es, 0);
-  bool same = __all(es == s0 || es < 0);
-  if (same) {
-    for (int o = 32; o > 0; o >>= 1) {
-      unsigned long long q = __shfl_xor(e, o);
-      e = q > e ? q : e;
-    }
-    if ((threadIdx.x & 63) == 0 && e && s0 >= 0) atomicMax(&a.stream_end[s0], e);
-  } else if (es >= 0 && e) {
-    atomicMax(&a.stream_end[es], e);
-  }
+  acc.done(a.stream_end);
 }
 
 // 5 waves per SIMD (<= 96 VGPRs) with pre-generated normals, 4 with the inline sampler

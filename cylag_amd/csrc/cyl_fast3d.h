@@ -367,32 +367,16 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
     }
   }
   __syncthreads();
-  unsigned long long e = 0;
-  int es = -1;
-  for_slots_xcd(a.n, [&](int jl) {
-    int j = a.j0 + jl;
-    unsigned long long qe = fast3d_particle<MODEL, RNG>(a3, j, zt);
-    if (qe) {
-      int s = a.scen ? a.scen[j] : 0;
-      if (es >= 0 && es != s) {
-        atomicMax(&a.stream_end[es], e);  // the thread moved on to another scenario
-        e = 0;
-      }
-      es = s;
-      e = qe > e ? qe : e;
-    }
-  });
-  int s0 = __shfl(es, 0);
-  bool same = __all(es == s0 || es < 0);
-  if (same) {
-    for (int o = 32; o > 0; o >>= 1) {
-      unsigned long long qq = __shfl_xor(e, o);
-      e = qq > e ? qq : e;
-    }
-    if ((threadIdx.x & 63) == 0 && e && s0 >= 0) atomicMax(&a.stream_end[s0], e);
-  } else if (es >= 0 && e) {
-    atomicMax(&a.stream_end[es], e);
-  }
+  StreamEndAcc acc;
+  for_slots_xcd(
+      a.n,
+      [&](int jl) {
+        int j = a.j0 + jl;
+        unsigned long long qe = fast3d_particle<MODEL, RNG>(a3, j, zt);
+        if (qe) acc.add(a.scen ? a.scen[j] : 0, qe);
+      },
+      [&] { acc.flush(a.stream_end); });
+  acc.done(a.stream_end);
 }
 
 }  // namespace cyl

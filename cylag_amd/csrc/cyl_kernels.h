@@ -183,40 +183,71 @@ __device__ __forceinline__ unsigned long long step_slot(const StepArgs& a, int j
   return dr.live ? (unsigned long long)(dr.pos0 + dr.ncalls) : 0ull;
 }
 
+// Max of v into base[key] for the lanes of a wave (lanes with key < 0 or
+// v == 0 take no part): one atomic per distinct key in the wave.  Called by
+// every lane of the wave.
+__device__ __forceinline__ void wave_max_by_key(int key, unsigned long long v, unsigned long long* base) {
+  bool pending = key >= 0 && v != 0;
+  for (;;) {
+    const unsigned long long m = __ballot(pending);
+    if (m == 0) break;
+    const int leader = __ffsll((long long)m) - 1;
+    const int k0 = __shfl(key, leader);
+    const bool mine = pending && key == k0;
+    unsigned long long x = mine ? v : 0ull;
+    for (int o = 32; o > 0; o >>= 1) {
+      unsigned long long y = __shfl_xor(x, o);
+      x = y > x ? y : x;
+    }
+    if ((int)(threadIdx.x & 63) == leader) atomicMax(&base[k0], x);
+    if (mine) pending = false;
+  }
+}
+
+// Running stream end of the scenario whose rows a thread steps.  A thread
+// that moves on to another scenario parks the finished one's end; flush(),
+// called by every lane of the wave once per loop iteration, merges parked ends
+// with one atomic per scenario in the wave, and done() merges the rest.
+struct StreamEndAcc {
+  unsigned long long e = 0, pe = 0;
+  int es = -1, ps = -1;
+  __device__ __forceinline__ void add(int s, unsigned long long q) {
+    if (es >= 0 && es != s) {  // at most once per iteration: flush() follows each
+      ps = es;
+      pe = e;
+      e = 0;
+    }
+    es = s;
+    e = q > e ? q : e;
+  }
+  __device__ __forceinline__ void flush(unsigned long long* base) {
+    if (__any(pe != 0)) {
+      wave_max_by_key(ps, pe, base);
+      pe = 0;
+      ps = -1;
+    }
+  }
+  __device__ __forceinline__ void done(unsigned long long* base) { wave_max_by_key(es, e, base); }
+};
+
 // Grid-stride over the slots of the launch (the grid is at most what fits on
 // the device at once, so a guarded launch that finds nothing to do ends at once).
 template <int DIM, int MODEL, bool COUNT>
 __global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
   if (a.guarded && a.flag[0] == 0) return;
-  unsigned long long e = 0;
-  int es = -1;
+  StreamEndAcc acc;
+  // the trip count is block-uniform, so every lane reaches flush()
   for (int64_t b = (int64_t)blockIdx.x * BLK; b < a.n; b += (int64_t)gridDim.x * BLK) {
     int64_t jl = b + threadIdx.x;
-    if (jl >= a.n) continue;
-    int scen = -1;
-    unsigned long long q = step_slot<DIM, MODEL, COUNT>(a, a.j0 + (int)jl, scen);
-    if (!COUNT && q) {
-      if (es >= 0 && es != scen) {
-        atomicMax(&a.stream_end[es], e);  // the thread moved on to another scenario
-        e = 0;
-      }
-      es = scen;
-      e = q > e ? q : e;
+    if (jl < a.n) {
+      int scen = -1;
+      unsigned long long q = step_slot<DIM, MODEL, COUNT>(a, a.j0 + (int)jl, scen);
+      if (!COUNT && q) acc.add(scen, q);
     }
+    if (!COUNT) acc.flush(a.stream_end);
   }
   if (COUNT) return;
-  // stream end: one atomic per wavefront when its rows share a scenario
-  int s0 = __shfl(es, 0);
-  bool same = __all(es == s0 || es < 0);
-  if (same) {
-    for (int o = 32; o > 0; o >>= 1) {
-      unsigned long long q = __shfl_xor(e, o);
-      e = q > e ? q : e;
-    }
-    if ((threadIdx.x & 63) == 0 && e && s0 >= 0) atomicMax(&a.stream_end[s0], e);
-  } else if (es >= 0 && e) {
-    atomicMax(&a.stream_end[es], e);
-  }
+  acc.done(a.stream_end);
 }
 
 // tmp[fid][i] = tf*frame_rec[i] + (1-tf)*frame_rec1[i]  (eulerian_field_set.pyx:926-938)
