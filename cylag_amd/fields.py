@@ -88,6 +88,54 @@ class EulerianFieldSet:
                                               init_trifinder, friction_model, friction_coefficient)
 
     @staticmethod
+    def from_hdf5(fset_file="cylag_fset_file.h5", mesh_file="cylag_mesh_file.h5", optional_fields=None,
+                  init_trifinder=True, friction_model=1, friction_coefficient=40.):
+        """Field set from a CyLag field-set file and mesh file
+        (eulerian_field_set.pyx:486-566)."""
+        from .hdf5 import H5Reader
+        from .mesh import TriangularMesh
+        mesh = TriangularMesh.from_hdf5(file_name=mesh_file)
+        trifinder = None
+        if init_trifinder:
+            import matplotlib.tri as mtri
+            trifinder = mtri.Triangulation(mesh.x, mesh.y, mesh.triangles).get_trifinder()
+        f = H5Reader(fset_file)
+        dim, nplan = int(f["dim"][0]), int(f["nplan"][0])
+        if dim not in (2, 3):
+            raise ValueError("Wrong dimension in fset_file")
+        velz = f["velocity_z"] if dim == 3 else None
+        opt = None
+        if int(f["nopt"][0]) != 0:
+            if optional_fields is None:
+                if "opt_fields_names" in f:
+                    optional_fields = [n.decode("utf-8") if isinstance(n, bytes) else str(n)
+                                       for n in f["opt_fields_names"]]
+                else:
+                    optional_fields = ["opt_field_{}".format(i) for i in range(int(f["nopt"][0]))]
+            opt = f["opt_fields"]
+        else:
+            optional_fields = None
+        return EulerianFieldSet(mesh, dim, nplan if dim == 3 else 1, f["times"], f["elevation"],
+                                f["velocity_x"], f["velocity_y"], velz, opt, optional_fields, trifinder,
+                                friction_model=friction_model, friction_coefficient=friction_coefficient)
+
+    def save_hdf5(self, fset_file="cylag_fset_file.h5", mesh_file="cylag_mesh_file.h5"):
+        """Write the mesh file and the field-set file (eulerian_field_set.pyx:568-604);
+        the optional field names are stored as fixed-length byte strings."""
+        from .hdf5 import write_h5
+        self.triangular_mesh.save_mesh_hdf5(mesh_file)
+        d = dict(dim=np.array([self.dim], dtype=np.int64), nplan=np.array([self.nlayers], dtype=np.int64),
+                 times=np.asarray(self.times), elevation=np.asarray(self.elevation),
+                 velocity_x=np.asarray(self.velocity_x), velocity_y=np.asarray(self.velocity_y),
+                 nopt=np.array([self.nopt], dtype=np.int64))
+        if self.dim == 3:
+            d["velocity_z"] = np.asarray(self.velocity_z)
+        if self.nopt != 0:
+            d["opt_fields"] = np.asarray(self.opt_fields)
+            d["opt_fields_names"] = [str(n) for n in self.opt_fields_names]
+        write_h5(fset_file, d)
+
+    @staticmethod
     def _from_telemac(file_name, dim, names, bnd_file, optional_fields, init_trifinder, friction_model,
                       friction_coefficient):
         from .mesh import TriangularMesh, read_cli

@@ -76,6 +76,29 @@ class TriangularMesh:
                               boundary_points=boundary_points, boundary_edges=boundary_edges,
                               nopen=nopen)
 
+    # the tables a mesh file holds, in the reference's names
+    HDF5_TABLES = ("x", "y", "triangles", "edges", "vertex_adjacency", "triangle_adjacency",
+                   "triangle_neighbors", "edgetri_map", "triedge_map", "boundary_points", "boundary_edges",
+                   "vertex_labels", "edges_labels", "signed_area", "gradx", "grady", "det",
+                   "triangle_centroids")
+
+    @staticmethod
+    def from_hdf5(file_name="cylag_mesh_file.h5"):
+        """Mesh from a CyLag mesh file: the tables are taken as stored, not
+        rebuilt (triangular_mesh.pyx:204-244)."""
+        from .hdf5 import H5Reader
+        f = H5Reader(file_name)
+        t = {k: (f[k] if k in f else None) for k in TriangularMesh.HDF5_TABLES}
+        return TriangularMesh(nopen=int(f["nopen"][0]), **t)
+
+    def save_mesh_hdf5(self, file_name="cylag_mesh_file.h5"):
+        """Write the mesh tables to a CyLag mesh file (triangular_mesh.pyx:246-286);
+        the triangle adjacency tables only when present."""
+        from .hdf5 import write_h5
+        d = {k: np.asarray(getattr(self, k)) for k in self.HDF5_TABLES if getattr(self, k, None) is not None}
+        d["nopen"] = np.array([self.nopen], dtype=np.int64)
+        write_h5(file_name, d)
+
     def _sides(self):
         t = self.triangles
         a = t.reshape(-1)
@@ -87,8 +110,8 @@ class TriangularMesh:
         return np.ascontiguousarray(np.unique(np.stack([lo, hi], axis=1), axis=0), dtype=np.int32)
 
     def _init_vertex_adjacency(self):
-        """Per-vertex triangle lists; the reference caps them at
-        MAX_VERTEX_ADJACENCY and raises beyond (triangular_mesh.pyx:311-367)."""
+        """(nv, MAX_VERTEX_ADJACENCY) table of the triangles of each vertex;
+        the reference raises beyond MAX_VERTEX_ADJACENCY (triangular_mesh.pyx:311-367)."""
         counts = np.bincount(self.triangles.reshape(-1), minlength=self.nv)
         if counts.size and counts.max() > MAX_VERTEX_ADJACENCY:
             over = np.where(counts > MAX_VERTEX_ADJACENCY)[0]
@@ -98,7 +121,16 @@ class TriangularMesh:
                 "Affected vertices ({} of {} shown): {}\n".format(
                     len(over), MAX_VERTEX_ADJACENCY, int(counts.max()), min(5, len(over)),
                     len(over), over[:5].tolist()))
-        return counts
+        # triangles of each vertex in increasing order (the reference's loop
+        # over triangles appends them that way), -1 past the last
+        flat = self.triangles.reshape(-1).astype(np.int64)
+        order = np.argsort(flat, kind="stable")
+        vert = flat[order]
+        first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        slot = np.arange(flat.size) - first[vert]
+        table = np.full((self.nv, MAX_VERTEX_ADJACENCY), -1, dtype=np.int32)
+        table[vert, slot] = (order // 3).astype(np.int32)
+        return table
 
     def _init_triangles_edges_connectivity(self):
         nt, ne, nv = self.nt, self.ne, self.nv

@@ -9,8 +9,7 @@ cylag/extra/particle_source.pyx and polygon_area (cylag/geom/utils.pyx:200-225).
 import numpy as np
 
 from . import random_utils
-
-INTERN_REF = 0  # label of an interior mesh vertex (constants.pyx)
+from .constants import INTERN_REF  # label of an interior mesh vertex (-1)
 
 
 # ---------------------------------------------------------------------------

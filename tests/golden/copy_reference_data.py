@@ -1,6 +1,7 @@
 """Copy the data files of the reference's own tests that the host tests use
 as fixtures (TELEMAC result and boundary files, the triangulation exported by
-the reference's reader, a particle result file) into tests/golden/reference_data.
+the reference's reader, a particle result file, the mesh tables the reference
+saved to HDF5 from the TELEMAC flume) into tests/golden/reference_data.
 
     python tests/golden/copy_reference_data.py /root/reference/tests/data
 """
@@ -9,7 +10,7 @@ import shutil
 import sys
 
 FILES = ["r2d_flume.slf", "geo_flume.cli", "test_triangulation.slf", "test_triangulation_meshx.txt",
-         "test_triangulation_meshy.txt", "test_triangulation_meshtri.txt", "particles_2d.txt"]
+         "test_triangulation_meshy.txt", "test_triangulation_meshtri.txt", "particles_2d.txt", "geo_flume.h5"]
 
 if __name__ == "__main__":
     src = sys.argv[1]
