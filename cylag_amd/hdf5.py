@@ -307,7 +307,7 @@ def _datatype_message(dt):
         return struct.pack("<B3BIHHBBBBI", 0x11, 0x20, sign, 0, dt.itemsize, 0, 8 * dt.itemsize,
                            exp_loc, exp_size, 0, mant, bias)
     if dt.kind == "S":
-        return struct.pack("<B3BI", 0x13, 0, 0, 0, max(dt.itemsize, 1))
+        return struct.pack("<B3BI", 0x13, 1, 0, 0, max(dt.itemsize, 1))  # null-padded ASCII
     raise NotImplementedError("dtype %s" % dt)
 
 
@@ -340,7 +340,10 @@ def write_h5(file_name, datasets):
     for n in names:
         name_off.append(len(heap))
         heap += _pad8(n.encode("utf-8") + b"\0")
-    heap += bytes(8)  # never empty past the last name
+    # one free block closes the data segment, as the HDF5 library writes it:
+    # (offset of the next free block, 1 = none; size of this block)
+    free_off = len(heap)
+    heap += struct.pack("<QQ", 1, 16)
     snods = [names[i:i + 2 * LEAF_K] for i in range(0, len(names), 2 * LEAF_K)] or [[]]
     if len(snods) > 2 * NODE_K:
         raise NotImplementedError("more than %d datasets in one file" % (2 * NODE_K * 2 * LEAF_K))
@@ -373,7 +376,7 @@ def write_h5(file_name, datasets):
         data_addr.append(alloc(len(_pad8(a.tobytes()))) if a.nbytes else UNDEF)
 
     put(root_hdr, _object_header([_message(MSG_SYMTAB, struct.pack("<QQ", btree, heap_hdr))]))
-    put(heap_hdr, b"HEAP" + struct.pack("<B3xQQQ", 0, len(heap), UNDEF, heap_data))
+    put(heap_hdr, b"HEAP" + struct.pack("<B3xQQQ", 0, len(heap), free_off, heap_data))
     put(heap_data, bytes(heap))
     node = bytearray(b"TREE" + struct.pack("<BBHQQ", 0, 0, len(snods), UNDEF, UNDEF))
     node += struct.pack("<Q", 0)

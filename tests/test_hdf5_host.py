@@ -128,3 +128,28 @@ def test_source_from_field_dual_cells():
         r = np.hypot(pos[s:s + n, 0] - m.x[k], pos[s:s + n, 1] - m.y[k])
         assert r.max() <= np.sqrt(area[k] / np.pi) * (1 + 1e-12)
         s += n
+
+
+def _heap_free_list(f):
+    """(data segment size, free-list head, next of the head block) of the root group's local heap."""
+    b = f.buf
+    root = int.from_bytes(b[24 + 4 * 8 + 8:24 + 4 * 8 + 16], "little")
+    for t, d, _ in f._messages(root):
+        if t == 0x11:
+            hp = int.from_bytes(b[d + 8:d + 16], "little")
+            size, head = int.from_bytes(b[hp + 8:hp + 16], "little"), int.from_bytes(b[hp + 16:hp + 24], "little")
+            data = int.from_bytes(b[hp + 24:hp + 32], "little")
+            return size, head, int.from_bytes(b[data + head:data + head + 8], "little")
+    raise AssertionError("no symbol table")
+
+
+def test_writer_heap_free_list_as_library(tmp_path):
+    """The HDF5 library rejects a local heap whose free-list head is not
+    inside the data segment, and ends the list with offset 1: the reference's
+    file (written by the library) and ours agree on both."""
+    size, head, nxt = _heap_free_list(H5Reader(DATA + "geo_flume.h5"))
+    assert head < size and nxt == 1
+    p = str(tmp_path / "mesh.h5")
+    flume_fset().triangular_mesh.save_mesh_hdf5(p)
+    size, head, nxt = _heap_free_list(H5Reader(p))
+    assert head + 16 <= size and nxt == 1
