@@ -117,12 +117,6 @@ def c3_workload(npts_x=1000, npts_y=250, nlayers=10, seed=7):
     return x, y, tri, h, fields
 
 
-def settling_diameters(rng, n, rho_p=2650.):
-    """Stokes diameters of settling velocities w_s ~ LN(median 1 mm/s, sigma 0.5)."""
-    ws = np.exp(np.log(1e-3) + 0.5 * rng.standard_normal(n))
-    return np.sqrt(18. * 1e-6 * ws / ((rho_p / 1000. - 1.) * 9.81))
-
-
 def bytes_per_particle_step(nt, nv, npart):
     """Algorithmic bytes per particle-step: the evolving state read and written
     once (x, y f64 + element id i32 = 20 B each way), plus the mesh and node
@@ -335,6 +329,7 @@ def c5_requests(cy, x, y, h, window, lam=20., seed=13, dt=10.):
         n = int(npart[i])
         ps = cy.LagrangianParticleSet(np.repeat([[px[i], py[i], pz[i]]], n, 0), dim=3, particle_density=2650.,
                                       buoyancy_velocity_model=1)
+        from tests.cases import settling_diameters
         d = settling_diameters(np.random.default_rng((seed, i)), n)
         ps.diameter = d
         ps.volume = np.pi * d ** 3 / 6.

@@ -149,6 +149,9 @@ _sig("ref_forward_n", C.c_int, [_SP, C.c_int])
 _sig("ref_initialize_model_coefficients", None, [C.POINTER(PsetS)])
 _sig("ref_buoyancy_velocity", C.c_double, [C.POINTER(PsetS), C.c_double, C.c_int, C.c_double])
 _sig("ref_drag_coefficient", C.c_double, [C.POINTER(PsetS), C.c_double, C.c_double])
+_sig("ref_interp_acceleration_2d", C.c_int, [C.POINTER(FsetS), _MP, C.c_int] + [C.c_double] * 5 + [_dp, _dp])
+_sig("ref_interp_acceleration_3d", C.c_int, [C.POINTER(FsetS), _MP, C.c_int, C.c_int] + [C.c_double] * 7
+     + [_dp, _dp, _dp])
 
 
 def P(a, kind):

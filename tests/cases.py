@@ -45,7 +45,7 @@ class Case:
     """mesh + fields + initial rows; `fields` is a dict of EulerianFieldSet kwargs."""
 
     def __init__(self, x, y, tri, fields, position, velocity=None, pool=None, boundary_points=None,
-                 nopen=0, use_trifinder=True, pset_kw=None, mpl_tri=None, pdim=None):
+                 nopen=0, use_trifinder=True, pset_kw=None, mpl_tri=None, pdim=None, diameter=None):
         self.x, self.y, self.tri = x, y, tri
         self.fields = fields
         self.pdim = pdim if pdim is not None else fields.get("dim", 2)
@@ -56,6 +56,7 @@ class Case:
         self.nopen = nopen
         self.use_trifinder = use_trifinder
         self.pset_kw = pset_kw or {}
+        self.diameter = diameter        # per-row diameters (None: the particle set's own)
         self._mpl = mpl_tri
 
     def trifinder(self):
@@ -73,6 +74,11 @@ class Case:
         fs = cy.EulerianFieldSet(mesh, trifinder=self.trifinder(), **self.fields)
         ps = cy.LagrangianParticleSet(self.position, self.velocity, dim=self.pdim,
                                       initial_pool_size=self.pool, **self.pset_kw)
+        if self.diameter is not None:
+            d = np.zeros(ps.npart)
+            d[:len(self.diameter)] = self.diameter
+            ps.diameter = d
+            ps.volume = np.pi * d ** 3 / 6.
         cy.pcg32_seed(*SEED)
         return cy.Solver(fs, ps, params, **solver_kw)
 
@@ -95,6 +101,11 @@ class Case:
         f = dict(self.fields)
         fs = R.RefFields(mesh, **{k: v for k, v in f.items() if k != "trifinder"})
         kw = {k: v for k, v in self.pset_kw.items()}
+        if self.diameter is not None:
+            n = self.pool if self.pool is not None else len(self.position)
+            d = np.zeros(n)
+            d[:len(self.diameter)] = self.diameter
+            kw["diameter"] = d
         ps = R.RefParticles(self.position, self.velocity, dim=self.pdim,
                             initial_pool_size=self.pool, **kw)
         rng = R.RefRng(*SEED)
@@ -266,3 +277,31 @@ def open_channel_case(n=21, npart=400, u=0.8, seed=5):
     rng = np.random.default_rng(seed)
     pos = np.stack([rng.uniform(5., 95., npart), rng.uniform(5., 95., npart)], 1)
     return Case(x, y, tri, fields, pos, pool=npart + 16, boundary_points=bp, nopen=2)
+
+
+def settling_diameters(rng, n, rho_p=2650.):
+    """Stokes diameters of settling velocities w_s ~ LN(median 1 mm/s, sigma 0.5)
+    for sediment of density rho_p (a way to draw a diameter spread; the step
+    itself uses the reference's constants)."""
+    ws = np.exp(np.log(1e-3) + 0.5 * rng.standard_normal(n))
+    return np.sqrt(18. * 1e-6 * ws / ((rho_p / 1000. - 1.) * 9.81))
+
+
+def settling_3d_case(npart=1500, seed=21, spread="regimes", **kw):
+    """vortex_3d_case with sediment (density 2650, analytic settling velocity)
+    of per-row diameters: "regimes" spreads them log-uniformly over
+    [5e-6, 3e-3] m, across the Stokes (< 1e-4), Ruby-Zanke (1e-4 - 1e-3) and
+    constant-drag (> 1e-3) branches of the settling law; "coarse" over
+    [1.5e-3, 5e-3] m (relaxation times of a tenth of a second, for the explicit
+    LSM-2 schemes); "lognormal" draws settling_diameters (most rows in the
+    Stokes branch)."""
+    case = vortex_3d_case(npart=npart, **kw)
+    rng = np.random.default_rng(seed)
+    if spread == "regimes":
+        case.diameter = np.exp(rng.uniform(np.log(5e-6), np.log(3e-3), npart))
+    elif spread == "coarse":
+        case.diameter = np.exp(rng.uniform(np.log(1.5e-3), np.log(5e-3), npart))
+    else:
+        case.diameter = settling_diameters(rng, npart)
+    case.pset_kw = dict(particle_density=2650., buoyancy_velocity_model=1)
+    return case

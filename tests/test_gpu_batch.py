@@ -1,12 +1,14 @@
 """Batched scenarios: each scenario's result in the batch equals the result of
-running it alone with a plain Solver (same stream, same steps): element ids,
-flags, tags and random stream identical, positions within 1e-9 relative."""
+running it alone with a plain Solver (same stream, same steps) and the
+oracle's serial run of it (the reference's step with that stream): element
+ids, flags, tags and random stream identical, positions within 1e-9
+relative."""
 import numpy as np
 import pytest
 
 import cylag_amd as cy
 from oracle import refpy as R
-from tests.cases import open_channel_case, vortex_case
+from tests.cases import open_channel_case, square_case, vortex_case
 
 pytestmark = pytest.mark.gpu
 
@@ -23,10 +25,29 @@ def streams(n):
     return out
 
 
-def make_pset(case, npart, seed):
+def pick(case, npart, seed):
     rng = np.random.default_rng(seed)
-    pos = case.position[rng.choice(case.position.shape[0], npart, replace=False)]
-    return cy.LagrangianParticleSet(pos.copy(), dim=2)
+    return case.position[rng.choice(case.position.shape[0], npart, replace=False)].copy()
+
+
+def make_pset(case, npart, seed):
+    return cy.LagrangianParticleSet(pick(case, npart, seed), dim=2)
+
+
+def run_oracle(case, pos, params, state, nsteps, trifinder=True, pset_kw=None, diameter=None):
+    """The scenario through the oracle: the reference's serial step on its own
+    pool, from the scenario's stream (solver_parallel.pyx:113-115 gives each
+    rank such a stream)."""
+    mesh = R.RefMesh(case.x, case.y, case.tri, boundary_points=case.boundary_points, nopen=case.nopen)
+    fs = R.RefFields(mesh, **case.fields)
+    ps = R.RefParticles(pos, dim=pos.shape[1], diameter=diameter, **(pset_kw or {}))
+    rng = R.RefRng()
+    rng.set_state(*state)
+    o = R.RefSolver(fs, ps, params, trifinder=case.trifinder() if trifinder else None, rng=rng)
+    o.forward(nsteps)
+    return dict(position=ps.position.copy(), tri=ps.tri.copy(), inactive=ps.inactive.copy(), tag=ps.tag.copy(),
+                velocity=ps.velocity.copy(), lowerlayer=ps.lowerlayer.copy(), rng=rng.state(),
+                bnd=np.array(o.bnd_history, dtype=np.int64).reshape(nsteps, mesh.nopen))
 
 
 def run_alone(case, pset, params, state, nsteps):
@@ -47,6 +68,8 @@ def compare(b, a, rng_b):
     assert np.array_equal(b.inactive, a["inactive"])
     assert np.array_equal(b.tri, a["tri"])
     assert np.array_equal(b.tag, a["tag"])
+    if "lowerlayer" in a:
+        assert np.array_equal(b.lowerlayer, a["lowerlayer"])
     scale = np.maximum(np.abs(a["position"]), 1.)
     assert np.max(np.abs(b.position - a["position"]) / scale) <= TOL
     vs = np.maximum(np.abs(a["velocity"]), 1.)
@@ -80,6 +103,7 @@ def test_batch_equals_alone(rng_method):
     for s, (p, n, k) in enumerate(MIX):
         alone = run_alone(case, make_pset(case, n, 100 + s), params[s], st[s], k)
         compare(out[s], alone, b.rng_state(s))
+        compare(out[s], run_oracle(case, pick(case, n, 100 + s), params[s], st[s], k), b.rng_state(s))
     b.ctx.close()
 
 
@@ -115,6 +139,12 @@ def test_batch_deletions_tags_and_statistics(tmp_path):
         compare(out[s], alone, b.rng_state(s))
         np.testing.assert_array_equal(stats[s], solo.bnd_stats[1][:k])
         solo.ctx.close()
+        if params[s].get("stranding_probability", 0.) == 0.:
+            # stranding draws come from per-row sub-streams on the device, so a
+            # scenario that strands is compared with its plain run only
+            ref = run_oracle(case, pick(case, n, 200 + s), params[s], st[s], k)
+            compare(out[s], ref, b.rng_state(s))
+            np.testing.assert_array_equal(stats[s], ref["bnd"])
     b.ctx.close()
 
 
@@ -157,8 +187,10 @@ def test_continuous_batching_equals_alone(rng_method):
         done[sid] = (b.collect(s), b.rng_state(s))
     b.close()
     for sid, (p, n, k, join) in enumerate(spec):
-        alone = run_alone(case, make_pset(case, n, 300 + sid), dict(params[sid], initial_time=join * dt), st[sid], k)
+        pj = dict(params[sid], initial_time=join * dt)
+        alone = run_alone(case, make_pset(case, n, 300 + sid), pj, st[sid], k)
         compare(done[sid][0], alone, done[sid][1])
+        compare(done[sid][0], run_oracle(case, pick(case, n, 300 + sid), pj, st[sid], k), done[sid][1])
 
 
 def test_batch_graph_replay_equals_alone():
@@ -178,5 +210,55 @@ def test_batch_graph_replay_equals_alone():
     out = b.results()
     for s, (p, n, k) in enumerate(MIX):
         alone = run_alone(case, make_pset(case, n, 100 + s), params[s], st[s], nsteps[s])
+        compare(out[s], alone, b.rng_state(s))
+        compare(out[s], run_oracle(case, pick(case, n, 100 + s), params[s], st[s], nsteps[s]), b.rng_state(s))
+    b.ctx.close()
+
+
+def square_scenarios(nscen=8, seed=11):
+    """Release scenarios on the plain workload's mesh (101 x 101 nodes, 20,000
+    triangles, uniform 0.1 m/s flow, walls): uneven particle counts
+    (log-uniform in [5, 5000]), uneven step counts, LSM-1 or 2D LSM-2 per
+    scenario, random interior release points and one on the centre vertex."""
+    rng = np.random.default_rng(seed)
+    npart = np.exp(rng.uniform(np.log(5), np.log(5000), nscen)).astype(int)
+    nsteps = rng.integers(10, 300, nscen)
+    model = np.array([1, 2] * (nscen // 2))
+    xy = np.stack([rng.uniform(5., 95., nscen), rng.uniform(5., 95., nscen)], 1)
+    xy[0] = 50.
+    return npart, nsteps, model, xy
+
+
+def square_params(model, rng_method):
+    p = dict(final_time=1e9, time_step=1., frozen_eulerian_fields=True, diffusion_model=1,
+             horizontal_diffusivity=0.01, rng_method=rng_method)
+    if model == 1:
+        p.update(model=1)
+    else:
+        p.update(model=2, time_scheme=5, diffusion_lsm2_option=1, particle_velocity_init=1)
+    return p
+
+
+@pytest.mark.parametrize("rng_method", [1, 2, 3])
+def test_batch_square_mesh_uneven_mixed(rng_method):
+    """Uneven scenarios of mixed LSM-1 / LSM-2 on the 20,000-triangle square,
+    each against the oracle and against its plain run alone; the device
+    locates the releases itself (no trifinder), as the batched workload does."""
+    case = square_case(n=101, npart=1)
+    case.use_trifinder = False
+    npart, nsteps, model, xy = square_scenarios()
+    params = [square_params(m, rng_method) for m in model]
+    st = streams(len(npart))
+    pos = [np.repeat(xy[s][None], npart[s], 0) for s in range(len(npart))]
+    mesh = cy.TriangularMesh(case.x, case.y, case.tri)
+    fs = cy.EulerianFieldSet(mesh, **case.fields)
+    b = cy.BatchSolver(fs, [cy.LagrangianParticleSet(q.copy(), dim=2) for q in pos], params, st, nsteps=nsteps,
+                       sort_interval=5)
+    b.run()
+    out = b.results()
+    for s in range(len(npart)):
+        ref = run_oracle(case, pos[s], params[s], st[s], int(nsteps[s]), trifinder=False)
+        compare(out[s], ref, b.rng_state(s))
+        alone = run_alone(case, cy.LagrangianParticleSet(pos[s].copy(), dim=2), params[s], st[s], int(nsteps[s]))
         compare(out[s], alone, b.rng_state(s))
     b.ctx.close()
