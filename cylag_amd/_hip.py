@@ -80,6 +80,8 @@ SIGNATURES = {
     "cyl_scenario_rng_get": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "cyl_batch_step": (C.c_int, [C.c_void_p, _dp, C.c_double, C.c_int, _ip]),
     "cyl_batch_collect": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _ip, _ip, _ip, _ip, C.c_int, _ip]),
+    "cyl_batch_collect_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, _ip, _ip, _ip, _ip, C.c_int,
+                                            _ip]),
     "cyl_batch_reshape": (C.c_int, [C.c_void_p, C.c_int, _ip, C.c_int, _ip, C.c_void_p, C.POINTER(C.c_uint64),
                                     C.POINTER(C.c_uint64), _ip, _dp, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _ip,
                                     C.c_double, C.c_double]),

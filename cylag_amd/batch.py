@@ -240,9 +240,12 @@ class BatchSolver:
         return st.value, inc.value
 
     # -- results ------------------------------------------------------------------
-    def collect(self, s):
+    def collect(self, s, allocate=True):
         """Write scenario s's rows back into its particle set (tags of deleted
-        rows renumbered in deletion order, lagrangian_particle_set.pyx:786-788)."""
+        rows renumbered in deletion order, lagrangian_particle_set.pyx:786-788).
+        allocate=False stages the rows in the library's static device array
+        instead of device memory allocated for the call (scenarios of at most
+        CYL_NOALLOC_MAX_ROWS rows)."""
         ps, dim = self.scen[s].pset, self.dim
         n = ps.npart
         h = ps._h
@@ -253,9 +256,10 @@ class BatchSolver:
         dele = np.zeros(max(n, 1), dtype=np.int32)
         cnt = C.c_int()
         P = _hip.ptr
-        self.ctx.check(self.ctx.lib.cyl_batch_collect(
+        fn = "cyl_batch_collect" if allocate else "cyl_batch_collect_noalloc"
+        self.ctx.check(getattr(self.ctx.lib, fn)(
             self.ctx.h, s, n, P(h["position"]), P(h["velocity"]), P(h["fluid_velocity_seen"]), P(h["inactive"], "i"),
-            P(h["tri"], "i"), P(h["lowerlayer"], "i"), P(dele, "i"), n, C.byref(cnt)), "cyl_batch_collect")
+            P(h["tri"], "i"), P(h["lowerlayer"], "i"), P(dele, "i"), n, C.byref(cnt)), fn)
         rows = dele[:cnt.value]
         tag = h["tag"]
         tag[rows] = np.arange(ps.maxtag + 1, ps.maxtag + 1 + len(rows), dtype=np.int32)

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -245,6 +246,38 @@ static uint64_t hilbert_index(uint32_t x, uint32_t y) {
 }
 
 // ---------------------------------------------------------------------------
+// Staging of cyl_batch_collect_noalloc: one file-scope device array per GPU
+// (the code object's own copy on each device), sized for CYL_NOALLOC_MAX_ROWS
+// rows of a 3D scenario.  Regions, each CYL_NOALLOC_MAX_ROWS entries long:
+// position, velocity, seen velocity (3 doubles per row each), element id,
+// layer id, inactive flag (int), deletion key (uint64).  Host code never
+// passes `scratch` itself to a kernel or copy: it uses the device address
+// hipGetSymbolAddress returns for the current device, fetched once per device
+// when a context is created there and kept in noalloc_ptr[].
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_ROWS = CYL_NOALLOC_MAX_ROWS;
+constexpr size_t NOALLOC_BYTES = NOALLOC_ROWS * (9 * sizeof(double) + 3 * sizeof(int) + sizeof(uint64_t));
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;                     // guards noalloc_ptr and serialises the noalloc calls
+static void* noalloc_ptr[NOALLOC_MAX_DEVICES];    // device address of `scratch` on each device
+static uint64_t noalloc_keys[NOALLOC_ROWS];       // host side of the deletion keys
+
+// With `device` current: fetch (once) the device address of this device's
+// copy of `scratch`.
+static hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (noalloc_ptr[device]) return hipSuccess;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  noalloc_ptr[device] = p;
+  return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
 extern "C" int cyl_ctx_create(int device, int flags, cyl_ctx** out) {
@@ -254,6 +287,7 @@ extern "C" int cyl_ctx_create(int device, int flags, cyl_ctx** out) {
   ctx->device = device;
   *out = ctx;
   HIPCHK(hipSetDevice(device));
+  HIPCHK(noalloc_scratch_init(device));
   HIPCHK(hipStreamCreate(&ctx->stream));
   HIPCHK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, device));
   HIPCHK(hipEventCreate(&ctx->ev0));
@@ -1241,6 +1275,115 @@ extern "C" int cyl_batch_collect(cyl_ctx* ctx, int s, int npart, double* pos, do
   *ndeleted = (int)del.size();
   for (int q = 0; q < (int)del.size() && q < cap; ++q)
     if (deleted) deleted[q] = del[q].second;
+  return CYL_OK;
+}
+
+// ---- cyl_batch_collect_noalloc: the same rows through the static staging array
+struct CollectOut {
+  PoolDev P;
+  const int *gpool, *scen, *del_step;
+  int s, base, n, dim, npart;
+  double *pos, *vel, *us;   // (npart, dim) each, in the scenario's pool order
+  int *tri, *ll, *inactive;
+  unsigned long long* key;  // deleted since the last collect: (deletion step << 32) | row; else ~0
+};
+
+// every output row cleared to an inactive row: the array keeps the previous call's contents
+__global__ void k_collect_clear(CollectOut a) {
+  for (int i = blockIdx.x * BLK + threadIdx.x; i < a.npart; i += gridDim.x * BLK) {
+    for (int c = 0; c < a.dim; ++c) {
+      a.pos[(size_t)i * a.dim + c] = 0.;
+      a.vel[(size_t)i * a.dim + c] = 0.;
+      a.us[(size_t)i * a.dim + c] = 0.;
+    }
+    a.tri[i] = -1;
+    a.ll[i] = 0;
+    a.inactive[i] = 1;
+    a.key[i] = ~0ull;
+  }
+}
+
+// the rows of scenario s in the output layout of cyl_batch_collect (inactive
+// rows zeroed); reported deletions become plain inactive rows
+__global__ void k_collect_rows(CollectOut a) {
+  int j = blockIdx.x * BLK + threadIdx.x;
+  if (j >= a.n || a.scen[j] != a.s) return;
+  int i = a.gpool[j] - a.base;
+  if (i < 0 || i >= a.npart) return;
+  int ina = a.P.inactive[j];
+  if (ina == 0) {
+    const double* px[3] = {a.P.x, a.P.y, a.P.z};
+    for (int c = 0; c < a.dim; ++c) {
+      a.pos[(size_t)i * a.dim + c] = px[c][j];
+      a.vel[(size_t)i * a.dim + c] = a.P.up[c][j];
+      a.us[(size_t)i * a.dim + c] = a.P.us[c][j];
+    }
+    a.tri[i] = a.P.tri[j];
+    a.ll[i] = a.P.ll[j];
+    a.inactive[i] = 0;
+  } else if (ina == 2) {
+    a.key[i] = ((unsigned long long)(unsigned)a.del_step[j] << 32) | (unsigned)i;
+    a.P.inactive[j] = 1;
+  }
+}
+
+extern "C" int cyl_batch_collect_noalloc(cyl_ctx* ctx, int s, int npart, double* pos, double* vel, double* us,
+                                         int* inactive, int* tri, int* ll, int* deleted, int cap, int* ndeleted) {
+  if (!ctx || !ctx->batched || s < 0 || s >= ctx->nscen || !ndeleted) return CYL_E_ARG;
+  int base = ctx->scen_base_h[s], np = ctx->scen_base_h[s + 1] - base, dim = ctx->dim;
+  if (npart != np) return ctx->set_err(CYL_E_ARG, -1, "npart differs from the scenario's pool size");
+  if ((size_t)np > NOALLOC_ROWS)
+    return ctx->set_err(CYL_E_ARG, -1,
+                        "scenario of " + std::to_string(np) + " rows exceeds CYL_NOALLOC_MAX_ROWS (" +
+                            std::to_string(NOALLOC_ROWS) + "); use cyl_batch_collect");
+  if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES || !noalloc_ptr[ctx->device])
+    return ctx->set_err(CYL_E_STATE, -1, "staging array of this device not initialised");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  *ndeleted = 0;
+  if (np == 0) return CYL_OK;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  unsigned char* base_ptr = (unsigned char*)noalloc_ptr[ctx->device];
+  const size_t R = NOALLOC_ROWS;
+  CollectOut a{};
+  a.pos = (double*)base_ptr;
+  a.vel = a.pos + 3 * R;
+  a.us = a.vel + 3 * R;
+  a.tri = (int*)(a.us + 3 * R);
+  a.ll = a.tri + R;
+  a.inactive = a.ll + R;
+  a.key = (unsigned long long*)(a.inactive + R);
+  a.P = ctx->pool[ctx->cur];
+  a.gpool = ctx->d_gpool;
+  a.scen = ctx->d_scen;
+  a.del_step = ctx->d_del_step;
+  a.s = s;
+  a.base = base;
+  a.n = ctx->n;
+  a.dim = dim;
+  a.npart = np;
+  k_collect_clear<<<std::min(nblk(np), (unsigned)(4 * ctx->ncu)), BLK, 0, ctx->stream>>>(a);
+  HIPCHK(hipGetLastError());
+  k_collect_rows<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(a);
+  HIPCHK(hipGetLastError());
+  const size_t nd = (size_t)np * dim;
+  if (pos) HIPCHK(hipMemcpyAsync(pos, a.pos, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (vel) HIPCHK(hipMemcpyAsync(vel, a.vel, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (us) HIPCHK(hipMemcpyAsync(us, a.us, nd * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  if (tri) HIPCHK(hipMemcpyAsync(tri, a.tri, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (ll) HIPCHK(hipMemcpyAsync(ll, a.ll, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  if (inactive)
+    HIPCHK(hipMemcpyAsync(inactive, a.inactive, (size_t)np * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(noalloc_keys, a.key, (size_t)np * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  // deletion order: by step, then by row (the serial loop's order within a step)
+  size_t nk = 0;
+  for (int i = 0; i < np; ++i)
+    if (noalloc_keys[i] != ~0ull) noalloc_keys[nk++] = noalloc_keys[i];
+  std::sort(noalloc_keys, noalloc_keys + nk);
+  *ndeleted = (int)nk;
+  if (deleted)
+    for (size_t q = 0; q < nk && q < (size_t)std::max(cap, 0); ++q) deleted[q] = (int)(noalloc_keys[q] & 0xffffffffu);
   return CYL_OK;
 }
 

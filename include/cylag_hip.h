@@ -148,6 +148,17 @@ int cyl_batch_step(cyl_ctx *ctx, double *time_inout, double dt, int nsteps, int 
  * rows).  Arrays may be NULL. */
 int cyl_batch_collect(cyl_ctx *ctx, int s, int npart, double *pos, double *vel, double *us, int *inactive,
                       int *tri, int *lowerlayer, int *deleted, int cap, int *ndeleted);
+/* Allocation-free variant of cyl_batch_collect, same arguments and results.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  It calls no hipMalloc/hipFree: the rows are staged in a
+ * static device array that each GPU the library runs on holds until the
+ * process ends (CYL_NOALLOC_MAX_ROWS * 92 bytes = 46 MiB); scenarios with more
+ * rows are rejected with CYL_E_ARG.  Calls share that array and a host buffer
+ * of deletion keys, so the library serialises them with a process-wide lock. */
+#define CYL_NOALLOC_MAX_ROWS 524288
+int cyl_batch_collect_noalloc(cyl_ctx *ctx, int s, int npart, double *pos, double *vel, double *us,
+                              int *inactive, int *tri, int *lowerlayer, int *deleted, int cap, int *ndeleted);
 /* A new batch on the same device state: the nkeep scenarios keep[] (increasing
  * ids) keep their rows, random streams and remaining steps and become
  * scenarios 0..nkeep-1; nnew scenarios follow, with their rows given as in
