@@ -24,6 +24,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from tests.cases import c2_workload, c3_workload  # noqa: E402  (the workload builders, shared with the tests)
+
 HBM_PEAK = 8.0e12
 METRIC = "particle-steps/s plain vs batched (1 GPU); p50/p99 scenario latency at fixed QPS"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lsm1_2d_unstructured_keps_10M_counters.md")
@@ -68,55 +70,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def c2_workload(npts_x=2000, npts_y=500, npart=10_000_000, seed=7):
-    from tests.cases import jittered_delaunay
-    lx, ly = 20000., 5000.
-    x, y, tri, mpl = jittered_delaunay(npts_x, npts_y, lx, ly, seed=seed)
-    n = x.size
-    k = 2. * np.pi / 2000.
-    times = np.array([0., 10000.])
-    u = np.stack([np.sin(k * x + ph) * np.cos(k * y) for ph in (0., 0.5)])
-    v = np.stack([-np.cos(k * x + ph) * np.sin(k * y) for ph in (0., 0.5)])
-    rng = np.random.default_rng(seed)
-    depth = 2. + 18. * (0.5 + 0.5 * np.sin(2 * np.pi * x / 7000. + 1.) * np.cos(2 * np.pi * y / 3000.))
-    K = 0.1 + 4.9 * (0.5 + 0.5 * np.cos(2 * np.pi * x / 5000.) * np.sin(2 * np.pi * y / 4000. + 0.3))
-    eps = np.full(n, 1e-3)
-    tke = np.sqrt(K * eps / (0.09 / 0.72))
-    fields = dict(dim=2, nlayers=1, times=times, elevation=np.stack([depth, depth]),
-                  velocity_x=u, velocity_y=v,
-                  opt_fields=np.stack([np.stack([tke, tke]), np.stack([eps, eps])]),
-                  opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
-    pos = np.stack([rng.uniform(0.01 * lx, 0.99 * lx, npart), rng.uniform(0.01 * ly, 0.99 * ly, npart)], 1)
-    params = dict(final_time=10000., time_step=5., model=1, diffusion_model=2, schmidt_number=0.72)
-    return x, y, tri, fields, pos, params
-
-
-def c3_workload(npts_x=1000, npts_y=250, nlayers=10, seed=7):
-    """The 3D sigma-layer channel: the 20 x 5 km domain with 250k jittered
-    points (~500k triangles), 10 uniform sigma planes between the bottom -h
-    (h = 2-20 m, smooth) and the surface 0; log-law u(z) along x, v = w = 0;
-    k and eps giving a parabolic K(z) = kappa u* z' (1 - z'/h); frozen (one frame)."""
-    from tests.cases import jittered_delaunay
-    lx, ly = 20000., 5000.
-    x, y, tri, mpl = jittered_delaunay(npts_x, npts_y, lx, ly, seed=seed)
-    n = x.size
-    h = 2. + 18. * (0.5 + 0.5 * np.sin(2 * np.pi * x / 7000. + 1.) * np.cos(2 * np.pi * y / 3000.))
-    kappa, ustar, z0, cmu = 0.41, 0.05, 1e-3, 0.09
-    sig = np.linspace(0., 1., nlayers)
-    zp = np.concatenate([np.maximum(s_ * h, z0 * 10.) for s_ in sig])   # height above the bottom
-    hh = np.tile(h, nlayers)
-    z = np.concatenate([-h + s_ * h for s_ in sig])
-    u = (ustar / kappa) * np.log(zp / z0)
-    frac = np.maximum(1. - zp / hh, 0.05)
-    tke = ustar ** 2 / np.sqrt(cmu) * frac
-    eps = ustar ** 3 / (kappa * zp) * frac
-    fields = dict(dim=3, nlayers=nlayers, times=np.zeros(1), elevation=z[None], velocity_x=u[None],
-                  velocity_y=np.zeros((1, n * nlayers)), velocity_z=np.zeros((1, n * nlayers)),
-                  opt_fields=np.stack([tke[None], eps[None]]),
-                  opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
-    return x, y, tri, h, fields
-
-
 def bytes_per_particle_step(nt, nv, npart):
     """Algorithmic bytes per particle-step: the evolving state read and written
     once (x, y f64 + element id i32 = 20 B each way), plus the mesh and node
@@ -128,31 +81,107 @@ def bytes_per_particle_step(nt, nv, npart):
     return state + mesh / npart, state, mesh
 
 
-def cpu_baseline(x, y, tri, fields, pos, params, nsample=100_000, budget_s=15.):
-    """The oracle (serial restatement, one core) on a bounded sample: the first
-    `nsample` particles of the same workload, stepped until ~budget_s of CPU time."""
-    from matplotlib.tri import Triangulation
+def cpu_threads():
+    """Threads the CPU baseline may use: this process's CPU affinity set,
+    capped by OMP_NUM_THREADS when set (a GPU box grants each GPU a share of
+    its host's cores and sets that variable to the share)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(x, y, tri, fields, pos, params, nsample=100_000, budget_s=15., nthreads=None):
+    """The oracle (the serial C restatement of the reference's step) timed on
+    this host, on bounded samples of the same workload:
+
+    * one core: the first `nsample` particles, one pool-order stream, stepped
+      until ~budget_s of wall time (the reference's serial Solver.solve loop,
+      solver.pyx:459);
+    * all granted cores: thread r steps its own `nsample` particles (rows
+      r*nsample...) from the stream advanced by r << 48, with its own field set
+      state, as SolverParallel splits a release over ranks
+      (solver_parallel.pyx:113-115,127-179); every thread runs as many steps as
+      the one-core run did, and the clock spans them all.
+
+    The releases are located by the reference's KD-tree search (no trifinder,
+    as in the GPU legs)."""
+    import threading
     from oracle import refpy as R
     t0 = time.perf_counter()
     mesh = R.RefMesh(x, y, tri)
-    fs = R.RefFields(mesh, **fields)
-    ps = R.RefParticles(pos[:nsample].copy(), dim=2)
-    tf = Triangulation(x, y, tri).get_trifinder()
-    s = R.RefSolver(fs, ps, params, trifinder=tf, rng=R.RefRng())
+    locate = mesh.kdtree_locator()
+
+    def solver(rank):
+        rows = pos[rank * nsample:(rank + 1) * nsample].copy()
+        rng = R.RefRng()
+        rng.advance(rank << 48)
+        return R.RefSolver(R.RefFields(mesh, **fields), R.RefParticles(rows, dim=2), params, trifinder=locate,
+                           rng=rng)
+
+    s = solver(0)
     log("cpu baseline: oracle ready ({:.1f} s)".format(time.perf_counter() - t0))
     t0 = time.perf_counter()
-    nsteps = 0
+    nsteps, shown = 0, 0.
     while True:
         s.forward(1)
         nsteps += 1
         el = time.perf_counter() - t0
-        if nsteps % 5 == 0:
-            log("cpu baseline: {} steps, {:.1f} s".format(nsteps, el))
+        if el - shown >= 3.:
+            shown = el
+            log("cpu baseline, 1 core: {} steps, {:.1f} s".format(nsteps, el))
         if el >= budget_s:
             break
-    return dict(value=nsample * nsteps / el, unit="particle-steps/s", cores=1, kind="port",
-                sample="{} particles x {} steps of the same workload ({:.1f} s, oracle/cpu_ref.c "
-                       "-O2, serial pool-order loop)".format(nsample, nsteps, el))
+    one = dict(value=nsample * nsteps / el, unit="particle-steps/s", cores=1,
+               sample="{} particles x {} steps of the same workload ({:.1f} s, serial pool-order loop)".format(
+                   nsample, nsteps, el))
+    nth = cpu_threads() if nthreads is None else nthreads
+    nth = min(nth, pos.shape[0] // nsample)
+    out = dict(one, kind="port", cpu_model=cpu_model(), host_cpus=os.cpu_count())
+    if nth <= 1:
+        out["single_core"] = dict(one)
+        return out
+    t0 = time.perf_counter()
+    solvers = [solver(r) for r in range(nth)]
+    log("cpu baseline: {} oracle solvers ready ({:.1f} s)".format(nth, time.perf_counter() - t0))
+    start = threading.Barrier(nth + 1)
+    errors = []
+
+    def work(sv):
+        start.wait()
+        try:
+            sv.forward(nsteps)
+        except Exception as e:  # reported after the join
+            errors.append(e)
+
+    th = [threading.Thread(target=work, args=(sv,)) for sv in solvers]
+    for t in th:
+        t.start()
+    start.wait()
+    t0 = time.perf_counter()
+    for t in th:
+        t.join()
+    el = time.perf_counter() - t0
+    if errors:
+        raise errors[0]
+    log("cpu baseline, {} threads: {} x {} particles x {} steps in {:.1f} s".format(nth, nth, nsample, nsteps, el))
+    out.update(value=nth * nsample * nsteps / el, cores=nth,
+               sample="{} threads x {} particles x {} steps of the same workload ({:.1f} s; thread r on rows "
+                      "r*{}.. with the stream advanced by r<<48)".format(nth, nsample, nsteps, el, nsample),
+               single_core=one)
+    return out
 
 
 def c4_scenarios(nscen=256, seed=11, lx=20000., ly=5000.):
@@ -396,6 +425,8 @@ def main():
     ap.add_argument("--npart", type=int, default=10_000_000)
     ap.add_argument("--sort-interval", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=None,
+                    help="threads of the all-core CPU baseline (default: the CPU share granted to this process)")
     ap.add_argument("--inline-normals", action="store_true",
                     help="plain leg: sample normals inside the fused kernel instead of the pool-order generator")
     ap.add_argument("--legs", default="plain,batched,sequential,served",
@@ -445,7 +476,10 @@ def main():
         line["served"] = served_leg(cy, window=args.served_window, horizon=args.served_window + 15.)
     if "plain" in legs and not args.no_cpu:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(x, y, tri, fields, pos, params)
+        cb = cpu_baseline(x, y, tri, fields, pos, params, nthreads=args.cpu_threads)
+        cb["gpu_over_cpu"] = line["value"] / cb["value"]
+        cb["gpu_over_one_core"] = line["value"] / cb["single_core"]["value"]
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 

@@ -342,6 +342,18 @@ int ref_point_in_tri(const ref_mesh *m, int k, double x, double y)
     return -1;
 }
 
+void ref_locate_candidates(const ref_mesh *m, int n, const double *xs, const double *ys, int k,
+                           const long long *cand, int *out)
+{   /* xy_localize_point_kdtree, xylocalizer.pyx:255-286: the k nearest centroids
+     * (cand[i*k ...], in the KD-tree's order) are tested with point_in_tri; a
+     * point none of them holds takes the expanding search */
+    for (int i = 0; i < n; i++) {
+        int found = -1;
+        for (int j = 0; j < k && found == -1; j++) found = ref_point_in_tri(m, (int)cand[(long)i * k + j], xs[i], ys[i]);
+        out[i] = found != -1 ? found : ref_localize_kdtree(m, xs[i], ys[i], k);
+    }
+}
+
 static inline int orientation(const double *v0, const double *v1, const double *v2)
 {   /* utils.pyx:228-252 */
     double delta = (v0[1] - v1[1]) * (v2[0] - v0[0]) - (v0[0] - v1[0]) * (v2[1] - v0[1]);

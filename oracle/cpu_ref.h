@@ -83,6 +83,8 @@ typedef struct {
 } ref_mesh;
 
 /* ---- geometry (xylocalizer.pyx / utils.pyx / intersections.pyx) ---- */
+void ref_locate_candidates(const ref_mesh *m, int n, const double *xs, const double *ys, int k,
+                           const long long *cand, int *out);
 int ref_point_in_tri(const ref_mesh *m, int k, double x, double y);
 int ref_xy_localize(const ref_mesh *m, int k, double p0x, double p0y, double x, double y,
                     int method, int *bnd_j);

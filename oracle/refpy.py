@@ -132,6 +132,7 @@ _sig("ref_xy_localize", C.c_int, [_MP, C.c_int, C.c_double, C.c_double, C.c_doub
                                   C.c_double, C.c_int, _ip])
 _sig("ref_localize_kdtree", C.c_int, [_MP, C.c_double, C.c_double, C.c_int])
 _sig("ref_localize_brute", C.c_int, [_MP, C.c_double, C.c_double])
+_sig("ref_locate_candidates", None, [_MP, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_longlong), _ip])
 _sig("ref_intersec2d", C.c_int, [_dp, _dp, _dp, _dp])
 _sig("ref_symmetric_point2d", None, [_dp, _dp, _dp, C.c_double, _dp])
 _sig("ref_lower_layer_index", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double])
@@ -292,6 +293,31 @@ class RefMesh:
 
     def kdtree(self, x, y):
         return lib.ref_localize_kdtree(C.byref(self.s), x, y, 10)
+
+    def kdtree_locator(self):
+        """Callable (xs, ys) -> element ids by xy_localize_point_kdtree
+        (xylocalizer.pyx:200-286), the initial search of a field set without a
+        trifinder (solver.pyx:262-268), over scipy's cKDTree of the centroids as
+        the reference builds it (triangular_mesh.pyx:617-629): the 10 nearest
+        centroids are tested in cKDTree's order with point_in_tri; points none
+        of them holds take the oracle's own expanding search."""
+        from scipy.spatial import cKDTree
+        tree = cKDTree(self.centroids)
+
+        def locate(xs, ys):
+            xs, ys = np.atleast_1d(np.asarray(xs, float)), np.atleast_1d(np.asarray(ys, float))
+            out = np.full(xs.shape[0], -1, np.int32)
+            if not xs.size:
+                return out
+            k = min(10, self.nt)
+            _, cand = tree.query(np.stack([xs, ys], 1), k=k)
+            cand = np.ascontiguousarray(cand.reshape(xs.shape[0], k), dtype=np.int64)
+            xs, ys = np.ascontiguousarray(xs), np.ascontiguousarray(ys)
+            lib.ref_locate_candidates(C.byref(self.s), xs.shape[0], P(xs, "d"), P(ys, "d"), k,
+                                      cand.ctypes.data_as(C.POINTER(C.c_longlong)), P(out, "i"))
+            return out
+
+        return locate
 
     def brute(self, x, y):
         return lib.ref_localize_brute(C.byref(self.s), x, y)

@@ -305,3 +305,50 @@ def settling_3d_case(npart=1500, seed=21, spread="regimes", **kw):
         case.diameter = settling_diameters(rng, npart)
     case.pset_kw = dict(particle_density=2650., buoyancy_velocity_model=1)
     return case
+
+
+def c2_workload(npts_x=2000, npts_y=500, npart=10_000_000, seed=7):
+    lx, ly = 20000., 5000.
+    x, y, tri, mpl = jittered_delaunay(npts_x, npts_y, lx, ly, seed=seed)
+    n = x.size
+    k = 2. * np.pi / 2000.
+    times = np.array([0., 10000.])
+    u = np.stack([np.sin(k * x + ph) * np.cos(k * y) for ph in (0., 0.5)])
+    v = np.stack([-np.cos(k * x + ph) * np.sin(k * y) for ph in (0., 0.5)])
+    rng = np.random.default_rng(seed)
+    depth = 2. + 18. * (0.5 + 0.5 * np.sin(2 * np.pi * x / 7000. + 1.) * np.cos(2 * np.pi * y / 3000.))
+    K = 0.1 + 4.9 * (0.5 + 0.5 * np.cos(2 * np.pi * x / 5000.) * np.sin(2 * np.pi * y / 4000. + 0.3))
+    eps = np.full(n, 1e-3)
+    tke = np.sqrt(K * eps / (0.09 / 0.72))
+    fields = dict(dim=2, nlayers=1, times=times, elevation=np.stack([depth, depth]),
+                  velocity_x=u, velocity_y=v,
+                  opt_fields=np.stack([np.stack([tke, tke]), np.stack([eps, eps])]),
+                  opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
+    pos = np.stack([rng.uniform(0.01 * lx, 0.99 * lx, npart), rng.uniform(0.01 * ly, 0.99 * ly, npart)], 1)
+    params = dict(final_time=10000., time_step=5., model=1, diffusion_model=2, schmidt_number=0.72)
+    return x, y, tri, fields, pos, params
+
+
+def c3_workload(npts_x=1000, npts_y=250, nlayers=10, seed=7):
+    """The 3D sigma-layer channel: the 20 x 5 km domain with 250k jittered
+    points (~500k triangles), 10 uniform sigma planes between the bottom -h
+    (h = 2-20 m, smooth) and the surface 0; log-law u(z) along x, v = w = 0;
+    k and eps giving a parabolic K(z) = kappa u* z' (1 - z'/h); frozen (one frame)."""
+    lx, ly = 20000., 5000.
+    x, y, tri, mpl = jittered_delaunay(npts_x, npts_y, lx, ly, seed=seed)
+    n = x.size
+    h = 2. + 18. * (0.5 + 0.5 * np.sin(2 * np.pi * x / 7000. + 1.) * np.cos(2 * np.pi * y / 3000.))
+    kappa, ustar, z0, cmu = 0.41, 0.05, 1e-3, 0.09
+    sig = np.linspace(0., 1., nlayers)
+    zp = np.concatenate([np.maximum(s_ * h, z0 * 10.) for s_ in sig])   # height above the bottom
+    hh = np.tile(h, nlayers)
+    z = np.concatenate([-h + s_ * h for s_ in sig])
+    u = (ustar / kappa) * np.log(zp / z0)
+    frac = np.maximum(1. - zp / hh, 0.05)
+    tke = ustar ** 2 / np.sqrt(cmu) * frac
+    eps = ustar ** 3 / (kappa * zp) * frac
+    fields = dict(dim=3, nlayers=nlayers, times=np.zeros(1), elevation=z[None], velocity_x=u[None],
+                  velocity_y=np.zeros((1, n * nlayers)), velocity_z=np.zeros((1, n * nlayers)),
+                  opt_fields=np.stack([tke[None], eps[None]]),
+                  opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
+    return x, y, tri, h, fields
