@@ -427,8 +427,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=None,
                     help="threads of the all-core CPU baseline (default: the CPU share granted to this process)")
-    ap.add_argument("--inline-normals", action="store_true",
-                    help="plain leg: sample normals inside the fused kernel instead of the pool-order generator")
+    ap.add_argument("--normals", type=int, default=0, choices=[0, 1, 2],
+                    help="plain leg: where the fused kernel takes its normals (0: pool-order generator, 1: sampled "
+                         "inline, 2: slot-order generator)")
     ap.add_argument("--legs", default="plain,batched,sequential,served",
                     help="comma-separated: plain (the 10M-particle step), batched (256 scenarios in one batch), "
                          "sequential (the same scenarios one after another with the plain step), served "
@@ -489,7 +490,7 @@ def plain_leg(cy, args, mesh, fields, pos, params):
     cy.pcg32_seed(42, 0xda3e06d11bb2b203)
     t0 = time.time()
     solver = cy.Solver(fs, ps, params)
-    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0, int(args.inline_normals), 0)
+    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0, args.normals, 0)
     log("solver ready ({:.1f} s): {}".format(time.time() - t0, solver.ctx.device_info()[0]))
     solver._run(args.warmup)
     log("warmup done")

@@ -102,9 +102,10 @@ class BatchSolver:
 
     def set_tuning(self, sort_interval, blocks_per_cu=0, inline_normals=0, use_graph=None):
         """Element re-sort interval (steps, 0 = never), resident blocks per CU of
-        the persistent kernels (0 = as many as fit), inline sampling of the 2D
-        fused kernels' normals, and graph replay of multi-step calls (None: keep);
-        results do not depend on them."""
+        the persistent kernels (0 = as many as fit), the source of the 2D fused
+        kernels' normals (0 pool-order generator, 1 inline sampling, 2
+        slot-order generator), and graph replay of multi-step calls (None:
+        keep); results do not depend on them."""
         if use_graph is not None:
             self.use_graph = int(use_graph)
         self.ctx.check(self.ctx.lib.cyl_set_tuning(self.ctx.h, int(sort_interval), int(blocks_per_cu),

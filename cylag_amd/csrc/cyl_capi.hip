@@ -29,6 +29,9 @@ using namespace cyl;
 
 namespace {
 
+// steps_since_sort value that makes the next step re-sort the slots
+constexpr int SORT_NOW = 1 << 28;
+
 struct Bufs {
   std::vector<void*> ptrs;
   void free_all() {
@@ -155,9 +158,9 @@ struct cyl_ctx {
   int bshift = 6;
 
   // element re-sort
-  int sort_interval = 0, steps_since_sort = 0;
+  int sort_interval = 8, steps_since_sort = 0;  // SORT_NOW: re-sort at the next step
   int blocks_per_cu = 0;  // persistent grids: resident blocks per CU (0 = as many as fit)
-  int inline_normals = 0; // 2D fused kernels: sample normals inline instead of the pool-order generator
+  int inline_normals = 0; // 2D fused kernels' normals: 0 pool-order generator, 1 inline sampling, 2 slot-order generator
   int use_graph = 0;      // replay multi-step loops from a captured HIP graph
   int* d_clock = nullptr; // graph replays: {step id, batch step} on the device
   int last_graph_steps = 0;
@@ -889,7 +892,8 @@ extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const doub
   HIPCHK(hipMemset(ctx->d_n_deleted, 0, sizeof(int)));
   HIPCHK(hipMemset(ctx->d_n_deleted_total, 0, sizeof(long long)));
   ctx->have_last = false;
-  ctx->steps_since_sort = 0;
+  // the uploaded rows are in pool order, not element order: the first step re-sorts
+  ctx->steps_since_sort = SORT_NOW;
   return CYL_OK;
 }
 
@@ -987,6 +991,10 @@ extern "C" int cyl_particles_add(cyl_ctx* ctx, int nadd, const int* slots, const
     if (vol) HIPCHK(hipMemcpy(ctx->d_vol + j, vol + q, sizeof(double), hipMemcpyHostToDevice));
   }
   ctx->pending_init = true;
+  // a large arrival (an eighth of the pool or more) re-sorts right after the
+  // step that initializes it, when the new rows have their elements
+  if (ctx->sort_interval > 0 && (int64_t)nadd * 8 >= n)
+    ctx->steps_since_sort = std::max(ctx->steps_since_sort, ctx->sort_interval - 1);
   int one = 1;
   HIPCHK(hipMemcpy(ctx->d_flag + 1, &one, sizeof(int), hipMemcpyHostToDevice));
   return CYL_OK;
@@ -2044,6 +2052,8 @@ static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
   f.xi_cap = ctx->xi_cap;
   f.nact = ctx->d_scen_active;
   f.chunk_base = ctx->d_gen_base;
+  f.pregen = ctx->d_pregen;
+  f.xi_by_slot = ctx->inline_normals == 2 ? 1 : 0;
   return f;
 }
 
@@ -2057,7 +2067,19 @@ static unsigned persistent_grid(cyl_ctx* ctx, const void* fn, int64_t nblocks, b
 }
 
 // normals of the step for every scenario of the fused kernel (uniform assignment)
-static void launch_gen(cyl_ctx* ctx, const FastArgs& f, int m) {
+static void launch_gen(cyl_ctx* ctx, const FastArgs& f0, int m) {
+  if (ctx->inline_normals == 2) {
+    // slot order: every slot of the pool (the scenarios' pregen flags select the rows)
+    FastArgs f = f0;
+    f.j0 = 0;
+    f.n = ctx->n;
+    int64_t nb = nblk(f.n);
+    if (m == 1) k_gen_slots<1><<<persistent_grid(ctx, (const void*)k_gen_slots<1>, nb, true), BLK, 0, ctx->stream>>>(f);
+    else if (m == 2) k_gen_slots<2><<<persistent_grid(ctx, (const void*)k_gen_slots<2>, nb, true), BLK, 0, ctx->stream>>>(f);
+    else k_gen_slots<3><<<persistent_grid(ctx, (const void*)k_gen_slots<3>, nb, true), BLK, 0, ctx->stream>>>(f);
+    return;
+  }
+  const FastArgs& f = f0;
   int64_t nb = nblk(((int64_t)ctx->n + GEN_PER - 1) / GEN_PER + ctx->nscen);
   if (m == 1) k_gen_normals<1><<<persistent_grid(ctx, (const void*)k_gen_normals<1>, nb, false), BLK, 0, ctx->stream>>>(f);
   else if (m == 2) k_gen_normals<2><<<persistent_grid(ctx, (const void*)k_gen_normals<2>, nb, false), BLK, 0, ctx->stream>>>(f);
@@ -2135,7 +2157,7 @@ static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int model, int m, 
   }
   // the inline variant (draw start + sampling in the kernel) also serves the
   // uniform assignment when the generator is switched off
-  bool inl = general || ctx->inline_normals;
+  bool inl = general || ctx->inline_normals == 1;
   if (model == 1) launch_fast_model<1>(ctx, f, m, inl);
   else launch_fast_model<2>(ctx, f, m, inl);
 }
@@ -2302,7 +2324,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     {
       std::vector<int> pregen(ns, 0);
       // the normal generator serves the 2D fused kernels (the 3D one samples inline)
-      bool gen2d = ctx->dim == 2 && !ctx->inline_normals;
+      bool gen2d = ctx->dim == 2 && ctx->inline_normals != 1;
       if (!ctx->batched) pregen[0] = cls[0].fast && gen2d ? 1 : 0;
       else
         for (int s = 0; s < ns; ++s) pregen[s] = ctx->classes_fast_of_scen(s) && any_fast && gen2d ? 1 : 0;
@@ -2377,7 +2399,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       k_uniform_units<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, ns, a.scen_end,
                                                          a.batch_step, ctx->d_scen_units, a.clock);
       ++launches;
-      if (any_fast) {
+      if (any_fast && ctx->inline_normals == 0) {
         k_gen_chunks<<<1, BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, a.scen_end, ctx->d_pregen,
                                                a.batch_step, ns, ctx->d_gen_base, a.clock);
         ++launches;
@@ -2394,7 +2416,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         }
         HIPCHK(hipEventRecord(ctx->kev[3 * ti], ctx->stream));
       }
-      if (any_fast && ctx->dim == 2 && !ctx->inline_normals) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
+      if (any_fast && ctx->dim == 2 && ctx->inline_normals != 1) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
       if (!cap) HIPCHK(hipEventRecord(ctx->kev[3 * ti + 1], ctx->stream));
       for (auto& c : cls) {
         if (c.hi <= c.lo) continue;
@@ -2660,12 +2682,12 @@ extern "C" int cyl_init_particles(cyl_ctx* ctx, const cyl_params* p, double time
 
 extern "C" int cyl_set_tuning(cyl_ctx* ctx, int sort_interval, int blocks_per_cu, int inline_normals,
                               int use_graph) {
-  if (!ctx || sort_interval < 0 || blocks_per_cu < 0) return CYL_E_ARG;
+  if (!ctx || sort_interval < 0 || blocks_per_cu < 0 || inline_normals < 0 || inline_normals > 2) return CYL_E_ARG;
   ctx->use_graph = use_graph ? 1 : 0;
   ctx->sort_interval = sort_interval;
   ctx->blocks_per_cu = blocks_per_cu;
   if (inline_normals != ctx->inline_normals) ctx->pregen_h.clear();  // re-upload the generator flags
-  ctx->inline_normals = inline_normals ? 1 : 0;
+  ctx->inline_normals = inline_normals;
   return CYL_OK;
 }
 

@@ -262,6 +262,8 @@ struct FastArgs {
   int64_t xi_cap;     // entries of xi
   const int64_t* nact;        // [nscen] active rows
   const int64_t* chunk_base;  // [nscen + 1] first normal-generator chunk of each scenario
+  const int* pregen;          // [nscen] scenario's normals come from a generator this step
+  int xi_by_slot;             // the generator stored each slot's pair at xi[slot] (k_gen_slots)
 };
 
 
@@ -386,16 +388,16 @@ __global__ void __launch_bounds__(BLK) k_gen_chunks(const StreamDev* st, const i
   if (threadIdx.x == 0) chunk_base[nscen] = carry;
 }
 
-// Normals of generator chunk c (scenario s): the pairs of its ranks, in the
-// serial stream order from the chunk's first position.  Returns the stream
-// position after its last draw (0 if the scenario draws nothing this step).
-template <int RNG>
-__device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64_t c, int s, const JumpTable8& t8,
-                                                        uint64_t t8_inc, const ZigTables& zt) {
+// Normals of ranks [r0, r1) of scenario s, drawn sequentially from the stream
+// position of rank r0's first unit: store(r, v0, v1) receives each rank's
+// pair.  Returns the stream position after the last draw; sets *err on a
+// window overflow or a slow Ziggurat start missing from the event table.
+template <int RNG, class Store>
+__device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, int64_t r0, int64_t r1,
+                                                      const JumpTable8& t8, uint64_t t8_inc, const ZigTables& zt,
+                                                      int& err, Store&& store) {
   const int64_t NONE = 0x7fffffffffffffffLL;
   const StreamDev& st = a.stream[s];
-  if (st.units_per_part == 0) return 0;
-  int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
   int64_t u = r0 * st.units_per_part;
   Draws d;
   d.st = &st;
@@ -409,7 +411,7 @@ __device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64
   } else {
     pos = 2 * (u + d.events_before(u));
   }
-  int err = (RNG != 1 && pos >= st.window) ? E_RNG_WINDOW : 0;
+  if (RNG != 1 && pos >= st.window) err = E_RNG_WINDOW;
   Pcg g;
   g.inc = st.inc;
   g.state = st.inc == t8_inc ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
@@ -451,12 +453,28 @@ __device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64
       v[0] = x1 * w;
       v[1] = x2 * w;
     }
-    int64_t xidx = xi_tile_index(c, r - r0);
-    if (xidx < a.xi_cap) a.xi[xidx] = make_double2(v[0], v[1]);
+    store(r, v[0], v[1]);
   }
+  return (unsigned long long)pos;
+}
+
+// Normals of generator chunk c (scenario s): the pairs of its ranks, in the
+// serial stream order from the chunk's first position.  Returns the stream
+// position after its last draw (0 if the scenario draws nothing this step).
+template <int RNG>
+__device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64_t c, int s, const JumpTable8& t8,
+                                                        uint64_t t8_inc, const ZigTables& zt) {
+  const StreamDev& st = a.stream[s];
+  if (st.units_per_part == 0) return 0;
+  int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
+  int err = 0;
+  unsigned long long pend = gen_run<RNG>(a, s, r0, r1, t8, t8_inc, zt, err, [&](int64_t r, double v0, double v1) {
+    int64_t xidx = xi_tile_index(c, r - r0);
+    if (xidx < a.xi_cap) a.xi[xidx] = make_double2(v0, v1);
+  });
   // reported against pool index 0: the rank's slot is not known here
   if (err) atomicMin(&a.err[s], (unsigned long long)(unsigned)err);
-  return (unsigned long long)pos;
+  return pend;
 }
 
 template <int RNG>
@@ -504,6 +522,43 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   }
 }
 
+// Normals of the step in slot order: every active slot of a scenario that
+// draws (pregen[s]) jumps to the stream position of its own rank's first unit
+// and draws its pair there, stored at xi[slot], so that the fused kernel reads
+// them coalesced.  The pool-order generator above draws 32 ranks per jump but
+// its output, read at xi[rank], is a scattered read once the slots are sorted
+// by element; here each slot pays a jump (4 table lookups in LDS and 64-bit
+// multiply-adds) and every access is coalesced.
+template <int RNG>
+__global__ void __launch_bounds__(BLK) k_gen_slots(FastArgs a) {
+  __shared__ JumpTable8 t8;
+  __shared__ ZigTables zt;
+  if (RNG == 2)
+    for (int q = threadIdx.x; q < 128; q += BLK) {
+      zt.kn[q] = a.zt->kn[q];
+      zt.wn[q] = a.zt->wn[q];
+    }
+  load_jump8(t8, a.jt8);
+  __syncthreads();
+  const uint64_t t8_inc = a.stream[0].inc;
+  for_slots_xcd(a.n, [&](int jl) {
+    int j = a.j0 + jl;
+    if (a.inactive[j] != 0) return;
+    int s = a.scen ? a.scen[j] : 0;
+    if (!a.pregen[s] || a.stream[s].units_per_part == 0) return;
+    if (a.scen_end && batch_step_of(a) >= a.scen_end[s]) return;
+    int err = 0;
+    int64_t r = a.rank[j];
+    unsigned long long pend = gen_run<RNG>(a, s, r, r + 1, t8, t8_inc, zt, err,
+                                           [&](int64_t, double v0, double v1) {
+                                             if (j < a.xi_cap) a.xi[j] = make_double2(v0, v1);
+                                           });
+    if (err) atomicMin(&a.err[s], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)err);
+    // stream positions grow with the rank: the scenario's last rank ends its draws
+    if (r == a.nact[s] - 1) atomicMax(&a.stream_end[s], pend);
+  });
+}
+
 // One particle of the step; returns the stream position after its last draw
 // (0 if it drew nothing or its normals were generated beforehand).
 // MODEL 1: euler_lsm1 + diffusion_lsm1.  MODEL 2: di_lsm2 without added mass
@@ -541,7 +596,7 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   // the pair of normals of this slot's active rank (load issued early)
   double2 xi = make_double2(0., 0.);
   if (PREGEN && q.diffusion_model != 0 && st.units_per_part > 0) {
-    int64_t xidx = xi_index(a.chunk_base, a.nscen, s, a.rank[j]);
+    int64_t xidx = a.xi_by_slot ? (int64_t)j : xi_index(a.chunk_base, a.nscen, s, a.rank[j]);
     if (xidx < a.xi_cap) xi = a.xi[xidx];
   }
   const ElemF& ef = a.fm.ef[tri0];

@@ -1,7 +1,7 @@
 """Autotuner for a running batch: the element re-sort interval, the resident
 blocks per CU of the persistent kernels, and where the 2D fused kernels get
-their normals (the pool-order generator, or inline sampling from each slot's
-prepared stream position).
+their normals (the pool-order generator, the slot-order generator, or inline
+sampling from each slot's prepared stream position).
 
 Each candidate setting is timed over real steps of the job (the results do
 not depend on any setting), by coordinate search: the re-sort interval first,
@@ -11,7 +11,7 @@ of the job, so tuning costs only the steps run under slower settings.
 import time
 
 
-def autotune(batch, sort_intervals=(4, 8, 16, 32), blocks_per_cu=(0, 3, 2), inline_normals=(0, 1),
+def autotune(batch, sort_intervals=(4, 8, 16, 32), blocks_per_cu=(0, 3, 2), inline_normals=(0, 2, 1),
              trial_steps=32, clock=time.perf_counter, log=None):
     """Tune `batch` (a BatchSolver) in place; returns (sort_interval,
     blocks_per_cu, inline_normals, {setting: seconds per step})."""
@@ -33,7 +33,7 @@ def autotune(batch, sort_intervals=(4, 8, 16, 32), blocks_per_cu=(0, 3, 2), inli
         table[(si, bpc, inl)] = dt
         if log:
             log("autotune: sort every {} steps, {} blocks/CU, {} normals: {:.3f} ms/step".format(
-                si, bpc or "max", "inline" if inl else "generated", dt * 1e3))
+                si, bpc or "max", ("pool-order generated", "inline", "slot-order generated")[inl], dt * 1e3))
         return dt
 
     cur = tuple(getattr(batch, "tuning", (sort_intervals[0], blocks_per_cu[0], inline_normals[0])))

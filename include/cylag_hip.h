@@ -173,12 +173,15 @@ int cyl_batch_reshape(cyl_ctx *ctx, int nkeep, const int *keep, int nnew, const 
                       const double *volume, const int *init_tri, double time, double dt);
 
 /* Tuning (no counterpart in the reference): element re-sort interval in steps
- * (0 = never), the resident 256-thread blocks per CU of the persistent grids
- * (0 = as many as fit), whether the 2D fused kernels sample their normals
- * inline (1) or read them from the pool-order generator (0), and whether
- * multi-step calls replay a captured HIP graph of 2 * sort_interval steps
- * (frozen fields, uniform draw assignment, no boundary statistics; otherwise
- * the steps are enqueued one by one).  Results do not depend on any of them. */
+ * (0 = never; default 8, and the first step after an upload re-sorts), the
+ * resident 256-thread blocks per CU of the persistent grids (0 = as many as
+ * fit), where the 2D fused kernels take their normals from (0: a generator
+ * that draws runs of ranks in pool order, the default; 1: sampled inline in
+ * the kernel; 2: a generator that draws each slot's pair at its rank's stream
+ * position and stores it in slot order), and whether multi-step calls replay
+ * a captured HIP graph of 2 * sort_interval steps (frozen fields, uniform draw
+ * assignment, no boundary statistics; otherwise the steps are enqueued one by
+ * one).  Results do not depend on any of them. */
 int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int blocks_per_cu, int inline_normals, int use_graph);
 /* Timing of the last cyl_step: summed duration of the fused step-kernel
  * launches (HIP events on the context's stream), wall time of the call, and

@@ -88,7 +88,8 @@ MIX = [
 
 
 @pytest.mark.parametrize("rng_method", [1, 2, 3])
-def test_batch_equals_alone(rng_method):
+@pytest.mark.parametrize("normals", [0, 2], ids=["pool_generator", "slot_generator"])
+def test_batch_equals_alone(normals, rng_method):
     case = vortex_case(npart=4000)
     base = dict(final_time=10000., time_step=5., rng_method=rng_method)
     params = [dict(base, **p) for p, _, _ in MIX]
@@ -98,6 +99,7 @@ def test_batch_equals_alone(rng_method):
     mesh = cy.TriangularMesh(case.x, case.y, case.tri)
     fs = cy.EulerianFieldSet(mesh, trifinder=case.trifinder(), **case.fields)
     b = cy.BatchSolver(fs, psets, params, st, nsteps=nsteps, sort_interval=3)
+    b.set_tuning(3, 0, normals)
     b.run()
     out = b.results()
     for s, (p, n, k) in enumerate(MIX):

@@ -241,13 +241,15 @@ def test_2d_kernel_path(model):
 
 
 @pytest.mark.parametrize("rng_method", [1, 2, 3])
-def test_inline_normals_keps_two_frames(rng_method):
+@pytest.mark.parametrize("mode", [1, 2], ids=["inline", "slot_generator"])
+def test_inline_normals_keps_two_frames(mode, rng_method):
     """The 2D fused kernel with its normals sampled inline from each slot's
-    prepared stream position (uniform assignment, generator off)."""
+    prepared stream position (mode 1), or drawn per slot at its rank's stream
+    position by the slot-order generator (mode 2)."""
     case = vortex_case(npart=3000)
     params = dict(final_time=200., time_step=5., model=1, diffusion_model=2, rng_method=rng_method)
     s = case.build_gpu(cy, params)
-    s.ctx.lib.cyl_set_tuning(s.ctx.h, 4, 0, 1, 0)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 4, 0, mode, 0)
     s.solve()
     ps = s.particle_set
     g = SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(), tri=ps.tri.copy(),

@@ -30,11 +30,11 @@ class FakeBatch:
 
 def test_autotune_picks_fastest_and_steps_are_real():
     cost = lambda si, b, n: (1.0 + abs(si - 16) * 0.01 + (0.05 if b == 0 else 0.) + (0.02 if b == 2 else 0.)
-                             - 0.03 * n)
+                             - (0.03 if n == 1 else 0.01 * n))
     fb = FakeBatch(10_000, cost)
     si, b, n, table = autotune(fb, trial_steps=10, clock=fb.clock)
     assert (si, b, n) == (16, 3, 1) and fb.tuning == (16, 3, 1)
-    assert fb.ran == 10 * (1 + 4 + 1 + 2)
+    assert fb.ran == 10 * (1 + 4 + 2 + 2)
     assert min(table, key=table.get) == (16, 3, 1)
 
 
