@@ -2066,6 +2066,9 @@ static unsigned persistent_grid(cyl_ctx* ctx, const void* fn, int64_t nblocks, b
   return mult8 ? (unsigned)std::max<int64_t>(8, g / 8 * 8) : (unsigned)g;
 }
 
+// the fused kernels run one slot per thread: 8 * ceil(chunks / 8) blocks (xcd_chunk)
+static unsigned fast_grid(int64_t nblocks) { return (unsigned)(std::max<int64_t>(1, (nblocks + 7) / 8) * 8); }
+
 // normals of the step for every scenario of the fused kernel (uniform assignment)
 static void launch_gen(cyl_ctx* ctx, const FastArgs& f0, int m) {
   if (ctx->inline_normals == 2) {
@@ -2092,10 +2095,9 @@ template <int MODEL>
 static void launch_fast_model(cyl_ctx* ctx, const FastArgs& f, int m, bool general) {
   int64_t nb = nblk(f.n);
   hipStream_t s = ctx->stream;
-#define CYL_FAST_K(R, P) (MODEL == 1 ? (const void*)k_lsm1_2d<R, P> : (const void*)k_lsm2_2d<R, P>)
 #define CYL_FAST_LAUNCH(R, P)                                                                           \
   do {                                                                                                  \
-    unsigned g = persistent_grid(ctx, CYL_FAST_K(R, P), nb, true);                                      \
+    unsigned g = fast_grid(nb);                                                                         \
     if (MODEL == 1) k_lsm1_2d<R, P><<<g, BLK, 0, s>>>(f);                                               \
     else k_lsm2_2d<R, P><<<g, BLK, 0, s>>>(f);                                                          \
   } while (0)
@@ -2112,7 +2114,6 @@ static void launch_fast_model(cyl_ctx* ctx, const FastArgs& f, int m, bool gener
   else if (m == 2) CYL_FAST_LAUNCH(2, false);
   else CYL_FAST_LAUNCH(3, false);
 #undef CYL_FAST_LAUNCH
-#undef CYL_FAST_K
 }
 
 // the fused 3D kernel: stream positions of every slot's first draw (uniform or
@@ -2125,9 +2126,9 @@ static void launch_fast3(cyl_ctx* ctx, const Fast3Args& f3, int m) {
   if (m == 1) k_draw_start<1><<<persistent_grid(ctx, (const void*)k_draw_start<1>, nb, true), BLK, 0, s>>>(f);
   else if (m == 2) k_draw_start<2><<<persistent_grid(ctx, (const void*)k_draw_start<2>, nb, true), BLK, 0, s>>>(f);
   else k_draw_start<3><<<persistent_grid(ctx, (const void*)k_draw_start<3>, nb, true), BLK, 0, s>>>(f);
-  if (m == 1) k_fast_3d<MODEL, 1><<<persistent_grid(ctx, (const void*)k_fast_3d<MODEL, 1>, nb, true), BLK, 0, s>>>(f3);
-  else if (m == 2) k_fast_3d<MODEL, 2><<<persistent_grid(ctx, (const void*)k_fast_3d<MODEL, 2>, nb, true), BLK, 0, s>>>(f3);
-  else k_fast_3d<MODEL, 3><<<persistent_grid(ctx, (const void*)k_fast_3d<MODEL, 3>, nb, true), BLK, 0, s>>>(f3);
+  if (m == 1) k_fast_3d<MODEL, 1><<<fast_grid(nb), BLK, 0, s>>>(f3);
+  else if (m == 2) k_fast_3d<MODEL, 2><<<fast_grid(nb), BLK, 0, s>>>(f3);
+  else k_fast_3d<MODEL, 3><<<fast_grid(nb), BLK, 0, s>>>(f3);
 }
 
 static Fast3Args make_fast3_args(cyl_ctx* ctx, const FastArgs& f, const StepArgs& a) {
@@ -2489,7 +2490,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
           ctx->d_sec_of, ctx->nsecseg, ctx->d_sec_counts + (size_t)step * ctx->nsec);
       ++launches;
     }
-    k_finish<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_stream_end, ns, ctx->d_flag,
+    k_finish<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_jt, ctx->d_scen_units, method, ns, ctx->d_flag,
                                                 ctx->d_n_deleted, ctx->d_n_deleted_total, ctx->step_counter,
                                                 ctx->d_step_done, ctx->d_err_step, ctx->d_err,
                                                 a.clock ? ctx->d_clock : nullptr, a.strand_count,

@@ -140,54 +140,77 @@ __device__ __forceinline__ double pick3(int i, double a0, double a1, double a2) 
   return i == 0 ? a0 : (i == 1 ? a1 : a2);
 }
 
+// orientation class of a determinant (utils.pyx:246-252): 0 collinear, 1, 2
+__device__ __forceinline__ int orient_class(double d) { return d == 0. ? 0 : (d > 0. ? 1 : 2); }
+
+__device__ __forceinline__ int pick3i(int i, int a0, int a1, int a2) { return i == 0 ? a0 : (i == 1 ? a1 : a2); }
+
 // xy_localize_point_from_path on the packed tables (xylocalizer.pyx:338-435),
-// starting in element k (k >= 0) whose record is e0.  Each step needs the
-// current element's walk record only: the edge end points are its own
-// vertices, and the record of the next element is loaded for its
-// point-in-triangle test and kept for the next step's edges.
+// starting in element k (k >= 0) whose record is e0.  Each pass tests the
+// current element's edges in its triedge order, skipping the edge it entered
+// by, and moves across the first one the path [p0, p1] intersects
+// (intersec2d_segment_segment, intersections.pyx:106-148).
+//
+// The four orientations of an edge test are the reference's, value for
+// value: orient(p0, p1, v) = (p0y - p1y)(vx - p0x) - (p0x - p1x)(vy - p0y)
+// depends on the vertex only, so it is computed once per vertex of the
+// element and shared by the two edges through that vertex; orient(a, b, p0)
+// and orient(a, b, p1) share the edge's own (ay - by) and (ax - bx).  The
+// collinear cases (on_segment, utils.pyx:282-303) only run when one of an
+// edge's orientations is zero.  The next element's walk record is loaded with
+// its vertices.
 __device__ inline int walk_fast(const FastMesh& fm, int k, const ElemF& e0, double p0x, double p0y, double x,
                                 double y, int& bnd_j) {
-  int new_k = -1, niter = 0, last_edge = -1;
   bnd_j = -1;
+  const double ly = p0y - y, lx = p0x - x;
   double vx0 = e0.vx[0], vx1 = e0.vx[1], vx2 = e0.vx[2];
   double vy0 = e0.vy[0], vy1 = e0.vy[1], vy2 = e0.vy[2];
-  while (new_k == -1 && niter <= 50) {
-    bool hit = false;
-    int kk = k < 0 ? k + fm.nt : k;
-    const WalkRec w = fm.wr[kk];
-    for (int i = 0; i < 3; ++i) {
-      int j = w.e[i];
-      if (j == last_edge) continue;
-      int ia = (w.vv >> (4 * i)) & 3, ib = (w.vv >> (4 * i + 2)) & 3;
-      double ax = pick3(ia, vx0, vx1, vx2), ay = pick3(ia, vy0, vy1, vy2);
-      double bx = pick3(ib, vx0, vx1, vx2), by = pick3(ib, vy0, vy1, vy2);
-      if (seg_intersect(p0x, p0y, x, y, ax, ay, bx, by)) {
-        hit = true;
-        int kn = w.nb[i];
-        if (kn == -1) {
-          bnd_j = j;
-          return -1;
-        }
-        const ElemF& en = fm.ef[kn];
-        vx0 = en.vx[0]; vx1 = en.vx[1]; vx2 = en.vx[2];
-        vy0 = en.vy[0]; vy1 = en.vy[1]; vy2 = en.vy[2];
-        // point_in_triangle_dotp on the new element (in_elem)
-        bool in = pintri(x, y, vx0, vy0, vx1, vy1, vx2, vy2) >= 0 && pintri(x, y, vx1, vy1, vx2, vy2, vx0, vy0) >= 0 &&
-                  pintri(x, y, vx2, vy2, vx0, vy0, vx1, vy1) >= 0;
-        if (in) return kn;
-        k = kn;
-        last_edge = j;
-        break;
-      }
+  WalkRec w = fm.wr[k < 0 ? k + fm.nt : k];
+  int last_edge = -1;
+  for (int niter = 0; niter <= 50; ++niter) {
+    const int ov0 = orient_class(ly * (vx0 - p0x) - lx * (vy0 - p0y));
+    const int ov1 = orient_class(ly * (vx1 - p0x) - lx * (vy1 - p0y));
+    const int ov2 = orient_class(ly * (vx2 - p0x) - lx * (vy2 - p0y));
+    int hit_i = -1;
+#pragma unroll
+    for (int i = 2; i >= 0; --i) {  // the first intersected edge in triedge order wins
+      const int ia = (w.vv >> (4 * i)) & 3, ib = (w.vv >> (4 * i + 2)) & 3;
+      const double ax = pick3(ia, vx0, vx1, vx2), ay = pick3(ia, vy0, vy1, vy2);
+      const double bx = pick3(ib, vx0, vx1, vx2), by = pick3(ib, vy0, vy1, vy2);
+      const int o1 = pick3i(ia, ov0, ov1, ov2), o2 = pick3i(ib, ov0, ov1, ov2);
+      const double ey = ay - by, ex = ax - bx;
+      const int o3 = orient_class(ey * (p0x - ax) - ex * (p0y - ay));
+      const int o4 = orient_class(ey * (x - ax) - ex * (y - ay));
+      bool hit = o1 != o2 && o3 != o4;
+      if (!hit && (o1 == 0 || o2 == 0 || o3 == 0 || o4 == 0))
+        hit = (o1 == 0 && on_seg(p0x, p0y, x, y, ax, ay)) || (o2 == 0 && on_seg(p0x, p0y, x, y, bx, by)) ||
+              (o3 == 0 && on_seg(ax, ay, bx, by, p0x, p0y)) || (o4 == 0 && on_seg(ax, ay, bx, by, x, y));
+      if (hit && w.e[i] != last_edge) hit_i = i;
     }
-    if (!hit) {
+    if (hit_i < 0) {
+      // no edge crossed: the point is in the current element or nowhere
       bool in = pintri(x, y, vx0, vy0, vx1, vy1, vx2, vy2) >= 0 && pintri(x, y, vx1, vy1, vx2, vy2, vx0, vy0) >= 0 &&
                 pintri(x, y, vx2, vy2, vx0, vy0, vx1, vy1) >= 0;
       return in ? k : -1;
     }
-    ++niter;
+    const int j = pick3i(hit_i, w.e[0], w.e[1], w.e[2]), kn = pick3i(hit_i, w.nb[0], w.nb[1], w.nb[2]);
+    if (kn == -1) {
+      bnd_j = j;
+      return -1;
+    }
+    const ElemF& en = fm.ef[kn];
+    const WalkRec wn = fm.wr[kn];
+    vx0 = en.vx[0]; vx1 = en.vx[1]; vx2 = en.vx[2];
+    vy0 = en.vy[0]; vy1 = en.vy[1]; vy2 = en.vy[2];
+    // point_in_triangle_dotp on the new element (in_elem)
+    bool in = pintri(x, y, vx0, vy0, vx1, vy1, vx2, vy2) >= 0 && pintri(x, y, vx1, vy1, vx2, vy2, vx0, vy0) >= 0 &&
+              pintri(x, y, vx2, vy2, vx0, vy0, vx1, vy1) >= 0;
+    if (in) return kn;
+    k = kn;
+    last_edge = j;
+    w = wn;
   }
-  return new_k;
+  return -1;
 }
 
 // xy_localize with method 3 (xylocalizer.pyx:65-120)
@@ -516,9 +539,7 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
       s = (c < a.chunk_base[sb + 1]) ? sb : scen_of(a.chunk_base, a.nscen, c);
       if (!(a.scen_end && batch_step_of(a) >= a.scen_end[s])) pend = gen_chunk<RNG>(a, c, s, t8, t8_inc, zt);
     }
-    // the stream of each scenario advances past its last draw: one atomic per
-    // scenario present in the wave
-    wave_max_by_key(s, pend, a.stream_end);
+    (void)pend;  // the streams' ends come from the units drawn (k_finish)
   }
 }
 
@@ -553,9 +574,8 @@ __global__ void __launch_bounds__(BLK) k_gen_slots(FastArgs a) {
                                            [&](int64_t, double v0, double v1) {
                                              if (j < a.xi_cap) a.xi[j] = make_double2(v0, v1);
                                            });
+    (void)pend;  // the streams' ends come from the units drawn (k_finish)
     if (err) atomicMin(&a.err[s], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)err);
-    // stream positions grow with the rank: the scenario's last rank ends its draws
-    if (r == a.nact[s] - 1) atomicMax(&a.stream_end[s], pend);
   });
 }
 
@@ -769,8 +789,20 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   return dr.live ? (unsigned long long)(dr.pos0 + dr.ncalls) : 0ull;
 }
 
-// Persistent launch over the slots (see for_slots_xcd); the Ziggurat tables
-// are staged in LDS once per block.
+// One slot per thread.  Blocks are dispatched round-robin over the 8 XCDs, so
+// block b takes chunk (b % 8) * ceil(chunks / 8) + b / 8 of BLK slots: each
+// XCD sweeps one contiguous eighth of the slots, and neighbouring slots
+// (neighbouring elements once sorted) share its L2.  The grid is 8 *
+// ceil(chunks / 8) blocks (fast_grid).  Without the grid-stride loop of a
+// persistent launch, the kernel's uniform values need not stay live across
+// slots, which spares scalar registers.  The Ziggurat tables are staged in LDS
+// per block for the inline sampler.
+__device__ __forceinline__ int64_t xcd_chunk(int n) {
+  int64_t nch = ((int64_t)n + BLK - 1) / BLK, cpx = (nch + 7) / 8;
+  int64_t chunk = (blockIdx.x % 8) * cpx + blockIdx.x / 8;
+  return chunk < nch ? chunk : -1;
+}
+
 template <int MODEL, int RNG, bool PREGEN>
 __device__ __forceinline__ void fast_step_body(const FastArgs& a) {
   if (a.guarded && a.flag[0] == 0) return;
@@ -781,21 +813,12 @@ __device__ __forceinline__ void fast_step_body(const FastArgs& a) {
       zt.wn[q] = a.zt->wn[q];
       zt.fn[q] = a.zt->fn[q];
     }
+    __syncthreads();
   }
-  __syncthreads();
-  StreamEndAcc acc;
-  for_slots_xcd(
-      a.n,
-      [&](int jl) {
-        int j = a.j0 + jl;
-        unsigned long long q = fast_particle<MODEL, RNG, PREGEN>(a, j, zt);
-        if (!PREGEN && q) acc.add(a.scen ? a.scen[j] : 0, q);
-      },
-      [&] {
-        if (!PREGEN) acc.flush(a.stream_end);
-      });
-  if (PREGEN) return;  // the normal generator advanced the streams
-  acc.done(a.stream_end);
+  const int64_t chunk = xcd_chunk(a.n);
+  if (chunk < 0) return;
+  const int64_t jl = chunk * BLK + threadIdx.x;
+  if (jl < a.n) (void)fast_particle<MODEL, RNG, PREGEN>(a, a.j0 + (int)jl, zt);
 }
 
 // 5 waves per SIMD (<= 96 VGPRs) with pre-generated normals, 4 with the inline sampler

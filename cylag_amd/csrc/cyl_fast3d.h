@@ -367,16 +367,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
     }
   }
   __syncthreads();
-  StreamEndAcc acc;
-  for_slots_xcd(
-      a.n,
-      [&](int jl) {
-        int j = a.j0 + jl;
-        unsigned long long qe = fast3d_particle<MODEL, RNG>(a3, j, zt);
-        if (qe) acc.add(a.scen ? a.scen[j] : 0, qe);
-      },
-      [&] { acc.flush(a.stream_end); });
-  acc.done(a.stream_end);
+  const int64_t chunk = xcd_chunk(a.n);
+  if (chunk < 0) return;
+  const int64_t jl = chunk * BLK + threadIdx.x;
+  if (jl < a.n) (void)fast3d_particle<MODEL, RNG>(a3, a.j0 + (int)jl, zt);
 }
 
 }  // namespace cyl

@@ -232,22 +232,14 @@ struct StreamEndAcc {
 
 // Grid-stride over the slots of the launch (the grid is at most what fits on
 // the device at once, so a guarded launch that finds nothing to do ends at once).
+// The streams' ends come from the units drawn (k_finish).
 template <int DIM, int MODEL, bool COUNT>
 __global__ void __launch_bounds__(BLK) k_step(StepArgs a) {
   if (a.guarded && a.flag[0] == 0) return;
-  StreamEndAcc acc;
-  // the trip count is block-uniform, so every lane reaches flush()
-  for (int64_t b = (int64_t)blockIdx.x * BLK; b < a.n; b += (int64_t)gridDim.x * BLK) {
-    int64_t jl = b + threadIdx.x;
-    if (jl < a.n) {
-      int scen = -1;
-      unsigned long long q = step_slot<DIM, MODEL, COUNT>(a, a.j0 + (int)jl, scen);
-      if (!COUNT && q) acc.add(scen, q);
-    }
-    if (!COUNT) acc.flush(a.stream_end);
+  for (int64_t jl = (int64_t)blockIdx.x * BLK + threadIdx.x; jl < a.n; jl += (int64_t)gridDim.x * BLK) {
+    int scen = -1;
+    (void)step_slot<DIM, MODEL, COUNT>(a, a.j0 + (int)jl, scen);
   }
-  if (COUNT) return;
-  acc.done(a.stream_end);
 }
 
 // tmp[fid][i] = tf*frame_rec[i] + (1-tf)*frame_rec1[i]  (eulerian_field_set.pyx:926-938)
@@ -526,15 +518,27 @@ __global__ void __launch_bounds__(BLK) k_slot_of_pool(const int* gpool, int n, i
 
 // end of step: advance each scenario's stream past its last draw, reset counters
 // clock (graph replays): the device-side step counters, advanced here
-__global__ void k_finish(StreamDev* st, const JumpTable* jt, unsigned long long* stream_end, int nscen, int* flag,
-                         int* n_deleted, long long* n_deleted_total, int step_id, int* step_done,
+// End of a step: each scenario's stream advances past the step's last draw.
+// Its rows drew scen_units[s] units in pool order (the uniform count, or the
+// general-mode scan total), so the position after the last draw is the
+// position of unit scen_units[s]: the unit itself for rectangular normals,
+// shifted by the slow Ziggurat normals' extra draws or the rejected polar
+// attempts before it (the step's event tables, Draws::zig_position /
+// polar_position).  No kernel reduces the stream end.
+__global__ void k_finish(StreamDev* st, const JumpTable* jt, const int64_t* scen_units, int method, int nscen,
+                         int* flag, int* n_deleted, long long* n_deleted_total, int step_id, int* step_done,
                          unsigned long long* err_step, unsigned long long* err, int* clock, int* strand_count,
                          int* unchecked) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < nscen) {
-    unsigned long long e = stream_end[s];
+    const int64_t u = scen_units[s];
+    unsigned long long e = 0;
+    if (u > 0) {
+      Draws d;
+      d.st = &st[s];
+      e = (unsigned long long)(method == 1 ? u : (method == 2 ? d.zig_position(u) : d.polar_position(u)));
+    }
     if (e) st[s].base = pcg_jump(st[s].base, e, jt[s]);
-    stream_end[s] = 0;
     if (err_step[s] < err[s]) err[s] = err_step[s];
     err_step[s] = ~0ull;
   }
