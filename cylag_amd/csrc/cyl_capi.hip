@@ -94,6 +94,11 @@ struct cyl_ctx {
   ParticleConst pc{};
   bool pending_init = false;   // rows with tri == -2 exist
   bool have_last = false;
+  // 2D LSM-1 in a plain context: the fused kernel leaves us unwritten, as
+  // fluid_velocity_seen equals velocity for every 2D LSM-1 row after a step
+  // (update_state_lsm1.pyx:169-172 and the other schemes; rebounds reflect no
+  // LSM-1 velocity, compute_bc.pyx:195); downloads then derive it
+  bool us_from_up = false;
 
   // scenarios (a single one for the plain solver; several after cyl_scenarios_set)
   bool batched = false;
@@ -892,6 +897,7 @@ extern "C" int cyl_particles_upload(cyl_ctx* ctx, int npart, int dim, const doub
   HIPCHK(hipMemset(ctx->d_n_deleted, 0, sizeof(int)));
   HIPCHK(hipMemset(ctx->d_n_deleted_total, 0, sizeof(long long)));
   ctx->have_last = false;
+  ctx->us_from_up = false;
   // the uploaded rows are in pool order, not element order: the first step re-sorts
   ctx->steps_since_sort = SORT_NOW;
   return CYL_OK;
@@ -921,7 +927,7 @@ extern "C" int cyl_particles_download(cyl_ctx* ctx, int npart, double* pos, doub
   for (int c = 0; c < dim; ++c) {
     if (pos) HIPCHK(getd(px[c], pos, c, dim, true));
     if (vel) HIPCHK(getd(P.up[c], vel, c, dim, true));
-    if (us) HIPCHK(getd(P.us[c], us, c, dim, true));
+    if (us) HIPCHK(getd(ctx->us_from_up ? P.up[c] : P.us[c], us, c, dim, true));
     if (lpos) HIPCHK(getd(ctx->have_last ? lx[c] : px[c], lpos, c, dim, true));
     if (lvel) HIPCHK(getd(ctx->have_last ? L.up[c] : P.up[c], lvel, c, dim, true));
   }
@@ -2054,6 +2060,7 @@ static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
   f.chunk_base = ctx->d_gen_base;
   f.pregen = ctx->d_pregen;
   f.xi_by_slot = ctx->inline_normals == 2 ? 1 : 0;
+  f.skip_us = !ctx->batched && ctx->dim == 2 ? 1 : 0;  // read by the LSM-1 kernel only
   return f;
 }
 
@@ -2336,6 +2343,15 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         ctx->pregen_h = pregen;
       }
     }
+    // us left unstored by 2D LSM-1 steps is materialized (= up) before a step
+    // of another kind, which may read it
+    const bool lsm1_fused_2d = !ctx->batched && !init_only && cls[0].fast && p->model == 1 && ctx->dim == 2;
+    if (ctx->us_from_up && !ctx->batched && !lsm1_fused_2d && !(p->model == 1 && ctx->dim == 2)) {
+      const PoolDev& C = ctx->pool[ctx->cur];
+      for (int c = 0; c < ctx->dim; ++c)
+        HIPCHK(hipMemcpyAsync(C.us[c], C.up[c], (size_t)ctx->n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+      ctx->us_from_up = false;
+    }
     bool do_sort = !init_only && ctx->sort_interval > 0 && ctx->steps_since_sort >= ctx->sort_interval;
     if (do_sort) {
       // the fused kernels rewrite us for every row they step; rows of finished
@@ -2498,6 +2514,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     ++launches;
     ctx->cur = 1 - ctx->cur;
     ctx->have_last = !init_only;
+    if (lsm1_fused_2d) ctx->us_from_up = true;
     ctx->pending_init = false;
     ++ctx->step_counter;
     ++ctx->steps_since_sort;

@@ -287,6 +287,7 @@ struct FastArgs {
   const int64_t* chunk_base;  // [nscen + 1] first normal-generator chunk of each scenario
   const int* pregen;          // [nscen] scenario's normals come from a generator this step
   int xi_by_slot;             // the generator stored each slot's pair at xi[slot] (k_gen_slots)
+  int skip_us;                // LSM-1: us (= up) is not stored; downloads derive it from up
 };
 
 
@@ -784,8 +785,10 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   a.triB[j] = tri;
   a.upB0[j] = upx;
   a.upB1[j] = upy;
-  a.usB0[j] = ux;
-  a.usB1[j] = uy;
+  if (MODEL != 1 || !a.skip_us) {
+    a.usB0[j] = ux;
+    a.usB1[j] = uy;
+  }
   return dr.live ? (unsigned long long)(dr.pos0 + dr.ncalls) : 0ull;
 }
 

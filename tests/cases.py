@@ -91,6 +91,7 @@ class Case:
                 s.forward(s.simutime.time_step)
         ps = s.particle_set
         return SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(),
+                               fluid_velocity_seen=ps.fluid_velocity_seen.copy(),
                                tri=ps.tri.copy(), inactive=ps.inactive.copy(),
                                lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy(), solver=s)
 
@@ -115,6 +116,7 @@ class Case:
         else:
             s.forward(nsteps)
         return SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(),
+                               fluid_velocity_seen=ps.us.copy(),
                                tri=ps.tri.copy(), inactive=ps.inactive.copy(),
                                lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy(), rng=rng,
                                last_position=ps.last_position.copy(),

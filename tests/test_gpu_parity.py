@@ -148,7 +148,32 @@ def test_unstructured_keps_two_frames(rng_method):
     case = vortex_case(npart=3000)
     params = dict(final_time=200., time_step=5., model=1, diffusion_model=2,
                   rng_method=rng_method)
-    assert_parity(case.run_gpu(cy, params), case.run_oracle(R, params))
+    g, r = case.run_gpu(cy, params), case.run_oracle(R, params)
+    assert_parity(g, r)
+    # fluid_velocity_seen (not stored by the fused 2D LSM-1 step, derived on download)
+    us_err = np.max(np.abs(g.fluid_velocity_seen - r.fluid_velocity_seen) / np.maximum(np.abs(r.fluid_velocity_seen), 1.))
+    assert us_err <= TOL
+
+
+def test_seen_velocity_materialized_on_model_switch():
+    """A context stepped with 2D LSM-1 (seen velocity not stored, derived from
+    the velocity on download) and then with the test model, which neither
+    sets nor reads it: the stored seen velocity must be the last LSM-1
+    velocity, materialized at the switch."""
+    case = vortex_case(npart=1500)
+    p1 = dict(final_time=100., time_step=5., model=1, diffusion_model=2)
+    s = case.build_gpu(cy, p1)
+    for _ in range(10):
+        s.forward()
+    ps = s.particle_set
+    up_lsm1 = ps.velocity.copy()
+    from cylag_amd.parameters import Parameters
+    from cylag_amd.solver import _params_struct
+    s.parameters = Parameters(dict(p1, model=0, particle_velocity_init=0, boundary_conditions=False))
+    s._par = _params_struct(s.parameters, ps)
+    s.forward()
+    act = ps.inactive == 0
+    np.testing.assert_array_equal(ps.fluid_velocity_seen[act], up_lsm1[act])
 
 
 @pytest.mark.parametrize("rng_method", [1, 2, 3])
