@@ -24,7 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from tests.cases import c2_workload, c3_workload  # noqa: E402  (the workload builders, shared with the tests)
+from tests.cases import c2_workload, c3_release, c3_workload  # noqa: E402  (workload builders shared with the tests)
 
 HBM_PEAK = 8.0e12
 METRIC = "particle-steps/s plain vs batched (1 GPU); p50/p99 scenario latency at fixed QPS"
@@ -182,6 +182,63 @@ def cpu_baseline(x, y, tri, fields, pos, params, nsample=100_000, budget_s=15., 
                       "r*{}.. with the stream advanced by r<<48)".format(nth, nsample, nsteps, el, nsample),
                single_core=one)
     return out
+
+
+def c3_bytes_per_particle_step(nt, nv, nl, npart):
+    """Algorithmic bytes per particle-step of the 3D LSM-2 step: the evolving
+    state read and written once (x, y, z and the particle velocity f64, element
+    and layer ids i32: 2 x 56 B) and the diameter read (8 B), plus the mesh and
+    the one (frozen) frame of the node fields read once per step, shared by the
+    N particles (element record 84 B; 2D node x, y 16 B; per node and plane
+    elevation, u, v, w, k, eps 48 B)."""
+    state = 2 * (6 * 8 + 2 * 4) + 8
+    mesh = nt * 84 + nv * 16 + nv * nl * 6 * 8
+    return state + mesh / npart, state, mesh
+
+
+def c3_leg(cy, steps=100, warmup=10, npart=5_000_000, sort_interval=8):
+    """3D LSM-2 Langevin with settling: the 3D channel (500k elements x 10
+    sigma planes), 5M sediment particles (density 2650, settling diameters),
+    direct integrator, k-epsilon K(z) with its gradient drift, dt = 2 s,
+    frozen fields; `steps` steps timed after `warmup`."""
+    t0 = time.time()
+    x, y, tri, h, fields = c3_workload()
+    pos, d = c3_release(x, y, h, npart)
+    mesh = cy.TriangularMesh(x, y, tri)
+    fs = cy.EulerianFieldSet(mesh, **fields)
+    ps = cy.LagrangianParticleSet(pos, dim=3, particle_density=2650., buoyancy_velocity_model=1)
+    ps.diameter = d
+    ps.volume = np.pi * d ** 3 / 6.
+    params = dict(final_time=6000., time_step=2., frozen_eulerian_fields=True, model=2, time_scheme=5,
+                  diffusion_lsm2_option=1, diffusion_model=3, particle_velocity_init=1)
+    cy.pcg32_seed(42, 0xda3e06d11bb2b203)
+    solver = cy.Solver(fs, ps, params)
+    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, sort_interval, 0, 0, 0)
+    log("c3: {} elements x {} planes, {} particles, ready ({:.1f} s)".format(tri.shape[0], fields["nlayers"], npart,
+                                                                            time.time() - t0))
+    solver._run(warmup)
+    t0 = time.perf_counter()
+    solver._run(steps)
+    wall = time.perf_counter() - t0
+    k_ms, _, nlaunch = solver.ctx.last_timing()
+    prep_ms = solver.ctx.last_prep_ms()
+    fused = solver.ctx.last_path()[0]
+    ps._sync()
+    nact = ps.npart_active
+    solver.ctx.close()
+    b_alg, _, _ = c3_bytes_per_particle_step(mesh.nt, mesh.nv, fields["nlayers"], nact)
+    avg_ms = k_ms / max(nlaunch, 1)
+    achieved = b_alg * nact / (avg_ms * 1e-3)
+    log("c3: {} steps in {:.3f} s, fused kernel {:.3f} ms/launch ({} fused launches), draw starts {:.3f} ms".format(
+        steps, wall, avg_ms, fused, prep_ms / max(nlaunch, 1)))
+    return dict(value=nact * steps / wall, unit="particle-steps/s", ms_per_step=wall * 1e3 / steps, steps=steps,
+                warmup=warmup, particles=int(nact),
+                workload="3D LSM-2 direct integrator with settling (lognormal settling velocities, density 2650), "
+                         "k-eps K(z) with drift, sigma-layer channel {} elements x {} planes, {} particles, dt=2 s, "
+                         "frozen".format(mesh.nt, fields["nlayers"], nact),
+                roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK, "kernel": "k_fast_3d<2,2>", "kernel_avg_ms": avg_ms,
+                          "bytes_per_particle_step": b_alg, "draw_start_avg_ms": prep_ms / max(nlaunch, 1)})
 
 
 def c4_scenarios(nscen=256, seed=11, lx=20000., ly=5000.):
@@ -430,8 +487,9 @@ def main():
     ap.add_argument("--normals", type=int, default=0, choices=[0, 1, 2],
                     help="plain leg: where the fused kernel takes its normals (0: pool-order generator, 1: sampled "
                          "inline, 2: slot-order generator)")
-    ap.add_argument("--legs", default="plain,batched,sequential,served",
-                    help="comma-separated: plain (the 10M-particle step), batched (256 scenarios in one batch), "
+    ap.add_argument("--legs", default="plain,c3,batched,sequential,served",
+                    help="comma-separated: plain (the 10M-particle step), c3 (3D LSM-2 with settling, 5M particles), "
+                         "batched (256 scenarios in one batch), "
                          "sequential (the same scenarios one after another with the plain step), served "
                          "(Poisson requests on the 3D channel)")
     ap.add_argument("--served-window", type=float, default=60.,
@@ -457,6 +515,8 @@ def main():
             "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
     if "plain" in legs:
         line.update(plain_leg(cy, args, mesh, fields, pos, params))
+    if "c3" in legs:
+        line["c3"] = c3_leg(cy)
     bt = None
     if "batched" in legs:
         bt = batched_leg(cy, mesh, fields, args.sort_interval, args.batch_steps)

@@ -2096,10 +2096,20 @@ static void launch_gen(cyl_ctx* ctx, const FastArgs& f0, int m) {
   else k_gen_normals<3><<<persistent_grid(ctx, (const void*)k_gen_normals<3>, nb, false), BLK, 0, ctx->stream>>>(f);
 }
 
+// stream position and generator state of every slot's first draw (the inline
+// samplers of the fused kernels)
+static void launch_draw_start(cyl_ctx* ctx, const FastArgs& f, int m) {
+  int64_t nb = nblk(f.n);
+  hipStream_t s = ctx->stream;
+  if (m == 1) k_draw_start<1><<<persistent_grid(ctx, (const void*)k_draw_start<1>, nb, true), BLK, 0, s>>>(f);
+  else if (m == 2) k_draw_start<2><<<persistent_grid(ctx, (const void*)k_draw_start<2>, nb, true), BLK, 0, s>>>(f);
+  else k_draw_start<3><<<persistent_grid(ctx, (const void*)k_draw_start<3>, nb, true), BLK, 0, s>>>(f);
+}
+
 // the fused kernel of `model` over f's slot range: pre-generated normals
 // (uniform), or prepared stream positions and the inline sampler (general)
 template <int MODEL>
-static void launch_fast_model(cyl_ctx* ctx, const FastArgs& f, int m, bool general) {
+static void launch_fast_model(cyl_ctx* ctx, const FastArgs& f, int m, bool general, bool start_done) {
   int64_t nb = nblk(f.n);
   hipStream_t s = ctx->stream;
 #define CYL_FAST_LAUNCH(R, P)                                                                           \
@@ -2114,25 +2124,20 @@ static void launch_fast_model(cyl_ctx* ctx, const FastArgs& f, int m, bool gener
     else CYL_FAST_LAUNCH(3, true);
     return;
   }
-  if (m == 1) k_draw_start<1><<<persistent_grid(ctx, (const void*)k_draw_start<1>, nb, true), BLK, 0, s>>>(f);
-  else if (m == 2) k_draw_start<2><<<persistent_grid(ctx, (const void*)k_draw_start<2>, nb, true), BLK, 0, s>>>(f);
-  else k_draw_start<3><<<persistent_grid(ctx, (const void*)k_draw_start<3>, nb, true), BLK, 0, s>>>(f);
+  if (!start_done) launch_draw_start(ctx, f, m);
   if (m == 1) CYL_FAST_LAUNCH(1, false);
   else if (m == 2) CYL_FAST_LAUNCH(2, false);
   else CYL_FAST_LAUNCH(3, false);
 #undef CYL_FAST_LAUNCH
 }
 
-// the fused 3D kernel: stream positions of every slot's first draw (uniform or
-// general assignment), then the step with the inline sampler
+// the fused 3D kernel with the inline sampler (its slots' stream positions
+// prepared by launch_draw_start, here unless start_done)
 template <int MODEL>
-static void launch_fast3(cyl_ctx* ctx, const Fast3Args& f3, int m) {
+static void launch_fast3(cyl_ctx* ctx, const Fast3Args& f3, int m, bool start_done) {
   int64_t nb = nblk(f3.f.n);
   hipStream_t s = ctx->stream;
-  const FastArgs& f = f3.f;
-  if (m == 1) k_draw_start<1><<<persistent_grid(ctx, (const void*)k_draw_start<1>, nb, true), BLK, 0, s>>>(f);
-  else if (m == 2) k_draw_start<2><<<persistent_grid(ctx, (const void*)k_draw_start<2>, nb, true), BLK, 0, s>>>(f);
-  else k_draw_start<3><<<persistent_grid(ctx, (const void*)k_draw_start<3>, nb, true), BLK, 0, s>>>(f);
+  if (!start_done) launch_draw_start(ctx, f3.f, m);
   if (m == 1) k_fast_3d<MODEL, 1><<<fast_grid(nb), BLK, 0, s>>>(f3);
   else if (m == 2) k_fast_3d<MODEL, 2><<<fast_grid(nb), BLK, 0, s>>>(f3);
   else k_fast_3d<MODEL, 3><<<fast_grid(nb), BLK, 0, s>>>(f3);
@@ -2155,19 +2160,24 @@ static Fast3Args make_fast3_args(cyl_ctx* ctx, const FastArgs& f, const StepArgs
   return r;
 }
 
+// whether the fused kernels of this context sample their normals inline
+// (from stream positions prepared by launch_draw_start) under the uniform
+// assignment: always in 3D, in 2D when the generator is switched off
+static bool samples_inline(const cyl_ctx* ctx) { return ctx->dim == 3 || ctx->inline_normals == 1; }
+
+// start_done: the uniform assignment's draw starts were prepared before the
+// timed kernel window (general-mode reruns prepare their own)
 static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int model, int m, bool general,
-                             const StepArgs* a = nullptr) {
+                             const StepArgs* a = nullptr, bool start_done = false) {
   if (ctx->dim == 3) {
     Fast3Args f3 = make_fast3_args(ctx, f, *a);
-    if (model == 1) launch_fast3<1>(ctx, f3, m);
-    else launch_fast3<2>(ctx, f3, m);
+    if (model == 1) launch_fast3<1>(ctx, f3, m, start_done);
+    else launch_fast3<2>(ctx, f3, m, start_done);
     return;
   }
-  // the inline variant (draw start + sampling in the kernel) also serves the
-  // uniform assignment when the generator is switched off
   bool inl = general || ctx->inline_normals == 1;
-  if (model == 1) launch_fast_model<1>(ctx, f, m, inl);
-  else launch_fast_model<2>(ctx, f, m, inl);
+  if (model == 1) launch_fast_model<1>(ctx, f, m, inl, start_done);
+  else launch_fast_model<2>(ctx, f, m, inl, start_done);
 }
 
 static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches);
@@ -2433,12 +2443,16 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         }
         HIPCHK(hipEventRecord(ctx->kev[3 * ti], ctx->stream));
       }
-      if (any_fast && ctx->dim == 2 && ctx->inline_normals != 1) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
+      // normals of the fused kernels: generated, or the inline samplers' draw starts
+      if (any_fast && !samples_inline(ctx)) launch_gen(ctx, make_fast_args(ctx, a, 0, 0), method);
+      if (any_fast && samples_inline(ctx))
+        for (auto& c : cls)
+          if (c.fast && c.hi > c.lo) launch_draw_start(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), method);
       if (!cap) HIPCHK(hipEventRecord(ctx->kev[3 * ti + 1], ctx->stream));
       for (auto& c : cls) {
         if (c.hi <= c.lo) continue;
         if (c.fast) {
-          launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), c.model, method, false, &a);
+          launch_fast_main(ctx, make_fast_args(ctx, a, c.lo, c.hi - c.lo), c.model, method, false, &a, true);
           ++nfast;
         } else {
           a.j0 = c.lo;

@@ -203,8 +203,9 @@ int cyl_stranded_get(cyl_ctx *ctx, double *xyz, int cap, int *n);
  * do are not counted; steps inside graph replays are not counted), and the
  * number of steps run by graph replays. */
 int cyl_last_path(cyl_ctx *ctx, int *fused_launches, int *generic_launches, int *graph_steps);
-/* Summed duration of the draw-start pass that precedes the fused 2D LSM-1
- * step kernel in the same timed steps (0 when the generic kernel ran). */
+/* Summed duration of what prepares the fused kernels' normals in the timed
+ * steps of the last cyl_step: the normal generator, or the draw-start pass of
+ * the inline samplers (always in 3D); 0 when the generic kernel ran. */
 int cyl_last_prep_timing(cyl_ctx *ctx, double *prep_ms);
 
 #ifdef __cplusplus

@@ -354,3 +354,16 @@ def c3_workload(npts_x=1000, npts_y=250, nlayers=10, seed=7):
                   opt_fields=np.stack([tke[None], eps[None]]),
                   opt_fields_names=['TURBULENT ENERG.', 'DISSIPATION'])
     return x, y, tri, h, fields
+
+
+def c3_release(x, y, h, npart, seed=17, lx=20000., ly=5000.):
+    """Rows of the 3D sediment workload on the c3_workload channel: uniform over
+    the interior, at a uniform fraction 0.1-0.9 of the local depth below the
+    surface (the nearest node's depth places them in the column), with
+    settling_diameters."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    px, py = rng.uniform(0.02 * lx, 0.98 * lx, npart), rng.uniform(0.02 * ly, 0.98 * ly, npart)
+    hn = h[cKDTree(np.stack([x, y], 1)).query(np.stack([px, py], 1))[1]]
+    pos = np.stack([px, py, -hn * rng.uniform(0.1, 0.9, npart)], 1)
+    return pos, settling_diameters(rng, npart)

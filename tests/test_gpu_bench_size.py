@@ -18,7 +18,7 @@ import pytest
 
 import cylag_amd as cy
 from oracle import refpy as R
-from tests.cases import SEED, c2_workload, c3_workload, settling_diameters
+from tests.cases import SEED, c2_workload, c3_release, c3_workload
 from tests.test_gpu_parity import assert_parity
 
 pytestmark = pytest.mark.gpu
@@ -69,14 +69,8 @@ def test_plain_workload_mesh_resorted(c2, rng_method):
 @pytest.fixture(scope="module")
 def c3():
     x, y, tri, h, fields = c3_workload()
-    rng = np.random.default_rng(17)
-    n = 60_000
-    lx, ly = 20000., 5000.
-    px, py = rng.uniform(0.02 * lx, 0.98 * lx, n), rng.uniform(0.02 * ly, 0.98 * ly, n)
-    from scipy.spatial import cKDTree
-    hn = h[cKDTree(np.stack([x, y], 1)).query(np.stack([px, py], 1))[1]]
-    pos = np.stack([px, py, -hn * rng.uniform(0.1, 0.9, n)], 1)
-    return x, y, tri, fields, pos, settling_diameters(rng, n)
+    pos, d = c3_release(x, y, h, 60_000)
+    return x, y, tri, fields, pos, d
 
 
 @pytest.mark.parametrize("model", [1, 2])
