@@ -29,20 +29,40 @@ from tests.cases import c2_workload, c3_release, c3_workload  # noqa: E402  (wor
 HBM_PEAK = 8.0e12
 METRIC = "particle-steps/s plain vs batched (1 GPU); p50/p99 scenario latency at fixed QPS"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_lsm1_2d_unstructured_keps_10M_counters.md")
+TRAFFIC_FILE_C3 = os.path.join(ROOT, "profiles", "k_fast_3d_channel_settling_5M_counters.md")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the fused step kernel, from the committed
-    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration); None when no summary is present."""
+def kernel_source_digest():
+    """sha256 (16 hex digits) of the device sources the library is built from
+    (cylag_amd/csrc/*.h, *.hip, in name order)."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    for f in sorted(glob.glob(os.path.join(ROOT, "cylag_amd", "csrc", "*.h")) +
+                    glob.glob(os.path.join(ROOT, "cylag_amd", "csrc", "*.hip"))):
+        with open(f, "rb") as fh:
+            h.update(os.path.basename(f).encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per launch of the fused step kernel from a committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    with the source digest the summary was measured at; the figure is None
+    when the summary is absent or was measured on other kernel sources."""
+    fields = {}
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if ":" in line and line.split(":", 1)[0] in ("traffic_bytes_per_launch", "kernel_source_sha256"):
+                    k, v = line.split(":", 1)
+                    fields[k] = v.strip()
     except OSError:
-        pass
-    return None
+        return None, None
+    digest = fields.get("kernel_source_sha256")
+    if "traffic_bytes_per_launch" not in fields or digest != kernel_source_digest():
+        return None, digest
+    return float(fields["traffic_bytes_per_launch"]), digest
 
 
 def calibrate_copy(nbytes=1 << 30):
@@ -227,6 +247,7 @@ def c3_leg(cy, steps=100, warmup=10, npart=5_000_000, sort_interval=8):
     nact = ps.npart_active
     solver.ctx.close()
     b_alg, _, _ = c3_bytes_per_particle_step(mesh.nt, mesh.nv, fields["nlayers"], nact)
+    traffic, digest = measured_traffic(TRAFFIC_FILE_C3)
     avg_ms = k_ms / max(nlaunch, 1)
     achieved = b_alg * nact / (avg_ms * 1e-3)
     log("c3: {} steps in {:.3f} s, fused kernel {:.3f} ms/launch ({} fused launches), draw starts {:.3f} ms".format(
@@ -237,7 +258,9 @@ def c3_leg(cy, steps=100, warmup=10, npart=5_000_000, sort_interval=8):
                          "k-eps K(z) with drift, sigma-layer channel {} elements x {} planes, {} particles, dt=2 s, "
                          "frozen".format(mesh.nt, fields["nlayers"], nact),
                 roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                          "frac": achieved / HBM_PEAK, "kernel": "k_fast_3d<2,2>", "kernel_avg_ms": avg_ms,
+                          "frac": achieved / HBM_PEAK, "traffic": traffic,
+                          "traffic_source": "profiles/k_fast_3d_channel_settling_5M_counters.md",
+                          "traffic_kernel_sources": digest, "kernel": "k_fast_3d<2,2>", "kernel_avg_ms": avg_ms,
                           "bytes_per_particle_step": b_alg, "draw_start_avg_ms": prep_ms / max(nlaunch, 1)})
 
 
@@ -497,7 +520,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=None,
                     help="run only the first steps of the batched job (profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="after the timed steps, copy 1 GiB device to device (for counter calibration)")
+                    help="after the legs, copy 1 GiB device to device (for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the step runs on one MI355X holding the whole mesh "
@@ -541,6 +564,8 @@ def main():
         cb["gpu_over_cpu"] = line["value"] / cb["value"]
         cb["gpu_over_one_core"] = line["value"] / cb["single_core"]["value"]
         line["cpu_baseline"] = cb
+    if args.calibrate_copy:
+        calibrate_copy()
     print(json.dumps(line), flush=True)
 
 
@@ -563,11 +588,10 @@ def plain_leg(cy, args, mesh, fields, pos, params):
     nact = ps.npart_active
     log("timed: {} steps in {:.3f} s, step kernel {:.3f} ms/launch over {} launches, normals {:.3f} ms".format(
         args.steps, wall, k_ms / max(nlaunch, 1), nlaunch, prep_ms / max(nlaunch, 1)))
-    if args.calibrate_copy:
-        calibrate_copy()
     solver.ctx.close()
     psteps = nact * args.steps / wall
     b_alg, b_state, b_mesh = bytes_per_particle_step(mesh.nt, mesh.nv, nact)
+    traffic, digest = measured_traffic()
     avg_ms = k_ms / max(nlaunch, 1)
     achieved = b_alg * nact / (avg_ms * 1e-3)
     return {
@@ -580,7 +604,9 @@ def plain_leg(cy, args, mesh, fields, pos, params):
                    "elements": int(mesh.nt), "nodes": int(mesh.nv), "particles": int(nact),
                    "rng_method": 2, "sort_interval": args.sort_interval},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": measured_traffic(),
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "traffic_source": "profiles/k_lsm1_2d_unstructured_keps_10M_counters.md",
+                     "traffic_kernel_sources": digest,
                      "kernel": "k_lsm1_2d<2,true>", "kernel_avg_ms": avg_ms,
                      "bytes_per_particle_step": b_alg,
                      "normals_kernel_avg_ms": prep_ms / max(nlaunch, 1)},

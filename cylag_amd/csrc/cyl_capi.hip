@@ -61,6 +61,11 @@ struct cyl_ctx {
   FastMesh fm{};
   NodeF* d_nodes = nullptr;
   NodeL* d_nodes3 = nullptr;  // 3D: per node and sigma plane
+  // packed node records are valid for the frozen fields of stamp pack_stamp,
+  // with k-epsilon K on or off (packed_keps) and factor CMU / packed_sigc
+  int64_t field_stamp = 0, pack_stamp = -1;
+  int packed_keps = -1;
+  double packed_sigc = 0.;
 
   // fields
   FieldsDev fields{};
@@ -530,6 +535,7 @@ extern "C" int cyl_mesh_upload(cyl_ctx* ctx, int nv, const double* x, const doub
 extern "C" int cyl_fields_upload(cyl_ctx* ctx, int dim, int nlayers, int npoints, int ntimes, const double* times,
                                  const double* elev, const double* vx, const double* vy, const double* vz, int nopt,
                                  const double* opt, const int* fids, int friction_model, double friction_coef) {
+  if (ctx) ++ctx->field_stamp;
   if (!ctx || (dim != 2 && dim != 3) || nlayers < 1 || npoints < 1 || ntimes < 1 || nopt < 0 || nopt > MAX_OPT)
     return CYL_E_ARG;
   if (!elev || !vx || !vy || (dim == 3 && !vz) || (nopt > 0 && !opt)) return CYL_E_ARG;
@@ -580,25 +586,25 @@ extern "C" int cyl_fields_upload(cyl_ctx* ctx, int dim, int nlayers, int npoints
       f.prev[fid] = pv;
     }
   }
-  // k-epsilon bounds (k of every node and frame strictly positive and finite)
+  // k-epsilon bounds: the range of k^2 / max(eps, EPSILON) over every node
+  // and frame (nodal K without its constant factor), with k strictly positive
+  // and finite everywhere
   ctx->keps_ok = false;
   if (f.fid_turbeng >= 0 && f.fid_dissip >= 0 && src[f.fid_turbeng] && src[f.fid_dissip]) {
     const double* kt = src[f.fid_turbeng];
     const double* ep = src[f.fid_dissip];
-    double kmin = HUGE_VAL, kmax = 0., emin = HUGE_VAL, emax = 0.;
+    double lo = HUGE_VAL, hi = 0.;
     bool ok = true;
     for (int64_t i = 0; i < (int64_t)ntimes * size && ok; ++i) {
       ok = kt[i] > 0. && std::isfinite(kt[i]) && std::isfinite(ep[i]);
-      double e = std::max(ep[i], EPS);
-      kmin = std::min(kmin, kt[i]);
-      kmax = std::max(kmax, kt[i]);
-      emin = std::min(emin, e);
-      emax = std::max(emax, e);
+      double r = kt[i] * kt[i] / std::max(ep[i], EPS);
+      lo = std::min(lo, r);
+      hi = std::max(hi, r);
     }
-    if (ok) {
+    if (ok && std::isfinite(hi)) {
       ctx->keps_ok = true;
-      ctx->keps_lo = kmin * kmin / emax;
-      ctx->keps_hi = kmax * kmax / emin;
+      ctx->keps_lo = lo;
+      ctx->keps_hi = hi;
     }
   }
   // time-interpolation pointer tables: fields 0..dim (main) and every optional field
@@ -643,6 +649,7 @@ static int lower_index(const std::vector<double>& a, double v, int last) {
 
 extern "C" int cyl_fields_reset(cyl_ctx* ctx, int frozen, double t0) {
   if (!ctx || !ctx->has_fields) return CYL_E_STATE;
+  ++ctx->field_stamp;
   ctx->frozen = frozen ? 1 : 0;
   ctx->fields.frozen = ctx->frozen;
   if (!ctx->frozen) ctx->record = lower_index(ctx->times, t0, 0);
@@ -1984,6 +1991,7 @@ static int ensure_fast(cyl_ctx* ctx) {
   ctx->fm.edge_labels = m.edge_labels;
   ctx->fm.nt = m.nt;
   ctx->fast_ready = true;
+  ctx->pack_stamp = -1;
   return CYL_OK;
 }
 
@@ -2396,18 +2404,26 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
           keps |= P[s].diffusion_model >= 2;
           sigc = P[s].schmidt_number;
         }
-      if (ctx->dim == 3) {
-        int64_t nrec = (int64_t)fd.npoints * fd.nlayers;
-        k_pack_nodes3<<<nblk(nrec), BLK, 0, ctx->stream>>>(
-            fd.tmp[fd.fid_zs], fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], fd.tmp[fd.fid_uz],
-            keps ? fd.tmp[fd.fid_turbeng] : nullptr, keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / sigc,
-            fd.npoints, fd.nlayers, ctx->d_node_old, ctx->d_nodes3);
-      } else {
-        k_pack_nodes<<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(
-            fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], keps ? fd.tmp[fd.fid_turbeng] : nullptr,
-            keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / sigc, fd.npoints, ctx->d_node_old, ctx->d_nodes);
+      // frozen fields: the records packed at an earlier step are still the step's
+      const bool fresh = ctx->frozen && ctx->pack_stamp == ctx->field_stamp && ctx->packed_keps == (int)keps &&
+                         ctx->packed_sigc == sigc;
+      if (!fresh) {
+        if (ctx->dim == 3) {
+          int64_t nrec = (int64_t)fd.npoints * fd.nlayers;
+          k_pack_nodes3<<<nblk(nrec), BLK, 0, ctx->stream>>>(
+              fd.tmp[fd.fid_zs], fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], fd.tmp[fd.fid_uz],
+              keps ? fd.tmp[fd.fid_turbeng] : nullptr, keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / sigc,
+              fd.npoints, fd.nlayers, ctx->d_node_old, ctx->d_nodes3);
+        } else {
+          k_pack_nodes<<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(
+              fd.tmp[fd.fid_ux], fd.tmp[fd.fid_uy], keps ? fd.tmp[fd.fid_turbeng] : nullptr,
+              keps ? fd.tmp[fd.fid_dissip] : nullptr, CMU / sigc, fd.npoints, ctx->d_node_old, ctx->d_nodes);
+        }
+        ++launches;
+        ctx->pack_stamp = ctx->field_stamp;
+        ctx->packed_keps = (int)keps;
+        ctx->packed_sigc = sigc;
       }
-      ++launches;
     }
     a.A = ctx->pool[ctx->cur];
     a.B = ctx->pool[1 - ctx->cur];
