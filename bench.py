@@ -456,18 +456,18 @@ def c5_requests(cy, x, y, h, window, lam=20., seed=13, dt=10.):
                        npart=int(npart[i]), dim=3) for i in range(m)]
 
 
-def served_leg(cy, window=60., horizon=75., lam=20., max_rows=20_000_000, tick_steps=10, sort_interval=8):
-    """The request loop on the 3D channel: arrivals over `window` seconds, the
-    loop stopped `horizon` seconds after the start; latencies are those of the
-    requests completed by then."""
+def c5_mean_work(dt=10.):
+    """Expected particle-steps of one request: E[N] E[steps] for N ~ logU(1e3,
+    2e6) and steps = round(U[1, 48] h / dt) (independent draws)."""
+    lo, hi = 1e3, 2e6
+    return (hi - lo) / np.log(hi / lo) * (24.5 * 3600. / dt)
+
+
+def served_run(cy, fs, x, y, h, lam, window, horizon, max_rows, tick_steps, sort_interval):
+    """One request loop on the 3D channel: Poisson arrivals at `lam` per second
+    over `window` seconds, stopped `horizon` seconds after the start."""
     from cylag_amd.serve import latency_stats
-    t0 = time.time()
-    x, y, tri, h, fields = c3_workload()
-    mesh = cy.TriangularMesh(x, y, tri)
-    fs = cy.EulerianFieldSet(mesh, **fields)
     reqs = c5_requests(cy, x, y, h, window, lam=lam)
-    log("served: 3D channel {} elements, {} requests over {:.0f} s ({:.1f} s)".format(
-        tri.shape[0], len(reqs), window, time.time() - t0))
     last = [0.]
 
     def progress(msg):
@@ -480,20 +480,49 @@ def served_leg(cy, window=60., horizon=75., lam=20., max_rows=20_000_000, tick_s
                         keep_results=False, log=progress)
     done = loop.serve(reqs, horizon=horizon)
     loop.close()
-    st = latency_stats(done)
+    st = latency_stats(done, loop.unfinished, horizon=horizon)
     offered = sum(r.npart * r.nsteps for r in reqs) / window
     out = dict(lam_per_s=lam, window_s=window, horizon_s=horizon, arrived=len(reqs),
                admitted=sum(r.admitted is not None for r in reqs), completed=st["completed"],
+               completed_frac=st.get("completed_frac"),
                latency_p50_s=st["p50_s"], latency_p99_s=st["p99_s"],
+               latency_p50_censored_s=st.get("p50_censored_s"), latency_p99_censored_s=st.get("p99_censored_s"),
                value=loop.particle_steps / loop.elapsed, unit="particle-steps/s",
                offered_particle_steps_per_s=offered, max_running=loop.max_running, capacity_rows=loop.capacity,
-               tick_steps=tick_steps,
-               workload=("3D sigma-layer channel (~500k elements, 10 planes, frozen); Poisson {} req/s (seed 13); "
-                         "N ~ logU(1e3, 2e6), 1-48 h at dt = 10 s, LSM1 or LSM2; admission by free HBM "
-                         "capped at {} rows; latencies of requests completed within {:.0f} s").format(
-                             lam, max_rows, horizon))
-    log("served: {} arrived, {} admitted, {} completed, p50 {} s, p99 {} s, {:.3e} particle-steps/s".format(
-        out["arrived"], out["admitted"], out["completed"], out["latency_p50_s"], out["latency_p99_s"], out["value"]))
+               tick_steps=tick_steps)
+    log("served: lambda {} /s: {} arrived, {} admitted, {} completed ({:.0%}), p50 {} s, p99 {} s, censored p50 {} s, "
+        "p99 {} s, {:.3e} particle-steps/s".format(lam, out["arrived"], out["admitted"], out["completed"],
+                                                  out["completed_frac"] or 0., out["latency_p50_s"],
+                                                  out["latency_p99_s"], out["latency_p50_censored_s"],
+                                                  out["latency_p99_censored_s"], out["value"]))
+    return out
+
+
+def served_leg(cy, window=60., horizon=75., lam=20., max_rows=20_000_000, tick_steps=10, sort_interval=8,
+               stable_window=40., stable_horizon=75., stable_margin=0.7):
+    """The request loop on the 3D channel at the stated rate (lam = 20
+    requests/s, far above what one GPU serves), then at a rate below
+    saturation: the saturated throughput over the requests' mean work gives
+    the largest stable rate, and a second run at stable_margin times that
+    rate measures the latencies there.  Latencies are reported over the
+    completed requests and over every arrival (unfinished requests counted
+    with horizon - arrival, which makes those percentiles lower bounds)."""
+    t0 = time.time()
+    x, y, tri, h, fields = c3_workload()
+    mesh = cy.TriangularMesh(x, y, tri)
+    fs = cy.EulerianFieldSet(mesh, **fields)
+    log("served: 3D channel {} elements ({:.1f} s)".format(tri.shape[0], time.time() - t0))
+    out = served_run(cy, fs, x, y, h, lam, window, horizon, max_rows, tick_steps, sort_interval)
+    work = c5_mean_work()
+    lam_stable = out["value"] / work
+    out["workload"] = ("3D sigma-layer channel (~500k elements, 10 planes, frozen); Poisson {} req/s (seed 13); "
+                       "N ~ logU(1e3, 2e6), 1-48 h at dt = 10 s, LSM1 or LSM2 with settling; admission by free HBM "
+                       "capped at {} rows, small requests pass a blocked large one until it has waited 10 s").format(
+                           lam, max_rows)
+    out["mean_request_particle_steps"] = work
+    out["lambda_stable_per_s"] = lam_stable
+    out["stable"] = served_run(cy, fs, x, y, h, round(stable_margin * lam_stable, 3), stable_window, stable_horizon,
+                               max_rows, tick_steps, sort_interval)
     return out
 
 

@@ -38,6 +38,13 @@ struct Bufs {
     for (void* p : ptrs) (void)hipFree(p);
     ptrs.clear();
   }
+  // free one buffer of the set
+  hipError_t release(void* p) {
+    auto it = std::find(ptrs.begin(), ptrs.end(), p);
+    if (it == ptrs.end()) return hipSuccess;
+    ptrs.erase(it);
+    return hipFree(p);
+  }
 };
 
 }  // namespace
@@ -1119,9 +1126,15 @@ extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const
   int e = ensure_scen_tables(ctx);
   if (e) return e;
   if (!ctx->d_scen) HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scen, ctx->n));
-  // normals buffer for the per-scenario chunk padding (see k_gen_normals)
-  HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)ctx->n + 32 * ((size_t)nscen + 2) + 64 * 32));
-  ctx->xi_cap = (int64_t)ctx->n + 32 * ((int64_t)nscen + 2) + 64 * 32;
+  // normals buffer with the per-scenario chunk padding (see k_gen_normals):
+  // the upload's buffer when it is large enough, else a larger one
+  const int64_t need_xi = (int64_t)ctx->n + 32 * ((int64_t)nscen + 2) + 64 * 32;
+  if (need_xi > ctx->xi_cap) {
+    HIPCHK(ctx->part_bufs.release(ctx->d_xi));
+    ctx->d_xi = nullptr;
+    HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)need_xi));
+    ctx->xi_cap = need_xi;
+  }
   k_scen_of_slot<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_gpool, ctx->d_scen_base, nscen, ctx->n, ctx->d_scen);
   ctx->rng_state.assign(state, state + nscen);
   ctx->rng_inc.assign(inc, inc + nscen);

@@ -7,6 +7,11 @@ batch, advances the batch until the next request finishes (or a tick of
 steps), writes finished requests' rows back into their particle sets and drops
 them from the device.  Rows never leave the device while a request runs.
 
+Admission goes through the queue in arrival order and takes every request
+that fits, so small requests pass a large one that waits for room; once the
+oldest waiting request has waited reserve_after_s, nothing passes it any more
+and the room freed by finishing requests goes to it (no starvation).
+
 Each request's result is the result of a plain Solver run of that request
 alone with the same stream and steps (with frozen fields, at any admission
 time; with time-dependent fields, for a Solver built at the batch time at
@@ -65,13 +70,19 @@ class ServeLoop:
     holds the old and the new pool at once, hence the default fraction),
     capped at max_rows if given (a bound on the step time).
     tick_steps: most steps between two admission points.
-    keep_results: False drops a finished request's rows after collecting them."""
+    keep_results: False drops a finished request's rows after collecting them.
+    bypass: False admits strictly first come, first served (a request that does
+    not fit blocks the ones behind it); True lets later requests that fit pass
+    it until it has waited reserve_after_s seconds."""
 
     def __init__(self, field_set, device=0, capacity=None, hbm_fraction=0.4, tick_steps=50, sort_interval=8,
-                 clock=time.perf_counter, log=None, batch_factory=BatchSolver, keep_results=True, max_rows=None):
+                 clock=time.perf_counter, log=None, batch_factory=BatchSolver, keep_results=True, max_rows=None,
+                 bypass=True, reserve_after_s=10.):
         self.field_set = field_set
         self.keep_results = keep_results
         self.max_rows = max_rows
+        self.bypass = bypass
+        self.reserve_after_s = reserve_after_s
         self.batch_factory = batch_factory
         self.device = device
         self.capacity = capacity
@@ -152,14 +163,15 @@ class ServeLoop:
                 waiting.append(pending.popleft())
             if self.batch is not None:
                 self._finish(running, now)
-            # admission: first come, first served, while the device has room
+            # admission in arrival order while the device has room
             adm = []
             dim = waiting[0].dim if waiting else 2
             room = self._room(dim)
-            while waiting and (room is None or waiting[0].npart <= room - sum(r.npart for r in adm)):
-                adm.append(waiting.popleft())
-                if room is None:
-                    break  # the first batch fixes the capacity
+            if room is None:
+                if waiting:
+                    adm.append(waiting.popleft())  # the first batch fixes the capacity
+            else:
+                adm = self._pick(waiting, room, now)
             if waiting and not adm and not running:
                 adm.append(waiting.popleft())  # larger than the device's room: it runs alone
             if adm:
@@ -190,15 +202,42 @@ class ServeLoop:
         self.unfinished = list(running.values()) + list(waiting) + list(pending)
         return self.done
 
+    def _pick(self, waiting, room, now):
+        """Requests admitted now (removed from `waiting`): in arrival order, each
+        one that fits the room left; one that does not fit blocks the rest
+        unless bypassing is on and it has waited less than reserve_after_s."""
+        adm, free = [], room
+        for r in list(waiting):
+            if r.npart <= free:
+                adm.append(r)
+                free -= r.npart
+            elif not self.bypass or now - r.arrival >= self.reserve_after_s:
+                break
+        for r in adm:
+            waiting.remove(r)
+        return adm
+
     def close(self):
         if self.batch is not None:
             self.batch.close()
             self.batch = None
 
 
-def latency_stats(reqs):
-    lat = np.array([r.latency for r in reqs if r.latency is not None])
-    if not lat.size:
-        return dict(completed=0, p50_s=None, p99_s=None)
-    return dict(completed=int(lat.size), p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                mean_s=float(lat.mean()), max_s=float(lat.max()))
+def latency_stats(done, unfinished=(), horizon=None):
+    """Latency percentiles of the completed requests, and over every arrival:
+    a request unfinished at `horizon` (seconds after the start) counts with
+    horizon - arrival, a lower bound of its latency, so the all-arrival
+    percentiles are lower bounds of the true ones (censored at the horizon)."""
+    lat = np.array([r.latency for r in done if r.latency is not None])
+    out = dict(completed=int(lat.size), p50_s=None, p99_s=None)
+    if lat.size:
+        out.update(p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                   mean_s=float(lat.mean()), max_s=float(lat.max()))
+    if horizon is not None:
+        cens = np.array([horizon - r.arrival for r in unfinished if r.arrival <= horizon])
+        allv = np.concatenate([lat, cens])
+        out.update(arrived=int(allv.size), unfinished=int(cens.size),
+                   completed_frac=float(lat.size / allv.size) if allv.size else None)
+        if allv.size:
+            out.update(p50_censored_s=float(np.percentile(allv, 50)), p99_censored_s=float(np.percentile(allv, 99)))
+    return out

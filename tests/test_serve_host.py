@@ -100,7 +100,9 @@ def test_serve_admits_by_room_and_overlaps():
 def test_serve_runs_oversized_request_alone_and_stops_at_horizon():
     FakeBatch.log = []
     reqs = [make(20000, 6, 0.), make(100, 1000, 0.)]
-    loop = ServeLoop(None, capacity=5000, tick_steps=10, clock=FakeClock(), batch_factory=FakeBatch)
+    # reserve_after_s=0: nothing passes the waiting head, which runs alone as it never fits
+    loop = ServeLoop(None, capacity=5000, tick_steps=10, clock=FakeClock(), batch_factory=FakeBatch,
+                     reserve_after_s=0.)
     loop.serve(reqs, horizon=0.05)
     assert reqs[0].finished is not None
     assert reqs[1].finished is None and reqs[1] in loop.unfinished
@@ -117,3 +119,36 @@ def test_serve_room_from_free_memory():
     # 0.5 * (free + resident rows) / row bytes = 0.5 * (10000 + 2000) rows = 6000 rows
     assert loop.capacity == 6000
     assert FakeBatch.log[0][1] == [0, 1, 2]
+
+
+def test_small_requests_pass_a_blocked_large_one():
+    """A 5000-row request that does not fit waits; a 2000-row request behind
+    it that fits is admitted (bypass); with bypass off, or once the large
+    request has waited reserve_after_s, the small one waits behind it."""
+    def run(**kw):
+        FakeBatch.log = []
+        reqs = [make(5000, 200, 0.), make(5000, 10, 0.001), make(2000, 10, 0.002)]
+        loop = ServeLoop(None, capacity=8000, tick_steps=4, clock=FakeClock(), batch_factory=FakeBatch, **kw)
+        loop.serve(reqs)
+        assert all(r.finished is not None for r in reqs)
+        return reqs
+    a, b, c = run()
+    assert c.admitted < b.admitted and c.finished < b.admitted
+    a, b, c = run(bypass=False)
+    assert c.admitted >= b.admitted
+    a, b, c = run(reserve_after_s=0.)
+    assert c.admitted >= b.admitted
+
+
+def test_latency_stats_censored_over_all_arrivals():
+    done = [make(1, 1, a) for a in (0., 1., 2.)]
+    for r, f in zip(done, (3., 5., 4.)):
+        r.finished = f
+    late = [make(1, 1, a) for a in (4., 8.)]
+    st = latency_stats(done, late, horizon=10.)
+    assert st["completed"] == 3 and st["arrived"] == 5 and st["unfinished"] == 2
+    assert st["completed_frac"] == 3 / 5
+    lat = np.array([3., 4., 2., 6., 2.])   # completed latencies, then horizon - arrival
+    assert st["p50_censored_s"] == np.percentile(lat, 50)
+    assert st["p99_censored_s"] == np.percentile(lat, 99)
+    assert st["p50_s"] == 3.
