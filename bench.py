@@ -320,7 +320,7 @@ def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250, tune
         len(npart), int(npart.sum()), int(nsteps.min()), int(nsteps.max()), time.time() - t0))
     total = int(nsteps.max()) if max_steps is None else min(int(max_steps), int(nsteps.max()))
     t0 = time.perf_counter()
-    done, ncollected, tuned = 0, 0, []
+    done, ncollected, tuned, graph_steps = 0, 0, [], 0
     tune_at = [0, total // 2] if tune else []
     while done < total:
         if tune_at and done >= tune_at[0] and total - done > 400:
@@ -333,6 +333,7 @@ def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250, tune
             continue
         k = min(chunk - done % chunk, total - done)
         b.run(k)
+        graph_steps += b.ctx.last_graph_steps()
         done += k
         # finished scenarios: rows back to the host, then off the device
         fin = [s for s in range(b.nscen) if b.steps_left(s) == 0]
@@ -351,7 +352,7 @@ def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250, tune
     work = float(np.sum(npart * np.minimum(nsteps, total)))
     b.ctx.close()
     return dict(scenarios=len(npart), particles=int(npart.sum()), lsm2_scenarios=int(np.sum(model == 2)),
-                particle_steps=work, steps=total, wall_s=wall, value=work / wall,
+                particle_steps=work, steps=total, wall_s=wall, value=work / wall, graph_steps=graph_steps,
                 tuning=tuned if tune else [dict(step=0, sort_interval=sort_interval, blocks_per_cu=0)])
 
 

@@ -87,6 +87,11 @@ struct cyl_ctx {
   double* d_prev[4] = {};
   int record = 0, interp_done = 0, prev_valid = 0, frozen = 1;
   double interp_hist[2] = {0., 0.};         // times of the last two interpolations (most recent last)
+  // graph replays of time-varying fields: the (record, weight) of each step of
+  // the call, indexed by step id - interp_step0 (host-computed)
+  InterpStep* d_interp_tab = nullptr;
+  int64_t interp_tab_cap = 0;
+  int interp_step0 = 0;
   int interp_count = 0;
   const double** d_ti_src = nullptr;        // [ntimes][nti] frame pointers of the interpolated fields
   double** d_ti_dst = nullptr;              // [2][nti] destination per buffer parity
@@ -361,6 +366,7 @@ extern "C" int cyl_ctx_destroy(cyl_ctx* ctx) {
   ctx->fast_ready = false;
   if (ctx->d_pos0) (void)hipFree(ctx->d_pos0);
   if (ctx->d_state0) (void)hipFree(ctx->d_state0);
+  if (ctx->d_interp_tab) (void)hipFree(ctx->d_interp_tab);
   ctx->field_bufs.free_all();
   ctx->part_bufs.free_all();
   ctx->ev_bufs.free_all();
@@ -677,8 +683,9 @@ extern "C" int cyl_deleted_count(cyl_ctx* ctx, long long* n) {
 // enqueue _time_interpolation(time) (eulerian_field_set.pyx:866-938).  The
 // main fields are double-buffered: the previous interpolation stays in the
 // other buffer, which is what the reference copies into _tmp_*0.
-static int enqueue_time_interp(cyl_ctx* ctx, double time, int& launches) {
-  if (ctx->frozen) return CYL_OK;
+// Host state of one time interpolation at `time` (record, weight, history,
+// buffer parity); returns CYL_OK or the reference's error.
+static int time_interp_state(cyl_ctx* ctx, double time, double& tf) {
   ctx->prev_valid = ctx->interp_done;
   ctx->interp_done = 1;
   ctx->interp_hist[0] = ctx->interp_hist[1];
@@ -690,22 +697,39 @@ static int enqueue_time_interp(cyl_ctx* ctx, double time, int& launches) {
   double dtt = ctx->times[rec] - ctx->times[rec + 1];
   if (std::fabs(dtt) < EPS)
     return ctx->set_err(CYL_E_STEP, -1, "Time interpolation failed: duplicate or too-close time values detected");
-  double tf = (time - ctx->times[rec + 1]) / dtt;
-  int par = 1 - ctx->tmp_parity;  // write into the buffer that held the older values
-  dim3 g(nblk(ctx->fsize), ctx->nti);
-  k_time_interp<<<g, BLK, 0, ctx->stream>>>(ctx->d_ti_src + (size_t)rec * ctx->nti,
-                                           ctx->d_ti_src + (size_t)(rec + 1) * ctx->nti,
-                                           ctx->d_ti_dst + (size_t)par * ctx->nti, ctx->nti, ctx->fsize, tf);
-  ++launches;
-  ctx->tmp_parity = par;
+  tf = (time - ctx->times[rec + 1]) / dtt;
+  ctx->tmp_parity = 1 - ctx->tmp_parity;  // the new values go into the buffer that held the older ones
   for (int fid = 0; fid < 4; ++fid) {
     if (!ctx->d_tmp[fid] || !ctx->d_tmpbuf2[fid]) continue;
-    double* cur = par == 0 ? ctx->d_tmp[fid] : ctx->d_tmpbuf2[fid];
-    double* old = par == 0 ? ctx->d_tmpbuf2[fid] : ctx->d_tmp[fid];
+    double* cur = ctx->tmp_parity == 0 ? ctx->d_tmp[fid] : ctx->d_tmpbuf2[fid];
+    double* old = ctx->tmp_parity == 0 ? ctx->d_tmpbuf2[fid] : ctx->d_tmp[fid];
     ctx->fields.tmp[fid] = cur;
     ctx->fields.prev[fid] = old;
   }
   ctx->fields.prev_valid = ctx->prev_valid;
+  return CYL_OK;
+}
+
+// enqueue _time_interpolation(time) (eulerian_field_set.pyx:866-938).  The
+// main fields are double-buffered: the previous interpolation stays in the
+// other buffer, which is what the reference copies into _tmp_*0.  clock: the
+// step is being captured into a graph; the record and weight then come from
+// the call's table through the device step clock.
+static int enqueue_time_interp(cyl_ctx* ctx, double time, int& launches, const int* clock = nullptr) {
+  if (ctx->frozen) return CYL_OK;
+  double tf = 0.;
+  int e = time_interp_state(ctx, time, tf);
+  if (e) return e;
+  const int rec = ctx->record, par = ctx->tmp_parity;
+  dim3 g(nblk(ctx->fsize), ctx->nti);
+  if (clock)
+    k_time_interp_tab<<<g, BLK, 0, ctx->stream>>>(ctx->d_ti_src, ctx->d_ti_dst + (size_t)par * ctx->nti, ctx->nti,
+                                                 ctx->fsize, ctx->d_interp_tab, clock, ctx->interp_step0);
+  else
+    k_time_interp<<<g, BLK, 0, ctx->stream>>>(ctx->d_ti_src + (size_t)rec * ctx->nti,
+                                             ctx->d_ti_src + (size_t)(rec + 1) * ctx->nti,
+                                             ctx->d_ti_dst + (size_t)par * ctx->nti, ctx->nti, ctx->fsize, tf);
+  ++launches;
   return CYL_OK;
 }
 
@@ -2392,7 +2416,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       ctx->steps_since_sort = 0;
     }
     if (!init_only) {
-      e = enqueue_time_interp(ctx, t, launches);
+      e = enqueue_time_interp(ctx, t, launches, cap ? ctx->d_clock : nullptr);
       if (e) return e;
     }
     // per-slot arrays (their buffers are swapped by the sort)
@@ -2570,12 +2594,14 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
   int step = 0;
   ctx->last_graph_steps = 0;
   // Graph replay of the loop: a period of 2 * sort_interval steps returns the
-  // state copies, the spare buffers and the sort phase to where they were, so
-  // one captured period is launched repeatedly; the step counters live on the
-  // device.  Needs the uniform draw assignment, frozen fields (the captured
-  // interpolation would be the first step's) and no per-step statistics.
+  // state copies, the spare buffers, the sort phase and the interpolated
+  // fields' buffer parity to where they were, so one captured period is
+  // launched repeatedly; the step counters live on the device, and with
+  // time-varying fields each step's record and weight come from a table the
+  // host computes for the call.  Needs the uniform draw assignment and no
+  // per-step statistics.
   const int period = 2 * std::max(1, ctx->sort_interval);
-  if (ctx->use_graph && !init_only && can_uniform && ctx->frozen && !per_step_records && nsteps >= 1 + 2 * period) {
+  if (ctx->use_graph && !init_only && can_uniform && !per_step_records && nsteps >= 1 + 2 * period) {
     // one eager step first: arrivals initialized, generator flags and packed records in place
     e = one_step(step++, false);
     if (e) return e;
@@ -2583,8 +2609,35 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     if (e) return e;
     e = ensure_fast(ctx);  // no allocation or packing inside the capture
     if (e) return e;
-    if (!ctx->pending_init) {
-      const int nrep = (nsteps - step) / period;
+    const int nrep = (nsteps - step) / period;
+    // time-varying fields: the record and weight of every replayed step, from
+    // the host's own interpolation state (a step past the last record leaves
+    // the steps to the eager loop, which reports it where it happens)
+    bool tab_ok = true;
+    if (!ctx->frozen && !ctx->pending_init) {
+      const int64_t ntab = (int64_t)nrep * period;
+      std::vector<InterpStep> tab((size_t)ntab);
+      int rec = ctx->record;
+      double tt = t;
+      for (int64_t q = 0; q < ntab && tab_ok; ++q) {
+        rec = lower_index(ctx->times, tt, rec);
+        double dtt = rec + 1 < ctx->ntimes ? ctx->times[rec] - ctx->times[rec + 1] : 0.;
+        tab_ok = rec + 1 < ctx->ntimes && std::fabs(dtt) >= EPS;
+        if (tab_ok) tab[(size_t)q] = InterpStep{(tt - ctx->times[rec + 1]) / dtt, (int64_t)rec};
+        tt += dt;
+      }
+      if (tab_ok) {
+        if (ntab > ctx->interp_tab_cap) {
+          if (ctx->d_interp_tab) HIPCHK(hipFree(ctx->d_interp_tab));
+          ctx->d_interp_tab = nullptr;
+          HIPCHK(hipMalloc(&ctx->d_interp_tab, (size_t)ntab * sizeof(InterpStep)));
+          ctx->interp_tab_cap = ntab;
+        }
+        HIPCHK(hipMemcpy(ctx->d_interp_tab, tab.data(), (size_t)ntab * sizeof(InterpStep), hipMemcpyHostToDevice));
+        ctx->interp_step0 = ctx->step_counter;
+      }
+    }
+    if (!ctx->pending_init && tab_ok) {
       HIPCHK(hipStreamSynchronize(ctx->stream));
       int clk[2] = {ctx->step_counter, ctx->batch_steps};
       HIPCHK(hipMemcpy(ctx->d_clock, clk, sizeof(clk), hipMemcpyHostToDevice));
@@ -2618,7 +2671,14 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
         if (r > 0) {
           ctx->step_counter += period;
           ctx->batch_steps += period;
-          for (int q = 0; q < period; ++q) t += dt;
+          for (int q = 0; q < period; ++q) {
+            if (!ctx->frozen) {
+              double tf = 0.;
+              int ie = time_interp_state(ctx, t, tf);  // the host's record of the replayed interpolation
+              if (ie) return ie;
+            }
+            t += dt;
+          }
         }
       }
       step += nrep * period;

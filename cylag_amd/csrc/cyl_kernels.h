@@ -251,6 +251,27 @@ __global__ void __launch_bounds__(BLK) k_time_interp(const double* const* f0, co
   out[q][i] = tf * f0[q][i] + (1. - tf) * f1[q][i];
 }
 
+// Record and weight of one step's time interpolation, computed by the host for
+// the steps of a replayed graph (the record bracketing the step's time and
+// tf = (t - t[rec+1]) / (t[rec] - t[rec+1]), eulerian_field_set.pyx:909-916)
+struct InterpStep {
+  double tf;
+  int64_t rec;
+};
+
+// k_time_interp of a captured step: the step's entry of the table, found
+// through the device step clock, selects the frames and the weight
+__global__ void __launch_bounds__(BLK) k_time_interp_tab(const double* const* frames, double* const* out, int nf,
+                                                         int64_t size, const InterpStep* tab, const int* clock,
+                                                         int step0) {
+  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  int q = blockIdx.y;
+  if (i >= size || q >= nf) return;
+  const InterpStep e = tab[clock[0] - step0];
+  const double* const* f0 = frames + e.rec * nf;
+  out[q][i] = e.tf * f0[q][i] + (1. - e.tf) * f0[nf + q][i];
+}
+
 // ---------------------------------------------------------------------------
 // Pool-order scans
 // ---------------------------------------------------------------------------

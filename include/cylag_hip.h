@@ -179,9 +179,9 @@ int cyl_batch_reshape(cyl_ctx *ctx, int nkeep, const int *keep, int nnew, const 
  * that draws runs of ranks in pool order, the default; 1: sampled inline in
  * the kernel; 2: a generator that draws each slot's pair at its rank's stream
  * position and stores it in slot order), and whether multi-step calls replay
- * a captured HIP graph of 2 * sort_interval steps (frozen fields, uniform draw
- * assignment, no boundary statistics; otherwise the steps are enqueued one by
- * one).  Results do not depend on any of them. */
+ * a captured HIP graph of 2 * sort_interval steps (uniform draw assignment,
+ * no boundary statistics; otherwise the steps are enqueued one by one).
+ * Results do not depend on any of them. */
 int cyl_set_tuning(cyl_ctx *ctx, int sort_interval, int blocks_per_cu, int inline_normals, int use_graph);
 /* Timing of the last cyl_step: summed duration of the fused step-kernel
  * launches (HIP events on the context's stream), wall time of the call, and

@@ -294,13 +294,16 @@ GRAPH_CASES = [
 
 
 @pytest.mark.parametrize("rng_method", [1, 2, 3])
+@pytest.mark.parametrize("frozen", [True, False], ids=["frozen", "two_frames"])
 @pytest.mark.parametrize("which", range(len(GRAPH_CASES)), ids=[c[0] for c in GRAPH_CASES])
-def test_graph_replay(which, rng_method):
+def test_graph_replay(which, frozen, rng_method):
     """Multi-step calls replayed from a captured HIP graph (device-side step
-    counters, a period of two re-sorts) give the oracle's results."""
+    counters, a period of two re-sorts) give the oracle's results, with frozen
+    fields and with two frames interpolated in time (each replayed step's
+    record and weight from the host's table)."""
     name, mk, p = GRAPH_CASES[which]
     case = mk()
-    params = dict(final_time=300., time_step=5., frozen_eulerian_fields=True, rng_method=rng_method, **p)
+    params = dict(final_time=300., time_step=5., frozen_eulerian_fields=frozen, rng_method=rng_method, **p)
     s = case.build_gpu(cy, params)
     s.ctx.lib.cyl_set_tuning(s.ctx.h, 3, 0, 0, 1)
     s._run(s._steps_left())
@@ -443,3 +446,21 @@ def test_stranded_positions_output(scheme, tmp_path):
     assert got.shape == ref.shape
     assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.)) <= TOL
     assert ps.stranded_count == 0  # reset by the flush
+
+
+@pytest.mark.parametrize("model", [1, 2])
+def test_graph_replay_crosses_field_records(model):
+    """Replayed steps whose times cross several field records (61 frames,
+    one every 167 s, over 600 s of steps)."""
+    case = vortex_case(npart=2000, ntimes=61)
+    p = dict(model=1, diffusion_model=2) if model == 1 else dict(
+        model=2, time_scheme=5, diffusion_model=2, diffusion_lsm2_option=1, particle_velocity_init=1)
+    params = dict(final_time=600., time_step=5., **p)
+    s = case.build_gpu(cy, params)
+    s.ctx.lib.cyl_set_tuning(s.ctx.h, 4, 0, 0, 1)
+    s._run(s._steps_left())
+    assert s.ctx.last_graph_steps() >= 100
+    ps = s.particle_set
+    g = SimpleNamespace(position=ps.position.copy(), velocity=ps.velocity.copy(), tri=ps.tri.copy(),
+                        inactive=ps.inactive.copy(), lowerlayer=ps.lowerlayer.copy(), tag=ps.tag.copy())
+    assert_parity(g, case.run_oracle(R, params))
