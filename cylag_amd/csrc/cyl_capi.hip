@@ -174,6 +174,7 @@ struct cyl_ctx {
   double* d_cand_val = nullptr;                    // value per candidate
   uint64_t* d_ev_state = nullptr;                  // generator state per candidate
   int64_t *d_ev_begin = nullptr, *d_nev = nullptr, *d_ev_bsum = nullptr, *d_xscratch = nullptr;
+  int64_t* d_chunk_live = nullptr;                 // end of the step's live chunks
   int* d_bucket = nullptr;          // bucket index of the event keys, all scenarios
   int64_t* d_bucket_base = nullptr;
   int64_t nbucket = 0;
@@ -1830,21 +1831,29 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(dalloc(b, &ctx->d_xscratch, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_begin, ns + 1));
   HIPCHK(dalloc(b, &ctx->d_nev, 1));
+  HIPCHK(dalloc(b, &ctx->d_chunk_live, 1));
   return CYL_OK;
+}
+
+// blocks of a scan pass: one per tile, at most 8 per CU
+static unsigned scan_grid(const cyl_ctx* ctx, int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(n, SCAN_TILE), 8LL * ctx->ncu));
 }
 
 // exclusive scan of n values (int or int64) into out (int64, n entries) with total
 template <class T>
 static void enqueue_scan(cyl_ctx* ctx, const T* in, int64_t* out, int64_t n, int64_t* bsum, int64_t* total,
                          const int64_t* ndev, const int* cond, int& launches) {
-  unsigned nb = nblk(n, SCAN_TILE);
+  // tiles are visited grid-stride, only the live ones when a device count bounds the input
+  unsigned nb = scan_grid(ctx, n);
   k_scan_tiles<T, OpSum><<<nb, BLK, 0, ctx->stream>>>(in, out, bsum, n, ndev, cond);
-  k_scan_top<OpSum><<<1, 1024, 0, ctx->stream>>>(bsum, nb, total, cond);
-  k_scan_fix<<<nb, BLK, 0, ctx->stream>>>(out, bsum, n, cond);
+  k_scan_top<OpSum><<<1, 1024, 0, ctx->stream>>>(bsum, n, ndev, total, cond);
+  k_scan_fix<<<nb, BLK, 0, ctx->stream>>>(out, bsum, n, ndev, cond);
   launches += 3;
 }
 
-// active rank of every row within its scenario (pool order)
+// active rank of every row within its scenario (pool order); the scenario
+// totals follow in k_step_prep
 static void enqueue_rank(cyl_ctx* ctx, const int* cond, int& launches) {
   int n = ctx->n;
   const PoolDev& A = ctx->pool[ctx->cur];
@@ -1854,10 +1863,7 @@ static void enqueue_rank(cyl_ctx* ctx, const int* cond, int& launches) {
                launches);
   k_gather_rank<<<gg, BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_gpool, ctx->d_scen, ctx->d_scen_base, n,
                                                   ctx->d_rank, nullptr, cond);
-  k_scen_totals<<<nblk(ctx->nscen), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_total, ctx->d_scen_base, ctx->nscen,
-                                                           n, ctx->d_scen_active, cond);
-  k_clear_flag<<<1, 1, 0, ctx->stream>>>(ctx->d_flag, 1);
-  launches += 4;
+  launches += 2;
 }
 
 // event tables of the step's stream window (rng_method 2 and 3)
@@ -1878,6 +1884,10 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   e.ev_begin = ctx->d_ev_begin;
   e.ev_pos = ctx->d_ev_pos;
   e.ev_state = ctx->d_ev_state;
+  e.ev_len = ctx->d_ev_len;
+  e.ev_val = ctx->d_cand_val;
+  e.ev_key = ctx->d_xscratch;
+  e.chunk_live = ctx->d_chunk_live;
   e.ev_cap = ctx->ev_cap;
   e.overflow = ctx->d_flag + 2;
   e.cond = cond;
@@ -1890,33 +1900,32 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   if (method == 2) k_ev_scan<2, false><<<pgrid((const void*)k_ev_scan<2, false>), BLK, 0, s>>>(e);
   else k_ev_scan<3, false><<<pgrid((const void*)k_ev_scan<3, false>), BLK, 0, s>>>(e);
   enqueue_scan(ctx, ctx->d_chunk_count, ctx->d_chunk_off, ctx->nchunks, ctx->d_ev_bsum, ctx->d_nev,
-               (const int64_t*)nullptr, cond, launches);
+               (const int64_t*)ctx->d_chunk_live, cond, launches);
   if (method == 2) k_ev_scan<2, true><<<pgrid((const void*)k_ev_scan<2, true>), BLK, 0, s>>>(e);
   else k_ev_scan<3, true><<<pgrid((const void*)k_ev_scan<3, true>), BLK, 0, s>>>(e);
   launches += 2;
   int64_t cap = ctx->ev_cap;
-  unsigned nb = std::max(nblk(cap), nblk(ctx->nscen));
+  // the candidate kernels loop over the step's candidates (device count), far
+  // fewer than the capacity: 4 blocks per CU
+  unsigned nb = std::max(std::min<unsigned>(nblk(cap), 4u * (unsigned)ctx->ncu), nblk(ctx->nscen));
   if (method == 2) {
-    k_ev_sample<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->d_zt, ctx->d_ev_begin, ctx->nscen, ctx->d_nev, ctx->d_ev_pos,
-                                   ctx->d_ev_state, ctx->d_ev_len, ctx->d_cand_val, ctx->d_xscratch, cond);
-    unsigned nt = nblk(cap, SCAN_TILE);
-    k_scan_tiles<int64_t, OpMax><<<nt, BLK, 0, s>>>(ctx->d_xscratch, ctx->d_reach, ctx->d_bmax, cap, ctx->d_nev,
-                                                    cond);
-    k_scan_top<OpMax><<<1, 1024, 0, s>>>(ctx->d_bmax, nt, nullptr, cond);
+    k_scan_tiles<int64_t, OpMax><<<scan_grid(ctx, cap), BLK, 0, s>>>(ctx->d_xscratch, ctx->d_reach, ctx->d_bmax,
+                                                                     cap, ctx->d_nev, cond);
+    k_scan_top<OpMax><<<1, 1024, 0, s>>>(ctx->d_bmax, cap, ctx->d_nev, nullptr, cond);
     k_ev_resolve<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->nscen, ctx->d_nev,
                                     ctx->d_reach, ctx->d_bmax, ctx->d_E, cond);
     enqueue_scan(ctx, ctx->d_E, ctx->d_C, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, ctx->d_nev, cond, launches);
     k_ev_compact<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos,
                                     ctx->d_ev_len, ctx->d_cand_val, ctx->d_E, ctx->d_C, ctx->d_key, ctx->d_xtra,
                                     ctx->d_ev_val, ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);
-    launches += 5;
+    launches += 4;
   } else {
     k_ev_polar<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos, ctx->d_key,
                                   ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);
     launches += 1;
   }
-  k_ev_buckets<<<nblk(ctx->nbucket), BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_bucket_base, ctx->nbucket,
-                                                  ctx->d_bucket, cond);
+  const unsigned nbb = std::min<unsigned>(nblk(ctx->nbucket), 8u * (unsigned)ctx->ncu);
+  k_ev_buckets<<<nbb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_bucket_base, ctx->nbucket, ctx->d_bucket, cond);
   launches += 1;
 }
 
@@ -2474,16 +2483,30 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     // active rank when rows were deleted or added since the last scan
     enqueue_rank(ctx, ctx->d_flag + 1, launches);
     bool uniform = can_uniform && !ctx->pending_init;
+    {
+      // scenario totals (after a rank scan), the scan request cleared, units
+      // of the step and generator chunks: one block
+      PrepArgs pa{};
+      pa.scan = ctx->d_scan;
+      pa.total = ctx->d_total;
+      pa.scen_base = ctx->d_scen_base;
+      pa.nscen = ns;
+      pa.npool = ctx->n;
+      pa.rank_flag = ctx->d_flag + 1;
+      pa.nact = ctx->d_scen_active;
+      pa.st = ctx->d_stream;
+      pa.scen_end = a.scen_end;
+      pa.batch_step = a.batch_step;
+      pa.clock = a.clock;
+      pa.scen_units = ctx->d_scen_units;
+      pa.uniform = uniform ? 1 : 0;
+      pa.pregen = ctx->d_pregen;
+      pa.chunk_base = uniform && any_fast && ctx->inline_normals == 0 ? ctx->d_gen_base : nullptr;
+      k_step_prep<<<1, BLK, 0, ctx->stream>>>(pa);
+      ++launches;
+    }
     if (step == 0 && !cap) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (uniform) {
-      k_uniform_units<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, ns, a.scen_end,
-                                                         a.batch_step, ctx->d_scen_units, a.clock);
-      ++launches;
-      if (any_fast && ctx->inline_normals == 0) {
-        k_gen_chunks<<<1, BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_active, a.scen_end, ctx->d_pregen,
-                                               a.batch_step, ns, ctx->d_gen_base, a.clock);
-        ++launches;
-      }
       enqueue_events(ctx, method, one, launches);
       a.general = 0;
       a.guarded = 0;

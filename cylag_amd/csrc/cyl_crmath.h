@@ -106,6 +106,17 @@ __device__ inline double cr_exp(double x) {
   return ldexp(e.hi, k);
 }
 
+// l < exp(a) decided as with the correctly rounded exp: the device library's
+// exp is within one ulp of e^a, so a comparison outside a band of 2^-50
+// relative around it (four ulps and more) is already decided; inside the band
+// the correctly rounded value decides.  a >= -708 (normal results).
+__device__ __forceinline__ bool lt_exp(double l, double a) {
+  double e = exp(a);
+  if (l < e * (1. - 0x1p-50)) return true;
+  if (l > e * (1. + 0x1p-50)) return false;
+  return l < cr_exp(a);
+}
+
 // log(x) in double-double, x positive, normal and finite
 __device__ __noinline__ DD dd_log(double x) {
   double t = x - 1.;  // exact for x in [1/2, 2]

@@ -16,8 +16,8 @@
 // Draws.events_before / zig_position / polar_position then map a unit to its
 // position, and the normal generator replays runs of the stream from them.
 //
-// Pipeline (rng_method 2): count -> scan -> write -> sample -> reach max-scan
-// -> resolve -> (extra, count) scan -> compact -> buckets.
+// Pipeline (rng_method 2): count -> scan -> write and sample -> reach
+// max-scan -> resolve -> (extra, count) scan -> compact -> buckets.
 #pragma once
 #include "cyl_kernels.h"
 
@@ -42,6 +42,10 @@ struct EventArgs {
   int64_t* ev_begin;          // [nscen + 1] first candidate of each scenario
   int64_t* ev_pos;            // candidate position (relative to the scenario's window start)
   uint64_t* ev_state;         // generator state at the candidate (zig)
+  int64_t* ev_len;            // length of the normal starting at the candidate (zig)
+  double* ev_val;             // its value (zig)
+  int64_t* ev_key;            // its reach key (zig)
+  int64_t* chunk_live;        // end of the live chunks (set by the count pass)
   int64_t ev_cap;
   int* overflow;
   const int* cond;            // run only when *cond != 0
@@ -71,14 +75,41 @@ __device__ __forceinline__ bool zig_fast(uint32_t u, const uint32_t* kn) {
   return ahz < kn[iz];
 }
 
-// Scan of the stream window, one chunk of ZCHUNK positions per thread (grid
-// stride).  WRITE = false: count candidates per chunk (and set each
-// scenario's window); WRITE = true: write them at the scanned offsets.
+// Scan of the stream window, one chunk of ZCHUNK positions per thread, blocks
+// striding over spans of BLK chunks.  WRITE = false: count candidates per
+// chunk, set each scenario's window, and (block 0) the end of the live chunks
+// (no chunk at or past it holds a position of any window).  WRITE = true:
+// write the candidates of the live chunks at the scanned offsets; under
+// rng_method 2 the block then samples the candidates of its span (one
+// contiguous range of the table, one candidate per thread): the value and
+// length of the Ziggurat normal that starts there and its reach key
+// (scenario << REACH_SHIFT) | (pos + len - 1).
+// A candidate whose normal reaches the base strip's tail loop (two correctly
+// rounded logarithms per attempt; a few percent of candidates) is queued in
+// LDS and finished by the block at the end, so that the waves sampling the
+// other candidates do not wait on it.
+constexpr int TAIL_CAP = 256;
+struct TailRec {
+  uint64_t state;
+  int64_t i;
+  int32_t hz, ncalls;
+};
+
+__device__ __forceinline__ void ev_finish(const EventArgs& e, int64_t i, double v, int64_t ncalls) {
+  e.ev_val[i] = v;
+  e.ev_len[i] = ncalls;
+  e.ev_key[i] = e.ev_key[i] | (e.ev_pos[i] + ncalls - 1);
+}
+
 template <int METHOD, bool WRITE>
 __global__ void __launch_bounds__(BLK) k_ev_scan(EventArgs e) {
   if (e.cond && *e.cond == 0) return;
   __shared__ JumpTable8 t8;
   __shared__ uint32_t kn[128];
+  __shared__ int64_t lds[BLK / 64];
+  __shared__ TailRec tails[WRITE && METHOD == 2 ? TAIL_CAP : 1];
+  __shared__ int ntail;
+  if (threadIdx.x == 0) ntail = 0;
   const int s0 = scen_of(e.chunk_base, e.nscen, (int64_t)blockIdx.x * BLK);
   const uint64_t inc0 = e.stream[s0].inc;
   // the byte-window table of the block's first scenario; it depends on the
@@ -92,84 +123,125 @@ __global__ void __launch_bounds__(BLK) k_ev_scan(EventArgs e) {
       for (int q = threadIdx.x; q < 128; q += BLK) kn[q] = e.zt->kn[q];
   }
   __syncthreads();
-  if (!WRITE && blockIdx.x == 0)
-    for (int q = threadIdx.x; q < e.nscen; q += BLK) e.stream[q].window = window_for(METHOD, e.scen_units[q]);
+  if (!WRITE && blockIdx.x == 0) {
+    int64_t live = 0;
+    for (int q = threadIdx.x; q < e.nscen; q += BLK) {
+      int64_t W = window_for(METHOD, e.scen_units[q]);
+      e.stream[q].window = W;
+      if (W > 0) {
+        int64_t end = e.chunk_base[q] + (W + ZCHUNK - 1) / ZCHUNK;
+        live = end > live ? end : live;
+      }
+    }
+    int64_t tot;
+    block_excl_scan<OpMax, BLK>(live, tot, lds);
+    // a window past its scenario's chunk capacity is reported (overflow) and
+    // never scanned past the table
+    if (threadIdx.x == 0) *e.chunk_live = tot < 0 ? 0 : (tot < e.nchunks ? tot : e.nchunks);
+  }
+  // the count pass visits every chunk; the write pass only the live ones
+  // (the chunk scan is exact up to index chunk_live, which holds the total)
+  const int64_t nch = WRITE ? *e.chunk_live : e.nchunks;
   if (WRITE && blockIdx.x == 0)
     for (int q = threadIdx.x; q <= e.nscen; q += BLK) {
-      int64_t v = q < e.nscen ? e.chunk_off[e.chunk_base[q]] : *e.nev;
+      int64_t v = q < e.nscen && e.chunk_base[q] < nch ? e.chunk_off[e.chunk_base[q]] : *e.nev;
       e.ev_begin[q] = v < e.ev_cap ? v : e.ev_cap;  // an overflow is reported, never indexed past
     }
-  for (int64_t ch = (int64_t)blockIdx.x * BLK + threadIdx.x; ch < e.nchunks; ch += (int64_t)gridDim.x * BLK) {
-    int s = scen_of(e.chunk_base, e.nscen, ch);
-    int64_t local = ch - e.chunk_base[s];
-    int64_t W = window_for(METHOD, e.scen_units[s]);
-    int64_t cap_chunks = (s + 1 < e.nscen ? e.chunk_base[s + 1] : e.nchunks) - e.chunk_base[s];
-    int64_t p0 = local * ZCHUNK;
-    int cnt = 0;
-    int64_t w = WRITE ? e.chunk_off[ch] : 0;
-    if (p0 < W) {
-      if (!WRITE && (local + 1) * ZCHUNK < W && local + 1 == cap_chunks) atomicOr(e.overflow, 1);
-      const StreamDev& st = e.stream[s];
-      Pcg g;
-      g.inc = st.inc;
-      if (METHOD == 2) {
-        g.state = st.inc == inc0 ? pcg_jump8(st.base, (uint64_t)p0, t8, e.jt[s]) : pcg_jump(st.base, (uint64_t)p0, e.jt[s]);
-        for (int q = 0; q < ZCHUNK; ++q) {
-          uint64_t at = g.state;
-          if (zig_fast(pcg_next(g), kn)) continue;
-          if (WRITE) {
-            if (w < e.ev_cap) {
-              e.ev_pos[w] = p0 + q;
-              e.ev_state[w] = at;
+  for (int64_t span = (int64_t)blockIdx.x * BLK; span < nch; span += (int64_t)gridDim.x * BLK) {
+    const int64_t ch = span + threadIdx.x;
+    if (ch < nch) {
+      int s = scen_of(e.chunk_base, e.nscen, ch);
+      int64_t local = ch - e.chunk_base[s];
+      int64_t W = window_for(METHOD, e.scen_units[s]);
+      int64_t cap_chunks = (s + 1 < e.nscen ? e.chunk_base[s + 1] : e.nchunks) - e.chunk_base[s];
+      int64_t p0 = local * ZCHUNK;
+      int cnt = 0;
+      int64_t w = WRITE ? e.chunk_off[ch] : 0;
+      const int64_t skey = (int64_t)s << REACH_SHIFT;
+      if (p0 < W) {
+        if (!WRITE && (local + 1) * ZCHUNK < W && local + 1 == cap_chunks) atomicOr(e.overflow, 1);
+        const StreamDev& st = e.stream[s];
+        Pcg g;
+        g.inc = st.inc;
+        if (METHOD == 2) {
+          g.state = st.inc == inc0 ? pcg_jump8(st.base, (uint64_t)p0, t8, e.jt[s]) : pcg_jump(st.base, (uint64_t)p0, e.jt[s]);
+          for (int q = 0; q < ZCHUNK; ++q) {
+            uint64_t at = g.state;
+            if (zig_fast(pcg_next(g), kn)) continue;
+            if (WRITE) {
+              if (w < e.ev_cap) {
+                e.ev_pos[w] = p0 + q;
+                e.ev_state[w] = at;
+                e.ev_key[w] = skey;
+              } else {
+                atomicOr(e.overflow, 2);
+              }
+              ++w;
             } else {
-              atomicOr(e.overflow, 2);
+              ++cnt;
             }
-            ++w;
-          } else {
-            ++cnt;
           }
-        }
-      } else {
-        // polar: attempt a uses positions 2a, 2a+1
-        int64_t a0 = p0 / 2;
-        uint64_t d = (uint64_t)(2 * a0);
-        g.state = st.inc == inc0 ? pcg_jump8(st.base, d, t8, e.jt[s]) : pcg_jump(st.base, d, e.jt[s]);
-        for (int q = 0; q < ZCHUNK / 2; ++q) {
-          uint32_t u1 = pcg_next(g), u2 = pcg_next(g);
-          if (polar_accept(u1, u2)) continue;
-          if (WRITE) {
-            if (w < e.ev_cap) e.ev_pos[w] = a0 + q;
-            else atomicOr(e.overflow, 2);
-            ++w;
-          } else {
-            ++cnt;
+        } else {
+          // polar: attempt a uses positions 2a, 2a+1
+          int64_t a0 = p0 / 2;
+          uint64_t d = (uint64_t)(2 * a0);
+          g.state = st.inc == inc0 ? pcg_jump8(st.base, d, t8, e.jt[s]) : pcg_jump(st.base, d, e.jt[s]);
+          for (int q = 0; q < ZCHUNK / 2; ++q) {
+            uint32_t u1 = pcg_next(g), u2 = pcg_next(g);
+            if (polar_accept(u1, u2)) continue;
+            if (WRITE) {
+              if (w < e.ev_cap) e.ev_pos[w] = a0 + q;
+              else atomicOr(e.overflow, 2);
+              ++w;
+            } else {
+              ++cnt;
+            }
           }
         }
       }
+      if (!WRITE) e.chunk_count[ch] = cnt;
     }
-    if (!WRITE) e.chunk_count[ch] = cnt;
+    if (WRITE && METHOD == 2) {
+      // the span's candidates, written above by this block, in one range
+      __syncthreads();
+      int64_t c0 = e.chunk_off[span];
+      int64_t c1 = span + BLK < nch ? e.chunk_off[span + BLK] : *e.nev;
+      c0 = c0 < e.ev_cap ? c0 : e.ev_cap;
+      c1 = c1 < e.ev_cap ? c1 : e.ev_cap;
+      for (int64_t i = c0 + threadIdx.x; i < c1; i += BLK) {
+        Draws d;
+        d.zt = e.zt;
+        d.pre_pos = nullptr;
+        d.g.inc = e.stream[e.ev_key[i] >> REACH_SHIFT].inc;
+        d.g.state = e.ev_state[i];
+        d.ncalls = 0;
+        double v;
+        int32_t hz;
+        if (d.zig_head(v, hz)) {
+          ev_finish(e, i, v, d.ncalls);
+        } else {
+          int k = atomicAdd(&ntail, 1);
+          if (k < TAIL_CAP) tails[k] = TailRec{d.g.state, i, hz, (int32_t)d.ncalls};
+          else ev_finish(e, i, d.zig_tail(hz), d.ncalls);  // queue full: finish here
+        }
+      }
+    }
   }
-}
-
-// Value and length of the Ziggurat normal that starts at each candidate, and
-// its reach key (scenario << REACH_SHIFT) | (pos + len - 1).
-__global__ void __launch_bounds__(BLK) k_ev_sample(const StreamDev* st, const ZigTables* zt, const int64_t* ev_begin,
-                                                   int nscen, const int64_t* nev, const int64_t* pos,
-                                                   const uint64_t* state, int64_t* len, double* val, int64_t* key,
-                                                   const int* cond) {
-  if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= *nev) return;
-  int s = scen_of(ev_begin, nscen, i);
-  Draws d;
-  d.zt = zt;
-  d.pre_pos = nullptr;
-  d.g.inc = st[s].inc;
-  d.g.state = state[i];
-  d.ncalls = 0;
-  val[i] = d.zig();
-  len[i] = d.ncalls;
-  key[i] = ((int64_t)s << REACH_SHIFT) | (pos[i] + d.ncalls - 1);
+  if (WRITE && METHOD == 2) {
+    __syncthreads();
+    const int nt = ntail < TAIL_CAP ? ntail : TAIL_CAP;
+    for (int k = threadIdx.x; k < nt; k += BLK) {
+      const TailRec r = tails[k];
+      Draws d;
+      d.zt = e.zt;
+      d.pre_pos = nullptr;
+      d.g.inc = e.stream[e.ev_key[r.i] >> REACH_SHIFT].inc;
+      d.g.state = r.state;
+      d.ncalls = r.ncalls;
+      double v = d.zig_tail(r.hz);
+      ev_finish(e, r.i, v, d.ncalls);
+    }
+  }
 }
 
 // Ziggurat chain resolution.  Candidate p is off the chain iff an on-chain
@@ -182,28 +254,29 @@ __global__ void __launch_bounds__(BLK) k_ev_resolve(const int64_t* pos, const in
                                                     int nscen, const int64_t* nev, const int64_t* reach,
                                                     const int64_t* tmax, int64_t* pk, const int* cond) {
   if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= *nev) return;
-  int s = scen_of(ev_begin, nscen, i);
-  int64_t b = ev_begin[s];
-  auto reach_of = [&](int64_t j) -> int64_t {
-    int64_t m = reach[j], t = tmax[j / SCAN_TILE];
-    m = t > m ? t : m;
-    return (m >= 0 && (m >> REACH_SHIFT) == s) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
-  };
-  int64_t j0 = i;
-  while (j0 > b && reach_of(j0) >= pos[j0]) --j0;
-  int64_t cur_end = pos[j0] + len[j0] - 1;
-  int on = 1;
-  for (int64_t l = j0 + 1; l <= i; ++l) {
-    if (pos[l] > cur_end) {
-      cur_end = pos[l] + len[l] - 1;
-      on = 1;
-    } else {
-      on = 0;
+  const int64_t ne = ev_begin[nscen];  // candidates, clamped to the capacity
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < ne; i += (int64_t)gridDim.x * BLK) {
+    int s = scen_of(ev_begin, nscen, i);
+    int64_t b = ev_begin[s];
+    auto reach_of = [&](int64_t j) -> int64_t {
+      int64_t m = reach[j], t = tmax[j / SCAN_TILE];
+      m = t > m ? t : m;
+      return (m >= 0 && (m >> REACH_SHIFT) == s) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
+    };
+    int64_t j0 = i;
+    while (j0 > b && reach_of(j0) >= pos[j0]) --j0;
+    int64_t cur_end = pos[j0] + len[j0] - 1;
+    int on = 1;
+    for (int64_t l = j0 + 1; l <= i; ++l) {
+      if (pos[l] > cur_end) {
+        cur_end = pos[l] + len[l] - 1;
+        on = 1;
+      } else {
+        on = 0;
+      }
     }
+    pk[i] = on ? ((len[i] - 1) << 32) | 1 : 0;
   }
-  pk[i] = on ? ((len[i] - 1) << 32) | 1 : 0;
 }
 
 // Point each scenario's stream at its slice of the event tables (threads
@@ -215,25 +288,28 @@ __global__ void __launch_bounds__(BLK) k_ev_compact(StreamDev* st, int nscen, co
                                                     int64_t* key, int64_t* xtra, double* cval, const int* bucket,
                                                     const int64_t* bucket_base, int bshift, const int* cond) {
   if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i < nscen) {
-    int64_t c0 = ec[ev_begin[i]] & EC_MASK, c1 = ec[ev_begin[i + 1]] & EC_MASK;
-    st[i].ev_key = key + c0;
-    st[i].ev_extra = xtra + c0;
-    st[i].ev_val = cval + c0;
-    st[i].n_ev = c1 - c0;
-    st[i].ev_bucket = bucket + bucket_base[i];
-    st[i].n_bucket = bucket_base[i + 1] - bucket_base[i];
-    st[i].ev_bshift = bshift;
+  const int64_t ne = ev_begin[nscen];  // candidates, clamped to the capacity
+  const int64_t end = ne > nscen ? ne : nscen;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < end; i += (int64_t)gridDim.x * BLK) {
+    if (i < nscen) {
+      int64_t c0 = ec[ev_begin[i]] & EC_MASK, c1 = ec[ev_begin[i + 1]] & EC_MASK;
+      st[i].ev_key = key + c0;
+      st[i].ev_extra = xtra + c0;
+      st[i].ev_val = cval + c0;
+      st[i].n_ev = c1 - c0;
+      st[i].ev_bucket = bucket + bucket_base[i];
+      st[i].n_bucket = bucket_base[i + 1] - bucket_base[i];
+      st[i].ev_bshift = bshift;
+    }
+    if (i >= ne || !(pk[i] & 1)) continue;
+    int s = scen_of(ev_begin, nscen, i);
+    int64_t eb = ec[ev_begin[s]] >> 32;
+    int64_t e_excl = (ec[i] >> 32) - eb;
+    int64_t c = ec[i] & EC_MASK;
+    key[c] = pos[i] - e_excl;
+    xtra[c] = e_excl + len[i] - 1;
+    cval[c] = val[i];
   }
-  if (i >= *nev || !(pk[i] & 1)) return;
-  int s = scen_of(ev_begin, nscen, i);
-  int64_t eb = ec[ev_begin[s]] >> 32;
-  int64_t e_excl = (ec[i] >> 32) - eb;
-  int64_t c = ec[i] & EC_MASK;
-  key[c] = pos[i] - e_excl;
-  xtra[c] = e_excl + len[i] - 1;
-  cval[c] = val[i];
 }
 
 // polar: key_j = r_j - j (j counted within the scenario), and the stream slices
@@ -242,37 +318,42 @@ __global__ void __launch_bounds__(BLK) k_ev_polar(StreamDev* st, int nscen, cons
                                                   const int* bucket, const int64_t* bucket_base, int bshift,
                                                   const int* cond) {
   if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i < nscen) {
-    st[i].ev_key = key + ev_begin[i];
-    st[i].ev_extra = nullptr;
-    st[i].ev_val = nullptr;
-    st[i].n_ev = ev_begin[i + 1] - ev_begin[i];
-    st[i].ev_bucket = bucket + bucket_base[i];
-    st[i].n_bucket = bucket_base[i + 1] - bucket_base[i];
-    st[i].ev_bshift = bshift;
+  const int64_t ne = ev_begin[nscen];  // candidates, clamped to the capacity
+  const int64_t end = ne > nscen ? ne : nscen;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < end; i += (int64_t)gridDim.x * BLK) {
+    if (i < nscen) {
+      st[i].ev_key = key + ev_begin[i];
+      st[i].ev_extra = nullptr;
+      st[i].ev_val = nullptr;
+      st[i].n_ev = ev_begin[i + 1] - ev_begin[i];
+      st[i].ev_bucket = bucket + bucket_base[i];
+      st[i].n_bucket = bucket_base[i + 1] - bucket_base[i];
+      st[i].ev_bshift = bshift;
+    }
+    if (i >= ne) continue;
+    int s = scen_of(ev_begin, nscen, i);
+    key[i] = pos[i] - (i - ev_begin[s]);
   }
-  if (i >= *nev) return;
-  int s = scen_of(ev_begin, nscen, i);
-  key[i] = pos[i] - (i - ev_begin[s]);
 }
 
-// bucket index of every scenario's event keys: entry b = lower_bound(b << shift)
+// bucket index of every scenario's event keys: entry b = lower_bound(b << shift);
+// buckets past the last key (the step's units end there) hold n_ev directly
 __global__ void __launch_bounds__(BLK) k_ev_buckets(const StreamDev* st, int nscen, const int64_t* bucket_base,
                                                     int64_t ntot, int* bucket, const int* cond) {
   if (cond && *cond == 0) return;
-  int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= ntot) return;
-  int s = scen_of(bucket_base, nscen, i);
-  int64_t b = i - bucket_base[s], u = b << st[s].ev_bshift;
-  const int64_t* k = st[s].ev_key;
-  int64_t lo = 0, hi = st[s].n_ev;
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (k[mid] < u) lo = mid + 1;
-    else hi = mid;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < ntot; i += (int64_t)gridDim.x * BLK) {
+    int s = scen_of(bucket_base, nscen, i);
+    int64_t b = i - bucket_base[s], u = b << st[s].ev_bshift;
+    const int64_t* k = st[s].ev_key;
+    int64_t lo = 0, hi = st[s].n_ev;
+    if (hi > 0 && k[hi - 1] < u) lo = hi;
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (k[mid] < u) lo = mid + 1;
+      else hi = mid;
+    }
+    bucket[i] = (int)lo;
   }
-  bucket[i] = (int)lo;
 }
 
 }  // namespace cyl

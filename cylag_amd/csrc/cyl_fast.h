@@ -378,24 +378,57 @@ __device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int nscen
   return xi_tile_index(chunk_base[s] + r / GEN_PER, r % GEN_PER);
 }
 
-// chunk_base[s] = first generator chunk of scenario s: ceil(active / GEN_PER)
-// chunks for scenarios of the fused kernel that draw this step, none otherwise
-__global__ void __launch_bounds__(BLK) k_gen_chunks(const StreamDev* st, const int64_t* nact, const int* scen_end,
-                                                    const int* pregen, int step_id, int nscen, int64_t* chunk_base,
-                                                    const int* clock) {
-  // one block: exclusive scan of the scenarios' chunk counts, BLK at a time
+// Step preparation in one block, in stream order of the four small passes it
+// replaces: each scenario's active total after a rank scan (only when rows
+// were deleted or added: *rank_flag != 0), the clearing of that flag, the
+// step's units per scenario under uniform assignment (units per row x active
+// rows, none once the scenario has ended), and the first generator chunk of
+// each scenario (exclusive scan of ceil(active / GEN_PER) over the scenarios
+// whose rows the generator serves).
+struct PrepArgs {
+  const int64_t* scan;       // pool-order scan of the active flags
+  const int64_t* total;      // its total
+  const int* scen_base;      // [nscen + 1] first pool index of each scenario
+  int nscen, npool;
+  int* rank_flag;            // rows deleted or added since the last scan
+  int64_t* nact;             // [nscen] active rows
+  const StreamDev* st;
+  const int* scen_end;       // step at which each scenario ends, or null
+  int batch_step;
+  const int* clock;          // device step clock of a captured period, or null
+  int64_t* scen_units;       // [nscen] (uniform only)
+  int uniform;
+  const int* pregen;         // [nscen] generator serves the scenario, or null
+  int64_t* chunk_base;       // [nscen] first generator chunk, or null (no generator)
+};
+
+__global__ void __launch_bounds__(BLK) k_step_prep(PrepArgs a) {
   __shared__ int64_t part[BLK];
   __shared__ int64_t carry;
-  if (clock) step_id = clock[1];
+  const bool rescan = *a.rank_flag != 0;
+  const int step = a.clock ? a.clock[1] : a.batch_step;
   if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < nscen; base += BLK) {
+  __syncthreads();  // every thread has read the flag
+  if (threadIdx.x == 0) *a.rank_flag = 0;
+  for (int base = 0; base < a.nscen; base += BLK) {
     const int s = base + threadIdx.x;
     int64_t v = 0;
-    if (s < nscen) {
-      bool ended = scen_end && step_id >= scen_end[s];
-      if ((!pregen || pregen[s]) && !ended && st[s].units_per_part > 0) v = (nact[s] + GEN_PER - 1) / GEN_PER;
+    if (s < a.nscen) {
+      int64_t act;
+      if (rescan) {
+        int e = a.scen_base[s + 1];
+        int64_t end = e >= a.npool ? *a.total : a.scan[e];
+        act = end - a.scan[a.scen_base[s]];
+        a.nact[s] = act;
+      } else {
+        act = a.nact[s];
+      }
+      const bool ended = a.scen_end && step >= a.scen_end[s];
+      if (a.uniform) a.scen_units[s] = ended ? 0 : (int64_t)a.st[s].units_per_part * act;
+      if (a.chunk_base && (!a.pregen || a.pregen[s]) && !ended && a.st[s].units_per_part > 0)
+        v = (act + GEN_PER - 1) / GEN_PER;
     }
+    if (!a.chunk_base) continue;  // uniform across the block
     part[threadIdx.x] = v;
     __syncthreads();
     for (int o = 1; o < BLK; o <<= 1) {
@@ -404,12 +437,12 @@ __global__ void __launch_bounds__(BLK) k_gen_chunks(const StreamDev* st, const i
       part[threadIdx.x] += t;
       __syncthreads();
     }
-    if (s < nscen) chunk_base[s] = carry + part[threadIdx.x] - v;
+    if (s < a.nscen) a.chunk_base[s] = carry + part[threadIdx.x] - v;
     __syncthreads();
     if (threadIdx.x == BLK - 1) carry += part[BLK - 1];
     __syncthreads();
   }
-  if (threadIdx.x == 0) chunk_base[nscen] = carry;
+  if (a.chunk_base && threadIdx.x == 0) a.chunk_base[a.nscen] = carry;
 }
 
 // Normals of ranks [r0, r1) of scenario s, drawn sequentially from the stream

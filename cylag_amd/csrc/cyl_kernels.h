@@ -328,6 +328,18 @@ __device__ __forceinline__ int64_t block_excl_scan(int64_t v, int64_t& total, in
   return op(wpre, excl);
 }
 
+// tiles that hold live entries: every tile when no device count is given,
+// else those covering indices 0..min(*ndev, n) (index *ndev receives the total)
+__device__ __forceinline__ int64_t scan_live_tiles(int64_t n, const int64_t* ndev) {
+  int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (!ndev) return nt;
+  int64_t lim = *ndev < n ? *ndev : n;
+  int64_t t = lim / SCAN_TILE + 1;
+  return t < nt ? t : nt;
+}
+
+// grid-stride over the live tiles; entries at or past the device count read
+// as the identity, so out[*ndev] is the total
 template <class T, class Op>
 __global__ void __launch_bounds__(BLK) k_scan_tiles(const T* in, int64_t* out, int64_t* tile_sum, int64_t n,
                                                     const int64_t* ndev, const int* cond) {
@@ -335,42 +347,48 @@ __global__ void __launch_bounds__(BLK) k_scan_tiles(const T* in, int64_t* out, i
   __shared__ int64_t lds[BLK / 64];
   __shared__ int64_t buf[SCAN_TILE];
   Op op;
-  int64_t lim = ndev ? (*ndev < n ? *ndev : n) : n;
-  int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
-  // coalesced load of the tile, then each thread takes SCAN_ITEMS consecutive values
+  const int64_t lim = ndev ? (*ndev < n ? *ndev : n) : n;
+  const int64_t ntl = scan_live_tiles(n, ndev);
+  for (int64_t t = blockIdx.x; t < ntl; t += gridDim.x) {
+    const int64_t base = t * SCAN_TILE;
+    // coalesced load of the tile, then each thread takes SCAN_ITEMS consecutive values
 #pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = base + k * BLK + threadIdx.x;
-    buf[k * BLK + threadIdx.x] = i < lim ? (int64_t)in[i] : Op::ident;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      int64_t i = base + k * BLK + threadIdx.x;
+      buf[k * BLK + threadIdx.x] = i < lim ? (int64_t)in[i] : Op::ident;
+    }
+    __syncthreads();
+    int64_t v[SCAN_ITEMS];
+    int64_t run = Op::ident;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      int64_t x = buf[threadIdx.x * SCAN_ITEMS + k];
+      v[k] = run;  // exclusive within the thread
+      run = op(run, x);
+    }
+    int64_t total;
+    int64_t pre = block_excl_scan<Op, BLK>(run, total, lds);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) buf[threadIdx.x * SCAN_ITEMS + k] = op(pre, v[k]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      int64_t i = base + k * BLK + threadIdx.x;
+      if (i < n) out[i] = buf[k * BLK + threadIdx.x];
+    }
+    if (threadIdx.x == 0) tile_sum[t] = total;
+    __syncthreads();  // buf and lds are refilled by the next tile
   }
-  __syncthreads();
-  int64_t v[SCAN_ITEMS];
-  int64_t run = Op::ident;
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t x = buf[threadIdx.x * SCAN_ITEMS + k];
-    v[k] = run;  // exclusive within the thread
-    run = op(run, x);
-  }
-  int64_t total;
-  int64_t pre = block_excl_scan<Op, BLK>(run, total, lds);
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) buf[threadIdx.x * SCAN_ITEMS + k] = op(pre, v[k]);
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = base + k * BLK + threadIdx.x;
-    if (i < n) out[i] = buf[k * BLK + threadIdx.x];
-  }
-  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
 }
 
-// exclusive scan of the tile totals by one block of 1024 threads
+// exclusive scan of the live tile totals by one block of 1024 threads
 template <class Op>
-__global__ void __launch_bounds__(1024) k_scan_top(int64_t* ts, int64_t nt, int64_t* total, const int* cond) {
+__global__ void __launch_bounds__(1024) k_scan_top(int64_t* ts, int64_t n, const int64_t* ndev, int64_t* total,
+                                                   const int* cond) {
   if (cond && *cond == 0) return;
   __shared__ int64_t lds[16];
   Op op;
+  const int64_t nt = scan_live_tiles(n, ndev);
   int64_t per = (nt + 1023) / 1024;
   int64_t i0 = (int64_t)threadIdx.x * per;
   int64_t run = Op::ident;
@@ -388,14 +406,18 @@ __global__ void __launch_bounds__(1024) k_scan_top(int64_t* ts, int64_t nt, int6
   if (threadIdx.x == 0 && total) *total = tot;
 }
 
-__global__ void __launch_bounds__(BLK) k_scan_fix(int64_t* out, const int64_t* ts, int64_t n, const int* cond) {
+__global__ void __launch_bounds__(BLK) k_scan_fix(int64_t* out, const int64_t* ts, int64_t n, const int64_t* ndev,
+                                                  const int* cond) {
   if (cond && *cond == 0) return;
-  int64_t add = ts[blockIdx.x];
-  int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  const int64_t ntl = scan_live_tiles(n, ndev);
+  for (int64_t t = blockIdx.x; t < ntl; t += gridDim.x) {
+    const int64_t add = ts[t];
+    const int64_t base = t * SCAN_TILE;
 #pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; ++k) {
-    int64_t i = base + k * BLK + threadIdx.x;
-    if (i < n) out[i] += add;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+      int64_t i = base + k * BLK + threadIdx.x;
+      if (i < n) out[i] += add;
+    }
   }
 }
 
@@ -593,17 +615,6 @@ __global__ void k_rerun_reset(unsigned long long* stream_end, unsigned long long
   if (i == 0) *n_deleted = 0;
 }
 
-// units needed this step in uniform mode: units_per_part * active rows of the scenario
-// (scenarios that have run all their steps draw nothing)
-__global__ void k_uniform_units(const StreamDev* st, const int64_t* scen_active, int nscen, const int* scen_end,
-                                int batch_step, int64_t* scen_units, const int* clock) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nscen) return;
-  if (clock) batch_step = clock[1];
-  bool ended = scen_end && batch_step >= scen_end[s];
-  scen_units[s] = ended ? 0 : (int64_t)st[s].units_per_part * scen_active[s];
-}
-
 // a general-mode rerun starts from the step's input: undo its deletion marks
 __global__ void __launch_bounds__(BLK) k_undelete(int* inactive, const int* del_step, int n, int step_id,
                                                   const int* cond, const int* clock) {
@@ -645,10 +656,6 @@ __global__ void __launch_bounds__(BLK) k_control_sections(const double* ax, cons
 __global__ void __launch_bounds__(BLK) k_fill_int(int* p, int n, int v) {
   int j = blockIdx.x * BLK + threadIdx.x;
   if (j < n) p[j] = v;
-}
-
-__global__ void k_clear_flag(int* flag, int idx) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) flag[idx] = 0;
 }
 
 __global__ void k_set_flag(int* flag, int idx, int v) {

@@ -116,20 +116,40 @@ struct Draws {
 
   __device__ double zig() {
     // random_gaussian_ziggurat, random_utils.pyx:154-194
+    double v;
+    int32_t hz;
+    if (zig_head(v, hz)) return v;
+    return zig_tail(hz);
+  }
+
+  // The Ziggurat loop up to a base-strip draw: true with the normal in v, or
+  // false with the base-strip draw hz, whose tail loop zig_tail finishes
+  __device__ bool zig_head(double& v, int32_t& hz_tail) {
     for (;;) {
       int32_t hz = (int32_t)next();
       uint32_t iz = (uint32_t)hz & 127u;
       uint32_t ahz = hz < 0 ? (uint32_t)(-(int64_t)hz) : (uint32_t)hz;
-      if (ahz < zt->kn[iz]) return hz * zt->wn[iz];
+      if (ahz < zt->kn[iz]) {
+        v = hz * zt->wn[iz];
+        return true;
+      }
       if (iz == 0) {
-        for (;;) {
-          double x = -cr_log(u32_to_uniform(next())) * 0.2904764;
-          double y = -cr_log(u32_to_uniform(next()));
-          if (y + y >= x * x) return hz > 0 ? ZIG_R + x : -ZIG_R - x;
-        }
+        hz_tail = hz;
+        return false;
       }
       double x = hz * zt->wn[iz];
-      if (zt->fn[iz] + u32_to_uniform(next()) * (zt->fn[iz - 1] - zt->fn[iz]) < cr_exp(-0.5 * x * x)) return x;
+      if (lt_exp(zt->fn[iz] + u32_to_uniform(next()) * (zt->fn[iz - 1] - zt->fn[iz]), -0.5 * x * x)) {
+        v = x;
+        return true;
+      }
+    }
+  }
+
+  __device__ double zig_tail(int32_t hz) {
+    for (;;) {
+      double x = -cr_log(u32_to_uniform(next())) * 0.2904764;
+      double y = -cr_log(u32_to_uniform(next()));
+      if (y + y >= x * x) return hz > 0 ? ZIG_R + x : -ZIG_R - x;
     }
   }
 
