@@ -455,6 +455,12 @@ __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, 
                                                       int& err, Store&& store) {
   const int64_t NONE = 0x7fffffffffffffffLL;
   const StreamDev& st = a.stream[s];
+  // the stream record in registers: the stores below may not alias it, but
+  // the compiler cannot tell
+  const int64_t window = st.window, n_ev = st.n_ev;
+  const int64_t* ev_key = st.ev_key;
+  const int64_t* ev_extra = st.ev_extra;
+  const double* ev_val = st.ev_val;
   int64_t u = r0 * st.units_per_part;
   Draws d;
   d.st = &st;
@@ -463,12 +469,12 @@ __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, 
     pos = u;
   } else if (RNG == 2) {
     ev = d.events_before(u);
-    pos = u + (ev > 0 ? st.ev_extra[ev - 1] : 0);
-    next_key = ev < st.n_ev ? st.ev_key[ev] : NONE;
+    pos = u + (ev > 0 ? ev_extra[ev - 1] : 0);
+    next_key = ev < n_ev ? ev_key[ev] : NONE;
   } else {
     pos = 2 * (u + d.events_before(u));
   }
-  if (RNG != 1 && pos >= st.window) err = E_RNG_WINDOW;
+  if (RNG != 1 && pos >= window) err = E_RNG_WINDOW;
   Pcg g;
   g.inc = st.inc;
   g.state = st.inc == t8_inc ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
@@ -481,14 +487,18 @@ __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, 
       pos += 2;
     } else if (RNG == 2) {
       for (int k = 0; k < 2; ++k, ++u) {
-        if (pos >= st.window) err = E_RNG_WINDOW;
+        if (pos >= window) err = E_RNG_WINDOW;
         if (u == next_key) {
-          int64_t len = st.ev_extra[ev] - (ev > 0 ? st.ev_extra[ev - 1] : 0) + 1;
-          v[k] = st.ev_val[ev];
+          int64_t len = ev_extra[ev] - (ev > 0 ? ev_extra[ev - 1] : 0) + 1;
+          v[k] = ev_val[ev];
           for (int64_t q = 0; q < len; ++q) pcg_next(g);
           pos += len;
           ++ev;
-          next_key = ev < st.n_ev ? st.ev_key[ev] : NONE;
+          next_key = ev < n_ev ? ev_key[ev] : NONE;
+          // wait for these loads here, in the rare branch: left pending, they
+          // put a wait for every outstanding memory operation, the previous
+          // pairs' stores included, at the head of each iteration
+          __builtin_amdgcn_s_waitcnt(0);
         } else {
           int32_t hz = (int32_t)pcg_next(g);
           uint32_t iz = (uint32_t)hz & 127u;
