@@ -632,10 +632,29 @@ __global__ void __launch_bounds__(BLK) k_gen_slots(FastArgs a) {
 // assignment); otherwise they are sampled here from the prepared start.
 template <int MODEL, int RNG, bool PREGEN>
 __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, int j, const ZigTables& zt) {
-  if (a.inactive[j] != 0) return 0;
+  // The slot's loads, then the element record, node ids and normals they
+  // index, are issued before any branch, so that their latencies overlap
+  // instead of adding up.  The indices are clamped into their tables; rows
+  // that do not use a value (inactive, no element, no draws) ignore it.
+  const int ina = a.inactive[j];
   const int s = a.scen ? a.scen[j] : 0;
-  int tri0 = a.triA[j];
-  double x0 = a.xA[j], y0 = a.yA[j];
+  const int tri0 = a.triA[j];
+  const double x0 = a.xA[j], y0 = a.yA[j];
+  int64_t xidx = 0;
+  if (PREGEN) xidx = a.xi_by_slot ? (int64_t)j : xi_index(a.chunk_base, a.nscen, s, a.rank[j]);
+  const int te = (unsigned)tri0 < (unsigned)a.fm.nt ? tri0 : 0;
+  const ElemF& ef = a.fm.ef[te];
+  const int4 en = a.fm.en[te];
+  // the P1 part of the record, read here with the node ids
+  double det[3], gx[3], gy[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    det[k] = ef.det[k];
+    gx[k] = ef.gx[k];
+    gy[k] = ef.gy[k];
+  }
+  const double2 xi_raw = PREGEN ? a.xi[(uint64_t)xidx < (uint64_t)a.xi_cap ? xidx : 0] : make_double2(0., 0.);
+  if (ina != 0) return 0;
   if (a.scen_end && batch_step_of(a) >= a.scen_end[s]) {
     // the scenario has run all its steps: its rows carry over unchanged
     a.xB[j] = x0;
@@ -657,18 +676,13 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
     a.triB[j] = tri0;
     return 0;
   }
-  // the pair of normals of this slot's active rank (load issued early)
+  // the pair of normals of this slot's active rank
   double2 xi = make_double2(0., 0.);
-  if (PREGEN && q.diffusion_model != 0 && st.units_per_part > 0) {
-    int64_t xidx = a.xi_by_slot ? (int64_t)j : xi_index(a.chunk_base, a.nscen, s, a.rank[j]);
-    if (xidx < a.xi_cap) xi = a.xi[xidx];
-  }
-  const ElemF& ef = a.fm.ef[tri0];
-  int4 en = a.fm.en[tri0];
+  if (PREGEN && q.diffusion_model != 0 && st.units_per_part > 0 && xidx < a.xi_cap) xi = xi_raw;
   // P1 factors at the start position (eulerian_field_set.pyx:1103-1105)
-  double f0 = ef.det[0] + x0 * ef.gx[0] + y0 * ef.gy[0];
-  double f1 = ef.det[1] + x0 * ef.gx[1] + y0 * ef.gy[1];
-  double f2 = ef.det[2] + x0 * ef.gx[2] + y0 * ef.gy[2];
+  double f0 = det[0] + x0 * gx[0] + y0 * gy[0];
+  double f1 = det[1] + x0 * gx[1] + y0 * gy[1];
+  double f2 = det[2] + x0 * gx[2] + y0 * gy[2];
   NodeF n0 = a.nodes[en.x], n1 = a.nodes[en.y], n2 = a.nodes[en.z];
   double ux = n0.u * f0 + n1.u * f1 + n2.u * f2;
   double uy = n0.v * f0 + n1.v * f1 + n2.v * f2;
@@ -716,8 +730,8 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
     double diffx, driftx = 0., drifty = 0.;
     if (q.diffusion_model >= 2) {
       double K = n0.K * f0 + n1.K * f1 + n2.K * f2;
-      double dkdx = n0.K * ef.gx[0] + n1.K * ef.gx[1] + n2.K * ef.gx[2];
-      double dkdy = n0.K * ef.gy[0] + n1.K * ef.gy[1] + n2.K * ef.gy[2];
+      double dkdx = n0.K * gx[0] + n1.K * gx[1] + n2.K * gx[2];
+      double dkdy = n0.K * gy[0] + n1.K * gy[1] + n2.K * gy[2];
       diffx = sqrt(2. * fmax(0., K)) * a.sqrtdt;
       driftx = dkdx * a.dt;
       drifty = dkdy * a.dt;
