@@ -2978,7 +2978,7 @@ static int enqueue_sort(cyl_ctx* ctx, bool need_us, int& launches) {
   pi(ctx->pool[0].inactive, ctx->sp_inactive);
   pi(ctx->d_rank, ctx->sp_rank);  // a slot's active rank moves with it
   if (ctx->d_scen) pi(ctx->d_scen, ctx->sp_scen);
-  k_permute_all<<<nblk(n), BLK, 0, s>>>(pa);
+  k_permute_all<<<fast_grid(nblk(n)), BLK, 0, s>>>(pa);
   launches += 3;
   HIPCHK(hipGetLastError());
   ctx->cur = 1 - ctx->cur;

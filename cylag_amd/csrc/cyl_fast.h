@@ -849,19 +849,10 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   return dr.live ? (unsigned long long)(dr.pos0 + dr.ncalls) : 0ull;
 }
 
-// One slot per thread.  Blocks are dispatched round-robin over the 8 XCDs, so
-// block b takes chunk (b % 8) * ceil(chunks / 8) + b / 8 of BLK slots: each
-// XCD sweeps one contiguous eighth of the slots, and neighbouring slots
-// (neighbouring elements once sorted) share its L2.  The grid is 8 *
-// ceil(chunks / 8) blocks (fast_grid).  Without the grid-stride loop of a
-// persistent launch, the kernel's uniform values need not stay live across
-// slots, which spares scalar registers.  The Ziggurat tables are staged in LDS
-// per block for the inline sampler.
-__device__ __forceinline__ int64_t xcd_chunk(int n) {
-  int64_t nch = ((int64_t)n + BLK - 1) / BLK, cpx = (nch + 7) / 8;
-  int64_t chunk = (blockIdx.x % 8) * cpx + blockIdx.x / 8;
-  return chunk < nch ? chunk : -1;
-}
+// One slot per thread, chunks mapped to XCDs by xcd_chunk (grid: fast_grid).
+// Without the grid-stride loop of a persistent launch, the kernel's uniform
+// values need not stay live across slots, which spares scalar registers.  The
+// Ziggurat tables are staged in LDS per block for the inline sampler.
 
 template <int MODEL, int RNG, bool PREGEN>
 __device__ __forceinline__ void fast_step_body(const FastArgs& a) {

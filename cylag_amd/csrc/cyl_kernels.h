@@ -445,6 +445,17 @@ __global__ void k_scen_totals(const int64_t* scan, const int64_t* total, const i
   scen_units[s] = end - scan[scen_base[s]];
 }
 
+// One chunk of BLK slots per block, chunks mapped to the XCDs.  Blocks are
+// dispatched round-robin over the 8 XCDs, so block b takes chunk (b % 8) *
+// ceil(chunks / 8) + b / 8: each XCD sweeps one contiguous eighth of the
+// slots, and neighbouring slots (neighbouring elements once sorted) share its
+// L2.  The grid is 8 * ceil(chunks / 8) blocks; returns -1 past the last chunk.
+__device__ __forceinline__ int64_t xcd_chunk(int n) {
+  int64_t nch = ((int64_t)n + BLK - 1) / BLK, cpx = (nch + 7) / 8;
+  int64_t chunk = (blockIdx.x % 8) * cpx + blockIdx.x / 8;
+  return chunk < nch ? chunk : -1;
+}
+
 // Element re-sort: counting sort of the slots by a key that groups 2^SORT_SHIFT
 // elements neighbouring along a Hilbert curve through the element centroids
 // (active rows), then rows without element, then inactive rows.  The input is
@@ -673,9 +684,16 @@ struct PermArgs {
   int* idst[PERM_MAX];
 };
 
+// Destinations in XCD-mapped chunks: the slots of one contiguous eighth of
+// the new order come, up to the local reshuffle since the last sort, from one
+// eighth of the old order, so the source lines an XCD gathers stay in its own
+// L2 instead of being fetched by all eight.
 __global__ void __launch_bounds__(BLK) k_permute_all(PermArgs p) {
-  int j = blockIdx.x * BLK + threadIdx.x;
-  if (j >= p.n) return;
+  const int64_t chunk = xcd_chunk(p.n);
+  if (chunk < 0) return;
+  const int64_t jj = chunk * BLK + threadIdx.x;
+  if (jj >= p.n) return;
+  const int j = (int)jj;
   int s = p.perm[j];
 #pragma unroll
   for (int q = 0; q < PERM_MAX; ++q)

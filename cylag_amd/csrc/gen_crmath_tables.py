@@ -45,6 +45,11 @@ def main():
     out.append("__constant__ DD CR_EXPT[65] = {")
     out.append(",\n".join("  " + fmt(split((Decimal(j) / 64).exp())) for j in range(-32, 33)))
     out.append("};")
+    # e^(j/256), j = -96..96 (the fast paths: |r| <= ln2/2 gives |j| <= 89)
+    out.append("constexpr int CR_NT256 = 96;")
+    out.append("__constant__ DD CR_EXPT256[2 * CR_NT256 + 1] = {")
+    out.append(",\n".join("  " + fmt(split((Decimal(j) / 256).exp())) for j in range(-96, 97)))
+    out.append("};")
     # 2/(2n+1) for log1p(t) = 2 atanh(u), u = t/(2+t), |u| < 1/64 (n = 0..11)
     out.append("constexpr int CR_NATANH = 12;")
     out.append("__constant__ DD CR_ATANH[CR_NATANH] = {")
