@@ -104,7 +104,7 @@ struct cyl_ctx {
   int cur = 0;
   Bufs part_bufs;
   int *d_gpool = nullptr, *d_rank = nullptr, *d_init_tri = nullptr, *d_del_step = nullptr;
-  double2* d_xi = nullptr;            // the step's normals per active rank (fused 2D step)
+  double2* d_xi = nullptr;            // the step's normals per active rank (pairs in 2D, XiRec3 in 3D)
   int64_t xi_cap = 0;
   int64_t* d_off = nullptr;
   double *d_diam = nullptr, *d_vol = nullptr, *d_depth = nullptr;
@@ -817,9 +817,10 @@ static int alloc_slot_bufs(cyl_ctx* ctx, int n, int nscen_pad) {
   HIPCHK(dalloc(bb, &ctx->d_gpool, n));
   HIPCHK(dalloc(bb, &ctx->d_rank, n));
   // normals in tiles of 64 chunks x 32: chunks <= n/32 + 1 per scenario,
-  // rounded up to a whole tile
-  HIPCHK(dalloc(bb, &ctx->d_xi, (size_t)n + 32 * (size_t)nscen_pad + 64 * 32));
+  // rounded up to a whole tile; an entry is a pair (16 B) in 2D, a 32 B
+  // record of three values in 3D
   ctx->xi_cap = (int64_t)n + 32 * (int64_t)nscen_pad + 64 * 32;
+  HIPCHK(dalloc(bb, &ctx->d_xi, (size_t)ctx->xi_cap * (dim == 3 ? 2 : 1)));
   HIPCHK(dalloc(bb, &ctx->d_off, n));
   HIPCHK(dalloc(bb, &ctx->d_init_tri, n));
   HIPCHK(dalloc(bb, &ctx->d_del_step, n));
@@ -1157,7 +1158,7 @@ extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const
   if (need_xi > ctx->xi_cap) {
     HIPCHK(ctx->part_bufs.release(ctx->d_xi));
     ctx->d_xi = nullptr;
-    HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)need_xi));
+    HIPCHK(dalloc(ctx->part_bufs, &ctx->d_xi, (size_t)need_xi * (ctx->dim == 3 ? 2 : 1)));  // 32 B entries in 3D
     ctx->xi_cap = need_xi;
   }
   k_scen_of_slot<<<nblk(ctx->n), BLK, 0, ctx->stream>>>(ctx->d_gpool, ctx->d_scen_base, nscen, ctx->n, ctx->d_scen);
@@ -2145,9 +2146,20 @@ static void launch_gen(cyl_ctx* ctx, const FastArgs& f0, int m) {
   }
   const FastArgs& f = f0;
   int64_t nb = nblk(((int64_t)ctx->n + GEN_PER - 1) / GEN_PER + ctx->nscen);
-  if (m == 1) k_gen_normals<1><<<persistent_grid(ctx, (const void*)k_gen_normals<1>, nb, false), BLK, 0, ctx->stream>>>(f);
-  else if (m == 2) k_gen_normals<2><<<persistent_grid(ctx, (const void*)k_gen_normals<2>, nb, false), BLK, 0, ctx->stream>>>(f);
-  else k_gen_normals<3><<<persistent_grid(ctx, (const void*)k_gen_normals<3>, nb, false), BLK, 0, ctx->stream>>>(f);
+  hipStream_t s = ctx->stream;
+#define CYL_GEN_LAUNCH(R, NV) \
+  k_gen_normals<R, NV><<<persistent_grid(ctx, (const void*)k_gen_normals<R, NV>, nb, false), BLK, 0, s>>>(f)
+  // three values per rank for the 3D kernel (pair and vertical single), a pair in 2D
+  if (ctx->dim == 3) {
+    if (m == 1) CYL_GEN_LAUNCH(1, 3);
+    else if (m == 2) CYL_GEN_LAUNCH(2, 3);
+    else CYL_GEN_LAUNCH(3, 3);
+  } else {
+    if (m == 1) CYL_GEN_LAUNCH(1, 2);
+    else if (m == 2) CYL_GEN_LAUNCH(2, 2);
+    else CYL_GEN_LAUNCH(3, 2);
+  }
+#undef CYL_GEN_LAUNCH
 }
 
 // stream position and generator state of every slot's first draw (the inline
@@ -2185,16 +2197,23 @@ static void launch_fast_model(cyl_ctx* ctx, const FastArgs& f, int m, bool gener
 #undef CYL_FAST_LAUNCH
 }
 
-// the fused 3D kernel with the inline sampler (its slots' stream positions
-// prepared by launch_draw_start, here unless start_done)
+// the fused 3D kernel: pre-generated normals (uniform assignment), or the
+// inline sampler (its slots' stream positions prepared by launch_draw_start,
+// here unless start_done)
 template <int MODEL>
-static void launch_fast3(cyl_ctx* ctx, const Fast3Args& f3, int m, bool start_done) {
+static void launch_fast3(cyl_ctx* ctx, const Fast3Args& f3, int m, bool inl, bool start_done) {
   int64_t nb = nblk(f3.f.n);
   hipStream_t s = ctx->stream;
+  if (!inl) {
+    if (m == 1) k_fast_3d<MODEL, 1, true><<<fast_grid(nb), BLK, 0, s>>>(f3);
+    else if (m == 2) k_fast_3d<MODEL, 2, true><<<fast_grid(nb), BLK, 0, s>>>(f3);
+    else k_fast_3d<MODEL, 3, true><<<fast_grid(nb), BLK, 0, s>>>(f3);
+    return;
+  }
   if (!start_done) launch_draw_start(ctx, f3.f, m);
-  if (m == 1) k_fast_3d<MODEL, 1><<<fast_grid(nb), BLK, 0, s>>>(f3);
-  else if (m == 2) k_fast_3d<MODEL, 2><<<fast_grid(nb), BLK, 0, s>>>(f3);
-  else k_fast_3d<MODEL, 3><<<fast_grid(nb), BLK, 0, s>>>(f3);
+  if (m == 1) k_fast_3d<MODEL, 1, false><<<fast_grid(nb), BLK, 0, s>>>(f3);
+  else if (m == 2) k_fast_3d<MODEL, 2, false><<<fast_grid(nb), BLK, 0, s>>>(f3);
+  else k_fast_3d<MODEL, 3, false><<<fast_grid(nb), BLK, 0, s>>>(f3);
 }
 
 static Fast3Args make_fast3_args(cyl_ctx* ctx, const FastArgs& f, const StepArgs& a) {
@@ -2216,20 +2235,23 @@ static Fast3Args make_fast3_args(cyl_ctx* ctx, const FastArgs& f, const StepArgs
 
 // whether the fused kernels of this context sample their normals inline
 // (from stream positions prepared by launch_draw_start) under the uniform
-// assignment: always in 3D, in 2D when the generator is switched off
-static bool samples_inline(const cyl_ctx* ctx) { return ctx->dim == 3 || ctx->inline_normals == 1; }
+// assignment: when the generator is switched off (the slot-order generator
+// serves the 2D kernels only)
+static bool samples_inline(const cyl_ctx* ctx) {
+  return ctx->inline_normals == 1 || (ctx->dim == 3 && ctx->inline_normals == 2);
+}
 
 // start_done: the uniform assignment's draw starts were prepared before the
 // timed kernel window (general-mode reruns prepare their own)
 static void launch_fast_main(cyl_ctx* ctx, const FastArgs& f, int model, int m, bool general,
                              const StepArgs* a = nullptr, bool start_done = false) {
+  const bool inl = general || samples_inline(ctx);
   if (ctx->dim == 3) {
     Fast3Args f3 = make_fast3_args(ctx, f, *a);
-    if (model == 1) launch_fast3<1>(ctx, f3, m, start_done);
-    else launch_fast3<2>(ctx, f3, m, start_done);
+    if (model == 1) launch_fast3<1>(ctx, f3, m, inl, start_done);
+    else launch_fast3<2>(ctx, f3, m, inl, start_done);
     return;
   }
-  bool inl = general || ctx->inline_normals == 1;
   if (model == 1) launch_fast_model<1>(ctx, f, m, inl, start_done);
   else launch_fast_model<2>(ctx, f, m, inl, start_done);
 }
@@ -2395,11 +2417,11 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     }
     {
       std::vector<int> pregen(ns, 0);
-      // the normal generator serves the 2D fused kernels (the 3D one samples inline)
-      bool gen2d = ctx->dim == 2 && ctx->inline_normals != 1;
-      if (!ctx->batched) pregen[0] = cls[0].fast && gen2d ? 1 : 0;
+      // the normal generator serves the fused kernels unless they sample inline
+      const bool gen = !samples_inline(ctx);
+      if (!ctx->batched) pregen[0] = cls[0].fast && gen ? 1 : 0;
       else
-        for (int s = 0; s < ns; ++s) pregen[s] = ctx->classes_fast_of_scen(s) && any_fast && gen2d ? 1 : 0;
+        for (int s = 0; s < ns; ++s) pregen[s] = ctx->classes_fast_of_scen(s) && any_fast && gen ? 1 : 0;
       if (pregen != ctx->pregen_h) {  // changes only when arrivals switch the fused kernel off or on
         if (cap) return ctx->set_err(CYL_E_STATE, -1, "generator flags changed inside a captured step");
         HIPCHK(hipStreamSynchronize(ctx->stream));

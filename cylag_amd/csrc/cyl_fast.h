@@ -445,11 +445,21 @@ __global__ void __launch_bounds__(BLK) k_step_prep(PrepArgs a) {
   if (a.chunk_base && threadIdx.x == 0) a.chunk_base[a.nscen] = carry;
 }
 
+// The three normals of a rank in 3D (the diffusion pair, then the vertical
+// single), one 32 B record per rank in the tiled layout of the pairs.
+struct __align__(32) XiRec3 {
+  double a, b, c, pad;
+};
+
 // Normals of ranks [r0, r1) of scenario s, drawn sequentially from the stream
-// position of rank r0's first unit: store(r, v0, v1) receives each rank's
-// pair.  Returns the stream position after the last draw; sets *err on a
-// window overflow or a slow Ziggurat start missing from the event table.
-template <int RNG, class Store>
+// position of rank r0's first unit, in the order the fused kernels call them:
+// NV = 2, the diffusion pair (generate_random_pair, random_utils.pyx:202-233);
+// NV = 3, the pair then the vertical single (generate_random_single,
+// random_utils.pyx:235-262; under rng_method 3 a single is one polar call of
+// which the first value is kept).  store(r, v) receives each rank's values.
+// Returns the stream position after the last draw; sets err on a window
+// overflow or a slow Ziggurat start missing from the event table.
+template <int RNG, int NV, class Store>
 __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, int64_t r0, int64_t r1,
                                                       const JumpTable8& t8, uint64_t t8_inc, const ZigTables& zt,
                                                       int& err, Store&& store) {
@@ -480,13 +490,13 @@ __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, 
   g.state = st.inc == t8_inc ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
                              : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
   for (int64_t r = r0; r < r1; ++r) {
-    double v[2];
+    double v[NV];
     if (RNG == 1) {
-      v[0] = rect_normal(g);
-      v[1] = rect_normal(g);
-      pos += 2;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[k] = rect_normal(g);
+      pos += NV;
     } else if (RNG == 2) {
-      for (int k = 0; k < 2; ++k, ++u) {
+      for (int k = 0; k < NV; ++k, ++u) {
         if (pos >= window) err = E_RNG_WINDOW;
         if (u == next_key) {
           int64_t len = ev_extra[ev] - (ev > 0 ? ev_extra[ev - 1] : 0) + 1;
@@ -509,42 +519,56 @@ __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, 
         }
       }
     } else {
-      double x1 = 0., x2 = 0., w = 2.0;
-      while (w >= 1.0 || w == 0.0) {
-        x1 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
-        x2 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
-        w = x1 * x1 + x2 * x2;
-        pos += 2;
+      for (int k = 0; k < NV; k += 2) {
+        double x1 = 0., x2 = 0., w = 2.0;
+        while (w >= 1.0 || w == 0.0) {
+          x1 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
+          x2 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
+          w = x1 * x1 + x2 * x2;
+          pos += 2;
+        }
+        w = sqrt((-2.0 * cr_log(w)) / w);
+        v[k] = x1 * w;
+        if (k + 1 < NV) v[k + 1] = x2 * w;
       }
-      w = sqrt((-2.0 * cr_log(w)) / w);
-      v[0] = x1 * w;
-      v[1] = x2 * w;
     }
-    store(r, v[0], v[1]);
+    store(r, v);
   }
   return (unsigned long long)pos;
 }
 
-// Normals of generator chunk c (scenario s): the pairs of its ranks, in the
-// serial stream order from the chunk's first position.  Returns the stream
-// position after its last draw (0 if the scenario draws nothing this step).
-template <int RNG>
+// Normals of generator chunk c (scenario s): the NV values of each of its
+// ranks, in the serial stream order from the chunk's first position.  Returns
+// the stream position after its last draw (0 if the scenario draws nothing).
+template <int RNG, int NV>
 __device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64_t c, int s, const JumpTable8& t8,
                                                         uint64_t t8_inc, const ZigTables& zt) {
   const StreamDev& st = a.stream[s];
   if (st.units_per_part == 0) return 0;
   int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
   int err = 0;
-  unsigned long long pend = gen_run<RNG>(a, s, r0, r1, t8, t8_inc, zt, err, [&](int64_t r, double v0, double v1) {
+  unsigned long long pend = gen_run<RNG, NV>(a, s, r0, r1, t8, t8_inc, zt, err, [&](int64_t r, const double* v) {
     int64_t xidx = xi_tile_index(c, r - r0);
-    if (xidx < a.xi_cap) a.xi[xidx] = make_double2(v0, v1);
+    if (xidx >= a.xi_cap) return;
+    if (NV == 2) {
+      a.xi[xidx] = make_double2(v[0], v[1]);
+    } else {
+      XiRec3 q;
+      q.a = v[0];
+      q.b = v[1];
+      q.c = v[NV - 1];
+      q.pad = 0.;
+      reinterpret_cast<XiRec3*>(a.xi)[xidx] = q;
+    }
   });
   // reported against pool index 0: the rank's slot is not known here
   if (err) atomicMin(&a.err[s], (unsigned long long)(unsigned)err);
   return pend;
 }
 
-template <int RNG>
+// NV values per rank: 2 for the 2D fused kernels (pairs in a.xi), 3 for the
+// 3D one (XiRec3 records in the same buffer, which then holds 32 B per entry)
+template <int RNG, int NV>
 __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   __shared__ JumpTable8 t8;
   __shared__ ZigTables zt;
@@ -581,7 +605,7 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
     unsigned long long pend = 0;
     if (c < nchunk) {
       s = (c < a.chunk_base[sb + 1]) ? sb : scen_of(a.chunk_base, a.nscen, c);
-      if (!(a.scen_end && batch_step_of(a) >= a.scen_end[s])) pend = gen_chunk<RNG>(a, c, s, t8, t8_inc, zt);
+      if (!(a.scen_end && batch_step_of(a) >= a.scen_end[s])) pend = gen_chunk<RNG, NV>(a, c, s, t8, t8_inc, zt);
     }
     (void)pend;  // the streams' ends come from the units drawn (k_finish)
   }
@@ -614,10 +638,10 @@ __global__ void __launch_bounds__(BLK) k_gen_slots(FastArgs a) {
     if (a.scen_end && batch_step_of(a) >= a.scen_end[s]) return;
     int err = 0;
     int64_t r = a.rank[j];
-    unsigned long long pend = gen_run<RNG>(a, s, r, r + 1, t8, t8_inc, zt, err,
-                                           [&](int64_t, double v0, double v1) {
-                                             if (j < a.xi_cap) a.xi[j] = make_double2(v0, v1);
-                                           });
+    unsigned long long pend = gen_run<RNG, 2>(a, s, r, r + 1, t8, t8_inc, zt, err,
+                                              [&](int64_t, const double* v) {
+                                                if (j < a.xi_cap) a.xi[j] = make_double2(v[0], v[1]);
+                                              });
     (void)pend;  // the streams' ends come from the units drawn (k_finish)
     if (err) atomicMin(&a.err[s], ((unsigned long long)(unsigned)a.pool[j] << 8) | (unsigned)err);
   });
@@ -697,12 +721,11 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
     double u0x = a.upA0[j], u0y = a.upA1[j];
     double dx = ux - u0x, dy = uy - u0y;
     double a1 = relax_rate_dp(a.pc, a.diameter[j], sqrt(dx * dx + dy * dy));
-    double beta, D1, D2, fxc;
-    if (!lsm2_drift_coefficients(a1, a.dt, beta, D1, D2, fxc)) {
+    double beta, D1, D2, fxc, um_beta;
+    if (!lsm2_drift_coefficients(a1, a.dt, beta, D1, D2, fxc, um_beta)) {
       perr = E_LSM2_COEF;
-      beta = D1 = D2 = fxc = 0.;
+      beta = D1 = D2 = fxc = um_beta = 0.;
     }
-    double um_beta = -cr_expm1(-a1 * a.dt);
     double Fx = a.pc.a2c * 0., Fy = a.pc.a2c * 0.;  // no added mass: zero fluid acceleration
     upx = u0x * beta + ux * um_beta + Fx * D1;
     upy = u0y * beta + uy * um_beta + Fy * D1;

@@ -89,11 +89,19 @@ __device__ __forceinline__ void column3(const Fast3Args& a, int k, double x, dou
   zsp = top[en.x].z * f0 + top[en.y].z * f1 + top[en.z].z * f2;
 }
 
-template <int MODEL, int RNG>
+// PREGEN: the slot's three normals are the generator's record of its active
+// rank (uniform draw assignment); otherwise they are sampled here from the
+// prepared start (k_draw_start).
+template <int MODEL, int RNG, bool PREGEN>
 __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a3, int j, const ZigTables& zt) {
   const FastArgs& a = a3.f;
   if (a.inactive[j] != 0) return 0;
   const int s = a.scen ? a.scen[j] : 0;
+  // the record's load is issued here, before the branches, so that its latency
+  // overlaps the element and node loads
+  const int64_t xidx = PREGEN ? xi_index(a.chunk_base, a.nscen, s, a.rank[j]) : 0;
+  XiRec3 xr{0., 0., 0., 0.};
+  if (PREGEN) xr = reinterpret_cast<const XiRec3*>(a.xi)[(uint64_t)xidx < (uint64_t)a.xi_cap ? xidx : 0];
   int tri0 = a.triA[j], ll0 = a3.llA[j];
   double x0 = a.xA[j], y0 = a.yA[j], z0 = a3.zA[j];
   if (a.scen_end && batch_step_of(a) >= a.scen_end[s]) {
@@ -156,12 +164,11 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
     double v0x = a.upA0[j], v0y = a.upA1[j], v0z = a3.upA2[j];
     double dx = ux - v0x, dy = uy - v0y, dz = uz - v0z;
     double a1 = relax_rate_dp(a.pc, a.diameter[j], sqrt(dx * dx + dy * dy + dz * dz));
-    double beta, D1, D2, fxc;
-    if (!lsm2_drift_coefficients(a1, a.dt, beta, D1, D2, fxc)) {
+    double beta, D1, D2, fxc, um_beta;
+    if (!lsm2_drift_coefficients(a1, a.dt, beta, D1, D2, fxc, um_beta)) {
       perr = E_LSM2_COEF;
-      beta = D1 = D2 = fxc = 0.;
+      beta = D1 = D2 = fxc = um_beta = 0.;
     }
-    double um_beta = -cr_expm1(-a1 * a.dt);
     double Fx = a.pc.a2c * 0., Fy = a.pc.a2c * 0.;  // no added mass: zero fluid acceleration
     double Fz = a.pc.a2c * 0. - GRAV * a.pc.a0c;
     upx = v0x * beta + ux * um_beta + Fx * D1;
@@ -175,7 +182,7 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
   dr.st = &st;
   dr.jt = &a.jt[s];
   dr.jt8 = nullptr;
-  dr.pre_pos = a.pos0;
+  dr.pre_pos = PREGEN ? nullptr : a.pos0;
   dr.pre_state = a.state0;
   dr.slot = j;
   dr.zt = &zt;
@@ -222,8 +229,24 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
       diffz = q.diff_const_z;
     }
     double xix = 0., xiy = 0., xiz = 0.;
-    if (diffx != 0.) dr.pair(xix, xiy);
-    if (diffz != 0.) xiz = dr.single();
+    if (PREGEN) {
+      // the generator's values stand for the first draws of the rank; a row
+      // that skips the pair (zero amplitude) draws fewer units than predicted
+      // and the step is rerun in general mode
+      const bool have = st.units_per_part > 0 && xidx < a.xi_cap;
+      if (diffx != 0.) {
+        xix = have ? xr.a : 0.;
+        xiy = have ? xr.b : 0.;
+        dr.nunits += RNG == 3 ? 1 : 2;
+      }
+      if (diffz != 0.) {
+        xiz = have ? xr.c : 0.;
+        dr.nunits += 1;
+      }
+    } else {
+      if (diffx != 0.) dr.pair(xix, xiy);
+      if (diffz != 0.) xiz = dr.single();
+    }
     x += driftx + diffx * xix;
     y += drifty + diffx * xiy;
     z += driftz + diffz * xiz;
@@ -354,12 +377,12 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
 
 #undef CYL_P1
 
-template <int MODEL, int RNG>
-__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k_fast_3d(Fast3Args a3) {
+template <int MODEL, int RNG, bool PREGEN>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(PREGEN && MODEL == 1 ? 4 : 3))) k_fast_3d(Fast3Args a3) {
   const FastArgs& a = a3.f;
   if (a.guarded && a.flag[0] == 0) return;
   __shared__ ZigTables zt;
-  if (RNG == 2) {
+  if (RNG == 2 && !PREGEN) {
     for (int q = threadIdx.x; q < 128; q += BLK) {
       zt.kn[q] = a.zt->kn[q];
       zt.wn[q] = a.zt->wn[q];
@@ -370,7 +393,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(3))) k
   const int64_t chunk = xcd_chunk(a.n);
   if (chunk < 0) return;
   const int64_t jl = chunk * BLK + threadIdx.x;
-  if (jl < a.n) (void)fast3d_particle<MODEL, RNG>(a3, a.j0 + (int)jl, zt);
+  if (jl < a.n) (void)fast3d_particle<MODEL, RNG, PREGEN>(a3, a.j0 + (int)jl, zt);
 }
 
 }  // namespace cyl

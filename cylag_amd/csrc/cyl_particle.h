@@ -615,12 +615,15 @@ __device__ inline void keps_at(Ctx& c, const Part& p, double x, double y, double
 }
 
 // lsm2_direct_coefficients, update_state_lsm2.pyx:1240-1327: the drift part
-// (beta, D1, D2 and the force coefficient); false when a1 or dt is invalid
+// (beta, D1, D2 and the force coefficient), and um_beta = -expm1(-a1 dt), the
+// weight of the fluid velocity in di_lsm2 (update_state_lsm2.pyx:1434), which
+// is also D1's numerator when a1 dt >= 0.1; false when a1 or dt is invalid
 __device__ inline bool lsm2_drift_coefficients(double a1, double dt, double& beta, double& D1, double& D2,
-                                               double& fxc) {
+                                               double& fxc, double& um_beta) {
   double r = a1 * dt;
   if (!isfinite(a1) || !isfinite(dt) || a1 < 0. || dt < 0. || !isfinite(r)) return false;
   beta = cr_exp(-r);
+  um_beta = -cr_expm1(-r);
   if (r < 0.1) {
     double phi1 = 1., phi2 = 0.5, term = 1.;
     for (int n = 1; n < 13; ++n) {
@@ -632,7 +635,7 @@ __device__ inline bool lsm2_drift_coefficients(double a1, double dt, double& bet
     D2 = dt * r * phi2;
     fxc = dt * dt * phi2;
   } else {
-    D1 = -cr_expm1(-r) / a1;
+    D1 = um_beta / a1;
     D2 = dt - D1;
     fxc = D2 / a1;
   }
@@ -641,8 +644,8 @@ __device__ inline bool lsm2_drift_coefficients(double a1, double dt, double& bet
 
 // lsm2_direct_coefficients with the covariance terms of the noise
 __device__ inline bool lsm2_coefficients(double a1, double dt, double& beta, double& D1, double& D2, double& fxc,
-                                         double& quu, double& qxu, double& qxx) {
-  if (!lsm2_drift_coefficients(a1, dt, beta, D1, D2, fxc)) return false;
+                                         double& um_beta, double& quu, double& qxu, double& qxx) {
+  if (!lsm2_drift_coefficients(a1, dt, beta, D1, D2, fxc, um_beta)) return false;
   double r = a1 * dt;
   quu = (r == 0.) ? dt : dt * (-cr_expm1(-2. * r) / (2. * r));
   qxu = 0.5 * D1 * D1;
@@ -673,12 +676,11 @@ __device__ inline void di_lsm2(Ctx& c, Part& p, Draws& dr, int dm, double nux, d
   accel_at(c, p.tri, ll, x0[0], x0[1], z0, ux, uy, uz, ax, ay, az);
   CYL_RET_ERR();
   double a1 = relax_rate(c, p, rel_norm(d, ux, uy, uz, u0[0], u0[1], u0[2]));
-  double beta, D1, D2, fxc, quu, qxu, qxx;
-  if (!lsm2_coefficients(a1, dt, beta, D1, D2, fxc, quu, qxu, qxx)) {
+  double beta, D1, D2, fxc, um_beta, quu, qxu, qxx;
+  if (!lsm2_coefficients(a1, dt, beta, D1, D2, fxc, um_beta, quu, qxu, qxx)) {
     c.err = E_LSM2_COEF;
     return;
   }
-  double um_beta = -cr_expm1(-a1 * dt);
   double b2[3] = {0., 0., 0.};
   if (dm == 2 || dm == 3) {
     double tk, ep;
