@@ -260,7 +260,7 @@ def c3_leg(cy, steps=100, warmup=10, npart=5_000_000, sort_interval=8):
                 roofline={"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK, "traffic": traffic,
                           "traffic_source": "profiles/k_fast_3d_channel_settling_5M_counters.md",
-                          "traffic_kernel_sources": digest, "kernel": "k_fast_3d<2,2>", "kernel_avg_ms": avg_ms,
+                          "traffic_kernel_sources": digest, "kernel": "k_fast_3d<2,2,true>", "kernel_avg_ms": avg_ms,
                           "bytes_per_particle_step": b_alg, "draw_start_avg_ms": prep_ms / max(nlaunch, 1)})
 
 
