@@ -27,7 +27,7 @@ class CylParams(C.Structure):
                 ("stranding_probability", C.c_double), ("compute_depth", C.c_int),
                 ("frozen", C.c_int), ("lsm3_edge_opt", C.c_int),
                 ("diffusion_lsm3_tl_horizontal", C.c_double), ("diffusion_lsm3_tl_vertical", C.c_double),
-                ("stranding_output", C.c_int)]
+                ("stranding_output", C.c_int), ("bottom_deposition", C.c_int)]
 
 
 class CylPconst(C.Structure):
@@ -69,6 +69,7 @@ SIGNATURES = {
     "cyl_init_particles": (C.c_int, [C.c_void_p, C.POINTER(CylParams), C.c_double, C.c_double]),
     "cyl_step": (C.c_int, [C.c_void_p, C.POINTER(CylParams), _dp, C.c_double, C.c_int, _ip]),
     "cyl_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cyl_deposited_get": (C.c_int, [C.c_void_p, _dp, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_int)]),
     "cyl_last_timing": (C.c_int, [C.c_void_p, _dp, _dp, _ip]),
     "cyl_last_prep_timing": (C.c_int, [C.c_void_p, _dp]),
     "cyl_last_path": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),

@@ -200,6 +200,8 @@ struct cyl_ctx {
   int* d_strand_count = nullptr;
   int strand_cap = 0;
   std::vector<double> last_strand;  // 3 per record, in (step, pool index) order
+  std::vector<double> last_deposit; // deposits on the bed: 3 per record, same order
+  std::vector<int> last_deposit_step;  // step of the call (0-based) of each deposit
   int *d_perm = nullptr, *d_skey = nullptr, *d_scnt = nullptr;
   int64_t* d_cursor = nullptr;
   int64_t* d_sort_tiles = nullptr;
@@ -1740,6 +1742,7 @@ static ParamsDev to_dev(const cyl_params* p) {
   d.lsm3_edge_opt = p->lsm3_edge_opt; d.lsm3_tl_horizontal = p->diffusion_lsm3_tl_horizontal;
   d.lsm3_tl_vertical = p->diffusion_lsm3_tl_vertical;
   d.stranding_output = p->stranding_output;
+  d.bottom_deposition = p->bottom_deposition;
   return d;
 }
 
@@ -2116,6 +2119,10 @@ static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
   f.pregen = ctx->d_pregen;
   f.xi_by_slot = ctx->inline_normals == 2 ? 1 : 0;
   f.skip_us = !ctx->batched && ctx->dim == 2 ? 1 : 0;  // read by the LSM-1 kernel only
+  f.rec_xyz = a.strand_xyz;  // set for steps that record deposits (or stranded rows)
+  f.rec_key = a.strand_key;
+  f.rec_count = a.strand_count;
+  f.rec_cap = a.strand_cap;
   return f;
 }
 
@@ -2310,6 +2317,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       fs[s].bnd_statistics = P[s].bnd_statistics;
       fs[s].damp = fmin(P[s].rebound_damping_coef, 1.0 - EPS_REBOUND);
       fs[s].strand = P[s].stranding_probability;
+      fs[s].deposition = P[s].bottom_deposition;
       send[s] = ctx->batched ? ctx->scen_nsteps[s] : 0;
     }
     HIPCHK(hipMemcpy(ctx->d_pars, pd.data(), ns * sizeof(ParamsDev), hipMemcpyHostToDevice));
@@ -2336,9 +2344,16 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     }
     HIPCHK(hipMemsetAsync(ctx->d_sec_counts, 0, need * sizeof(int), ctx->stream));
   }
-  // stranded positions (one record per row at most: a stranded row is deleted)
-  bool strand_rec = !init_only && !ctx->batched && P[0].stranding_output && P[0].stranding_probability > 0.;
+  // stranded positions and deposits on the bed (one record per row at most:
+  // a stranded or deposited row is deleted)
+  const bool deposit_rec = !init_only && !ctx->batched && ctx->dim == 3 && P[0].bottom_deposition &&
+                           P[0].boundary_conditions;
+  bool strand_rec =
+      (!init_only && !ctx->batched && P[0].stranding_output && P[0].stranding_probability > 0.) || deposit_rec;
+  const int step0 = ctx->step_counter;
   ctx->last_strand.clear();
+  ctx->last_deposit.clear();
+  ctx->last_deposit_step.clear();
   if (strand_rec) {
     if (ctx->strand_cap < ctx->n) {
       if (ctx->d_strand_xyz) HIPCHK(hipFree(ctx->d_strand_xyz));
@@ -2778,11 +2793,18 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     std::vector<int> ord(nr);
     for (int q = 0; q < nr; ++q) ord[q] = q;
     std::sort(ord.begin(), ord.end(), [&](int u, int v) {
-      return key[u].x != key[v].x ? key[u].x < key[v].x : key[u].y < key[v].y;
+      const int pu = key[u].y & ~REC_DEPOSIT, pv = key[v].y & ~REC_DEPOSIT;
+      return key[u].x != key[v].x ? key[u].x < key[v].x : pu < pv;
     });
-    ctx->last_strand.resize((size_t)nr * 3);
-    for (int q = 0; q < nr; ++q)
-      for (int c = 0; c < 3; ++c) ctx->last_strand[3 * q + c] = xyz[3 * ord[q] + c];
+    for (int q = 0; q < nr; ++q) {
+      const int r = ord[q];
+      if (key[r].y & REC_DEPOSIT) {
+        for (int c = 0; c < 3; ++c) ctx->last_deposit.push_back(xyz[3 * r + c]);
+        ctx->last_deposit_step.push_back(key[r].x - step0);
+      } else {
+        for (int c = 0; c < 3; ++c) ctx->last_strand.push_back(xyz[3 * r + c]);
+      }
+    }
   }
   if (ctx->nsecseg > 0 && !init_only) {
     ctx->last_sec_counts.assign((size_t)nsteps * ctx->nsec, 0);
@@ -2907,6 +2929,17 @@ extern "C" int cyl_stranded_get(cyl_ctx* ctx, double* xyz, int cap, int* n) {
   *n = (int)(ctx->last_strand.size() / 3);
   for (int q = 0; q < *n && q < cap && xyz; ++q)
     for (int c = 0; c < 3; ++c) xyz[3 * q + c] = ctx->last_strand[3 * q + c];
+  return CYL_OK;
+}
+
+extern "C" int cyl_deposited_get(cyl_ctx* ctx, double* xyz, int* step, int cap, int* n) {
+  if (!ctx || !n) return CYL_E_ARG;
+  *n = (int)ctx->last_deposit_step.size();
+  for (int q = 0; q < *n && q < cap; ++q) {
+    if (xyz)
+      for (int c = 0; c < 3; ++c) xyz[3 * q + c] = ctx->last_deposit[3 * q + c];
+    if (step) step[q] = ctx->last_deposit_step[q];
+  }
   return CYL_OK;
 }
 

@@ -231,6 +231,7 @@ struct FastScen {
   double diff_const_z;     // sqrt(2*nuz)*sqrtdt (3D, models 1 and 2)
   int boundary_conditions, max_wall_rebounds, bnd_statistics;
   double damp, strand;
+  int deposition;          // 3D: deposition on the bed instead of the bottom reflection
 };
 
 struct FastArgs {
@@ -288,6 +289,10 @@ struct FastArgs {
   const int* pregen;          // [nscen] scenario's normals come from a generator this step
   int xi_by_slot;             // the generator stored each slot's pair at xi[slot] (k_gen_slots)
   int skip_us;                // LSM-1: us (= up) is not stored; downloads derive it from up
+  double* rec_xyz;            // deposits (3D): positions on the bed, or null (not recorded)
+  int2* rec_key;              // (step, pool index | REC_DEPOSIT)
+  int* rec_count;             // [0] records so far
+  int rec_cap;
 };
 
 

@@ -332,6 +332,21 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
       double zbp, zsp;
       column3(a3, tri, x, y, zbp, zsp);
       double hp = fmax(0., zsp - zbp);
+      if (below && q.deposition) {
+        // deposition on the bed (an extension; the reference reflects): the
+        // particle is recorded at (x, y, bed elevation) and leaves the column
+        if (a.rec_xyz) {
+          int r = atomicAdd(a.rec_count, 1);
+          if (r < a.rec_cap) {
+            a.rec_xyz[3 * r] = x;
+            a.rec_xyz[3 * r + 1] = y;
+            a.rec_xyz[3 * r + 2] = zbp;
+            a.rec_key[r] = make_int2(step_of(a), a.pool[j] | REC_DEPOSIT);
+          }
+        }
+        deleted = true;
+        continue;
+      }
       if (below) {
         double dz = fabs(zbp - z);
         z = fmin(zbp + (1. - damp) * dz, zbp + 0.5 * hp);

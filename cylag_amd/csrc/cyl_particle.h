@@ -230,6 +230,9 @@ struct BcDraws {
 
 // Stranded-particle records (compute_bc.pyx:107-122): position at stranding,
 // keyed by (step, pool index) so that the host can restore the serial order.
+// Deposits on the bed (bottom_deposition) share the buffer, their pool index
+// marked with REC_DEPOSIT: a row is recorded at most once (both delete it).
+constexpr int REC_DEPOSIT = 1 << 30;
 struct StrandRec {
   double* xyz;  // cap * 3
   int2* key;    // cap: (step, pool index)
@@ -1391,7 +1394,7 @@ __device__ inline void compute_bc(Ctx& c, Part& p, int bnd_j, BcDraws& bd, int* 
   if (bnd_j != -1) {
     int lab = m.edge_labels[bnd_j];
     if (lab == 0 && strand > 0.) strand_cond = bd.uniform() < strand;
-    if (strand_cond && c.sr) {
+    if (strand_cond && c.sr && c.par.stranding_output) {
       int q = atomicAdd(c.sr->count, 1);
       if (q < c.sr->cap) {
         c.sr->xyz[3 * q] = p.x[0];
@@ -1461,6 +1464,21 @@ __device__ inline void compute_bc(Ctx& c, Part& p, int bnd_j, BcDraws& bd, int* 
         zsp = field2d(c.f, m, c.f.fid_zs, p.tri, c.f.nlayers - 1, p.x[0], p.x[1]);
       }
       double hp = fmax(0., zsp - zbp);
+      if (below && c.par.bottom_deposition) {
+        // deposition on the bed (an extension; the reference reflects): the
+        // particle is recorded at (x, y, bed elevation) and leaves the column
+        if (c.sr) {
+          int q = atomicAdd(c.sr->count, 1);
+          if (q < c.sr->cap) {
+            c.sr->xyz[3 * q] = p.x[0];
+            c.sr->xyz[3 * q + 1] = p.x[1];
+            c.sr->xyz[3 * q + 2] = zbp;
+            c.sr->key[q] = make_int2(c.sr->step, c.sr->pool | REC_DEPOSIT);
+          }
+        }
+        delete_part(p, d);
+        continue;
+      }
       if (below) {
         double dz = fabs(zbp - p.x[2]);
         p.x[2] = fmin(zbp + (1. - damp) * dz, zbp + 0.5 * hp);

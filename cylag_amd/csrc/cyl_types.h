@@ -115,6 +115,7 @@ struct ParamsDev {
   int lsm3_edge_opt;
   double lsm3_tl_horizontal, lsm3_tl_vertical;
   int stranding_output;
+  int bottom_deposition;  // deposition on the bed instead of the bottom reflection (extension)
 };
 
 // Random stream of one scenario for one step.

@@ -41,4 +41,10 @@ CYLAG_DEFAULT_PARAMETERS = {
     'control_sections_file_name': 'cs_stats',
     'stranding_output': False,
     'stranding_output_file_name': 'stranded_particles',
+    # EXTENSIONS (no counterpart in the reference; the defaults keep its behaviour):
+    # bottom_deposition: in 3D a particle below the bed after its step is deposited
+    # (its position on the bed and the iteration are recorded, then it is deleted)
+    # instead of being reflected into the column
+    'bottom_deposition': False,
+    'deposition_output_file_name': 'deposited_particles',
 }

@@ -36,7 +36,7 @@ class LagrangianParticleSet:
                  particle_diameter=0.01, particle_diameter_distribution=None,
                  drag_coefficient_model=1, drag_coefficient=0.44, added_mass_force=False,
                  added_mass_coef=0.5, buoyancy_velocity_model=0, buoyancy_velocity=0.0,
-                 compute_depth=False, max_stranded=None):
+                 compute_depth=False, max_stranded=None, max_deposited=None):
         self._h = {}
         self._solver = None
         self._stale = False
@@ -75,6 +75,11 @@ class LagrangianParticleSet:
         self.max_stranded = max_stranded if max_stranded is not None else n
         self.stranded_count = 0
         self.stranded_positions = np.zeros((self.max_stranded, 3))
+        # deposits on the bed (bottom_deposition, an extension): position and iteration
+        self.max_deposited = max_deposited if max_deposited is not None else n
+        self.deposited_count = 0
+        self.deposited_positions = np.zeros((self.max_deposited, 3))
+        self.deposited_iterations = np.zeros(self.max_deposited, dtype=np.int64)
         self._init_state_vector(position, velocity, fluid_velocity_seen)
         h["tri"] = -np.ones(n, dtype=np.int32)
         h["lowerlayer"] = np.zeros(n, dtype=np.int32)

@@ -130,6 +130,21 @@ def write_stranded_particles_txt(outputdir, file_name, stranded_positions, stran
         f.write(''.join('{:.16e}, {:.16e}, {:.16e}\n'.format(r[0], r[1], r[2]) for r in rows))
 
 
+def write_deposited_particles_txt(outputdir, file_name, positions, iterations, count):
+    """Deposits on the bed recorded since the last flush (bottom_deposition, an
+    extension): the stranded file's layout with the iteration of each deposit;
+    the file is rewritten at each flush that has records."""
+    if count <= 0:
+        return
+    os.makedirs(outputdir, exist_ok=True)
+    rows = np.asarray(positions)[:count]
+    its = np.asarray(iterations)[:count]
+    with open(os.path.join(outputdir, file_name + '.txt'), 'w') as f:
+        f.write('cylag deposited particles file\nVariables: Xp, Yp, Zp, Iteration\n')
+        f.write(''.join('{:.16e}, {:.16e}, {:.16e}, {:d}\n'.format(r[0], r[1], r[2], int(i))
+                        for r, i in zip(rows, its)))
+
+
 class ParticlesIO:
     """Records of a particle result file (particles_io.pyx:26-120, 453-648):
     per record the time, iteration and arrays of tags, positions and velocities

@@ -13,7 +13,8 @@ import numpy as np
 
 from . import _hip, random_utils
 from .control_shapes import ControlSection, section_segments
-from .particles_io import prepare_output, write_particles, write_stranded_particles_txt
+from .particles_io import (prepare_output, write_deposited_particles_txt, write_particles,
+                           write_stranded_particles_txt)
 from .write_statistics import prepare_secstats_output, write_secstats_buffer
 from .parameters import Parameters
 from .simutime import SimuTime
@@ -28,7 +29,7 @@ def _params_struct(p, pset):
         float(p.rebound_damping_coef), float(p.wall_roughness_angle),
         float(p.stranding_probability), int(bool(pset.compute_depth)),
         int(bool(p.frozen_eulerian_fields)), int(p.lsm3_edge_opt), float(p.diffusion_lsm3_tl_horizontal),
-        float(p.diffusion_lsm3_tl_vertical), int(bool(p.stranding_output)))
+        float(p.diffusion_lsm3_tl_vertical), int(bool(p.stranding_output)), int(bool(p.bottom_deposition)))
 
 
 def _pconst_struct(ps):
@@ -151,6 +152,7 @@ class Solver:
         self._bnd_stats_data = np.zeros((nt_buf, max(1, nopen)), dtype=np.int32)
         self._bnd_stats_count = 0
         self._bnd_flushed = []
+        self._deposited_flushed = []
         self._csec_stats_times = np.zeros(nt_buf)
         self._csec_stats_data = np.zeros((nt_buf, max(1, p.ncsec)), dtype=np.int32)
         self._csec_stats_count = 0
@@ -281,6 +283,8 @@ class Solver:
                 self._bnd_stats_count += 1
         if p.stranding_output and p.stranding_probability > 0.:
             self._collect_stranded()
+        if p.bottom_deposition and self.particle_set.dim == 3:
+            self._collect_deposited(st.iteration - nsteps + 1)
         if self.control_sections:
             nc = p.ncsec
             counts = np.zeros(max(1, nsteps * nc), dtype=np.int32)
@@ -314,6 +318,27 @@ class Solver:
                 ps.stranded_count += 1
             else:
                 print("Warning: stranded particle buffer is full. Increase max_stranded in LagrangianParticleSet")
+
+    def _collect_deposited(self, first_iteration):
+        """Append the rows deposited on the bed by the last call (bottom_deposition,
+        an extension) to the particle set's buffer: position on the bed and the
+        iteration of the step, in the serial order of the loop."""
+        ps = self.particle_set
+        n = C.c_int()
+        self.ctx.check(self.ctx.lib.cyl_deposited_get(self.ctx.h, None, None, 0, C.byref(n)), "cyl_deposited_get")
+        if n.value == 0:
+            return
+        xyz = np.zeros((n.value, 3))
+        step = np.zeros(n.value, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.cyl_deposited_get(self.ctx.h, _hip.ptr(xyz), _hip.ptr(step, "i"), n.value,
+                                                      C.byref(n)), "cyl_deposited_get")
+        for r, s in zip(xyz, step):
+            if ps.deposited_count < ps.max_deposited:
+                ps.deposited_positions[ps.deposited_count] = r
+                ps.deposited_iterations[ps.deposited_count] = first_iteration + s
+                ps.deposited_count += 1
+            else:
+                print("Warning: deposited particle buffer is full. Increase max_deposited in LagrangianParticleSet")
 
     def forward(self, time_step=None):
         """One time step (solver.pyx:413-585); time_step is unused, as in the reference."""
@@ -433,6 +458,13 @@ class Solver:
             write_stranded_particles_txt(p.output_rep, p.stranding_output_file_name, ps.stranded_positions,
                                          ps.stranded_count)
             ps.stranded_count = 0
+        if p.bottom_deposition:
+            ps = self.particle_set
+            n = ps.deposited_count
+            write_deposited_particles_txt(p.output_rep, p.deposition_output_file_name, ps.deposited_positions,
+                                          ps.deposited_iterations, n)
+            self._deposited_flushed.append((ps.deposited_positions[:n].copy(), ps.deposited_iterations[:n].copy()))
+            ps.deposited_count = 0
         n = self._csec_stats_count
         if n:
             write_secstats_buffer(self._csec_stats_times, self._csec_stats_data, n, p.ncsec, p.output_rep,
@@ -448,6 +480,15 @@ class Solver:
                                        self._csec_stats_data[:self._csec_stats_count])]
         return (np.concatenate([t for t, _ in parts]),
                 np.concatenate([d for _, d in parts])[:, :max(self.parameters.ncsec, 1)])
+
+    @property
+    def deposited(self):
+        """(positions[n, 3], iterations[n]) of every deposit on the bed so far
+        (bottom_deposition, an extension), flushed or not, in the loop's order."""
+        ps = self.particle_set
+        parts = self._deposited_flushed + [(ps.deposited_positions[:ps.deposited_count],
+                                            ps.deposited_iterations[:ps.deposited_count])]
+        return np.concatenate([q[0] for q in parts]), np.concatenate([q[1] for q in parts])
 
     @property
     def bnd_stats(self):

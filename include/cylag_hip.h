@@ -42,6 +42,9 @@ typedef struct {
   int lsm3_edge_opt;                                                  /* 1 series, 2 clipped closed form */
   double diffusion_lsm3_tl_horizontal, diffusion_lsm3_tl_vertical;  /* TL of the constant-K models */
   int stranding_output;                                               /* record stranded positions */
+  int bottom_deposition; /* 1: a particle below the bed is deposited (recorded, then deleted) instead of
+                            reflected; an extension with no counterpart in the reference (0 = its
+                            damped reflection, compute_bc.pyx:294-332) */
 } cyl_params;
 
 /* Particle-set constants (lagrangian_particle_set.pyx:158-261,349-375). */
@@ -197,6 +200,12 @@ int cyl_control_counts(cyl_ctx *ctx, int *out, int cap, int *nout);
 /* Stranded positions of the last cyl_step call (stranding_output; compute_bc.pyx:107-122):
  * n records of (x, y, z) in the reference's order (steps in turn, rows in pool order). */
 int cyl_stranded_get(cyl_ctx *ctx, double *xyz, int cap, int *n);
+/* Deposits of the last cyl_step call (bottom_deposition; no counterpart in the reference): for each
+ * particle that reached the bed, its position on the bed (x, y, bed elevation) and the step of the
+ * call (0-based) in which it deposited, in the serial order of the reference's loop (steps in
+ * turn, rows in pool order).  n receives the count; at most cap records are copied.  Plain
+ * (single-scenario) contexts only. */
+int cyl_deposited_get(cyl_ctx *ctx, double *xyz, int *step, int cap, int *n);
 /* Which kernels advanced the particles in the last cyl_step / cyl_batch_step
  * call (no counterpart in the reference): launches of the fused kernels and of
  * the generic kernel that did the step (guarded reruns that found nothing to

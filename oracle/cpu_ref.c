@@ -1371,6 +1371,21 @@ static int compute_bc(ref_solver *s, int i, int bnd_j)
             }
             double *z = p->position + i * d + 2;
             double hp = fmax(0., zsp - zbp);
+            if (isb && par->bottom_deposition) {
+                /* Deposition on the bed: no counterpart in the reference, which
+                   reflects (the branch below, compute_bc.pyx:294-332).  The particle
+                   is recorded at (x, y, bed elevation) with the iteration of the
+                   step, then deleted like any particle leaving the water column. */
+                if (p->deposited_count < p->max_deposited) {
+                    int q = p->deposited_count++;
+                    p->deposited_positions[3 * q] = x;
+                    p->deposited_positions[3 * q + 1] = y;
+                    p->deposited_positions[3 * q + 2] = zbp;
+                    p->deposited_iteration[q] = s->iteration + 1;
+                }
+                ref_delete_single(p, i);
+                break;
+            }
             if (isb) {
                 double dzz = fabs(zbp - *z);
                 *z = fmin(zbp + (1. - damp) * dzz, zbp + 0.5 * hp);

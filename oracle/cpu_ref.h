@@ -145,6 +145,9 @@ typedef struct {
     double a0c, a1c, a2c, a3c;
     int max_stranded, stranded_count;
     double *stranded_positions;         /* max_stranded*3 */
+    int max_deposited, deposited_count; /* bottom_deposition (an extension of the reference) */
+    double *deposited_positions;        /* max_deposited*3: position on the bed */
+    int *deposited_iteration;           /* iteration (1-based) of the step of each deposit */
 } ref_pset;
 
 void ref_delete_single(ref_pset *p, int i);
@@ -163,6 +166,7 @@ typedef struct {
     int boundary_conditions, boundary_conditions_type, max_wall_rebounds;
     double rebound_damping_coef, wall_roughness_angle, stranding_probability;
     int bnd_statistics, stranding_output;
+    int bottom_deposition;  /* extension: deposit on the bed instead of the bottom reflection */
 } ref_params;
 
 typedef struct {

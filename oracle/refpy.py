@@ -76,7 +76,9 @@ class PsetS(C.Structure):
                 ("drag_coef_model", C.c_int), ("added_mass_force", C.c_int),
                 ("buoyancy_velocity_model", C.c_int), ("a0c", C.c_double), ("a1c", C.c_double),
                 ("a2c", C.c_double), ("a3c", C.c_double), ("max_stranded", C.c_int),
-                ("stranded_count", C.c_int), ("stranded_positions", _dp)]
+                ("stranded_count", C.c_int), ("stranded_positions", _dp),
+                ("max_deposited", C.c_int), ("deposited_count", C.c_int), ("deposited_positions", _dp),
+                ("deposited_iteration", C.POINTER(C.c_int))]
 
 
 class ParS(C.Structure):
@@ -90,7 +92,7 @@ class ParS(C.Structure):
                 ("boundary_conditions_type", C.c_int), ("max_wall_rebounds", C.c_int),
                 ("rebound_damping_coef", C.c_double), ("wall_roughness_angle", C.c_double),
                 ("stranding_probability", C.c_double), ("bnd_statistics", C.c_int),
-                ("stranding_output", C.c_int)]
+                ("stranding_output", C.c_int), ("bottom_deposition", C.c_int)]
 
 
 class SolverS(C.Structure):
@@ -398,6 +400,8 @@ class RefParticles:
         self.last_velocity = self.velocity.copy()
         self.depth = np.zeros(n) if compute_depth else None
         self.stranded = np.zeros((n, 3))
+        self.deposited = np.zeros((n, 3))
+        self.deposited_iteration = np.zeros(n, np.int32)
         self.s = PsetS(n, dim, na, n - 1, P(self.inactive, "i"), P(self.tri, "i"),
                        P(self.lowerlayer, "i"), P(self.last_tri, "i"), P(self.tag, "i"),
                        P(self.position, "d"), P(self.velocity, "d"), P(self.us, "d"),
@@ -406,7 +410,7 @@ class RefParticles:
                        int(compute_depth), 1000., particle_density, drag_coefficient,
                        added_mass_coef, buoyancy_velocity, drag_coefficient_model,
                        int(added_mass_force), buoyancy_velocity_model, 0., 0., 0., 0., n, 0,
-                       P(self.stranded, "d"))
+                       P(self.stranded, "d"), n, 0, P(self.deposited, "d"), P(self.deposited_iteration, "i"))
 
     @property
     def npart_active(self):
@@ -423,7 +427,7 @@ DEFAULTS = dict(initial_time=0., final_time=3600., time_step=1., frozen_eulerian
                 diffusion_lsm3_tl_horizontal=1., diffusion_lsm3_tl_vertical=1.,
                 boundary_conditions=True, boundary_conditions_type=1, max_wall_rebounds=4,
                 rebound_damping_coef=0.5, wall_roughness_angle=15.0, stranding_probability=0.,
-                bnd_statistics=False, stranding_output=False)
+                bnd_statistics=False, stranding_output=False, bottom_deposition=False)
 
 
 def resolve_params(user):
@@ -461,7 +465,7 @@ class RefSolver:
                         pr["boundary_conditions_type"], pr["max_wall_rebounds"],
                         pr["rebound_damping_coef"], pr["wall_roughness_angle"],
                         pr["stranding_probability"], int(pr["bnd_statistics"]),
-                        int(pr["stranding_output"]))
+                        int(pr["stranding_output"]), int(pr["bottom_deposition"]))
         self.bnd_stats = np.zeros(max(1, self.mesh.nopen), np.int32)
         self.bnd_history = []
         self.init_tri = np.full(particles.npart, -1, np.int32)

@@ -281,6 +281,25 @@ def open_channel_case(n=21, npart=400, u=0.8, seed=5):
     return Case(x, y, tri, fields, pos, pool=npart + 16, boundary_points=bp, nopen=2)
 
 
+def still_column_case(npart=600, h=10., nlayers=5, seed=31, ws=1e-3):
+    """Still water over a flat bed at depth h (sigma planes from -h to 0, no
+    flow, no turbulence fields), particles spread through the column, sinking
+    at the constant settling velocity ws (buoyancy_velocity_model 0)."""
+    x, y, tri, mpl = jittered_delaunay(11, 11, 100., 100., seed=3)
+    n = x.size
+    sig = np.linspace(0., 1., nlayers)
+    z = np.concatenate([np.full(n, -h + s_ * h) for s_ in sig])
+    zero = np.zeros((1, n * nlayers))
+    fields = dict(dim=3, nlayers=nlayers, times=np.zeros(1), elevation=z[None, :], velocity_x=zero,
+                  velocity_y=zero, velocity_z=zero)
+    rng = np.random.default_rng(seed)
+    pos = np.stack([rng.uniform(10., 90., npart), rng.uniform(10., 90., npart),
+                    -h * rng.uniform(0.02, 0.98, npart)], 1)
+    case = Case(x, y, tri, fields, pos, mpl_tri=mpl)
+    case.pset_kw = dict(buoyancy_velocity_model=0, buoyancy_velocity=-ws)
+    return case
+
+
 def settling_diameters(rng, n, rho_p=2650.):
     """Stokes diameters of settling velocities w_s ~ LN(median 1 mm/s, sigma 0.5)
     for sediment of density rho_p (a way to draw a diameter spread; the step
