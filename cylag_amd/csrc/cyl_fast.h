@@ -159,6 +159,35 @@ __device__ __forceinline__ int pick3i(int i, int a0, int a1, int a2) { return i 
 // collinear cases (on_segment, utils.pyx:282-303) only run when one of an
 // edge's orientations is zero.  The next element's walk record is loaded with
 // its vertices.
+// One edge test of the walk (intersec2d_segment_segment with the collinear
+// cases): edge i of walk record w, the element's vertices and their
+// orientation classes c0..c2 with respect to the path.  o1, o2: the classes
+// of the edge's first and second vertex.
+__device__ __forceinline__ bool walk_edge_hit(const WalkRec& w, int i, double vx0, double vx1, double vx2,
+                                              double vy0, double vy1, double vy2, int c0, int c1, int c2,
+                                              double p0x, double p0y, double x, double y, int& o1, int& o2) {
+  const int ia = (w.vv >> (4 * i)) & 3, ib = (w.vv >> (4 * i + 2)) & 3;
+  const double ax = pick3(ia, vx0, vx1, vx2), ay = pick3(ia, vy0, vy1, vy2);
+  const double bx = pick3(ib, vx0, vx1, vx2), by = pick3(ib, vy0, vy1, vy2);
+  o1 = pick3i(ia, c0, c1, c2);
+  o2 = pick3i(ib, c0, c1, c2);
+  const double ey = ay - by, ex = ax - bx;
+  const int o3 = orient_class(ey * (p0x - ax) - ex * (p0y - ay));
+  const int o4 = orient_class(ey * (x - ax) - ex * (y - ay));
+  bool hit = o1 != o2 && o3 != o4;
+  if (!hit && (o1 == 0 || o2 == 0 || o3 == 0 || o4 == 0))
+    hit = (o1 == 0 && on_seg(p0x, p0y, x, y, ax, ay)) || (o2 == 0 && on_seg(p0x, p0y, x, y, bx, by)) ||
+          (o3 == 0 && on_seg(ax, ay, bx, by, p0x, p0y)) || (o4 == 0 && on_seg(ax, ay, bx, by, x, y));
+  return hit;
+}
+
+// From the second pass on, the edge the walk entered by is skipped (the
+// reference tests it and discards its result: its id is the last edge), and
+// the orientation classes of its two vertices are those the previous pass
+// computed (same nodes, same path, same operands), so a pass evaluates one
+// vertex orientation and the two other edges, in triedge order, instead of
+// three and three.  Each lane selects its two edges, so that the saving is
+// real in a wavefront whose lanes entered by different edge slots.
 __device__ inline int walk_fast(const FastMesh& fm, int k, const ElemF& e0, double p0x, double p0y, double x,
                                 double y, int& bnd_j) {
   bnd_j = -1;
@@ -166,26 +195,26 @@ __device__ inline int walk_fast(const FastMesh& fm, int k, const ElemF& e0, doub
   double vx0 = e0.vx[0], vx1 = e0.vx[1], vx2 = e0.vx[2];
   double vy0 = e0.vy[0], vy1 = e0.vy[1], vy2 = e0.vy[2];
   WalkRec w = fm.wr[k < 0 ? k + fm.nt : k];
-  int last_edge = -1;
+  // first pass: the three vertex classes; slot 2 is tested here and slots 0
+  // and 1 by the loop body, as if the walk had entered by slot 2
+  int c0 = orient_class(ly * (vx0 - p0x) - lx * (vy0 - p0y));
+  int c1 = orient_class(ly * (vx1 - p0x) - lx * (vy1 - p0y));
+  int c2 = orient_class(ly * (vx2 - p0x) - lx * (vy2 - p0y));
+  int hit_i = -1, h1 = 0, h2 = 0, ie = 2;
+  if (walk_edge_hit(w, 2, vx0, vx1, vx2, vy0, vy1, vy2, c0, c1, c2, p0x, p0y, x, y, h1, h2)) hit_i = 2;
   for (int niter = 0; niter <= 50; ++niter) {
-    const int ov0 = orient_class(ly * (vx0 - p0x) - lx * (vy0 - p0y));
-    const int ov1 = orient_class(ly * (vx1 - p0x) - lx * (vy1 - p0y));
-    const int ov2 = orient_class(ly * (vx2 - p0x) - lx * (vy2 - p0y));
-    int hit_i = -1;
-#pragma unroll
-    for (int i = 2; i >= 0; --i) {  // the first intersected edge in triedge order wins
-      const int ia = (w.vv >> (4 * i)) & 3, ib = (w.vv >> (4 * i + 2)) & 3;
-      const double ax = pick3(ia, vx0, vx1, vx2), ay = pick3(ia, vy0, vy1, vy2);
-      const double bx = pick3(ib, vx0, vx1, vx2), by = pick3(ib, vy0, vy1, vy2);
-      const int o1 = pick3i(ia, ov0, ov1, ov2), o2 = pick3i(ib, ov0, ov1, ov2);
-      const double ey = ay - by, ex = ax - bx;
-      const int o3 = orient_class(ey * (p0x - ax) - ex * (p0y - ay));
-      const int o4 = orient_class(ey * (x - ax) - ex * (y - ay));
-      bool hit = o1 != o2 && o3 != o4;
-      if (!hit && (o1 == 0 || o2 == 0 || o3 == 0 || o4 == 0))
-        hit = (o1 == 0 && on_seg(p0x, p0y, x, y, ax, ay)) || (o2 == 0 && on_seg(p0x, p0y, x, y, bx, by)) ||
-              (o3 == 0 && on_seg(ax, ay, bx, by, p0x, p0y)) || (o4 == 0 && on_seg(ax, ay, bx, by, x, y));
-      if (hit && w.e[i] != last_edge) hit_i = i;
+    // the two slots other than ie, in triedge order: the first that is hit wins
+    const int iA = ie == 0 ? 1 : 0, iB = ie == 2 ? 1 : 2;
+    int o1, o2;
+    if (walk_edge_hit(w, iB, vx0, vx1, vx2, vy0, vy1, vy2, c0, c1, c2, p0x, p0y, x, y, o1, o2)) {
+      hit_i = iB;
+      h1 = o1;
+      h2 = o2;
+    }
+    if (walk_edge_hit(w, iA, vx0, vx1, vx2, vy0, vy1, vy2, c0, c1, c2, p0x, p0y, x, y, o1, o2)) {
+      hit_i = iA;
+      h1 = o1;
+      h2 = o2;
     }
     if (hit_i < 0) {
       // no edge crossed: the point is in the current element or nowhere
@@ -199,7 +228,7 @@ __device__ inline int walk_fast(const FastMesh& fm, int k, const ElemF& e0, doub
       return -1;
     }
     const ElemF& en = fm.ef[kn];
-    const WalkRec wn = fm.wr[kn];
+    w = fm.wr[kn];
     vx0 = en.vx[0]; vx1 = en.vx[1]; vx2 = en.vx[2];
     vy0 = en.vy[0]; vy1 = en.vy[1]; vy2 = en.vy[2];
     // point_in_triangle_dotp on the new element (in_elem)
@@ -207,8 +236,16 @@ __device__ inline int walk_fast(const FastMesh& fm, int k, const ElemF& e0, doub
               pintri(x, y, vx2, vy2, vx0, vy0, vx1, vy1) >= 0;
     if (in) return kn;
     k = kn;
-    last_edge = j;
-    w = wn;
+    // next pass: the entry edge (slot ie of the new record) is skipped, its
+    // vertices keep the classes h1, h2 (the edge's first and second vertex)
+    ie = w.e[0] == j ? 0 : (w.e[1] == j ? 1 : 2);
+    const int iae = (w.vv >> (4 * ie)) & 3, ibe = (w.vv >> (4 * ie + 2)) & 3;
+    const int ic = 3 - iae - ibe;
+    const int cc = orient_class(ly * (pick3(ic, vx0, vx1, vx2) - p0x) - lx * (pick3(ic, vy0, vy1, vy2) - p0y));
+    c0 = iae == 0 ? h1 : (ibe == 0 ? h2 : cc);
+    c1 = iae == 1 ? h1 : (ibe == 1 ? h2 : cc);
+    c2 = iae == 2 ? h1 : (ibe == 2 ? h2 : cc);
+    hit_i = -1;
   }
   return -1;
 }
