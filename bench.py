@@ -289,6 +289,39 @@ def c4_params(model):
     return p
 
 
+def c4_square_scenarios(nscen=256, seed=11, length=100.):
+    """The same scenario set (counts, steps, models from the same draws) as
+    point releases on the config-1 square (101 x 101 nodes, 20,000 elements)."""
+    npart, nsteps, model, _ = c4_scenarios(nscen, seed)
+    rng = np.random.default_rng(seed + 1)
+    xy = rng.uniform(0.05 * length, 0.95 * length, (nscen, 2))
+    return npart, nsteps, model, xy
+
+
+def c4_square_params(model):
+    """The config-1 parameters (uniform flow 0.1 m/s, constant K = 0.01 m2/s,
+    dt = 1 s, frozen fields, walls), LSM1, or 2D LSM2 with the direct
+    integrator (time_scheme 5, diffusion_lsm2_option 1)."""
+    p = dict(final_time=6000., time_step=1., frozen_eulerian_fields=True, diffusion_model=1,
+             horizontal_diffusivity=0.01)
+    if model == 1:
+        p.update(model=1)
+    else:
+        p.update(model=2, time_scheme=5, diffusion_lsm2_option=1)
+    return p
+
+
+def c1_square_workload():
+    """The config-1 mesh and fields: 101 x 101 nodes on a 100 m square, each
+    square split into two triangles, uniform u = 0.1 m/s, v = 0, depth 1 m."""
+    from tests.cases import structured_square
+    x, y, tri = structured_square(101, 100.)
+    n = x.size
+    fields = dict(dim=2, nlayers=1, times=np.zeros(1), elevation=np.ones((1, n)),
+                  velocity_x=np.full((1, n), 0.1), velocity_y=np.zeros((1, n)))
+    return x, y, tri, fields
+
+
 def scenario_streams(cy, nscen):
     """Scenario s draws from the default stream advanced by s << 48, the split
     SolverParallel gives its ranks (solver_parallel.pyx:113-115)."""
@@ -304,17 +337,18 @@ def point_release(cy, xy, n):
     return cy.LagrangianParticleSet(np.repeat(np.asarray(xy, dtype=np.float64)[None], int(n), 0), dim=2)
 
 
-def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250, tune=True):
+def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250, tune=True, scenarios=c4_scenarios,
+                params=c4_params):
     """The whole batched job: every scenario runs all its steps (or the first
     max_steps of them); every 250 steps the finished scenarios' rows come back
     to the host and the scenarios leave the device.  The clock brackets the
     stepping, collection and retirement (inputs resident on the device)."""
-    npart, nsteps, model, xy = c4_scenarios()
+    npart, nsteps, model, xy = scenarios()
     fs = cy.EulerianFieldSet(mesh, **fields)
     streams = scenario_streams(cy, len(npart))
     t0 = time.time()
     psets = [point_release(cy, xy[s], npart[s]) for s in range(len(npart))]
-    b = cy.BatchSolver(fs, psets, [c4_params(m) for m in model], streams, nsteps=nsteps,
+    b = cy.BatchSolver(fs, psets, [params(m) for m in model], streams, nsteps=nsteps,
                        sort_interval=sort_interval)
     log("batched: {} scenarios, {} particles, {}..{} steps, ready ({:.1f} s)".format(
         len(npart), int(npart.sum()), int(nsteps.min()), int(nsteps.max()), time.time() - t0))
@@ -356,7 +390,8 @@ def batched_leg(cy, mesh, fields, sort_interval, max_steps=None, chunk=250, tune
                 tuning=tuned if tune else [dict(step=0, sort_interval=sort_interval, blocks_per_cu=0)])
 
 
-def plain_sequential_leg(cy, mesh, fields, ksteps=40, warm=3, nsteps_cap=None):
+def plain_sequential_leg(cy, mesh, fields, ksteps=40, warm=3, nsteps_cap=None, scenarios=c4_scenarios,
+                         params=c4_params):
     """The same scenarios, one after another, each with the plain step
     (cyl_step on a context holding the mesh and fields, the scenario's pool
     uploaded and initialized as Solver does).  Each scenario is timed over its
@@ -366,7 +401,7 @@ def plain_sequential_leg(cy, mesh, fields, ksteps=40, warm=3, nsteps_cap=None):
     from cylag_amd import _hip
     from cylag_amd.parameters import Parameters
     from cylag_amd.solver import _params_struct, _pconst_struct, upload_fields, upload_mesh
-    npart, nsteps, model, xy = c4_scenarios()
+    npart, nsteps, model, xy = scenarios()
     if nsteps_cap is not None:
         nsteps = np.minimum(nsteps, nsteps_cap)
     fs = cy.EulerianFieldSet(mesh, **fields)
@@ -378,7 +413,7 @@ def plain_sequential_leg(cy, mesh, fields, ksteps=40, warm=3, nsteps_cap=None):
     t_job = 0.
     t0 = time.perf_counter()
     for s in range(len(npart)):
-        p = Parameters(c4_params(model[s]))
+        p = Parameters(params(model[s]))
         ps = point_release(cy, xy[s], npart[s])
         ps.rho0 = p.water_density
         if p.model == 2:
@@ -389,18 +424,21 @@ def plain_sequential_leg(cy, mesh, fields, ksteps=40, warm=3, nsteps_cap=None):
         d = lambda k: np.ascontiguousarray(h_[k], dtype=np.float64)
         tri = np.full(n, -2, dtype=np.int32)
         init_tri = np.full(n, -3, dtype=np.int32)
-        ctx.check(lib.cyl_fields_reset(h, 0, 0.), "cyl_fields_reset")
-        ctx.check(lib.cyl_fields_interpolate(h, 0.), "cyl_fields_interpolate")
+        frozen = bool(p.frozen_eulerian_fields)
+        ctx.check(lib.cyl_fields_reset(h, int(frozen), 0.), "cyl_fields_reset")
+        if not frozen:
+            ctx.check(lib.cyl_fields_interpolate(h, 0.), "cyl_fields_interpolate")
         ctx.check(lib.cyl_particles_upload(h, n, 2, P(d("position")), P(d("velocity")),
                                            P(d("fluid_velocity_seen")), P(np.zeros(n, np.int32), "i"), P(tri, "i"),
                                            P(np.zeros(n, np.int32), "i"), P(d("diameter")), P(d("volume")),
                                            P(init_tri, "i"), C.byref(pc)), "cyl_particles_upload")
         ctx.check(lib.cyl_rng_set(h, *streams[s]), "cyl_rng_set")
-        ctx.check(lib.cyl_init_particles(h, C.byref(par), 0., 2.), "cyl_init_particles")
+        dt = float(p.time_step)
+        ctx.check(lib.cyl_init_particles(h, C.byref(par), 0., dt), "cyl_init_particles")
         t = C.c_double(0.)
-        ctx.check(lib.cyl_step(h, C.byref(par), C.byref(t), 2., warm, None), "cyl_step")
+        ctx.check(lib.cyl_step(h, C.byref(par), C.byref(t), dt, warm, None), "cyl_step")
         ta = time.perf_counter()
-        ctx.check(lib.cyl_step(h, C.byref(par), C.byref(t), 2., ksteps, None), "cyl_step")
+        ctx.check(lib.cyl_step(h, C.byref(par), C.byref(t), dt, ksteps, None), "cyl_step")
         per_step = (time.perf_counter() - ta) / ksteps
         t_job += per_step * int(nsteps[s])
         if s % 32 == 31:
@@ -540,10 +578,11 @@ def main():
     ap.add_argument("--normals", type=int, default=0, choices=[0, 1, 2],
                     help="plain leg: where the fused kernel takes its normals (0: pool-order generator, 1: sampled "
                          "inline, 2: slot-order generator)")
-    ap.add_argument("--legs", default="plain,c3,batched,sequential,served",
+    ap.add_argument("--legs", default="plain,c3,batched,sequential,batched_square,served",
                     help="comma-separated: plain (the 10M-particle step), c3 (3D LSM-2 with settling, 5M particles), "
                          "batched (256 scenarios in one batch), "
-                         "sequential (the same scenarios one after another with the plain step), served "
+                         "sequential (the same scenarios one after another with the plain step), batched_square "
+                         "(the scenario set on the config-1 square, batched and one after another), served "
                          "(Poisson requests on the 3D channel)")
     ap.add_argument("--served-window", type=float, default=60.,
                     help="served leg: seconds of arrivals (the loop stops 15 s later)")
@@ -586,6 +625,21 @@ def main():
             bt["speedup_vs_plain_sequential"] = bt["value"] / seq["value"]
         else:
             line["plain_sequential"] = seq
+    if "batched_square" in legs:
+        xs, ys, ts, fsq = c1_square_workload()
+        msq = cy.TriangularMesh(xs, ys, ts)
+        bq = batched_leg(cy, msq, fsq, args.sort_interval, args.batch_steps, scenarios=c4_square_scenarios,
+                         params=c4_square_params)
+        seq = plain_sequential_leg(cy, msq, fsq, nsteps_cap=args.batch_steps, scenarios=c4_square_scenarios,
+                                   params=c4_square_params)
+        bq["plain_sequential"] = seq
+        bq["speedup_vs_plain_sequential"] = bq["value"] / seq["value"]
+        bq["workload"] = ("the batched scenario set on the config-1 square (101 x 101 nodes, 20,000 elements, "
+                          "u = 0.1 m/s, K = 0.01 m2/s, dt = 1 s, frozen, walls): 256 point releases, "
+                          "N ~ logU(1e3, 1e6), 500-5000 steps, LSM1 or 2D LSM2")
+        log("batched on the square: {:.3e} particle-steps/s, {:.1f}x one after another".format(
+            bq["value"], bq["speedup_vs_plain_sequential"]))
+        line["batched_square"] = bq
     if "served" in legs:
         line["served"] = served_leg(cy, window=args.served_window, horizon=args.served_window + 15.)
     if "plain" in legs and not args.no_cpu:
