@@ -167,12 +167,14 @@ struct cyl_ctx {
   int64_t ev_units_cap = 0, nchunks = 0, ev_cap = 0;
   Bufs ev_bufs;
   int64_t* d_chunk_base = nullptr;
-  int* d_chunk_count = nullptr;
-  int64_t *d_chunk_off = nullptr, *d_ev_pos = nullptr, *d_ev_len = nullptr, *d_reach = nullptr;
+  int64_t nspan = 0;                               // spans of BLK chunks
+  int *d_chunk_loc = nullptr, *d_span_cnt = nullptr;
+  int64_t *d_span_off = nullptr, *d_reg_pos = nullptr, *d_reg_len = nullptr, *d_reg_key = nullptr;
+  double* d_reg_val = nullptr;                     // span regions of the candidate records
+  int64_t *d_ev_pos = nullptr, *d_ev_len = nullptr, *d_reach = nullptr;
   int64_t *d_bmax = nullptr, *d_E = nullptr, *d_C = nullptr, *d_key = nullptr, *d_xtra = nullptr;
   double* d_ev_val = nullptr;                      // compacted slow-normal values
   double* d_cand_val = nullptr;                    // value per candidate
-  uint64_t* d_ev_state = nullptr;                  // generator state per candidate
   int64_t *d_ev_begin = nullptr, *d_nev = nullptr, *d_ev_bsum = nullptr, *d_xscratch = nullptr;
   int64_t* d_chunk_live = nullptr;                 // end of the step's live chunks
   int* d_bucket = nullptr;          // bucket index of the event keys, all scenarios
@@ -1818,9 +1820,18 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(hipMemcpy(ctx->d_bucket_base, bb.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
   HIPCHK(dalloc(b, &ctx->d_chunk_base, ns));
   HIPCHK(hipMemcpy(ctx->d_chunk_base, cb.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice));
-  HIPCHK(dalloc(b, &ctx->d_chunk_count, nch + 1));
-  HIPCHK(dalloc(b, &ctx->d_chunk_off, nch + 1));
-  HIPCHK(dalloc(b, &ctx->d_ev_bsum, nblk(std::max(nch, evcap) + 1) + 1));
+  ctx->nspan = (nch + BLK - 1) / BLK;
+  const int64_t nreg = ctx->nspan * (method == 2 ? EV_SPAN_CAP_ZIG : EV_SPAN_CAP_POLAR);
+  HIPCHK(dalloc(b, &ctx->d_chunk_loc, nch + 1));
+  HIPCHK(dalloc(b, &ctx->d_span_cnt, ctx->nspan + 1));
+  HIPCHK(dalloc(b, &ctx->d_span_off, ctx->nspan + 1));
+  HIPCHK(dalloc(b, &ctx->d_reg_pos, nreg));
+  if (method == 2) {
+    HIPCHK(dalloc(b, &ctx->d_reg_len, nreg));
+    HIPCHK(dalloc(b, &ctx->d_reg_val, nreg));
+    HIPCHK(dalloc(b, &ctx->d_reg_key, nreg));
+  }
+  HIPCHK(dalloc(b, &ctx->d_ev_bsum, nblk(evcap + 1) + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_pos, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_len, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_reach, evcap + 1));
@@ -1831,7 +1842,6 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(dalloc(b, &ctx->d_xtra, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_val, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_cand_val, evcap + 1));
-  HIPCHK(dalloc(b, &ctx->d_ev_state, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_xscratch, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_begin, ns + 1));
   HIPCHK(dalloc(b, &ctx->d_nev, 1));
@@ -1882,12 +1892,18 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   e.chunk_base = ctx->d_chunk_base;
   e.nscen = ctx->nscen;
   e.nchunks = ctx->nchunks;
-  e.chunk_count = ctx->d_chunk_count;
-  e.chunk_off = ctx->d_chunk_off;
+  e.chunk_loc = ctx->d_chunk_loc;
+  e.span_cnt = ctx->d_span_cnt;
+  e.span_off = ctx->d_span_off;
+  e.nspan = ctx->nspan;
+  e.span_cap = method == 2 ? EV_SPAN_CAP_ZIG : EV_SPAN_CAP_POLAR;
+  e.reg_pos = ctx->d_reg_pos;
+  e.reg_len = ctx->d_reg_len;
+  e.reg_val = ctx->d_reg_val;
+  e.reg_key = ctx->d_reg_key;
   e.nev = ctx->d_nev;
   e.ev_begin = ctx->d_ev_begin;
   e.ev_pos = ctx->d_ev_pos;
-  e.ev_state = ctx->d_ev_state;
   e.ev_len = ctx->d_ev_len;
   e.ev_val = ctx->d_cand_val;
   e.ev_key = ctx->d_xscratch;
@@ -1901,13 +1917,18 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BLK, 0) != hipSuccess || per_cu < 1) per_cu = 1;
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(ctx->nchunks), (int64_t)ctx->ncu * per_cu));
   };
-  if (method == 2) k_ev_scan<2, false><<<pgrid((const void*)k_ev_scan<2, false>), BLK, 0, s>>>(e);
-  else k_ev_scan<3, false><<<pgrid((const void*)k_ev_scan<3, false>), BLK, 0, s>>>(e);
-  enqueue_scan(ctx, ctx->d_chunk_count, ctx->d_chunk_off, ctx->nchunks, ctx->d_ev_bsum, ctx->d_nev,
-               (const int64_t*)ctx->d_chunk_live, cond, launches);
-  if (method == 2) k_ev_scan<2, true><<<pgrid((const void*)k_ev_scan<2, true>), BLK, 0, s>>>(e);
-  else k_ev_scan<3, true><<<pgrid((const void*)k_ev_scan<3, true>), BLK, 0, s>>>(e);
-  launches += 2;
+  // the gather visits the spans of the live chunks, a fraction of the capacity
+  const unsigned ngat = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ctx->nspan, 4LL * ctx->ncu));
+  if (method == 2) {
+    k_ev_write<2><<<pgrid((const void*)k_ev_write<2>), BLK, 0, s>>>(e);
+    k_ev_span_scan<<<1, BLK, 0, s>>>(e);
+    k_ev_gather<2><<<ngat, BLK, 0, s>>>(e);
+  } else {
+    k_ev_write<3><<<pgrid((const void*)k_ev_write<3>), BLK, 0, s>>>(e);
+    k_ev_span_scan<<<1, BLK, 0, s>>>(e);
+    k_ev_gather<3><<<ngat, BLK, 0, s>>>(e);
+  }
+  launches += 3;
   int64_t cap = ctx->ev_cap;
   // the candidate kernels loop over the step's candidates (device count), far
   // fewer than the capacity: 4 blocks per CU

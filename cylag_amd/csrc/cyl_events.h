@@ -16,8 +16,11 @@
 // Draws.events_before / zig_position / polar_position then map a unit to its
 // position, and the normal generator replays runs of the stream from them.
 //
-// Pipeline (rng_method 2): count -> scan -> write and sample -> reach
-// max-scan -> resolve -> (extra, count) scan -> compact -> buckets.
+// Pipeline (rng_method 2): one pass over the window (each block finds the
+// candidates of its span of chunks, compacts them in LDS and samples them into
+// the span's region) -> scan of the span counts -> gather into dense,
+// position-ordered tables -> reach max-scan -> resolve -> (extra, count) scan
+// -> compact -> buckets.
 #pragma once
 #include "cyl_kernels.h"
 
@@ -36,20 +39,33 @@ struct EventArgs {
   const int64_t* chunk_base;  // first chunk of each scenario (host-sized capacity)
   int nscen;
   int64_t nchunks;
-  int* chunk_count;           // candidates per chunk
-  int64_t* chunk_off;         // scanned
-  int64_t* nev;               // total candidates (scan total)
+  int* chunk_loc;             // per chunk: its first candidate's offset in its span
+  int* span_cnt;              // per span: candidates (at most span_cap)
+  int64_t* span_off;          // per span: first dense index (scan of span_cnt)
+  int64_t nspan;              // spans of the chunk capacity
+  int span_cap;               // candidate records per span region
+  int64_t* reg_pos;           // span regions (nspan * span_cap): position, and for rng_method 2
+  int64_t* reg_len;           //   the length, value and reach key of the normal starting there
+  double* reg_val;
+  int64_t* reg_key;
+  int64_t* nev;               // total candidates
   int64_t* ev_begin;          // [nscen + 1] first candidate of each scenario
-  int64_t* ev_pos;            // candidate position (relative to the scenario's window start)
-  uint64_t* ev_state;         // generator state at the candidate (zig)
+  int64_t* ev_pos;            // dense tables: candidate position (relative to the scenario's window start)
   int64_t* ev_len;            // length of the normal starting at the candidate (zig)
   double* ev_val;             // its value (zig)
   int64_t* ev_key;            // its reach key (zig)
-  int64_t* chunk_live;        // end of the live chunks (set by the count pass)
+  int64_t* chunk_live;        // end of the live chunks
   int64_t ev_cap;
   int* overflow;
   const int* cond;            // run only when *cond != 0
 };
+
+// candidate records per span region: a span is BLK chunks of ZCHUNK positions;
+// Ziggurat slow starts come at about 1.2 % of positions (about 200 per span),
+// rejected polar attempts at 1 - pi/4 of BLK * ZCHUNK / 2 attempts (about 1760)
+constexpr int EV_SPAN_CAP_ZIG = 1024;
+constexpr int EV_SPAN_CAP_POLAR = 4096;
+static_assert(ZCHUNK == 64, "a chunk's candidates are one 64-bit mask");
 
 __device__ __forceinline__ int scen_of(const int64_t* base, int nscen, int64_t i) {
   int lo = 0, hi = nscen - 1;
@@ -75,40 +91,48 @@ __device__ __forceinline__ bool zig_fast(uint32_t u, const uint32_t* kn) {
   return ahz < kn[iz];
 }
 
-// Scan of the stream window, one chunk of ZCHUNK positions per thread, blocks
-// striding over spans of BLK chunks.  WRITE = false: count candidates per
-// chunk, set each scenario's window, and (block 0) the end of the live chunks
-// (no chunk at or past it holds a position of any window).  WRITE = true:
-// write the candidates of the live chunks at the scanned offsets; under
-// rng_method 2 the block then samples the candidates of its span (one
-// contiguous range of the table, one candidate per thread): the value and
-// length of the Ziggurat normal that starts there and its reach key
-// (scenario << REACH_SHIFT) | (pos + len - 1).
-// A candidate whose normal reaches the base strip's tail loop (two correctly
+// One pass over the stream window, one chunk of ZCHUNK positions per thread,
+// blocks striding over spans of BLK chunks.  Each thread draws its chunk and
+// marks its candidates in a 64-bit mask (rng_method 2: draws failing the fast
+// test; 3: rejected polar attempts); a block scan of the mask counts gives
+// each its offset in the span, and the span's candidates go, in position
+// order, to the span's region.  Under rng_method 2 the block then samples
+// them, one candidate per thread: the generator state there is one
+// byte-window jump from its chunk's start state (kept in LDS), and the value
+// and length of the Ziggurat normal starting there and its reach key
+// (scenario << REACH_SHIFT) | (pos + len - 1) are stored with it.  A
+// candidate whose normal reaches the base strip's tail loop (two correctly
 // rounded logarithms per attempt; a few percent of candidates) is queued in
 // LDS and finished by the block at the end, so that the waves sampling the
-// other candidates do not wait on it.
+// other candidates do not wait on it.  Every block bounds its spans by the end
+// of the live chunks (no chunk at or past it holds a position of any window);
+// block 0 stores it and each scenario's window.
 constexpr int TAIL_CAP = 256;
 struct TailRec {
   uint64_t state;
-  int64_t i;
-  int32_t hz, ncalls;
+  int64_t i, pos;
+  int32_t hz, ncalls, scen, pad;
 };
 
-__device__ __forceinline__ void ev_finish(const EventArgs& e, int64_t i, double v, int64_t ncalls) {
-  e.ev_val[i] = v;
-  e.ev_len[i] = ncalls;
-  e.ev_key[i] = e.ev_key[i] | (e.ev_pos[i] + ncalls - 1);
+__device__ __forceinline__ void reg_finish(const EventArgs& e, int64_t r, double v, int64_t ncalls, int64_t pos,
+                                           int s) {
+  e.reg_val[r] = v;
+  e.reg_len[r] = ncalls;
+  e.reg_key[r] = ((int64_t)s << REACH_SHIFT) | (pos + ncalls - 1);
 }
 
-template <int METHOD, bool WRITE>
-__global__ void __launch_bounds__(BLK) k_ev_scan(EventArgs e) {
+template <int METHOD>
+__global__ void __launch_bounds__(BLK) k_ev_write(EventArgs e) {
   if (e.cond && *e.cond == 0) return;
   __shared__ JumpTable8 t8;
   __shared__ uint32_t kn[128];
   __shared__ int64_t lds[BLK / 64];
-  __shared__ TailRec tails[WRITE && METHOD == 2 ? TAIL_CAP : 1];
+  __shared__ uint64_t cmask[BLK], cstate[BLK];
+  __shared__ int64_t cpos[BLK];
+  __shared__ int cscen[BLK], cbase[BLK];
+  __shared__ TailRec tails[METHOD == 2 ? TAIL_CAP : 1];
   __shared__ int ntail;
+  __shared__ int64_t live_s;
   if (threadIdx.x == 0) ntail = 0;
   const int s0 = scen_of(e.chunk_base, e.nscen, (int64_t)blockIdx.x * BLK);
   const uint64_t inc0 = e.stream[s0].inc;
@@ -122,124 +146,188 @@ __global__ void __launch_bounds__(BLK) k_ev_scan(EventArgs e) {
     if (METHOD == 2)
       for (int q = threadIdx.x; q < 128; q += BLK) kn[q] = e.zt->kn[q];
   }
-  __syncthreads();
-  if (!WRITE && blockIdx.x == 0) {
+  {
     int64_t live = 0;
     for (int q = threadIdx.x; q < e.nscen; q += BLK) {
       int64_t W = window_for(METHOD, e.scen_units[q]);
-      e.stream[q].window = W;
+      if (blockIdx.x == 0) e.stream[q].window = W;
       if (W > 0) {
         int64_t end = e.chunk_base[q] + (W + ZCHUNK - 1) / ZCHUNK;
         live = end > live ? end : live;
       }
     }
     int64_t tot;
-    block_excl_scan<OpMax, BLK>(live, tot, lds);
+    block_excl_scan<OpMax, BLK>(live, tot, lds);  // synchronizes: the tables above are staged
     // a window past its scenario's chunk capacity is reported (overflow) and
     // never scanned past the table
-    if (threadIdx.x == 0) *e.chunk_live = tot < 0 ? 0 : (tot < e.nchunks ? tot : e.nchunks);
-  }
-  // the count pass visits every chunk; the write pass only the live ones
-  // (the chunk scan is exact up to index chunk_live, which holds the total)
-  const int64_t nch = WRITE ? *e.chunk_live : e.nchunks;
-  if (WRITE && blockIdx.x == 0)
-    for (int q = threadIdx.x; q <= e.nscen; q += BLK) {
-      int64_t v = q < e.nscen && e.chunk_base[q] < nch ? e.chunk_off[e.chunk_base[q]] : *e.nev;
-      e.ev_begin[q] = v < e.ev_cap ? v : e.ev_cap;  // an overflow is reported, never indexed past
+    if (threadIdx.x == 0) {
+      live_s = tot < 0 ? 0 : (tot < e.nchunks ? tot : e.nchunks);
+      if (blockIdx.x == 0) *e.chunk_live = live_s;
     }
+    __syncthreads();
+  }
+  const int64_t nch = live_s;
+  const int span_cap = e.span_cap;
   for (int64_t span = (int64_t)blockIdx.x * BLK; span < nch; span += (int64_t)gridDim.x * BLK) {
     const int64_t ch = span + threadIdx.x;
+    uint64_t m = 0, st0 = 0;
+    int64_t p0 = 0;
+    int s = 0;
     if (ch < nch) {
-      int s = scen_of(e.chunk_base, e.nscen, ch);
-      int64_t local = ch - e.chunk_base[s];
-      int64_t W = window_for(METHOD, e.scen_units[s]);
-      int64_t cap_chunks = (s + 1 < e.nscen ? e.chunk_base[s + 1] : e.nchunks) - e.chunk_base[s];
-      int64_t p0 = local * ZCHUNK;
-      int cnt = 0;
-      int64_t w = WRITE ? e.chunk_off[ch] : 0;
-      const int64_t skey = (int64_t)s << REACH_SHIFT;
+      s = scen_of(e.chunk_base, e.nscen, ch);
+      const int64_t local = ch - e.chunk_base[s];
+      const int64_t W = window_for(METHOD, e.scen_units[s]);
+      const int64_t cap_chunks = (s + 1 < e.nscen ? e.chunk_base[s + 1] : e.nchunks) - e.chunk_base[s];
+      p0 = local * ZCHUNK;
       if (p0 < W) {
-        if (!WRITE && (local + 1) * ZCHUNK < W && local + 1 == cap_chunks) atomicOr(e.overflow, 1);
+        if ((local + 1) * ZCHUNK < W && local + 1 == cap_chunks) atomicOr(e.overflow, 1);
         const StreamDev& st = e.stream[s];
         Pcg g;
         g.inc = st.inc;
+        // polar: attempt a uses positions 2a, 2a+1; p0 is even
+        g.state = st.inc == inc0 ? pcg_jump8(st.base, (uint64_t)p0, t8, e.jt[s]) : pcg_jump(st.base, (uint64_t)p0, e.jt[s]);
+        st0 = g.state;
         if (METHOD == 2) {
-          g.state = st.inc == inc0 ? pcg_jump8(st.base, (uint64_t)p0, t8, e.jt[s]) : pcg_jump(st.base, (uint64_t)p0, e.jt[s]);
-          for (int q = 0; q < ZCHUNK; ++q) {
-            uint64_t at = g.state;
-            if (zig_fast(pcg_next(g), kn)) continue;
-            if (WRITE) {
-              if (w < e.ev_cap) {
-                e.ev_pos[w] = p0 + q;
-                e.ev_state[w] = at;
-                e.ev_key[w] = skey;
-              } else {
-                atomicOr(e.overflow, 2);
-              }
-              ++w;
-            } else {
-              ++cnt;
-            }
-          }
+          for (int q = 0; q < ZCHUNK; ++q)
+            if (!zig_fast(pcg_next(g), kn)) m |= 1ull << q;
         } else {
-          // polar: attempt a uses positions 2a, 2a+1
-          int64_t a0 = p0 / 2;
-          uint64_t d = (uint64_t)(2 * a0);
-          g.state = st.inc == inc0 ? pcg_jump8(st.base, d, t8, e.jt[s]) : pcg_jump(st.base, d, e.jt[s]);
           for (int q = 0; q < ZCHUNK / 2; ++q) {
             uint32_t u1 = pcg_next(g), u2 = pcg_next(g);
-            if (polar_accept(u1, u2)) continue;
-            if (WRITE) {
-              if (w < e.ev_cap) e.ev_pos[w] = a0 + q;
-              else atomicOr(e.overflow, 2);
-              ++w;
-            } else {
-              ++cnt;
-            }
+            if (!polar_accept(u1, u2)) m |= 1ull << q;
           }
         }
       }
-      if (!WRITE) e.chunk_count[ch] = cnt;
     }
-    if (WRITE && METHOD == 2) {
-      // the span's candidates, written above by this block, in one range
-      __syncthreads();
-      int64_t c0 = e.chunk_off[span];
-      int64_t c1 = span + BLK < nch ? e.chunk_off[span + BLK] : *e.nev;
-      c0 = c0 < e.ev_cap ? c0 : e.ev_cap;
-      c1 = c1 < e.ev_cap ? c1 : e.ev_cap;
-      for (int64_t i = c0 + threadIdx.x; i < c1; i += BLK) {
+    cmask[threadIdx.x] = m;
+    cstate[threadIdx.x] = st0;
+    cpos[threadIdx.x] = p0;
+    cscen[threadIdx.x] = s;
+    int64_t total;
+    const int64_t excl = block_excl_scan<OpSum, BLK>((int64_t)__popcll(m), total, lds);
+    cbase[threadIdx.x] = (int)excl;
+    if (ch < nch) e.chunk_loc[ch] = (int)excl;
+    const int64_t sp = span / BLK;
+    const int ncand = total < span_cap ? (int)total : span_cap;
+    if (threadIdx.x == 0) {
+      e.span_cnt[sp] = ncand;
+      if (total > span_cap) atomicOr(e.overflow, 2);  // reported, never stored past the region
+    }
+    __syncthreads();
+    const int64_t region = sp * span_cap;
+    for (int ci = threadIdx.x; ci < ncand; ci += BLK) {
+      // owner: the last thread whose first candidate is at or before ci
+      int lo = 0, hi = BLK - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cbase[mid] <= ci) lo = mid;
+        else hi = mid - 1;
+      }
+      uint64_t mm = cmask[lo];
+      for (int k = ci - cbase[lo]; k > 0; --k) mm &= mm - 1;  // drop the lower candidates
+      const int q = __ffsll((unsigned long long)mm) - 1;
+      const int64_t r = region + ci;
+      if (METHOD == 2) {
+        const int so = cscen[lo];
+        const int64_t pos = cpos[lo] + q;
+        e.reg_pos[r] = pos;
+        const uint64_t inc = e.stream[so].inc;
         Draws d;
         d.zt = e.zt;
         d.pre_pos = nullptr;
-        d.g.inc = e.stream[e.ev_key[i] >> REACH_SHIFT].inc;
-        d.g.state = e.ev_state[i];
+        d.g.inc = inc;
+        d.g.state = inc == inc0 ? t8.mult[0][q] * cstate[lo] + t8.plus[0][q] : pcg_jump(cstate[lo], (uint64_t)q, e.jt[so]);
         d.ncalls = 0;
         double v;
         int32_t hz;
         if (d.zig_head(v, hz)) {
-          ev_finish(e, i, v, d.ncalls);
+          reg_finish(e, r, v, d.ncalls, pos, so);
         } else {
           int k = atomicAdd(&ntail, 1);
-          if (k < TAIL_CAP) tails[k] = TailRec{d.g.state, i, hz, (int32_t)d.ncalls};
-          else ev_finish(e, i, d.zig_tail(hz), d.ncalls);  // queue full: finish here
+          if (k < TAIL_CAP) tails[k] = TailRec{d.g.state, r, pos, hz, (int32_t)d.ncalls, so, 0};
+          else reg_finish(e, r, d.zig_tail(hz), d.ncalls, pos, so);  // queue full: finish here
         }
+      } else {
+        e.reg_pos[r] = cpos[lo] / 2 + q;
       }
     }
+    __syncthreads();  // the span's LDS entries are refilled by the next span
   }
-  if (WRITE && METHOD == 2) {
-    __syncthreads();
+  if (METHOD == 2) {
     const int nt = ntail < TAIL_CAP ? ntail : TAIL_CAP;
     for (int k = threadIdx.x; k < nt; k += BLK) {
-      const TailRec r = tails[k];
+      const TailRec t = tails[k];
       Draws d;
       d.zt = e.zt;
       d.pre_pos = nullptr;
-      d.g.inc = e.stream[e.ev_key[r.i] >> REACH_SHIFT].inc;
-      d.g.state = r.state;
-      d.ncalls = r.ncalls;
-      double v = d.zig_tail(r.hz);
-      ev_finish(e, r.i, v, d.ncalls);
+      d.g.inc = e.stream[t.scen].inc;
+      d.g.state = t.state;
+      d.ncalls = t.ncalls;
+      double v = d.zig_tail(t.hz);
+      reg_finish(e, t.i, v, d.ncalls, t.pos, t.scen);
+    }
+  }
+}
+
+// Exclusive scan of the span counts (one block, tiles of BLK * 8 spans), the
+// total number of candidates, and the first candidate of each scenario: the
+// dense index of the first candidate of its first chunk.
+__global__ void __launch_bounds__(BLK) k_ev_span_scan(EventArgs e) {
+  if (e.cond && *e.cond == 0) return;
+  __shared__ int64_t lds[BLK / 64];
+  const int64_t nch = *e.chunk_live;
+  const int64_t nsp = (nch + BLK - 1) / BLK;
+  int64_t carry = 0;
+  constexpr int IT = 8;
+  for (int64_t t0 = 0; t0 < nsp; t0 += (int64_t)BLK * IT) {
+    int64_t v[IT], run = 0;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int64_t i = t0 + (int64_t)threadIdx.x * IT + k;
+      v[k] = i < nsp ? e.span_cnt[i] : 0;
+      run += v[k];
+    }
+    int64_t total;
+    int64_t pre = block_excl_scan<OpSum, BLK>(run, total, lds);
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int64_t i = t0 + (int64_t)threadIdx.x * IT + k;
+      if (i < nsp) e.span_off[i] = carry + pre;
+      pre += v[k];
+    }
+    carry += total;
+  }
+  __syncthreads();  // span_off is complete (written by this block)
+  if (threadIdx.x == 0) *e.nev = carry;
+  for (int q = threadIdx.x; q <= e.nscen; q += BLK) {
+    int64_t b = carry;
+    if (q < e.nscen && e.chunk_base[q] < nch) {
+      const int64_t ch = e.chunk_base[q];
+      b = e.span_off[ch / BLK] + e.chunk_loc[ch];
+    }
+    e.ev_begin[q] = b < e.ev_cap ? b : e.ev_cap;  // an overflow is reported, never indexed past
+  }
+  if (threadIdx.x == 0 && carry > e.ev_cap) atomicOr(e.overflow, 2);
+}
+
+// The span regions gathered into the dense, position-ordered tables: block b
+// copies the spans b, b + grid, ...
+template <int METHOD>
+__global__ void __launch_bounds__(BLK) k_ev_gather(EventArgs e) {
+  if (e.cond && *e.cond == 0) return;
+  const int64_t nsp = (*e.chunk_live + BLK - 1) / BLK;
+  const int span_cap = e.span_cap;
+  for (int64_t sp = blockIdx.x; sp < nsp; sp += gridDim.x) {
+    const int n = e.span_cnt[sp];
+    const int64_t o = e.span_off[sp], r0 = sp * span_cap;
+    for (int i = threadIdx.x; i < n; i += BLK) {
+      const int64_t d = o + i;
+      if (d >= e.ev_cap) break;
+      e.ev_pos[d] = e.reg_pos[r0 + i];
+      if (METHOD == 2) {
+        e.ev_len[d] = e.reg_len[r0 + i];
+        e.ev_val[d] = e.reg_val[r0 + i];
+        e.ev_key[d] = e.reg_key[r0 + i];
+      }
     }
   }
 }
