@@ -169,10 +169,10 @@ struct cyl_ctx {
   int64_t* d_chunk_base = nullptr;
   int64_t nspan = 0;                               // spans of BLK chunks
   int *d_chunk_loc = nullptr, *d_span_cnt = nullptr;
-  int64_t *d_span_off = nullptr, *d_reg_pos = nullptr, *d_reg_len = nullptr, *d_reg_key = nullptr;
+  int64_t *d_span_off = nullptr, *d_span_max = nullptr, *d_span_carry = nullptr, *d_reg_pos = nullptr, *d_reg_len = nullptr, *d_reg_key = nullptr;
   double* d_reg_val = nullptr;                     // span regions of the candidate records
   int64_t *d_ev_pos = nullptr, *d_ev_len = nullptr, *d_reach = nullptr;
-  int64_t *d_bmax = nullptr, *d_E = nullptr, *d_C = nullptr, *d_key = nullptr, *d_xtra = nullptr;
+  int64_t *d_E = nullptr, *d_C = nullptr, *d_key = nullptr, *d_xtra = nullptr;
   double* d_ev_val = nullptr;                      // compacted slow-normal values
   double* d_cand_val = nullptr;                    // value per candidate
   int64_t *d_ev_begin = nullptr, *d_nev = nullptr, *d_ev_bsum = nullptr, *d_xscratch = nullptr;
@@ -1827,6 +1827,8 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(dalloc(b, &ctx->d_span_off, ctx->nspan + 1));
   HIPCHK(dalloc(b, &ctx->d_reg_pos, nreg));
   if (method == 2) {
+    HIPCHK(dalloc(b, &ctx->d_span_max, ctx->nspan + 1));
+    HIPCHK(dalloc(b, &ctx->d_span_carry, ctx->nspan + 1));
     HIPCHK(dalloc(b, &ctx->d_reg_len, nreg));
     HIPCHK(dalloc(b, &ctx->d_reg_val, nreg));
     HIPCHK(dalloc(b, &ctx->d_reg_key, nreg));
@@ -1835,7 +1837,6 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(dalloc(b, &ctx->d_ev_pos, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_len, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_reach, evcap + 1));
-  HIPCHK(dalloc(b, &ctx->d_bmax, nblk(evcap + 1) + 1));
   HIPCHK(dalloc(b, &ctx->d_E, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_C, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_key, evcap + 1));
@@ -1895,6 +1896,8 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   e.chunk_loc = ctx->d_chunk_loc;
   e.span_cnt = ctx->d_span_cnt;
   e.span_off = ctx->d_span_off;
+  e.span_max = ctx->d_span_max;
+  e.span_carry = ctx->d_span_carry;
   e.nspan = ctx->nspan;
   e.span_cap = method == 2 ? EV_SPAN_CAP_ZIG : EV_SPAN_CAP_POLAR;
   e.reg_pos = ctx->d_reg_pos;
@@ -1906,7 +1909,7 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   e.ev_pos = ctx->d_ev_pos;
   e.ev_len = ctx->d_ev_len;
   e.ev_val = ctx->d_cand_val;
-  e.ev_key = ctx->d_xscratch;
+  e.reach = ctx->d_reach;
   e.chunk_live = ctx->d_chunk_live;
   e.ev_cap = ctx->ev_cap;
   e.overflow = ctx->d_flag + 2;
@@ -1921,11 +1924,11 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   const unsigned ngat = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ctx->nspan, 4LL * ctx->ncu));
   if (method == 2) {
     k_ev_write<2><<<pgrid((const void*)k_ev_write<2>), BLK, 0, s>>>(e);
-    k_ev_span_scan<<<1, BLK, 0, s>>>(e);
+    k_ev_span_scan<2><<<1, BLK, 0, s>>>(e);
     k_ev_gather<2><<<ngat, BLK, 0, s>>>(e);
   } else {
     k_ev_write<3><<<pgrid((const void*)k_ev_write<3>), BLK, 0, s>>>(e);
-    k_ev_span_scan<<<1, BLK, 0, s>>>(e);
+    k_ev_span_scan<3><<<1, BLK, 0, s>>>(e);
     k_ev_gather<3><<<ngat, BLK, 0, s>>>(e);
   }
   launches += 3;
@@ -1934,16 +1937,13 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
   // fewer than the capacity: 4 blocks per CU
   unsigned nb = std::max(std::min<unsigned>(nblk(cap), 4u * (unsigned)ctx->ncu), nblk(ctx->nscen));
   if (method == 2) {
-    k_scan_tiles<int64_t, OpMax><<<scan_grid(ctx, cap), BLK, 0, s>>>(ctx->d_xscratch, ctx->d_reach, ctx->d_bmax,
-                                                                     cap, ctx->d_nev, cond);
-    k_scan_top<OpMax><<<1, 1024, 0, s>>>(ctx->d_bmax, cap, ctx->d_nev, nullptr, cond);
-    k_ev_resolve<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->nscen, ctx->d_nev,
-                                    ctx->d_reach, ctx->d_bmax, ctx->d_E, cond);
+    k_ev_resolve<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->nscen, ctx->d_reach,
+                                    ctx->d_E, cond);
     enqueue_scan(ctx, ctx->d_E, ctx->d_C, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, ctx->d_nev, cond, launches);
     k_ev_compact<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos,
                                     ctx->d_ev_len, ctx->d_cand_val, ctx->d_E, ctx->d_C, ctx->d_key, ctx->d_xtra,
                                     ctx->d_ev_val, ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);
-    launches += 4;
+    launches += 2;
   } else {
     k_ev_polar<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos, ctx->d_key,
                                   ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);

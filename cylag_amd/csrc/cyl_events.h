@@ -18,9 +18,10 @@
 //
 // Pipeline (rng_method 2): one pass over the window (each block finds the
 // candidates of its span of chunks, compacts them in LDS and samples them into
-// the span's region) -> scan of the span counts -> gather into dense,
-// position-ordered tables -> reach max-scan -> resolve -> (extra, count) scan
-// -> compact -> buckets.
+// the span's region, and reduces each span's largest reach key) -> scan of
+// the span counts and maxima -> gather into dense, position-ordered tables
+// with the exclusive max-scan of the reach keys -> resolve -> (extra, count)
+// scan -> compact -> buckets.
 #pragma once
 #include "cyl_kernels.h"
 
@@ -42,6 +43,8 @@ struct EventArgs {
   int* chunk_loc;             // per chunk: its first candidate's offset in its span
   int* span_cnt;              // per span: candidates (at most span_cap)
   int64_t* span_off;          // per span: first dense index (scan of span_cnt)
+  int64_t* span_max;          // per span (rng_method 2): largest reach key, -1 if none
+  int64_t* span_carry;        // per span: exclusive max-scan of span_max
   int64_t nspan;              // spans of the chunk capacity
   int span_cap;               // candidate records per span region
   int64_t* reg_pos;           // span regions (nspan * span_cap): position, and for rng_method 2
@@ -53,7 +56,7 @@ struct EventArgs {
   int64_t* ev_pos;            // dense tables: candidate position (relative to the scenario's window start)
   int64_t* ev_len;            // length of the normal starting at the candidate (zig)
   double* ev_val;             // its value (zig)
-  int64_t* ev_key;            // its reach key (zig)
+  int64_t* reach;             // exclusive max-scan of the reach keys (zig)
   int64_t* chunk_live;        // end of the live chunks
   int64_t ev_cap;
   int* overflow;
@@ -100,7 +103,8 @@ __device__ __forceinline__ bool zig_fast(uint32_t u, const uint32_t* kn) {
 // them, one candidate per thread: the generator state there is one
 // byte-window jump from its chunk's start state (kept in LDS), and the value
 // and length of the Ziggurat normal starting there and its reach key
-// (scenario << REACH_SHIFT) | (pos + len - 1) are stored with it.  A
+// (scenario << REACH_SHIFT) | (pos + len - 1) are stored with it, and the
+// block stores the largest key of each of its spans at the end.  A
 // candidate whose normal reaches the base strip's tail loop (two correctly
 // rounded logarithms per attempt; a few percent of candidates) is queued in
 // LDS and finished by the block at the end, so that the waves sampling the
@@ -265,26 +269,45 @@ __global__ void __launch_bounds__(BLK) k_ev_write(EventArgs e) {
       double v = d.zig_tail(t.hz);
       reg_finish(e, t.i, v, d.ncalls, t.pos, t.scen);
     }
+    __syncthreads();  // every key of the block's spans is stored
+    for (int64_t span = (int64_t)blockIdx.x * BLK; span < nch; span += (int64_t)gridDim.x * BLK) {
+      const int64_t sp = span / BLK;
+      const int n = e.span_cnt[sp];
+      int64_t m = OpMax::ident;
+      for (int i = threadIdx.x; i < n; i += BLK) {
+        const int64_t k = e.reg_key[sp * span_cap + i];
+        m = k > m ? k : m;
+      }
+      int64_t mx;
+      block_excl_scan<OpMax, BLK>(m, mx, lds);
+      if (threadIdx.x == 0) e.span_max[sp] = mx;
+    }
   }
 }
 
-// Exclusive scan of the span counts (one block, tiles of BLK * 8 spans), the
-// total number of candidates, and the first candidate of each scenario: the
-// dense index of the first candidate of its first chunk.
+// Exclusive scans of the span counts and (rng_method 2) of the span maxima
+// (one block, tiles of BLK * 8 spans), the total number of candidates, and the
+// first candidate of each scenario: the dense index of the first candidate of
+// its first chunk.
+template <int METHOD>
 __global__ void __launch_bounds__(BLK) k_ev_span_scan(EventArgs e) {
   if (e.cond && *e.cond == 0) return;
   __shared__ int64_t lds[BLK / 64];
   const int64_t nch = *e.chunk_live;
   const int64_t nsp = (nch + BLK - 1) / BLK;
-  int64_t carry = 0;
+  int64_t carry = 0, mcarry = OpMax::ident;
   constexpr int IT = 8;
   for (int64_t t0 = 0; t0 < nsp; t0 += (int64_t)BLK * IT) {
-    int64_t v[IT], run = 0;
+    int64_t v[IT], mv[IT], run = 0, mrun = OpMax::ident;
 #pragma unroll
     for (int k = 0; k < IT; ++k) {
       const int64_t i = t0 + (int64_t)threadIdx.x * IT + k;
       v[k] = i < nsp ? e.span_cnt[i] : 0;
       run += v[k];
+      if (METHOD == 2) {
+        mv[k] = i < nsp ? e.span_max[i] : OpMax::ident;
+        mrun = mv[k] > mrun ? mv[k] : mrun;
+      }
     }
     int64_t total;
     int64_t pre = block_excl_scan<OpSum, BLK>(run, total, lds);
@@ -295,6 +318,18 @@ __global__ void __launch_bounds__(BLK) k_ev_span_scan(EventArgs e) {
       pre += v[k];
     }
     carry += total;
+    if (METHOD == 2) {
+      int64_t mtot;
+      int64_t mpre = block_excl_scan<OpMax, BLK>(mrun, mtot, lds);
+      mpre = mpre > mcarry ? mpre : mcarry;
+#pragma unroll
+      for (int k = 0; k < IT; ++k) {
+        const int64_t i = t0 + (int64_t)threadIdx.x * IT + k;
+        if (i < nsp) e.span_carry[i] = mpre;
+        mpre = mv[k] > mpre ? mv[k] : mpre;
+      }
+      mcarry = mtot > mcarry ? mtot : mcarry;
+    }
   }
   __syncthreads();  // span_off is complete (written by this block)
   if (threadIdx.x == 0) *e.nev = carry;
@@ -310,10 +345,13 @@ __global__ void __launch_bounds__(BLK) k_ev_span_scan(EventArgs e) {
 }
 
 // The span regions gathered into the dense, position-ordered tables: block b
-// copies the spans b, b + grid, ...
+// copies the spans b, b + grid, ...  Under rng_method 2 it also writes the
+// exclusive max-scan of the reach keys: the span's carry, then a block scan of
+// the span's keys (EV_SPAN_CAP_ZIG / BLK consecutive keys per thread).
 template <int METHOD>
 __global__ void __launch_bounds__(BLK) k_ev_gather(EventArgs e) {
   if (e.cond && *e.cond == 0) return;
+  __shared__ int64_t lds[BLK / 64];
   const int64_t nsp = (*e.chunk_live + BLK - 1) / BLK;
   const int span_cap = e.span_cap;
   for (int64_t sp = blockIdx.x; sp < nsp; sp += gridDim.x) {
@@ -326,7 +364,26 @@ __global__ void __launch_bounds__(BLK) k_ev_gather(EventArgs e) {
       if (METHOD == 2) {
         e.ev_len[d] = e.reg_len[r0 + i];
         e.ev_val[d] = e.reg_val[r0 + i];
-        e.ev_key[d] = e.reg_key[r0 + i];
+      }
+    }
+    if (METHOD == 2) {
+      constexpr int PT = EV_SPAN_CAP_ZIG / BLK;
+      int64_t k[PT], run = OpMax::ident;
+#pragma unroll
+      for (int q = 0; q < PT; ++q) {
+        const int i = threadIdx.x * PT + q;
+        k[q] = i < n ? e.reg_key[r0 + i] : OpMax::ident;
+        run = k[q] > run ? k[q] : run;
+      }
+      int64_t tot;
+      int64_t m = block_excl_scan<OpMax, BLK>(run, tot, lds);
+      const int64_t c = e.span_carry[sp];
+      m = m > c ? m : c;
+#pragma unroll
+      for (int q = 0; q < PT; ++q) {
+        const int i = threadIdx.x * PT + q;
+        if (i < n && o + i < e.ev_cap) e.reach[o + i] = m;
+        m = k[q] > m ? k[q] : m;
       }
     }
   }
@@ -335,20 +392,19 @@ __global__ void __launch_bounds__(BLK) k_ev_gather(EventArgs e) {
 // Ziggurat chain resolution.  Candidate p is off the chain iff an on-chain
 // slow start q < p has q + len(q) - 1 >= p.  A candidate that no earlier
 // candidate reaches is surely on-chain; from there the chain is followed
-// forward.  reach (exclusive max-scan of the keys, tile prefix in tmax) gives
-// the furthest position reached by earlier candidates of the same scenario.
+// forward.  reach (exclusive max-scan of the keys) gives the furthest
+// position reached by earlier candidates of the same scenario.
 // Writes (len - 1) << 32 | 1 for on-chain candidates, 0 otherwise.
 __global__ void __launch_bounds__(BLK) k_ev_resolve(const int64_t* pos, const int64_t* len, const int64_t* ev_begin,
-                                                    int nscen, const int64_t* nev, const int64_t* reach,
-                                                    const int64_t* tmax, int64_t* pk, const int* cond) {
+                                                    int nscen, const int64_t* reach, int64_t* pk,
+                                                    const int* cond) {
   if (cond && *cond == 0) return;
   const int64_t ne = ev_begin[nscen];  // candidates, clamped to the capacity
   for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < ne; i += (int64_t)gridDim.x * BLK) {
     int s = scen_of(ev_begin, nscen, i);
     int64_t b = ev_begin[s];
     auto reach_of = [&](int64_t j) -> int64_t {
-      int64_t m = reach[j], t = tmax[j / SCAN_TILE];
-      m = t > m ? t : m;
+      const int64_t m = reach[j];
       return (m >= 0 && (m >> REACH_SHIFT) == s) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
     };
     int64_t j0 = i;
