@@ -169,13 +169,15 @@ struct cyl_ctx {
   int64_t* d_chunk_base = nullptr;
   int64_t nspan = 0;                               // spans of BLK chunks
   int *d_chunk_loc = nullptr, *d_span_cnt = nullptr;
-  int64_t *d_span_off = nullptr, *d_span_max = nullptr, *d_span_carry = nullptr, *d_reg_pos = nullptr, *d_reg_len = nullptr, *d_reg_key = nullptr;
+  int64_t *d_span_off = nullptr, *d_span_max = nullptr, *d_span_carry = nullptr;
+  int64_t *d_span_pk = nullptr, *d_span_pkoff = nullptr, *d_scen_c = nullptr, *d_scen_c0 = nullptr;
+  int64_t *d_reg_pos = nullptr, *d_reg_len = nullptr, *d_reg_key = nullptr;
   double* d_reg_val = nullptr;                     // span regions of the candidate records
   int64_t *d_ev_pos = nullptr, *d_ev_len = nullptr, *d_reach = nullptr;
-  int64_t *d_E = nullptr, *d_C = nullptr, *d_key = nullptr, *d_xtra = nullptr;
+  int64_t *d_E = nullptr, *d_key = nullptr, *d_xtra = nullptr;
   double* d_ev_val = nullptr;                      // compacted slow-normal values
   double* d_cand_val = nullptr;                    // value per candidate
-  int64_t *d_ev_begin = nullptr, *d_nev = nullptr, *d_ev_bsum = nullptr, *d_xscratch = nullptr;
+  int64_t *d_ev_begin = nullptr, *d_nev = nullptr;
   int64_t* d_chunk_live = nullptr;                 // end of the step's live chunks
   int* d_bucket = nullptr;          // bucket index of the event keys, all scenarios
   int64_t* d_bucket_base = nullptr;
@@ -1790,7 +1792,6 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   ctx->ev_bufs.free_all();
   ctx->ev_units_cap = need;
   int ns = ctx->nscen;
-  std::vector<int64_t> cb(ns);
   int64_t nch = 0;
   // chunk capacity per scenario, proportional to its pool size
   std::vector<int64_t> cap_units(ns);
@@ -1800,7 +1801,6 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   }
   int64_t evcap = 0;
   for (int s = 0; s < ns; ++s) {
-    cb[s] = nch;
     int64_t u = cap_units[s];
     int64_t w = method == 2 ? u + u / 8 + 4096 : 2 * (u + u / 2 + 4096);
     nch += (w + ZCHUNK - 1) / ZCHUNK + 1;
@@ -1818,8 +1818,7 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(dalloc(b, &ctx->d_bucket, ctx->nbucket));
   HIPCHK(dalloc(b, &ctx->d_bucket_base, ns + 1));
   HIPCHK(hipMemcpy(ctx->d_bucket_base, bb.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
-  HIPCHK(dalloc(b, &ctx->d_chunk_base, ns));
-  HIPCHK(hipMemcpy(ctx->d_chunk_base, cb.data(), ns * sizeof(int64_t), hipMemcpyHostToDevice));
+  HIPCHK(dalloc(b, &ctx->d_chunk_base, ns + 1));  // the step's layout (ev_layout)
   ctx->nspan = (nch + BLK - 1) / BLK;
   const int64_t nreg = ctx->nspan * (method == 2 ? EV_SPAN_CAP_ZIG : EV_SPAN_CAP_POLAR);
   HIPCHK(dalloc(b, &ctx->d_chunk_loc, nch + 1));
@@ -1829,21 +1828,22 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   if (method == 2) {
     HIPCHK(dalloc(b, &ctx->d_span_max, ctx->nspan + 1));
     HIPCHK(dalloc(b, &ctx->d_span_carry, ctx->nspan + 1));
+    HIPCHK(dalloc(b, &ctx->d_span_pk, ctx->nspan + 1));
+    HIPCHK(dalloc(b, &ctx->d_span_pkoff, ctx->nspan + 1));
+    HIPCHK(dalloc(b, &ctx->d_scen_c, ns + 1));
+    HIPCHK(dalloc(b, &ctx->d_scen_c0, ns + 1));
     HIPCHK(dalloc(b, &ctx->d_reg_len, nreg));
     HIPCHK(dalloc(b, &ctx->d_reg_val, nreg));
     HIPCHK(dalloc(b, &ctx->d_reg_key, nreg));
   }
-  HIPCHK(dalloc(b, &ctx->d_ev_bsum, nblk(evcap + 1) + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_pos, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_len, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_reach, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_E, evcap + 1));
-  HIPCHK(dalloc(b, &ctx->d_C, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_key, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_xtra, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_val, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_cand_val, evcap + 1));
-  HIPCHK(dalloc(b, &ctx->d_xscratch, evcap + 1));
   HIPCHK(dalloc(b, &ctx->d_ev_begin, ns + 1));
   HIPCHK(dalloc(b, &ctx->d_nev, 1));
   HIPCHK(dalloc(b, &ctx->d_chunk_live, 1));
@@ -1882,14 +1882,20 @@ static void enqueue_rank(cyl_ctx* ctx, const int* cond, int& launches) {
 }
 
 // event tables of the step's stream window (rng_method 2 and 3)
-static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launches) {
+// layout: lay the chunks out here (the step's preparation has not: the units
+// come from a counting run)
+static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, bool layout, int& launches) {
   if (method == 1) return;
+  if (layout) {
+    k_ev_layout<<<1, BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_units, ctx->nscen, method, ctx->nchunks,
+                                             ctx->d_chunk_base, ctx->d_chunk_live, ctx->d_flag + 2, cond);
+    ++launches;
+  }
   EventArgs e{};
   e.stream = ctx->d_stream;
   e.jt = ctx->d_jt;
   e.jt8 = ctx->d_jt8;
   e.zt = ctx->d_zt;
-  e.scen_units = ctx->d_scen_units;
   e.chunk_base = ctx->d_chunk_base;
   e.nscen = ctx->nscen;
   e.nchunks = ctx->nchunks;
@@ -1932,25 +1938,46 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, int& launc
     k_ev_gather<3><<<ngat, BLK, 0, s>>>(e);
   }
   launches += 3;
-  int64_t cap = ctx->ev_cap;
   // the candidate kernels loop over the step's candidates (device count), far
   // fewer than the capacity: 4 blocks per CU
-  unsigned nb = std::max(std::min<unsigned>(nblk(cap), 4u * (unsigned)ctx->ncu), nblk(ctx->nscen));
+  unsigned nb = std::max(std::min<unsigned>(nblk(ctx->ev_cap), 4u * (unsigned)ctx->ncu), nblk(ctx->nscen));
   if (method == 2) {
-    k_ev_resolve<<<nb, BLK, 0, s>>>(ctx->d_ev_pos, ctx->d_ev_len, ctx->d_ev_begin, ctx->nscen, ctx->d_reach,
-                                    ctx->d_E, cond);
-    enqueue_scan(ctx, ctx->d_E, ctx->d_C, cap + 1, ctx->d_ev_bsum, ctx->d_total + 1, ctx->d_nev, cond, launches);
-    k_ev_compact<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos,
-                                    ctx->d_ev_len, ctx->d_cand_val, ctx->d_E, ctx->d_C, ctx->d_key, ctx->d_xtra,
-                                    ctx->d_ev_val, ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);
-    launches += 2;
+    ChainArgs ca{};
+    ca.pos = ctx->d_ev_pos;
+    ca.len = ctx->d_ev_len;
+    ca.val = ctx->d_cand_val;
+    ca.reach = ctx->d_reach;
+    ca.ev_begin = ctx->d_ev_begin;
+    ca.span_cnt = ctx->d_span_cnt;
+    ca.span_off = ctx->d_span_off;
+    ca.chunk_live = ctx->d_chunk_live;
+    ca.nscen = ctx->nscen;
+    ca.ev_cap = ctx->ev_cap;
+    ca.pk = ctx->d_E;
+    ca.span_pk = ctx->d_span_pk;
+    ca.span_pkoff = ctx->d_span_pkoff;
+    ca.scen_c = ctx->d_scen_c;
+    ca.scen_c0 = ctx->d_scen_c0;
+    ca.st = ctx->d_stream;
+    ca.key = ctx->d_key;
+    ca.xtra = ctx->d_xtra;
+    ca.cval = ctx->d_ev_val;
+    ca.bucket = ctx->d_bucket;
+    ca.bucket_base = ctx->d_bucket_base;
+    ca.bshift = ctx->bshift;
+    ca.cond = cond;
+    k_ev_resolve<<<ngat, BLK, 0, s>>>(ca);
+    k_ev_chain_scan<<<1, BLK, 0, s>>>(ca);
+    k_ev_compact<<<ngat, BLK, 0, s>>>(ca);
+    launches += 3;
   } else {
     k_ev_polar<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos, ctx->d_key,
                                   ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);
     launches += 1;
   }
   const unsigned nbb = std::min<unsigned>(nblk(ctx->nbucket), 8u * (unsigned)ctx->ncu);
-  k_ev_buckets<<<nbb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_bucket_base, ctx->nbucket, ctx->d_bucket, cond);
+  k_ev_buckets<<<nbb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_bucket_base, ctx->nbucket,
+                                   method == 2 ? ctx->d_scen_c0 : ctx->d_ev_begin, ctx->d_key, ctx->d_bucket, cond);
   launches += 1;
 }
 
@@ -2560,12 +2587,20 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       pa.uniform = uniform ? 1 : 0;
       pa.pregen = ctx->d_pregen;
       pa.chunk_base = uniform && any_fast && ctx->inline_normals == 0 ? ctx->d_gen_base : nullptr;
+      if (uniform && method != 1) {
+        pa.ev_st = ctx->d_stream;
+        pa.ev_method = method;
+        pa.ev_nchunks = ctx->nchunks;
+        pa.ev_base = ctx->d_chunk_base;
+        pa.ev_live = ctx->d_chunk_live;
+        pa.ev_overflow = ctx->d_flag + 2;
+      }
       k_step_prep<<<1, BLK, 0, ctx->stream>>>(pa);
       ++launches;
     }
     if (step == 0 && !cap) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     if (uniform) {
-      enqueue_events(ctx, method, one, launches);
+      enqueue_events(ctx, method, one, false, launches);
       a.general = 0;
       a.guarded = 0;
       const int ti = n_timed;  // event slots of the timed (eager) steps
@@ -2629,7 +2664,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
     k_scen_totals<<<nblk(ns), BLK, 0, ctx->stream>>>(ctx->d_scan, ctx->d_total, ctx->d_scen_base, ns, ctx->n,
                                                      ctx->d_scen_units, g);
     launches += 4;
-    enqueue_events(ctx, method, g, launches);
+    enqueue_events(ctx, method, g, true, launches);
     a.general = 1;
     a.guarded = 1;
     for (auto& c : cls) {

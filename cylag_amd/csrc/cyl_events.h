@@ -16,12 +16,14 @@
 // Draws.events_before / zig_position / polar_position then map a unit to its
 // position, and the normal generator replays runs of the stream from them.
 //
-// Pipeline (rng_method 2): one pass over the window (each block finds the
+// Pipeline (rng_method 2): chunk layout of the step's windows (ev_layout, in
+// the step's preparation) -> one pass over the windows (each block finds the
 // candidates of its span of chunks, compacts them in LDS and samples them into
 // the span's region, and reduces each span's largest reach key) -> scan of
 // the span counts and maxima -> gather into dense, position-ordered tables
-// with the exclusive max-scan of the reach keys -> resolve -> (extra, count)
-// scan -> compact -> buckets.
+// with the exclusive max-scan of the reach keys -> resolve with an (extra,
+// count) scan within each span -> scan of the span totals -> compact ->
+// buckets.
 #pragma once
 #include "cyl_kernels.h"
 
@@ -36,8 +38,7 @@ struct EventArgs {
   const JumpTable* jt;        // [nscen]
   const JumpTable8* jt8;      // [nscen]
   const ZigTables* zt;
-  const int64_t* scen_units;  // units needed per scenario this step
-  const int64_t* chunk_base;  // first chunk of each scenario (host-sized capacity)
+  const int64_t* chunk_base;  // [nscen + 1] first chunk of each scenario's window this step (ev_layout)
   int nscen;
   int64_t nchunks;
   int* chunk_loc;             // per chunk: its first candidate's offset in its span
@@ -57,7 +58,7 @@ struct EventArgs {
   int64_t* ev_len;            // length of the normal starting at the candidate (zig)
   double* ev_val;             // its value (zig)
   int64_t* reach;             // exclusive max-scan of the reach keys (zig)
-  int64_t* chunk_live;        // end of the live chunks
+  const int64_t* chunk_live;  // chunks of the step's windows (ev_layout)
   int64_t ev_cap;
   int* overflow;
   const int* cond;            // run only when *cond != 0
@@ -87,6 +88,42 @@ __device__ __forceinline__ int64_t window_for(int method, int64_t units) {
   return 2 * (units + units / 2 + 4096);  // polar: attempts of 2 positions, ~21% rejected
 }
 
+// Chunk layout of the step's stream windows (one block): scenario q's window
+// (window_for its units, stored in its stream) takes ceil(W / ZCHUNK)
+// chunks from base[q], an exclusive scan over the scenarios; base[nscen] and
+// *live receive the total.  A layout past the table's nchunks is reported
+// (overflow) and *live clamped to the table.
+__device__ void ev_layout(StreamDev* st, const int64_t* units, int nscen, int method, int64_t nchunks,
+                          int64_t* base, int64_t* live, int* overflow, int64_t* lds) {
+  int64_t carry = 0;
+  for (int t0 = 0; t0 < nscen; t0 += BLK) {
+    const int q = t0 + threadIdx.x;
+    int64_t c = 0;
+    if (q < nscen) {
+      const int64_t W = window_for(method, units[q]);
+      st[q].window = W;
+      c = (W + ZCHUNK - 1) / ZCHUNK;
+    }
+    int64_t tot;
+    const int64_t pre = block_excl_scan<OpSum, BLK>(c, tot, lds);
+    if (q < nscen) base[q] = carry + pre;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) {
+    base[nscen] = carry;
+    if (carry > nchunks) atomicOr(overflow, 1);
+    *live = carry < nchunks ? carry : nchunks;
+  }
+}
+
+__global__ void __launch_bounds__(BLK) k_ev_layout(StreamDev* st, const int64_t* units, int nscen, int method,
+                                                   int64_t nchunks, int64_t* base, int64_t* live, int* overflow,
+                                                   const int* cond) {
+  if (cond && *cond == 0) return;
+  __shared__ int64_t lds[BLK / 64];
+  ev_layout(st, units, nscen, method, nchunks, base, live, overflow, lds);
+}
+
 __device__ __forceinline__ bool zig_fast(uint32_t u, const uint32_t* kn) {
   int32_t hz = (int32_t)u;
   uint32_t iz = (uint32_t)hz & 127u;
@@ -108,9 +145,8 @@ __device__ __forceinline__ bool zig_fast(uint32_t u, const uint32_t* kn) {
 // candidate whose normal reaches the base strip's tail loop (two correctly
 // rounded logarithms per attempt; a few percent of candidates) is queued in
 // LDS and finished by the block at the end, so that the waves sampling the
-// other candidates do not wait on it.  Every block bounds its spans by the end
-// of the live chunks (no chunk at or past it holds a position of any window);
-// block 0 stores it and each scenario's window.
+// other candidates do not wait on it.  The chunks are those of the step's
+// layout (ev_layout).
 constexpr int TAIL_CAP = 256;
 struct TailRec {
   uint64_t state;
@@ -136,7 +172,6 @@ __global__ void __launch_bounds__(BLK) k_ev_write(EventArgs e) {
   __shared__ int cscen[BLK], cbase[BLK];
   __shared__ TailRec tails[METHOD == 2 ? TAIL_CAP : 1];
   __shared__ int ntail;
-  __shared__ int64_t live_s;
   if (threadIdx.x == 0) ntail = 0;
   const int s0 = scen_of(e.chunk_base, e.nscen, (int64_t)blockIdx.x * BLK);
   const uint64_t inc0 = e.stream[s0].inc;
@@ -150,27 +185,8 @@ __global__ void __launch_bounds__(BLK) k_ev_write(EventArgs e) {
     if (METHOD == 2)
       for (int q = threadIdx.x; q < 128; q += BLK) kn[q] = e.zt->kn[q];
   }
-  {
-    int64_t live = 0;
-    for (int q = threadIdx.x; q < e.nscen; q += BLK) {
-      int64_t W = window_for(METHOD, e.scen_units[q]);
-      if (blockIdx.x == 0) e.stream[q].window = W;
-      if (W > 0) {
-        int64_t end = e.chunk_base[q] + (W + ZCHUNK - 1) / ZCHUNK;
-        live = end > live ? end : live;
-      }
-    }
-    int64_t tot;
-    block_excl_scan<OpMax, BLK>(live, tot, lds);  // synchronizes: the tables above are staged
-    // a window past its scenario's chunk capacity is reported (overflow) and
-    // never scanned past the table
-    if (threadIdx.x == 0) {
-      live_s = tot < 0 ? 0 : (tot < e.nchunks ? tot : e.nchunks);
-      if (blockIdx.x == 0) *e.chunk_live = live_s;
-    }
-    __syncthreads();
-  }
-  const int64_t nch = live_s;
+  __syncthreads();  // the tables above are staged
+  const int64_t nch = *e.chunk_live;
   const int span_cap = e.span_cap;
   for (int64_t span = (int64_t)blockIdx.x * BLK; span < nch; span += (int64_t)gridDim.x * BLK) {
     const int64_t ch = span + threadIdx.x;
@@ -180,12 +196,9 @@ __global__ void __launch_bounds__(BLK) k_ev_write(EventArgs e) {
     if (ch < nch) {
       s = scen_of(e.chunk_base, e.nscen, ch);
       const int64_t local = ch - e.chunk_base[s];
-      const int64_t W = window_for(METHOD, e.scen_units[s]);
-      const int64_t cap_chunks = (s + 1 < e.nscen ? e.chunk_base[s + 1] : e.nchunks) - e.chunk_base[s];
+      const StreamDev& st = e.stream[s];
       p0 = local * ZCHUNK;
-      if (p0 < W) {
-        if ((local + 1) * ZCHUNK < W && local + 1 == cap_chunks) atomicOr(e.overflow, 1);
-        const StreamDev& st = e.stream[s];
+      if (p0 < st.window) {
         Pcg g;
         g.inc = st.inc;
         // polar: attempt a uses positions 2a, 2a+1; p0 is even
@@ -389,70 +402,178 @@ __global__ void __launch_bounds__(BLK) k_ev_gather(EventArgs e) {
   }
 }
 
-// Ziggurat chain resolution.  Candidate p is off the chain iff an on-chain
-// slow start q < p has q + len(q) - 1 >= p.  A candidate that no earlier
-// candidate reaches is surely on-chain; from there the chain is followed
-// forward.  reach (exclusive max-scan of the keys) gives the furthest
-// position reached by earlier candidates of the same scenario.
-// Writes (len - 1) << 32 | 1 for on-chain candidates, 0 otherwise.
-__global__ void __launch_bounds__(BLK) k_ev_resolve(const int64_t* pos, const int64_t* len, const int64_t* ev_begin,
-                                                    int nscen, const int64_t* reach, int64_t* pk,
-                                                    const int* cond) {
-  if (cond && *cond == 0) return;
-  const int64_t ne = ev_begin[nscen];  // candidates, clamped to the capacity
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < ne; i += (int64_t)gridDim.x * BLK) {
-    int s = scen_of(ev_begin, nscen, i);
-    int64_t b = ev_begin[s];
-    auto reach_of = [&](int64_t j) -> int64_t {
-      const int64_t m = reach[j];
-      return (m >= 0 && (m >> REACH_SHIFT) == s) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
-    };
-    int64_t j0 = i;
-    while (j0 > b && reach_of(j0) >= pos[j0]) --j0;
-    int64_t cur_end = pos[j0] + len[j0] - 1;
-    int on = 1;
-    for (int64_t l = j0 + 1; l <= i; ++l) {
-      if (pos[l] > cur_end) {
-        cur_end = pos[l] + len[l] - 1;
-        on = 1;
-      } else {
-        on = 0;
+// Ziggurat chain resolution, one span of the dense tables per block.
+// Candidate p is off the chain iff an on-chain slow start q < p has
+// q + len(q) - 1 >= p.  A candidate that no earlier candidate reaches is
+// surely on-chain; from there the chain is followed forward.  reach (exclusive
+// max-scan of the keys) gives the furthest position reached by earlier
+// candidates of the same scenario.  The packed value (len - 1) << 32 | 1 of an
+// on-chain candidate (0 otherwise) is scanned within the span, EV_SPAN_CAP_ZIG
+// / BLK consecutive candidates per thread: pk receives the exclusive prefix
+// within the span (ON_CHAIN set for an on-chain candidate) and span_pk the
+// span's total.
+constexpr int64_t ON_CHAIN = (int64_t)1 << 62;  // flag bit of the per-candidate prefix
+
+struct ChainArgs {
+  const int64_t* pos;
+  const int64_t* len;
+  const double* val;
+  const int64_t* reach;
+  const int64_t* ev_begin;  // [nscen + 1], the last entry clamped to the capacity
+  const int* span_cnt;
+  const int64_t* span_off;
+  const int64_t* chunk_live;
+  int nscen;
+  int64_t ev_cap;
+  int64_t* pk;              // per candidate: exclusive prefix within its span
+  int64_t* span_pk;         // per span: total
+  int64_t* span_pkoff;      // per span: exclusive scan of span_pk
+  int64_t* scen_c;          // [nscen + 1]: the exclusive prefix at each scenario's first candidate
+  int64_t* scen_c0;         // [nscen + 1]: its count half (first compacted event of the scenario)
+  StreamDev* st;
+  int64_t* key;             // compacted events: normal index
+  int64_t* xtra;            //   cumulative extra positions (inclusive)
+  double* cval;             //   value
+  const int* bucket;
+  const int64_t* bucket_base;
+  int bshift;
+  const int* cond;
+};
+
+__device__ __forceinline__ int64_t span_items(const ChainArgs& a, int64_t sp, int64_t& o) {
+  const int64_t n = a.span_cnt[sp];
+  o = a.span_off[sp];
+  const int64_t ne = a.ev_begin[a.nscen];
+  return o >= ne ? 0 : (o + n <= ne ? n : ne - o);
+}
+
+__global__ void __launch_bounds__(BLK) k_ev_resolve(ChainArgs a) {
+  if (a.cond && *a.cond == 0) return;
+  __shared__ int64_t lds[BLK / 64];
+  constexpr int PT = EV_SPAN_CAP_ZIG / BLK;
+  const int64_t nsp = (*a.chunk_live + BLK - 1) / BLK;
+  for (int64_t sp = blockIdx.x; sp < nsp; sp += gridDim.x) {
+    int64_t o;
+    const int64_t n = span_items(a, sp, o);
+    int64_t v[PT], run = 0;
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      v[q] = 0;
+      const int64_t li = (int64_t)threadIdx.x * PT + q;
+      if (li >= n) continue;
+      const int64_t i = o + li;
+      const int s = scen_of(a.ev_begin, a.nscen, i);
+      const int64_t b = a.ev_begin[s];
+      auto reach_of = [&](int64_t j) -> int64_t {
+        const int64_t m = a.reach[j];
+        return (m >= 0 && (m >> REACH_SHIFT) == s) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
+      };
+      int64_t j0 = i;
+      while (j0 > b && reach_of(j0) >= a.pos[j0]) --j0;
+      int64_t cur_end = a.pos[j0] + a.len[j0] - 1;
+      int on = 1;
+      for (int64_t l = j0 + 1; l <= i; ++l) {
+        if (a.pos[l] > cur_end) {
+          cur_end = a.pos[l] + a.len[l] - 1;
+          on = 1;
+        } else {
+          on = 0;
+        }
       }
+      v[q] = on ? ((a.len[i] - 1) << 32) | 1 : 0;
+      run += v[q];
     }
-    pk[i] = on ? ((len[i] - 1) << 32) | 1 : 0;
+    int64_t tot;
+    int64_t pre = block_excl_scan<OpSum, BLK>(run, tot, lds);
+#pragma unroll
+    for (int q = 0; q < PT; ++q) {
+      const int64_t li = (int64_t)threadIdx.x * PT + q;
+      if (li < n) a.pk[o + li] = pre | (v[q] & 1 ? ON_CHAIN : 0);
+      pre += v[q];
+    }
+    if (threadIdx.x == 0) a.span_pk[sp] = tot;
   }
 }
 
-// Point each scenario's stream at its slice of the event tables (threads
-// below nscen) and compact the on-chain candidates: normal index, cumulative
-// extra positions (inclusive) and value.  ec = exclusive scan of pk.
-__global__ void __launch_bounds__(BLK) k_ev_compact(StreamDev* st, int nscen, const int64_t* ev_begin,
-                                                    const int64_t* nev, const int64_t* pos, const int64_t* len,
-                                                    const double* val, const int64_t* pk, const int64_t* ec,
-                                                    int64_t* key, int64_t* xtra, double* cval, const int* bucket,
-                                                    const int64_t* bucket_base, int bshift, const int* cond) {
-  if (cond && *cond == 0) return;
-  const int64_t ne = ev_begin[nscen];  // candidates, clamped to the capacity
-  const int64_t end = ne > nscen ? ne : nscen;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < end; i += (int64_t)gridDim.x * BLK) {
-    if (i < nscen) {
-      int64_t c0 = ec[ev_begin[i]] & EC_MASK, c1 = ec[ev_begin[i + 1]] & EC_MASK;
-      st[i].ev_key = key + c0;
-      st[i].ev_extra = xtra + c0;
-      st[i].ev_val = cval + c0;
-      st[i].n_ev = c1 - c0;
-      st[i].ev_bucket = bucket + bucket_base[i];
-      st[i].n_bucket = bucket_base[i + 1] - bucket_base[i];
-      st[i].ev_bshift = bshift;
+// Exclusive scan of the span totals (one block), and each scenario's prefix at
+// its first candidate (the total past the last one).
+__global__ void __launch_bounds__(BLK) k_ev_chain_scan(ChainArgs a) {
+  if (a.cond && *a.cond == 0) return;
+  __shared__ int64_t lds[BLK / 64];
+  const int64_t nsp = (*a.chunk_live + BLK - 1) / BLK;
+  int64_t carry = 0;
+  constexpr int IT = 8;
+  for (int64_t t0 = 0; t0 < nsp; t0 += (int64_t)BLK * IT) {
+    int64_t v[IT], run = 0;
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int64_t i = t0 + (int64_t)threadIdx.x * IT + k;
+      v[k] = i < nsp ? a.span_pk[i] : 0;
+      run += v[k];
     }
-    if (i >= ne || !(pk[i] & 1)) continue;
-    int s = scen_of(ev_begin, nscen, i);
-    int64_t eb = ec[ev_begin[s]] >> 32;
-    int64_t e_excl = (ec[i] >> 32) - eb;
-    int64_t c = ec[i] & EC_MASK;
-    key[c] = pos[i] - e_excl;
-    xtra[c] = e_excl + len[i] - 1;
-    cval[c] = val[i];
+    int64_t total;
+    int64_t pre = block_excl_scan<OpSum, BLK>(run, total, lds);
+#pragma unroll
+    for (int k = 0; k < IT; ++k) {
+      const int64_t i = t0 + (int64_t)threadIdx.x * IT + k;
+      if (i < nsp) a.span_pkoff[i] = carry + pre;
+      pre += v[k];
+    }
+    carry += total;
+  }
+  __syncthreads();  // span_pkoff is complete (written by this block)
+  const int64_t ne = a.ev_begin[a.nscen];
+  for (int q = threadIdx.x; q <= a.nscen; q += BLK) {
+    const int64_t d = a.ev_begin[q];
+    int64_t c = carry;
+    if (d < ne) {
+      // the span holding candidate d: the last span whose first index is at or before it
+      int64_t lo = 0, hi = nsp - 1;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (a.span_off[mid] <= d) lo = mid;
+        else hi = mid - 1;
+      }
+      c = a.span_pkoff[lo] + (a.pk[d] & ~ON_CHAIN);
+    }
+    a.scen_c[q] = c;
+    a.scen_c0[q] = c & EC_MASK;
+  }
+}
+
+// Point each scenario's stream at its slice of the event tables (block 0,
+// threads below nscen) and compact the on-chain candidates, one span per
+// block: normal index, cumulative extra positions (inclusive) and value.
+__global__ void __launch_bounds__(BLK) k_ev_compact(ChainArgs a) {
+  if (a.cond && *a.cond == 0) return;
+  if (blockIdx.x == 0)
+    for (int q = threadIdx.x; q < a.nscen; q += BLK) {
+      const int64_t c0 = a.scen_c0[q], c1 = a.scen_c0[q + 1];
+      a.st[q].ev_key = a.key + c0;
+      a.st[q].ev_extra = a.xtra + c0;
+      a.st[q].ev_val = a.cval + c0;
+      a.st[q].n_ev = c1 - c0;
+      a.st[q].ev_bucket = a.bucket + a.bucket_base[q];
+      a.st[q].n_bucket = a.bucket_base[q + 1] - a.bucket_base[q];
+      a.st[q].ev_bshift = a.bshift;
+    }
+  const int64_t nsp = (*a.chunk_live + BLK - 1) / BLK;
+  for (int64_t sp = blockIdx.x; sp < nsp; sp += gridDim.x) {
+    int64_t o;
+    const int64_t n = span_items(a, sp, o);
+    const int64_t base = a.span_pkoff[sp];
+    for (int64_t li = threadIdx.x; li < n; li += BLK) {
+      const int64_t i = o + li;
+      const int64_t p = a.pk[i];
+      if (!(p & ON_CHAIN)) continue;
+      const int64_t C = base + (p & ~ON_CHAIN);
+      const int s = scen_of(a.ev_begin, a.nscen, i);
+      const int64_t e_excl = (C >> 32) - (a.scen_c[s] >> 32);
+      const int64_t c = C & EC_MASK;
+      a.key[c] = a.pos[i] - e_excl;
+      a.xtra[c] = e_excl + a.len[i] - 1;
+      a.cval[c] = a.val[i];
+    }
   }
 }
 
@@ -480,23 +601,32 @@ __global__ void __launch_bounds__(BLK) k_ev_polar(StreamDev* st, int nscen, cons
   }
 }
 
-// bucket index of every scenario's event keys: entry b = lower_bound(b << shift);
-// buckets past the last key (the step's units end there) hold n_ev directly
+// bucket index of every scenario's event keys: entry b = lower_bound(b << shift),
+// the first event whose key is at least b << shift.  Event c of a scenario
+// writes the entries b with key[c - 1] < b << shift <= key[c] (keys are
+// non-negative and increasing); the entries past the last key (the step's
+// units end there) hold n_ev.  cbase[s]: the scenario's first event in key
+// (cbase[nscen]: the total).
 __global__ void __launch_bounds__(BLK) k_ev_buckets(const StreamDev* st, int nscen, const int64_t* bucket_base,
-                                                    int64_t ntot, int* bucket, const int* cond) {
+                                                    int64_t ntot, const int64_t* cbase, const int64_t* key,
+                                                    int* bucket, const int* cond) {
   if (cond && *cond == 0) return;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < ntot; i += (int64_t)gridDim.x * BLK) {
-    int s = scen_of(bucket_base, nscen, i);
-    int64_t b = i - bucket_base[s], u = b << st[s].ev_bshift;
-    const int64_t* k = st[s].ev_key;
-    int64_t lo = 0, hi = st[s].n_ev;
-    if (hi > 0 && k[hi - 1] < u) lo = hi;
-    while (lo < hi) {
-      int64_t mid = (lo + hi) >> 1;
-      if (k[mid] < u) lo = mid + 1;
-      else hi = mid;
-    }
-    bucket[i] = (int)lo;
+  const int64_t stride = (int64_t)gridDim.x * BLK;
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < ntot; i += stride) {
+    const int s = scen_of(bucket_base, nscen, i);
+    const int64_t b = i - bucket_base[s], n = st[s].n_ev;
+    const int sh = st[s].ev_bshift;
+    if (n == 0 || b > (st[s].ev_key[n - 1] >> sh)) bucket[i] = (int)n;
+  }
+  const int64_t nev = cbase[nscen];
+  for (int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x; c < nev; c += stride) {
+    const int s = scen_of(cbase, nscen, c);
+    const int64_t l = c - cbase[s], nb = bucket_base[s + 1] - bucket_base[s];
+    const int sh = st[s].ev_bshift;
+    const int64_t lo = l == 0 ? 0 : (key[c - 1] >> sh) + 1;
+    int64_t hi = key[c] >> sh;
+    hi = hi < nb - 1 ? hi : nb - 1;
+    for (int64_t b = lo; b <= hi; ++b) bucket[bucket_base[s] + b] = (int)l;
   }
 }
 

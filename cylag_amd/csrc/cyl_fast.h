@@ -19,7 +19,7 @@
 // (di_lsm2 :1329-1543, lsm2_direct_coefficients :1240-1327), xylocalizer.pyx:338-435
 // and compute_bc.pyx:35-395.
 #pragma once
-#include "cyl_kernels.h"
+#include "cyl_events.h"
 
 namespace cyl {
 
@@ -442,6 +442,12 @@ struct PrepArgs {
   int uniform;
   const int* pregen;         // [nscen] generator serves the scenario, or null
   int64_t* chunk_base;       // [nscen] first generator chunk, or null (no generator)
+  StreamDev* ev_st;          // event chunk layout of the step (ev_layout), or null
+  int ev_method;
+  int64_t ev_nchunks;
+  int64_t* ev_base;
+  int64_t* ev_live;
+  int* ev_overflow;
 };
 
 __global__ void __launch_bounds__(BLK) k_step_prep(PrepArgs a) {
@@ -485,6 +491,11 @@ __global__ void __launch_bounds__(BLK) k_step_prep(PrepArgs a) {
     __syncthreads();
   }
   if (a.chunk_base && threadIdx.x == 0) a.chunk_base[a.nscen] = carry;
+  if (a.ev_base) {
+    __shared__ int64_t lds[BLK / 64];
+    __syncthreads();  // every scenario's units are stored
+    ev_layout(a.ev_st, a.scen_units, a.nscen, a.ev_method, a.ev_nchunks, a.ev_base, a.ev_live, a.ev_overflow, lds);
+  }
 }
 
 // The three normals of a rank in 3D (the diffusion pair, then the vertical
