@@ -171,6 +171,7 @@ struct cyl_ctx {
   int *d_chunk_loc = nullptr, *d_span_cnt = nullptr;
   int64_t *d_span_off = nullptr, *d_span_max = nullptr, *d_span_carry = nullptr;
   int64_t *d_span_pk = nullptr, *d_span_pkoff = nullptr, *d_scen_c = nullptr, *d_scen_c0 = nullptr;
+  int64_t* d_lb_base = nullptr;                    // the step's live bucket entries per scenario
   int64_t *d_reg_pos = nullptr, *d_reg_len = nullptr, *d_reg_key = nullptr;
   double* d_reg_val = nullptr;                     // span regions of the candidate records
   int64_t *d_ev_pos = nullptr, *d_ev_len = nullptr, *d_reach = nullptr;
@@ -1819,6 +1820,7 @@ static int ensure_events(cyl_ctx* ctx, int64_t units_per_scen_cap, int method) {
   HIPCHK(dalloc(b, &ctx->d_bucket_base, ns + 1));
   HIPCHK(hipMemcpy(ctx->d_bucket_base, bb.data(), (ns + 1) * sizeof(int64_t), hipMemcpyHostToDevice));
   HIPCHK(dalloc(b, &ctx->d_chunk_base, ns + 1));  // the step's layout (ev_layout)
+  HIPCHK(dalloc(b, &ctx->d_lb_base, ns + 1));
   ctx->nspan = (nch + BLK - 1) / BLK;
   const int64_t nreg = ctx->nspan * (method == 2 ? EV_SPAN_CAP_ZIG : EV_SPAN_CAP_POLAR);
   HIPCHK(dalloc(b, &ctx->d_chunk_loc, nch + 1));
@@ -1882,13 +1884,29 @@ static void enqueue_rank(cyl_ctx* ctx, const int* cond, int& launches) {
 }
 
 // event tables of the step's stream window (rng_method 2 and 3)
-// layout: lay the chunks out here (the step's preparation has not: the units
+// the step's event table layout (ev_layout)
+static LayoutArgs layout_args(cyl_ctx* ctx, int method) {
+  LayoutArgs l{};
+  l.st = ctx->d_stream;
+  l.units = ctx->d_scen_units;
+  l.nscen = ctx->nscen;
+  l.method = method;
+  l.bshift = ctx->bshift;
+  l.nchunks = ctx->nchunks;
+  l.bucket_base = ctx->d_bucket_base;
+  l.base = ctx->d_chunk_base;
+  l.live = ctx->d_chunk_live;
+  l.lb_base = ctx->d_lb_base;
+  l.overflow = ctx->d_flag + 2;
+  return l;
+}
+
+// layout: lay the tables out here (the step's preparation has not: the units
 // come from a counting run)
 static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, bool layout, int& launches) {
   if (method == 1) return;
   if (layout) {
-    k_ev_layout<<<1, BLK, 0, ctx->stream>>>(ctx->d_stream, ctx->d_scen_units, ctx->nscen, method, ctx->nchunks,
-                                             ctx->d_chunk_base, ctx->d_chunk_live, ctx->d_flag + 2, cond);
+    k_ev_layout<<<1, BLK, 0, ctx->stream>>>(layout_args(ctx, method), cond);
     ++launches;
   }
   EventArgs e{};
@@ -1964,6 +1982,7 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, bool layou
     ca.cval = ctx->d_ev_val;
     ca.bucket = ctx->d_bucket;
     ca.bucket_base = ctx->d_bucket_base;
+    ca.lb_base = ctx->d_lb_base;
     ca.bshift = ctx->bshift;
     ca.cond = cond;
     k_ev_resolve<<<ngat, BLK, 0, s>>>(ca);
@@ -1972,12 +1991,13 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, bool layou
     launches += 3;
   } else {
     k_ev_polar<<<nb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_ev_begin, ctx->d_nev, ctx->d_ev_pos, ctx->d_key,
-                                  ctx->d_bucket, ctx->d_bucket_base, ctx->bshift, cond);
+                                  ctx->d_bucket, ctx->d_bucket_base, ctx->d_lb_base, ctx->bshift, cond);
     launches += 1;
   }
-  const unsigned nbb = std::min<unsigned>(nblk(ctx->nbucket), 8u * (unsigned)ctx->ncu);
-  k_ev_buckets<<<nbb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_bucket_base, ctx->nbucket,
-                                   method == 2 ? ctx->d_scen_c0 : ctx->d_ev_begin, ctx->d_key, ctx->d_bucket, cond);
+  const unsigned nbb = std::min<unsigned>(nblk(ctx->nbucket), 4u * (unsigned)ctx->ncu);
+  k_ev_buckets<<<nbb, BLK, 0, s>>>(ctx->d_stream, ctx->nscen, ctx->d_lb_base,
+                                   method == 2 ? ctx->d_scen_c0 : ctx->d_ev_begin, ctx->d_key, ctx->d_bucket,
+                                   ctx->d_bucket_base, cond);
   launches += 1;
 }
 
@@ -2588,12 +2608,8 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       pa.pregen = ctx->d_pregen;
       pa.chunk_base = uniform && any_fast && ctx->inline_normals == 0 ? ctx->d_gen_base : nullptr;
       if (uniform && method != 1) {
-        pa.ev_st = ctx->d_stream;
-        pa.ev_method = method;
-        pa.ev_nchunks = ctx->nchunks;
-        pa.ev_base = ctx->d_chunk_base;
-        pa.ev_live = ctx->d_chunk_live;
-        pa.ev_overflow = ctx->d_flag + 2;
+        pa.layout = 1;
+        pa.ev = layout_args(ctx, method);
       }
       k_step_prep<<<1, BLK, 0, ctx->stream>>>(pa);
       ++launches;

@@ -88,40 +88,61 @@ __device__ __forceinline__ int64_t window_for(int method, int64_t units) {
   return 2 * (units + units / 2 + 4096);  // polar: attempts of 2 positions, ~21% rejected
 }
 
-// Chunk layout of the step's stream windows (one block): scenario q's window
+// Layout of the step's event tables (one block).  Chunks: scenario q's window
 // (window_for its units, stored in its stream) takes ceil(W / ZCHUNK)
 // chunks from base[q], an exclusive scan over the scenarios; base[nscen] and
-// *live receive the total.  A layout past the table's nchunks is reported
-// (overflow) and *live clamped to the table.
-__device__ void ev_layout(StreamDev* st, const int64_t* units, int nscen, int method, int64_t nchunks,
-                          int64_t* base, int64_t* live, int* overflow, int64_t* lds) {
-  int64_t carry = 0;
-  for (int t0 = 0; t0 < nscen; t0 += BLK) {
+// *live receive the total, and a layout past the table's nchunks is reported
+// (overflow) and *live clamped to the table.  Bucket index: the scenario's
+// lookups ask for units up to its unit count u, so its live entries are the
+// first (u >> bshift) + 2 of its capacity (beyond them a lookup searches the
+// keys); lb_base is their exclusive scan.
+struct LayoutArgs {
+  StreamDev* st;
+  const int64_t* units;
+  int nscen, method, bshift;
+  int64_t nchunks;
+  const int64_t* bucket_base;  // [nscen + 1] bucket capacity of each scenario
+  int64_t* base;               // [nscen + 1]
+  int64_t* live;
+  int64_t* lb_base;            // [nscen + 1]
+  int* overflow;
+};
+
+__device__ void ev_layout(const LayoutArgs& a, int64_t* lds) {
+  int64_t carry = 0, bcarry = 0;
+  for (int t0 = 0; t0 < a.nscen; t0 += BLK) {
     const int q = t0 + threadIdx.x;
-    int64_t c = 0;
-    if (q < nscen) {
-      const int64_t W = window_for(method, units[q]);
-      st[q].window = W;
+    int64_t c = 0, nb = 0;
+    if (q < a.nscen) {
+      const int64_t u = a.units[q], W = window_for(a.method, u);
+      a.st[q].window = W;
       c = (W + ZCHUNK - 1) / ZCHUNK;
+      const int64_t cap = a.bucket_base[q + 1] - a.bucket_base[q];
+      nb = (u >> a.bshift) + 2;
+      nb = nb < cap ? nb : cap;
     }
-    int64_t tot;
+    int64_t tot, btot;
     const int64_t pre = block_excl_scan<OpSum, BLK>(c, tot, lds);
-    if (q < nscen) base[q] = carry + pre;
+    const int64_t bpre = block_excl_scan<OpSum, BLK>(nb, btot, lds);
+    if (q < a.nscen) {
+      a.base[q] = carry + pre;
+      a.lb_base[q] = bcarry + bpre;
+    }
     carry += tot;
+    bcarry += btot;
   }
   if (threadIdx.x == 0) {
-    base[nscen] = carry;
-    if (carry > nchunks) atomicOr(overflow, 1);
-    *live = carry < nchunks ? carry : nchunks;
+    a.base[a.nscen] = carry;
+    a.lb_base[a.nscen] = bcarry;
+    if (carry > a.nchunks) atomicOr(a.overflow, 1);
+    *a.live = carry < a.nchunks ? carry : a.nchunks;
   }
 }
 
-__global__ void __launch_bounds__(BLK) k_ev_layout(StreamDev* st, const int64_t* units, int nscen, int method,
-                                                   int64_t nchunks, int64_t* base, int64_t* live, int* overflow,
-                                                   const int* cond) {
+__global__ void __launch_bounds__(BLK) k_ev_layout(LayoutArgs a, const int* cond) {
   if (cond && *cond == 0) return;
   __shared__ int64_t lds[BLK / 64];
-  ev_layout(st, units, nscen, method, nchunks, base, live, overflow, lds);
+  ev_layout(a, lds);
 }
 
 __device__ __forceinline__ bool zig_fast(uint32_t u, const uint32_t* kn) {
@@ -436,6 +457,7 @@ struct ChainArgs {
   double* cval;             //   value
   const int* bucket;
   const int64_t* bucket_base;
+  const int64_t* lb_base;   // [nscen + 1] live bucket entries (ev_layout)
   int bshift;
   const int* cond;
 };
@@ -554,7 +576,7 @@ __global__ void __launch_bounds__(BLK) k_ev_compact(ChainArgs a) {
       a.st[q].ev_val = a.cval + c0;
       a.st[q].n_ev = c1 - c0;
       a.st[q].ev_bucket = a.bucket + a.bucket_base[q];
-      a.st[q].n_bucket = a.bucket_base[q + 1] - a.bucket_base[q];
+      a.st[q].n_bucket = a.lb_base[q + 1] - a.lb_base[q];
       a.st[q].ev_bshift = a.bshift;
     }
   const int64_t nsp = (*a.chunk_live + BLK - 1) / BLK;
@@ -580,8 +602,8 @@ __global__ void __launch_bounds__(BLK) k_ev_compact(ChainArgs a) {
 // polar: key_j = r_j - j (j counted within the scenario), and the stream slices
 __global__ void __launch_bounds__(BLK) k_ev_polar(StreamDev* st, int nscen, const int64_t* ev_begin,
                                                   const int64_t* nev, const int64_t* pos, int64_t* key,
-                                                  const int* bucket, const int64_t* bucket_base, int bshift,
-                                                  const int* cond) {
+                                                  const int* bucket, const int64_t* bucket_base,
+                                                  const int64_t* lb_base, int bshift, const int* cond) {
   if (cond && *cond == 0) return;
   const int64_t ne = ev_begin[nscen];  // candidates, clamped to the capacity
   const int64_t end = ne > nscen ? ne : nscen;
@@ -592,7 +614,7 @@ __global__ void __launch_bounds__(BLK) k_ev_polar(StreamDev* st, int nscen, cons
       st[i].ev_val = nullptr;
       st[i].n_ev = ev_begin[i + 1] - ev_begin[i];
       st[i].ev_bucket = bucket + bucket_base[i];
-      st[i].n_bucket = bucket_base[i + 1] - bucket_base[i];
+      st[i].n_bucket = lb_base[i + 1] - lb_base[i];
       st[i].ev_bshift = bshift;
     }
     if (i >= ne) continue;
@@ -602,31 +624,32 @@ __global__ void __launch_bounds__(BLK) k_ev_polar(StreamDev* st, int nscen, cons
 }
 
 // bucket index of every scenario's event keys: entry b = lower_bound(b << shift),
-// the first event whose key is at least b << shift.  Event c of a scenario
-// writes the entries b with key[c - 1] < b << shift <= key[c] (keys are
-// non-negative and increasing); the entries past the last key (the step's
-// units end there) hold n_ev.  cbase[s]: the scenario's first event in key
-// (cbase[nscen]: the total).
-__global__ void __launch_bounds__(BLK) k_ev_buckets(const StreamDev* st, int nscen, const int64_t* bucket_base,
-                                                    int64_t ntot, const int64_t* cbase, const int64_t* key,
-                                                    int* bucket, const int* cond) {
+// the first event whose key is at least b << shift, for the scenario's live
+// entries b < n_bucket (lb_base: their layout).  Event c of a scenario writes
+// the entries b with key[c - 1] < b << shift <= key[c] (keys are non-negative
+// and increasing); the entries past the last key hold n_ev.  cbase[s]: the
+// scenario's first event in key (cbase[nscen]: the total).
+__global__ void __launch_bounds__(BLK) k_ev_buckets(const StreamDev* st, int nscen, const int64_t* lb_base,
+                                                    const int64_t* cbase, const int64_t* key, int* bucket,
+                                                    const int64_t* bucket_base, const int* cond) {
   if (cond && *cond == 0) return;
   const int64_t stride = (int64_t)gridDim.x * BLK;
-  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < ntot; i += stride) {
-    const int s = scen_of(bucket_base, nscen, i);
-    const int64_t b = i - bucket_base[s], n = st[s].n_ev;
-    const int sh = st[s].ev_bshift;
-    if (n == 0 || b > (st[s].ev_key[n - 1] >> sh)) bucket[i] = (int)n;
+  const int64_t nlive = lb_base[nscen];
+  for (int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x; i < nlive; i += stride) {
+    const int s = scen_of(lb_base, nscen, i);
+    const int64_t b = i - lb_base[s], n = st[s].n_ev;
+    if (n == 0 || b > (st[s].ev_key[n - 1] >> st[s].ev_bshift)) bucket[bucket_base[s] + b] = (int)n;
   }
   const int64_t nev = cbase[nscen];
   for (int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x; c < nev; c += stride) {
     const int s = scen_of(cbase, nscen, c);
-    const int64_t l = c - cbase[s], nb = bucket_base[s + 1] - bucket_base[s];
+    const int64_t l = c - cbase[s], nb = st[s].n_bucket;
     const int sh = st[s].ev_bshift;
     const int64_t lo = l == 0 ? 0 : (key[c - 1] >> sh) + 1;
     int64_t hi = key[c] >> sh;
     hi = hi < nb - 1 ? hi : nb - 1;
-    for (int64_t b = lo; b <= hi; ++b) bucket[bucket_base[s] + b] = (int)l;
+    int* bk = bucket + bucket_base[s];
+    for (int64_t b = lo; b <= hi; ++b) bk[b] = (int)l;
   }
 }
 

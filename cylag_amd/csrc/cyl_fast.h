@@ -442,12 +442,8 @@ struct PrepArgs {
   int uniform;
   const int* pregen;         // [nscen] generator serves the scenario, or null
   int64_t* chunk_base;       // [nscen] first generator chunk, or null (no generator)
-  StreamDev* ev_st;          // event chunk layout of the step (ev_layout), or null
-  int ev_method;
-  int64_t ev_nchunks;
-  int64_t* ev_base;
-  int64_t* ev_live;
-  int* ev_overflow;
+  int layout;                // lay out the step's event tables (ev_layout)
+  LayoutArgs ev;
 };
 
 __global__ void __launch_bounds__(BLK) k_step_prep(PrepArgs a) {
@@ -491,10 +487,10 @@ __global__ void __launch_bounds__(BLK) k_step_prep(PrepArgs a) {
     __syncthreads();
   }
   if (a.chunk_base && threadIdx.x == 0) a.chunk_base[a.nscen] = carry;
-  if (a.ev_base) {
+  if (a.layout) {
     __shared__ int64_t lds[BLK / 64];
     __syncthreads();  // every scenario's units are stored
-    ev_layout(a.ev_st, a.scen_units, a.nscen, a.ev_method, a.ev_nchunks, a.ev_base, a.ev_live, a.ev_overflow, lds);
+    ev_layout(a.ev, lds);
   }
 }
 
