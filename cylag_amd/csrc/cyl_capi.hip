@@ -823,10 +823,10 @@ static int alloc_slot_bufs(cyl_ctx* ctx, int n, int nscen_pad) {
   Bufs& bb = ctx->part_bufs;
   HIPCHK(dalloc(bb, &ctx->d_gpool, n));
   HIPCHK(dalloc(bb, &ctx->d_rank, n));
-  // normals in tiles of 64 chunks x 32: chunks <= n/32 + 1 per scenario,
+  // normals in tiles of 64 chunks x GEN_PER: chunks <= n/GEN_PER + 1 per scenario,
   // rounded up to a whole tile; an entry is a pair (16 B) in 2D, a 32 B
   // record of three values in 3D
-  ctx->xi_cap = (int64_t)n + 32 * (int64_t)nscen_pad + 64 * 32;
+  ctx->xi_cap = (int64_t)n + GEN_PER * (int64_t)nscen_pad + 64 * GEN_PER;
   HIPCHK(dalloc(bb, &ctx->d_xi, (size_t)ctx->xi_cap * (dim == 3 ? 2 : 1)));
   HIPCHK(dalloc(bb, &ctx->d_off, n));
   HIPCHK(dalloc(bb, &ctx->d_init_tri, n));
@@ -1161,7 +1161,7 @@ extern "C" int cyl_scenarios_set(cyl_ctx* ctx, int nscen, const int* base, const
   if (!ctx->d_scen) HIPCHK(dalloc(ctx->part_bufs, &ctx->d_scen, ctx->n));
   // normals buffer with the per-scenario chunk padding (see k_gen_normals):
   // the upload's buffer when it is large enough, else a larger one
-  const int64_t need_xi = (int64_t)ctx->n + 32 * ((int64_t)nscen + 2) + 64 * 32;
+  const int64_t need_xi = (int64_t)ctx->n + GEN_PER * ((int64_t)nscen + 2) + 64 * GEN_PER;
   if (need_xi > ctx->xi_cap) {
     HIPCHK(ctx->part_bufs.release(ctx->d_xi));
     ctx->d_xi = nullptr;

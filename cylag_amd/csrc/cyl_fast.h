@@ -365,6 +365,12 @@ __device__ __forceinline__ void load_jump8(JumpTable8& t8, const JumpTable8* src
   for (int q = threadIdx.x; q < (int)(sizeof(JumpTable8) / 8); q += BLK) d[q] = s[q];
 }
 
+// the fast-path Ziggurat tables (the generator never takes the slow path)
+struct ZigFast {
+  uint32_t kn[128];
+  double wn[128];
+};
+
 // First draw of every active slot: the serial stream position of its first
 // unit (uniform: units_per_part * active rank; general: pool-order scan) and
 // the generator state there.  Lookups of neighbouring slots are independent,
@@ -408,7 +414,7 @@ __global__ void __launch_bounds__(BLK) k_draw_start(FastArgs a) {
 // GEN_PER + k) * 64 + c % 64], so that each of a wave's stores is one coalesced
 // 1 KB row and its 32 stores stay within 32 KB.  Indices stay below nchunk
 // rounded up to 64, times 32, where nchunk <= (active rows) / 32 + (scenarios):
-// the buffer holds rows + 32 * (scenarios + 2) + 64 * 32 pairs.
+// the buffer holds rows + GEN_PER * (scenarios + 2) + 64 * GEN_PER pairs.
 constexpr int GEN_PER = 32;
 
 __device__ __forceinline__ int64_t xi_tile_index(int64_t c, int64_t k) {
@@ -500,19 +506,63 @@ struct __align__(32) XiRec3 {
   double a, b, c, pad;
 };
 
+// The next four slow events of a generator run (normal index relative to the
+// run's first unit, positions taken and value), held in registers: loaded
+// before the run's loop and again only once all four are used (a run of 64
+// normals holds 0.8 events on average and more than four in about one run of
+// 800).  Loads inside the loop would wait, at their use, for every store the
+// wave has in flight.  The run's bookkeeping is 32-bit: a run spans at most
+// 3 GEN_PER normals.
+struct EvQueue {
+  static constexpr int32_t NONE = 0x7fffffff;  // no event, or one past the run
+  int32_t k0, k1, k2, k3, l0, l1, l2, l3;
+  double v0, v1, v2, v3;
+
+  __device__ __forceinline__ void load(const int64_t* key, const int64_t* ex, const double* val, int64_t n,
+                                       int64_t ev, int64_t u0) {
+    int64_t prev = ev > 0 && ev - 1 < n ? ex[ev - 1] : 0, e;
+    auto one = [&](int64_t i, int32_t& k, int32_t& l, double& v) {
+      if (i < n) {
+        const int64_t kr = key[i] - u0;  // >= 0: the queue starts at the run's first event
+        k = kr < NONE ? (int32_t)kr : NONE;
+        e = ex[i];
+        l = (int32_t)(e - prev + 1);
+        prev = e;
+        v = val[i];
+      } else {
+        k = NONE;
+        l = 0;
+        v = 0.;
+      }
+    };
+    one(ev, k0, l0, v0);
+    one(ev + 1, k1, l1, v1);
+    one(ev + 2, k2, l2, v2);
+    one(ev + 3, k3, l3, v3);
+  }
+
+  __device__ __forceinline__ void pop() {
+    k0 = k1; l0 = l1; v0 = v1;
+    k1 = k2; l1 = l2; v1 = v2;
+    k2 = k3; l2 = l3; v2 = v3;
+    k3 = NONE;
+  }
+};
+
 // Normals of ranks [r0, r1) of scenario s, drawn sequentially from the stream
 // position of rank r0's first unit, in the order the fused kernels call them:
 // NV = 2, the diffusion pair (generate_random_pair, random_utils.pyx:202-233);
 // NV = 3, the pair then the vertical single (generate_random_single,
 // random_utils.pyx:235-262; under rng_method 3 a single is one polar call of
-// which the first value is kept).  store(r, v) receives each rank's values.
-// Returns the stream position after the last draw; sets err on a window
-// overflow or a slow Ziggurat start missing from the event table.
+// which the first value is kept).  store(i, v) receives the values of rank
+// r0 + i.  Returns the stream position after the last draw; sets err on a
+// window overflow (a value starting past the window: positions only grow, so
+// the last value's start is checked) or a slow Ziggurat start missing from
+// the event table.
 template <int RNG, int NV, class Store>
 __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, int64_t r0, int64_t r1,
-                                                      const JumpTable8& t8, uint64_t t8_inc, const ZigTables& zt,
+                                                      const JumpTable8& t8, uint64_t t8_inc, const ZigFast& zt,
                                                       int& err, Store&& store) {
-  const int64_t NONE = 0x7fffffffffffffffLL;
   const StreamDev& st = a.stream[s];
   // the stream record in registers: the stores below may not alias it, but
   // the compiler cannot tell
@@ -523,48 +573,48 @@ __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, 
   int64_t u = r0 * st.units_per_part;
   Draws d;
   d.st = &st;
-  int64_t pos, ev = 0, next_key = NONE;
+  int64_t pos0, ev = 0;
+  EvQueue q;
   if (RNG == 1) {
-    pos = u;
+    pos0 = u;
   } else if (RNG == 2) {
     ev = d.events_before(u);
-    pos = u + (ev > 0 ? ev_extra[ev - 1] : 0);
-    next_key = ev < n_ev ? ev_key[ev] : NONE;
+    pos0 = u + (ev > 0 ? ev_extra[ev - 1] : 0);
+    q.load(ev_key, ev_extra, ev_val, n_ev, ev, u);
   } else {
-    pos = 2 * (u + d.events_before(u));
+    pos0 = 2 * (u + d.events_before(u));
   }
-  if (RNG != 1 && pos >= window) err = E_RNG_WINDOW;
+  if (RNG != 1 && pos0 >= window) err = E_RNG_WINDOW;
   Pcg g;
   g.inc = st.inc;
-  g.state = st.inc == t8_inc ? pcg_jump8(st.base, (uint64_t)pos, t8, a.jt[s])
-                             : pcg_jump8(st.base, (uint64_t)pos, a.jt8[s], a.jt[s]);
-  for (int64_t r = r0; r < r1; ++r) {
+  g.state = st.inc == t8_inc ? pcg_jump8(st.base, (uint64_t)pos0, t8, a.jt[s])
+                             : pcg_jump8(st.base, (uint64_t)pos0, a.jt8[s], a.jt[s]);
+  int32_t prel = 0, plast = 0, urel = 0;  // positions taken, start of the last value, units drawn
+  const int nr = (int)(r1 - r0);
+  for (int i = 0; i < nr; ++i) {
     double v[NV];
     if (RNG == 1) {
 #pragma unroll
       for (int k = 0; k < NV; ++k) v[k] = rect_normal(g);
-      pos += NV;
+      prel += NV;
     } else if (RNG == 2) {
-      for (int k = 0; k < NV; ++k, ++u) {
-        if (pos >= window) err = E_RNG_WINDOW;
-        if (u == next_key) {
-          int64_t len = ev_extra[ev] - (ev > 0 ? ev_extra[ev - 1] : 0) + 1;
-          v[k] = ev_val[ev];
-          for (int64_t q = 0; q < len; ++q) pcg_next(g);
-          pos += len;
+      for (int k = 0; k < NV; ++k, ++urel) {
+        plast = prel;
+        if (urel == q.k0) {
+          const int32_t len = q.l0;
+          v[k] = q.v0;
+          for (int32_t t = 0; t < len; ++t) pcg_next(g);
+          prel += len;
           ++ev;
-          next_key = ev < n_ev ? ev_key[ev] : NONE;
-          // wait for these loads here, in the rare branch: left pending, they
-          // put a wait for every outstanding memory operation, the previous
-          // pairs' stores included, at the head of each iteration
-          __builtin_amdgcn_s_waitcnt(0);
+          q.pop();
+          if (q.k0 == EvQueue::NONE && ev < n_ev) q.load(ev_key, ev_extra, ev_val, n_ev, ev, u);  // all four used
         } else {
           int32_t hz = (int32_t)pcg_next(g);
           uint32_t iz = (uint32_t)hz & 127u;
           uint32_t ahz = hz < 0 ? (uint32_t)(-(int64_t)hz) : (uint32_t)hz;
           if (ahz >= zt.kn[iz]) err = E_RNG_EVENTS;  // a slow start missing from the table
           v[k] = hz * zt.wn[iz];
-          pos += 1;
+          prel += 1;
         }
       }
     } else {
@@ -574,16 +624,17 @@ __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, 
           x1 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
           x2 = 2.0 * u32_to_uniform(pcg_next(g)) - 1.0;
           w = x1 * x1 + x2 * x2;
-          pos += 2;
+          prel += 2;
         }
         w = sqrt((-2.0 * cr_log(w)) / w);
         v[k] = x1 * w;
         if (k + 1 < NV) v[k + 1] = x2 * w;
       }
     }
-    store(r, v);
+    store(i, v);
   }
-  return (unsigned long long)pos;
+  if (RNG == 2 && nr > 0 && pos0 + plast >= window) err = E_RNG_WINDOW;
+  return (unsigned long long)(pos0 + prel);
 }
 
 // Normals of generator chunk c (scenario s): the NV values of each of its
@@ -591,13 +642,13 @@ __device__ __forceinline__ unsigned long long gen_run(const FastArgs& a, int s, 
 // the stream position after its last draw (0 if the scenario draws nothing).
 template <int RNG, int NV>
 __device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64_t c, int s, const JumpTable8& t8,
-                                                        uint64_t t8_inc, const ZigTables& zt) {
+                                                        uint64_t t8_inc, const ZigFast& zt) {
   const StreamDev& st = a.stream[s];
   if (st.units_per_part == 0) return 0;
   int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
   int err = 0;
-  unsigned long long pend = gen_run<RNG, NV>(a, s, r0, r1, t8, t8_inc, zt, err, [&](int64_t r, const double* v) {
-    int64_t xidx = xi_tile_index(c, r - r0);
+  unsigned long long pend = gen_run<RNG, NV>(a, s, r0, r1, t8, t8_inc, zt, err, [&](int i, const double* v) {
+    int64_t xidx = xi_tile_index(c, i);
     if (xidx >= a.xi_cap) return;
     if (NV == 2) {
       a.xi[xidx] = make_double2(v[0], v[1]);
@@ -620,14 +671,13 @@ __device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64
 template <int RNG, int NV>
 __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
   __shared__ JumpTable8 t8;
-  __shared__ ZigTables zt;
+  __shared__ ZigFast zt;
   __shared__ uint64_t t8_inc;
   __shared__ int t8_set;
   if (RNG == 2) {
     for (int q = threadIdx.x; q < 128; q += BLK) {
       zt.kn[q] = a.zt->kn[q];
       zt.wn[q] = a.zt->wn[q];
-      zt.fn[q] = a.zt->fn[q];
     }
   }
   if (threadIdx.x == 0) t8_set = 0;
@@ -670,7 +720,7 @@ __global__ void __launch_bounds__(BLK) k_gen_normals(FastArgs a) {
 template <int RNG>
 __global__ void __launch_bounds__(BLK) k_gen_slots(FastArgs a) {
   __shared__ JumpTable8 t8;
-  __shared__ ZigTables zt;
+  __shared__ ZigFast zt;
   if (RNG == 2)
     for (int q = threadIdx.x; q < 128; q += BLK) {
       zt.kn[q] = a.zt->kn[q];
@@ -688,7 +738,7 @@ __global__ void __launch_bounds__(BLK) k_gen_slots(FastArgs a) {
     int err = 0;
     int64_t r = a.rank[j];
     unsigned long long pend = gen_run<RNG, 2>(a, s, r, r + 1, t8, t8_inc, zt, err,
-                                              [&](int64_t, const double* v) {
+                                              [&](int, const double* v) {
                                                 if (j < a.xi_cap) a.xi[j] = make_double2(v[0], v[1]);
                                               });
     (void)pend;  // the streams' ends come from the units drawn (k_finish)
