@@ -578,6 +578,8 @@ def main():
     ap.add_argument("--normals", type=int, default=0, choices=[0, 1, 2],
                     help="plain leg: where the fused kernel takes its normals (0: pool-order generator, 1: sampled "
                          "inline, 2: slot-order generator)")
+    ap.add_argument("--graph", type=int, default=0, choices=[0, 1],
+                    help="plain leg: replay the step loop from a captured HIP graph")
     ap.add_argument("--legs", default="plain,c3,batched,sequential,batched_square,served",
                     help="comma-separated: plain (the 10M-particle step), c3 (3D LSM-2 with settling, 5M particles), "
                          "batched (256 scenarios in one batch), "
@@ -659,7 +661,7 @@ def plain_leg(cy, args, mesh, fields, pos, params):
     cy.pcg32_seed(42, 0xda3e06d11bb2b203)
     t0 = time.time()
     solver = cy.Solver(fs, ps, params)
-    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0, args.normals, 0)
+    solver.ctx.lib.cyl_set_tuning(solver.ctx.h, args.sort_interval, 0, args.normals, args.graph)
     log("solver ready ({:.1f} s): {}".format(time.time() - t0, solver.ctx.device_info()[0]))
     solver._run(args.warmup)
     log("warmup done")
