@@ -212,7 +212,7 @@ __device__ __noinline__ double cr_exp_dd(double x) {
   return ldexp(e.hi, k);
 }
 
-__device__ __noinline__ double cr_exp(double x) {
+__device__ __forceinline__ double cr_exp(double x) {
   if (!(x == x)) return x;
   if (x > 709.782712893384) return HUGE_VAL;
   if (x < -708.) return exp(x);  // subnormal or zero results
@@ -259,7 +259,7 @@ __device__ __forceinline__ bool cr_log_try(double x, double& out) {
 
 __device__ __noinline__ double cr_log_dd(double x) { return dd_log(x).hi; }
 
-__device__ __noinline__ double cr_log(double x) {
+__device__ __forceinline__ double cr_log(double x) {
   if (!(x >= 2.2250738585072014e-308) || !(x < HUGE_VAL)) return log(x);  // 0, negative, subnormal, inf, nan
   if (x == 1.) return 0.;
   double r;
@@ -290,7 +290,7 @@ __device__ __noinline__ double cr_pow_dd(double x, double y) {
   return ldexp(e.hi, k);
 }
 
-__device__ __noinline__ double cr_pow(double x, double y) {
+__device__ __forceinline__ double cr_pow(double x, double y) {
   if (y == 0. || x == 1.) return 1.;
   if (!(x >= 2.2250738585072014e-308) || !(x < HUGE_VAL) || !(fabs(y) < HUGE_VAL)) return pow(x, y);
   double r;
@@ -300,7 +300,7 @@ __device__ __noinline__ double cr_pow(double x, double y) {
 
 __device__ __noinline__ double cr_expm1_dd(double x);
 
-__device__ __noinline__ double cr_expm1(double x) {
+__device__ __forceinline__ double cr_expm1(double x) {
   if (!(x == x)) return x;
   if (x > 709.782712893384) return HUGE_VAL;
   if (x < -38.) return -1.;  // e^x < 2^-54: -1 + e^x rounds to -1

@@ -429,10 +429,9 @@ __global__ void __launch_bounds__(BLK) k_ev_gather(EventArgs e) {
 // surely on-chain; from there the chain is followed forward.  reach (exclusive
 // max-scan of the keys) gives the furthest position reached by earlier
 // candidates of the same scenario.  The packed value (len - 1) << 32 | 1 of an
-// on-chain candidate (0 otherwise) is scanned within the span, EV_SPAN_CAP_ZIG
-// / BLK consecutive candidates per thread: pk receives the exclusive prefix
-// within the span (ON_CHAIN set for an on-chain candidate) and span_pk the
-// span's total.
+// on-chain candidate (0 otherwise) is scanned within the span: pk receives
+// the exclusive prefix within the span (ON_CHAIN set for an on-chain
+// candidate) and span_pk the span's total.
 constexpr int64_t ON_CHAIN = (int64_t)1 << 62;  // flag bit of the per-candidate prefix
 
 struct ChainArgs {
@@ -472,48 +471,44 @@ __device__ __forceinline__ int64_t span_items(const ChainArgs& a, int64_t sp, in
 __global__ void __launch_bounds__(BLK) k_ev_resolve(ChainArgs a) {
   if (a.cond && *a.cond == 0) return;
   __shared__ int64_t lds[BLK / 64];
-  constexpr int PT = EV_SPAN_CAP_ZIG / BLK;
   const int64_t nsp = (*a.chunk_live + BLK - 1) / BLK;
   for (int64_t sp = blockIdx.x; sp < nsp; sp += gridDim.x) {
     int64_t o;
     const int64_t n = span_items(a, sp, o);
-    int64_t v[PT], run = 0;
-#pragma unroll
-    for (int q = 0; q < PT; ++q) {
-      v[q] = 0;
-      const int64_t li = (int64_t)threadIdx.x * PT + q;
-      if (li >= n) continue;
-      const int64_t i = o + li;
-      const int s = scen_of(a.ev_begin, a.nscen, i);
-      const int64_t b = a.ev_begin[s];
-      auto reach_of = [&](int64_t j) -> int64_t {
-        const int64_t m = a.reach[j];
-        return (m >= 0 && (m >> REACH_SHIFT) == s) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
-      };
-      int64_t j0 = i;
-      while (j0 > b && reach_of(j0) >= a.pos[j0]) --j0;
-      int64_t cur_end = a.pos[j0] + a.len[j0] - 1;
-      int on = 1;
-      for (int64_t l = j0 + 1; l <= i; ++l) {
-        if (a.pos[l] > cur_end) {
-          cur_end = a.pos[l] + a.len[l] - 1;
-          on = 1;
-        } else {
-          on = 0;
+    // rounds of BLK candidates, one per thread (a span holds about 200), so
+    // that a thread's chain walks do not queue behind each other
+    int64_t carry = 0;
+    for (int64_t r0 = 0; r0 < n; r0 += BLK) {
+      const int64_t li = r0 + threadIdx.x;
+      int64_t v = 0;
+      if (li < n) {
+        const int64_t i = o + li;
+        const int s = scen_of(a.ev_begin, a.nscen, i);
+        const int64_t b = a.ev_begin[s];
+        auto reach_of = [&](int64_t j) -> int64_t {
+          const int64_t m = a.reach[j];
+          return (m >= 0 && (m >> REACH_SHIFT) == s) ? (m & ((1LL << REACH_SHIFT) - 1)) : -1;
+        };
+        int64_t j0 = i;
+        while (j0 > b && reach_of(j0) >= a.pos[j0]) --j0;
+        int64_t cur_end = a.pos[j0] + a.len[j0] - 1;
+        int on = 1;
+        for (int64_t l = j0 + 1; l <= i; ++l) {
+          if (a.pos[l] > cur_end) {
+            cur_end = a.pos[l] + a.len[l] - 1;
+            on = 1;
+          } else {
+            on = 0;
+          }
         }
+        v = on ? ((a.len[i] - 1) << 32) | 1 : 0;
       }
-      v[q] = on ? ((a.len[i] - 1) << 32) | 1 : 0;
-      run += v[q];
+      int64_t tot;
+      const int64_t pre = block_excl_scan<OpSum, BLK>(v, tot, lds);
+      if (li < n) a.pk[o + li] = (carry + pre) | (v & 1 ? ON_CHAIN : 0);
+      carry += tot;
     }
-    int64_t tot;
-    int64_t pre = block_excl_scan<OpSum, BLK>(run, tot, lds);
-#pragma unroll
-    for (int q = 0; q < PT; ++q) {
-      const int64_t li = (int64_t)threadIdx.x * PT + q;
-      if (li < n) a.pk[o + li] = pre | (v[q] & 1 ? ON_CHAIN : 0);
-      pre += v[q];
-    }
-    if (threadIdx.x == 0) a.span_pk[sp] = tot;
+    if (threadIdx.x == 0) a.span_pk[sp] = carry;
   }
 }
 
