@@ -2184,6 +2184,7 @@ static FastArgs make_fast_args(cyl_ctx* ctx, const StepArgs& a, int j0, int n) {
   f.xi_cap = ctx->xi_cap;
   f.nact = ctx->d_scen_active;
   f.chunk_base = ctx->d_gen_base;
+  f.gen_shift = ctx->dim == 3 ? GEN_SHIFT_3D : GEN_SHIFT_2D;
   f.pregen = ctx->d_pregen;
   f.xi_by_slot = ctx->inline_normals == 2 ? 1 : 0;
   f.skip_us = !ctx->batched && ctx->dim == 2 ? 1 : 0;  // read by the LSM-1 kernel only
@@ -2220,7 +2221,8 @@ static void launch_gen(cyl_ctx* ctx, const FastArgs& f0, int m) {
     return;
   }
   const FastArgs& f = f0;
-  int64_t nb = nblk(((int64_t)ctx->n + GEN_PER - 1) / GEN_PER + ctx->nscen);
+  const int gsh = ctx->dim == 3 ? GEN_SHIFT_3D : GEN_SHIFT_2D;
+  int64_t nb = nblk((((int64_t)ctx->n + (1 << gsh) - 1) >> gsh) + ctx->nscen);
   hipStream_t s = ctx->stream;
 #define CYL_GEN_LAUNCH(R, NV) \
   k_gen_normals<R, NV><<<persistent_grid(ctx, (const void*)k_gen_normals<R, NV>, nb, false), BLK, 0, s>>>(f)
@@ -2607,6 +2609,7 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       pa.uniform = uniform ? 1 : 0;
       pa.pregen = ctx->d_pregen;
       pa.chunk_base = uniform && any_fast && ctx->inline_normals == 0 ? ctx->d_gen_base : nullptr;
+      pa.gen_shift = ctx->dim == 3 ? GEN_SHIFT_3D : GEN_SHIFT_2D;
       if (uniform && method != 1) {
         pa.layout = 1;
         pa.ev = layout_args(ctx, method);

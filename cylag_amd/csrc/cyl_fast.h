@@ -323,6 +323,7 @@ struct FastArgs {
   int64_t xi_cap;     // entries of xi
   const int64_t* nact;        // [nscen] active rows
   const int64_t* chunk_base;  // [nscen + 1] first normal-generator chunk of each scenario
+  int gen_shift;              // log2 of the generator's ranks per chunk
   const int* pregen;          // [nscen] scenario's normals come from a generator this step
   int xi_by_slot;             // the generator stored each slot's pair at xi[slot] (k_gen_slots)
   int skip_us;                // LSM-1: us (= up) is not stored; downloads derive it from up
@@ -404,26 +405,29 @@ __global__ void __launch_bounds__(BLK) k_draw_start(FastArgs a) {
 }
 
 // Normals of the step in pool order.  The active ranks of scenario s are cut
-// into chunks of GEN_PER ranks, numbered globally from chunk_base[s]; thread c
+// into chunks of P = 2^gen_shift ranks (P = GEN_PER = 32 in 2D; 16 in 3D,
+// where a rank takes three normals and 32-rank chunks leave too few waves to
+// hide the draws' latency), numbered globally from chunk_base[s]; thread c
 // serves one chunk, i.e. a contiguous run of the scenario's serial stream.  It
 // finds the run's first position once (event table), jumps there, then draws
 // sequentially: a Ziggurat normal that starts on a slow event takes the
 // event's precomputed value and skips its positions; every other normal is a
-// one-draw fast normal.  The pair of rank r = GEN_PER c_local + k (chunk c =
-// chunk_base[s] + c_local) is stored in tiles of 64 chunks: xi[((c / 64) *
-// GEN_PER + k) * 64 + c % 64], so that each of a wave's stores is one coalesced
-// 1 KB row and its 32 stores stay within 32 KB.  Indices stay below nchunk
-// rounded up to 64, times 32, where nchunk <= (active rows) / 32 + (scenarios):
-// the buffer holds rows + GEN_PER * (scenarios + 2) + 64 * GEN_PER pairs.
+// one-draw fast normal.  The values of rank r = P c_local + k (chunk c =
+// chunk_base[s] + c_local) are stored in tiles of 64 chunks: xi[((c / 64) * P
+// + k) * 64 + c % 64], so that each of a wave's stores is one coalesced row and
+// its P stores stay within 64 P entries.  Indices stay below nchunk rounded up
+// to 64, times P, where nchunk <= (active rows) / P + (scenarios): the buffer
+// holds rows + GEN_PER * (scenarios + 2) + 64 * GEN_PER entries for any P <=
+// GEN_PER.
 constexpr int GEN_PER = 32;
+constexpr int GEN_SHIFT_2D = 5, GEN_SHIFT_3D = 4;
 
-__device__ __forceinline__ int64_t xi_tile_index(int64_t c, int64_t k) {
-  return ((c >> 6) * GEN_PER + k) * 64 + (c & 63);
+__device__ __forceinline__ int64_t xi_tile_index(int64_t c, int64_t k, int sh) {
+  return (((c >> 6) << sh) + k) * 64 + (c & 63);
 }
 
-__device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int nscen, int s, int64_t r) {
-  (void)nscen;
-  return xi_tile_index(chunk_base[s] + r / GEN_PER, r % GEN_PER);
+__device__ __forceinline__ int64_t xi_index(const int64_t* chunk_base, int sh, int s, int64_t r) {
+  return xi_tile_index(chunk_base[s] + (r >> sh), r & ((1 << sh) - 1), sh);
 }
 
 // Step preparation in one block, in stream order of the four small passes it
@@ -448,6 +452,7 @@ struct PrepArgs {
   int uniform;
   const int* pregen;         // [nscen] generator serves the scenario, or null
   int64_t* chunk_base;       // [nscen] first generator chunk, or null (no generator)
+  int gen_shift;             // log2 of the generator's ranks per chunk
   int layout;                // lay out the step's event tables (ev_layout)
   LayoutArgs ev;
 };
@@ -476,7 +481,7 @@ __global__ void __launch_bounds__(BLK) k_step_prep(PrepArgs a) {
       const bool ended = a.scen_end && step >= a.scen_end[s];
       if (a.uniform) a.scen_units[s] = ended ? 0 : (int64_t)a.st[s].units_per_part * act;
       if (a.chunk_base && (!a.pregen || a.pregen[s]) && !ended && a.st[s].units_per_part > 0)
-        v = (act + GEN_PER - 1) / GEN_PER;
+        v = (act + (1 << a.gen_shift) - 1) >> a.gen_shift;
     }
     if (!a.chunk_base) continue;  // uniform across the block
     part[threadIdx.x] = v;
@@ -645,10 +650,11 @@ __device__ __forceinline__ unsigned long long gen_chunk(const FastArgs& a, int64
                                                         uint64_t t8_inc, const ZigFast& zt) {
   const StreamDev& st = a.stream[s];
   if (st.units_per_part == 0) return 0;
-  int64_t r0 = (c - a.chunk_base[s]) * GEN_PER, r1 = r0 + GEN_PER < a.nact[s] ? r0 + GEN_PER : a.nact[s];
+  const int64_t per = 1 << a.gen_shift;
+  int64_t r0 = (c - a.chunk_base[s]) << a.gen_shift, r1 = r0 + per < a.nact[s] ? r0 + per : a.nact[s];
   int err = 0;
   unsigned long long pend = gen_run<RNG, NV>(a, s, r0, r1, t8, t8_inc, zt, err, [&](int i, const double* v) {
-    int64_t xidx = xi_tile_index(c, i);
+    int64_t xidx = xi_tile_index(c, i, a.gen_shift);
     if (xidx >= a.xi_cap) return;
     if (NV == 2) {
       a.xi[xidx] = make_double2(v[0], v[1]);
@@ -764,7 +770,7 @@ __device__ __forceinline__ unsigned long long fast_particle(const FastArgs& a, i
   const int tri0 = a.triA[j];
   const double x0 = a.xA[j], y0 = a.yA[j];
   int64_t xidx = 0;
-  if (PREGEN) xidx = a.xi_by_slot ? (int64_t)j : xi_index(a.chunk_base, a.nscen, s, a.rank[j]);
+  if (PREGEN) xidx = a.xi_by_slot ? (int64_t)j : xi_index(a.chunk_base, a.gen_shift, s, a.rank[j]);
   const int te = (unsigned)tri0 < (unsigned)a.fm.nt ? tri0 : 0;
   const ElemF& ef = a.fm.ef[te];
   const int4 en = a.fm.en[te];

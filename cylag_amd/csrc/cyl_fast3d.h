@@ -99,7 +99,7 @@ __device__ __forceinline__ unsigned long long fast3d_particle(const Fast3Args& a
   const int s = a.scen ? a.scen[j] : 0;
   // the record's load is issued here, before the branches, so that its latency
   // overlaps the element and node loads
-  const int64_t xidx = PREGEN ? xi_index(a.chunk_base, a.nscen, s, a.rank[j]) : 0;
+  const int64_t xidx = PREGEN ? xi_index(a.chunk_base, a.gen_shift, s, a.rank[j]) : 0;
   XiRec3 xr{0., 0., 0., 0.};
   if (PREGEN) xr = reinterpret_cast<const XiRec3*>(a.xi)[(uint64_t)xidx < (uint64_t)a.xi_cap ? xidx : 0];
   int tri0 = a.triA[j], ll0 = a3.llA[j];
