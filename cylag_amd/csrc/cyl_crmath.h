@@ -225,11 +225,13 @@ __device__ __forceinline__ double cr_exp(double x) {
 // exp is within one ulp of e^a, so a comparison outside a band of 2^-50
 // relative around it (four ulps and more) is already decided; inside the band
 // the correctly rounded value decides.  a >= -708 (normal results).
+__device__ __noinline__ double cr_exp_call(double x) { return cr_exp(x); }
+
 __device__ __forceinline__ bool lt_exp(double l, double a) {
   double e = exp(a);
   if (l < e * (1. - 0x1p-50)) return true;
   if (l > e * (1. + 0x1p-50)) return false;
-  return l < cr_exp(a);
+  return l < cr_exp_call(a);  // rare: out of line, so callers keep their registers
 }
 
 // log(x) in double-double, x positive, normal and finite
@@ -266,6 +268,9 @@ __device__ __forceinline__ double cr_log(double x) {
   if (cr_log_try(x, r)) return r;
   return cr_log_dd(x);
 }
+
+// cr_log out of line, for rarely taken paths
+__device__ __noinline__ double cr_log_call(double x) { return cr_log(x); }
 
 // pow = e^(y log x): the fast log's absolute error err (at most 2^-68.9)
 // becomes |y| err in a = y log x, with the product's rounding (2^-104 |a|),

@@ -168,7 +168,7 @@ __device__ __forceinline__ bool zig_fast(uint32_t u, const uint32_t* kn) {
 // LDS and finished by the block at the end, so that the waves sampling the
 // other candidates do not wait on it.  The chunks are those of the step's
 // layout (ev_layout).
-constexpr int TAIL_CAP = 256;
+constexpr int TAIL_CAP = 128;  // a block queues about 10-50 (a few % of its candidates); a full queue finishes in place
 struct TailRec {
   uint64_t state;
   int64_t i, pos;
@@ -183,7 +183,7 @@ __device__ __forceinline__ void reg_finish(const EventArgs& e, int64_t r, double
 }
 
 template <int METHOD>
-__global__ void __launch_bounds__(BLK) k_ev_write(EventArgs e) {
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(5))) k_ev_write(EventArgs e) {
   if (e.cond && *e.cond == 0) return;
   __shared__ JumpTable8 t8;
   __shared__ uint32_t kn[128];

@@ -147,8 +147,8 @@ struct Draws {
 
   __device__ double zig_tail(int32_t hz) {
     for (;;) {
-      double x = -cr_log(u32_to_uniform(next())) * 0.2904764;
-      double y = -cr_log(u32_to_uniform(next()));
+      double x = -cr_log_call(u32_to_uniform(next())) * 0.2904764;  // the base strip's tail: rare
+      double y = -cr_log_call(u32_to_uniform(next()));
       if (y + y >= x * x) return hz > 0 ? ZIG_R + x : -ZIG_R - x;
     }
   }
