@@ -1922,7 +1922,6 @@ static void enqueue_events(cyl_ctx* ctx, int method, const int* cond, bool layou
   e.span_off = ctx->d_span_off;
   e.span_max = ctx->d_span_max;
   e.span_carry = ctx->d_span_carry;
-  e.nspan = ctx->nspan;
   e.span_cap = method == 2 ? EV_SPAN_CAP_ZIG : EV_SPAN_CAP_POLAR;
   e.reg_pos = ctx->d_reg_pos;
   e.reg_len = ctx->d_reg_len;

@@ -46,9 +46,8 @@ struct EventArgs {
   int64_t* span_off;          // per span: first dense index (scan of span_cnt)
   int64_t* span_max;          // per span (rng_method 2): largest reach key, -1 if none
   int64_t* span_carry;        // per span: exclusive max-scan of span_max
-  int64_t nspan;              // spans of the chunk capacity
   int span_cap;               // candidate records per span region
-  int64_t* reg_pos;           // span regions (nspan * span_cap): position, and for rng_method 2
+  int64_t* reg_pos;           // span regions (span_cap records per span of the chunk capacity): position, and for rng_method 2
   int64_t* reg_len;           //   the length, value and reach key of the normal starting there
   double* reg_val;
   int64_t* reg_key;

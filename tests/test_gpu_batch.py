@@ -264,3 +264,28 @@ def test_batch_square_mesh_uneven_mixed(rng_method):
         alone = run_alone(case, cy.LagrangianParticleSet(pos[s].copy(), dim=2), params[s], st[s], int(nsteps[s]))
         compare(out[s], alone, b.rng_state(s))
     b.ctx.close()
+
+
+@pytest.mark.parametrize("rng_method", [2, 3])
+def test_batch_dead_zone_general_rerun(rng_method):
+    """Rows in a zone without turbulence draw nothing, so the draws per row are
+    not uniform: each batched step is rerun in general mode, whose event tables
+    are laid out from the counted units of every scenario.  Each scenario
+    against its plain run alone and against the oracle."""
+    case = vortex_case(npart=4000, dead_zone=True)
+    base = dict(final_time=10000., time_step=5., rng_method=rng_method, model=1, diffusion_model=2)
+    counts, steps = [900, 700, 400], [18, 11, 0]
+    params = [dict(base) for _ in counts]
+    st = streams(len(counts))
+    mesh = cy.TriangularMesh(case.x, case.y, case.tri)
+    fs = cy.EulerianFieldSet(mesh, trifinder=case.trifinder(), **case.fields)
+    psets = [make_pset(case, n, 300 + i) for i, n in enumerate(counts)]
+    assert all((ps.position[:, 0] < 500.).any() for ps in psets), "every scenario has rows in the dead zone"
+    b = cy.BatchSolver(fs, psets, params, st, nsteps=steps, sort_interval=4)
+    b.run()
+    out = b.results()
+    for s, n in enumerate(counts):
+        alone = run_alone(case, make_pset(case, n, 300 + s), params[s], st[s], steps[s])
+        compare(out[s], alone, b.rng_state(s))
+        compare(out[s], run_oracle(case, pick(case, n, 300 + s), params[s], st[s], steps[s]), b.rng_state(s))
+    b.ctx.close()
