@@ -559,12 +559,6 @@ __global__ void __launch_bounds__(BLK) k_sort_place(const int* key, int n, int64
   }
 }
 
-template <class T>
-__global__ void __launch_bounds__(BLK) k_permute(const T* src, T* dst, const int* perm, int n) {
-  int j = blockIdx.x * BLK + threadIdx.x;
-  if (j < n) dst[j] = src[perm[j]];
-}
-
 __global__ void __launch_bounds__(BLK) k_slot_of_pool(const int* gpool, int n, int* slot_of_pool) {
   int j = blockIdx.x * BLK + threadIdx.x;
   if (j < n) slot_of_pool[gpool[j]] = j;
