@@ -96,6 +96,7 @@ struct cyl_ctx {
   const double** d_ti_src = nullptr;        // [ntimes][nti] frame pointers of the interpolated fields
   double** d_ti_dst = nullptr;              // [2][nti] destination per buffer parity
   int nti = 0, tmp_parity = 0;
+  std::vector<int> ti_fid;                  // field id of each interpolated field
   double* d_tmpbuf2[4] = {};                // second buffer of the main fields (previous values)
 
   // particles
@@ -635,6 +636,7 @@ extern "C" int cyl_fields_upload(cyl_ctx* ctx, int dim, int nlayers, int npoints
     if (!(fid < 4 && fid >= ntmp)) ti.push_back(fid);
   int nti = (int)ti.size();
   ctx->nti = nti;
+  ctx->ti_fid = ti;
   std::vector<const double*> srct((size_t)ntimes * nti);
   for (int r = 0; r < ntimes; ++r)
     for (int q = 0; q < nti; ++q) srct[(size_t)r * nti + q] = ctx->frame_ptr[(size_t)ti[q] * ntimes + r];
@@ -723,12 +725,51 @@ static int time_interp_state(cyl_ctx* ctx, double time, double& tf) {
 // other buffer, which is what the reference copies into _tmp_*0.  clock: the
 // step is being captured into a graph; the record and weight then come from
 // the call's table through the device step clock.
-static int enqueue_time_interp(cyl_ctx* ctx, double time, int& launches, const int* clock = nullptr) {
+// pack: also write the 2D fused kernels' node records from the interpolated
+// values (k_interp_pack), fac = C_mu / sigma_c, keps: with the k-epsilon K.
+struct PackReq {
+  bool on = false, keps = false;
+  double fac = 0.;
+};
+
+static int enqueue_time_interp(cyl_ctx* ctx, double time, int& launches, const int* clock = nullptr,
+                               const PackReq& pack = PackReq()) {
   if (ctx->frozen) return CYL_OK;
   double tf = 0.;
   int e = time_interp_state(ctx, time, tf);
   if (e) return e;
   const int rec = ctx->record, par = ctx->tmp_parity;
+  if (pack.on) {
+    const FieldsDev& fd = ctx->fields;
+    auto qof = [&](int fid) {
+      for (int q = 0; q < ctx->nti; ++q)
+        if (ctx->ti_fid[q] == fid) return q;
+      return -1;
+    };
+    InterpPack ip{};
+    ip.frames = ctx->d_ti_src;
+    ip.out = ctx->d_ti_dst + (size_t)par * ctx->nti;
+    ip.nf = ctx->nti;
+    ip.n = fd.npoints;
+    ip.rec = rec;
+    ip.tf = tf;
+    ip.tab = ctx->d_interp_tab;
+    ip.clock = clock;
+    ip.step0 = ctx->interp_step0;
+    ip.q_u = qof(fd.fid_ux);
+    ip.q_v = qof(fd.fid_uy);
+    ip.q_k = pack.keps ? qof(fd.fid_turbeng) : -1;
+    ip.q_e = pack.keps ? qof(fd.fid_dissip) : -1;
+    ip.fac = pack.fac;
+    ip.nnew = ctx->d_node_new;
+    ip.rec_out = ctx->d_nodes;
+    if (ip.q_u < 0 || ip.q_v < 0 || (pack.keps && (ip.q_k < 0 || ip.q_e < 0)))
+      return ctx->set_err(CYL_E_STATE, -1, "node records: a field of the fused kernel is not interpolated");
+    if (clock) k_interp_pack<true><<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(ip);
+    else k_interp_pack<false><<<nblk(fd.npoints), BLK, 0, ctx->stream>>>(ip);
+    ++launches;
+    return CYL_OK;
+  }
   dim3 g(nblk(ctx->fsize), ctx->nti);
   if (clock)
     k_time_interp_tab<<<g, BLK, 0, ctx->stream>>>(ctx->d_ti_src, ctx->d_ti_dst + (size_t)par * ctx->nti, ctx->nti,
@@ -2530,8 +2571,26 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       if (e) return e;
       ctx->steps_since_sort = 0;
     }
+    // the fused kernels' node records: k-epsilon K when a fused scenario uses it
+    bool keps = false;
+    double sigc = P[0].schmidt_number;
+    for (int s = 0; s < ns; ++s)
+      if (!ctx->batched || ctx->classes_fast_of_scen(s)) {
+        keps |= P[s].diffusion_model >= 2;
+        sigc = P[s].schmidt_number;
+      }
+    // 2D fields that change every step: the records are written by the time
+    // interpolation itself (the nodes' values are in registers there)
+    PackReq pack;
+    pack.on = any_fast && !init_only && !ctx->frozen && ctx->dim == 2 && ctx->fields.nlayers == 1;
+    pack.keps = keps;
+    pack.fac = CMU / sigc;
+    if (pack.on) {
+      e = ensure_fast(ctx);
+      if (e) return e;
+    }
     if (!init_only) {
-      e = enqueue_time_interp(ctx, t, launches, cap ? ctx->d_clock : nullptr);
+      e = enqueue_time_interp(ctx, t, launches, cap ? ctx->d_clock : nullptr, pack);
       if (e) return e;
     }
     // per-slot arrays (their buffers are swapped by the sort)
@@ -2549,16 +2608,9 @@ static int run_steps(cyl_ctx* ctx, const cyl_params* p, double* time_inout, doub
       e = ensure_fast(ctx);
       if (e) return e;
       const FieldsDev& fd = ctx->fields;
-      bool keps = false;
-      double sigc = P[0].schmidt_number;
-      for (int s = 0; s < ns; ++s)
-        if (!ctx->batched || ctx->classes_fast_of_scen(s)) {
-          keps |= P[s].diffusion_model >= 2;
-          sigc = P[s].schmidt_number;
-        }
       // frozen fields: the records packed at an earlier step are still the step's
-      const bool fresh = ctx->frozen && ctx->pack_stamp == ctx->field_stamp && ctx->packed_keps == (int)keps &&
-                         ctx->packed_sigc == sigc;
+      const bool fresh = pack.on || (ctx->frozen && ctx->pack_stamp == ctx->field_stamp &&
+                                     ctx->packed_keps == (int)keps && ctx->packed_sigc == sigc);
       if (!fresh) {
         if (ctx->dim == 3) {
           int64_t nrec = (int64_t)fd.npoints * fd.nlayers;

@@ -74,6 +74,60 @@ __global__ void __launch_bounds__(BLK) k_pack_nodes(const double* u, const doubl
   out[i] = r;
 }
 
+// The step's time interpolation of every field at mesh node o (k_time_interp:
+// tmp = tf f[rec] + (1 - tf) f[rec + 1]) with the node's packed record
+// (k_pack_nodes, from the values just interpolated) in one pass: the frames
+// and tmp are read and written in mesh order, the record goes to the node's
+// packed index nnew[o].  q_u, q_v, q_k, q_e: the positions of u, v, k and eps
+// among the interpolated fields (q_k < 0: no k-epsilon diffusivity, K = 0).
+// TAB: a captured step, whose record and weight come from the step's entry
+// of the table through the device step clock.
+struct InterpPack {
+  const double* const* frames;  // [ntimes][nf] frame pointers of the interpolated fields
+  double* const* out;           // [nf] destinations
+  int nf;
+  int64_t n;                    // nodes
+  int64_t rec;                  // eager steps: the record and weight
+  double tf;
+  const InterpStep* tab;        // captured steps
+  const int* clock;
+  int step0;
+  int q_u, q_v, q_k, q_e;
+  double fac;
+  const int* nnew;
+  NodeF* rec_out;
+};
+
+template <bool TAB>
+__global__ void __launch_bounds__(BLK) k_interp_pack(InterpPack p) {
+  const int64_t o = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (o >= p.n) return;
+  int64_t rec = p.rec;
+  double tf = p.tf;
+  if (TAB) {
+    const InterpStep e = p.tab[p.clock[0] - p.step0];
+    rec = e.rec;
+    tf = e.tf;
+  }
+  const double* const* f0 = p.frames + rec * p.nf;
+  const double* const* f1 = f0 + p.nf;
+  double u = 0., v = 0., k = 0., ep = 0.;
+  for (int q = 0; q < p.nf; ++q) {
+    const double val = tf * f0[q][o] + (1. - tf) * f1[q][o];
+    p.out[q][o] = val;
+    if (q == p.q_u) u = val;
+    if (q == p.q_v) v = val;
+    if (q == p.q_k) k = val;
+    if (q == p.q_e) ep = val;
+  }
+  NodeF r;
+  r.u = u;
+  r.v = v;
+  r.K = p.q_k >= 0 ? p.fac * k * k / fmax(ep, EPS) : 0.;
+  r.pad = 0.;
+  p.rec_out[p.nnew[o]] = r;
+}
+
 // element records from the mesh tables (built once per mesh); node ids in
 // the packed numbering nnew[old]
 __global__ void __launch_bounds__(BLK) k_pack_elems(MeshDev m, const int* nnew, ElemF* ef, int4* en, int4* te) {
