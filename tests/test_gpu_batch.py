@@ -289,3 +289,27 @@ def test_batch_dead_zone_general_rerun(rng_method):
         compare(out[s], alone, b.rng_state(s))
         compare(out[s], run_oracle(case, pick(case, n, 300 + s), params[s], st[s], steps[s]), b.rng_state(s))
     b.ctx.close()
+
+
+@pytest.mark.parametrize("rng_method", [2, 3])
+def test_batch_large_scenarios_many_spans(rng_method):
+    """Three scenarios of 1e5-3e5 particles: each one's stream window covers
+    many spans of the event tables, whose reach keys, counts and bucket
+    entries carry across spans and scenario boundaries.  Each scenario against
+    its plain run alone and against the oracle."""
+    case = vortex_case(nx=201, ny=51, lx=10000., ly=2500., npart=600_000, seed=12, wavelength=2000.)
+    base = dict(final_time=10000., time_step=5., rng_method=rng_method, model=1, diffusion_model=2)
+    counts, steps = [300_000, 180_000, 100_000], [3, 2, 3]
+    params = [dict(base) for _ in counts]
+    st = streams(len(counts))
+    mesh = cy.TriangularMesh(case.x, case.y, case.tri)
+    fs = cy.EulerianFieldSet(mesh, trifinder=case.trifinder(), **case.fields)
+    b = cy.BatchSolver(fs, [make_pset(case, n, 400 + i) for i, n in enumerate(counts)], params, st, nsteps=steps,
+                       sort_interval=2)
+    b.run()
+    out = b.results()
+    for s, n in enumerate(counts):
+        alone = run_alone(case, make_pset(case, n, 400 + s), params[s], st[s], steps[s])
+        compare(out[s], alone, b.rng_state(s))
+        compare(out[s], run_oracle(case, pick(case, n, 400 + s), params[s], st[s], steps[s]), b.rng_state(s))
+    b.ctx.close()
